@@ -1,0 +1,191 @@
+"""Benchmark of the granular-media time step on one MI355X.
+
+Workload (configuration 1): 256 x 256 x 4 cells of 2 nm, two exchange-coupled 4 nm
+physical layers (Ku 2e6 top, 1e7 bottom, Ms 700, Voronoi grains of 8 nm pitch with
+one-cell boundaries), T = 300 K thermal field from the on-device Philox source, uniform
+field tilted 5 deg from -z ramped 0 -> 20 kOe over 2 ns, Heun at dt = 50 fs, periodic FFT
+demag.  A step is one full Heun step (two field evaluations, two pushes) of all cells.
+
+value = cells * steps / wall seconds of the FP64 plain form with every input resident on
+the device.  The FP32 precision form is timed on the same workload and compared with it.
+The JSON line also carries the roofline of the dominant kernel (HIP-event durations on
+the library's stream, algorithmic bytes per launch) and a CPU baseline (the oracle, the
+C/OpenMP restatement of the reference path, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from mrm_amd import abi, media  # noqa: E402
+from mrm_amd.llg import Llg  # noqa: E402
+
+METRIC = "LLG cell-updates/s FP32 vs FP64 at max|Δm|≤1e-4; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+DT = 5e-14
+SEED = 1
+
+
+def build_ctx(med, precision, tensor=None):
+    x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=precision)
+    N = None
+    if tensor is None:
+        N = x.build_tensor(num_images=2, return_real=True)
+    else:
+        x.set_tensor(tensor)
+    x.set_media(med)
+    for k, u in enumerate(media.ramp_field(med.n_lay)):
+        x.set_ufield(k, u)
+    x.set_llg(dt=DT)
+    return x, N
+
+
+def timed(x, steps, t0):
+    t = time.perf_counter()
+    d = x.step(steps, t0=t0, philox_seed=SEED)     # returns after the device work is done
+    return time.perf_counter() - t, d
+
+
+def col_bytes(med, P):
+    """Algorithmic bytes of one column launch: spectra in and out (12 n_lay Kc P) plus the
+    unique tensor (T_u Kc 2P, T_u = 6 n_lay (n_lay+1)/2)."""
+    nl = med.n_lay
+    kc = (med.nx // 2 + 1) * med.ny
+    tu = 6 * nl * (nl + 1) // 2
+    return 12 * nl * kc * P + tu * kc * 2 * P
+
+
+def kernel_bytes(med, P):
+    nl, C = med.n_lay, med.cells
+    kc = (med.nx // 2 + 1) * med.ny
+    G = med.G
+    return {
+        "row_fwd": 4 * C + 6 * nl * kc * P + 4 * P * G,
+        "col": col_bytes(med, P),
+        "row_inv": 6 * nl * kc * P + 3 * C * P,
+        "grain1": 3 * C * P + 4 * C + 25 * P * G,
+        "grain2": 3 * C * P + 4 * C + 25 * P * G,
+    }
+
+
+def cpu_baseline(med, tensor, target_s=15.0):
+    """The oracle (C restatement, OpenMP) on the same workload: a bounded sample of steps."""
+    from oracle import oracle as O
+    g = abi.make_geom(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz)
+    s = O.Sim(g)
+    s.set_tensor(tensor)
+    for k, L in enumerate(med.layers):
+        s.set_layer(k, L)
+    for k, u in enumerate(media.ramp_field(med.n_lay)):
+        s.set_ufield(k, u)
+    rng = np.random.default_rng(3)
+    probe = 2
+    noise = rng.standard_normal((probe, med.G, 3))
+    t = time.perf_counter()
+    s.step(probe, DT, noise=noise)
+    per = (time.perf_counter() - t) / probe
+    n = int(max(2, min(400, target_s / max(per, 1e-6))))
+    noise = rng.standard_normal((n, med.G, 3))
+    t = time.perf_counter()
+    s.step(n, DT, t0=probe * DT, noise=noise)
+    el = time.perf_counter() - t
+    return {"value": med.cells * n / el, "unit": "cell-updates/s", "cores": int(O.lib().ora_num_threads()),
+            "kind": "port",
+            "sample": f"{n} FP64 Heun steps of the same 256x256x4 workload (injected normals), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        return 2
+    log = (lambda *x: None) if a.quiet else (lambda *x: print(*x, file=sys.stderr, flush=True))
+
+    med = media.config_media("cfg1")
+    log(f"workload cfg1: {med.nx}x{med.ny}x{med.n_lay} cells, {med.G} grains")
+    x64, N = build_ctx(med, abi.MRMG_FP64)
+    x64.step(a.warmup, philox_seed=SEED)
+    t0 = a.warmup * DT
+    el64, _ = timed(x64, a.steps, t0)
+    v64 = med.cells * a.steps / el64
+    log(f"fp64: {el64 * 1e3 / a.steps:.4f} ms/step, {v64:.4e} cell-updates/s")
+
+    # per-kernel durations with HIP events on the library's stream, same workload
+    x64.profile(True)
+    x64.step(a.steps, t0=t0 + a.steps * DT, philox_seed=SEED)
+    st = x64.kernel_stats()
+    x64.profile(False)
+    kb = kernel_bytes(med, 8)
+    kern = {k: {"launches": n, "avg_us": 1e3 * ms / max(n, 1), "share": 0.0} for k, (n, ms) in st.items()}
+    tot = sum(ms for _, ms in st.values())
+    for k in kern:
+        kern[k]["share"] = st[k][1] / tot if tot else 0.0
+        kern[k]["gbs"] = kb[k] / (kern[k]["avg_us"] * 1e-6) / 1e9 if kern[k]["avg_us"] > 0 else 0.0
+    dom = max(kern, key=lambda k: kern[k]["share"])
+    log("kernels: " + ", ".join(f"{k} {v['avg_us']:.2f}us {v['share'] * 100:.0f}% {v['gbs']:.0f}GB/s" for k, v in kern.items()))
+    ach = kern[dom]["gbs"]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": kb[dom], "avg_launch_us": round(kern[dom]["avg_us"], 3),
+            "copy_ceiling_gbs": None}
+    try:
+        nb = 1 << 30
+        ms = x64.copy_probe(nb, reps=10)
+        roof["copy_ceiling_gbs"] = round(2 * nb / (ms * 1e-3) / 1e9, 1)
+    except Exception as e:  # pragma: no cover
+        log("copy probe failed:", e)
+
+    out = {"metric": METRIC, "value": v64, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": el64 * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded Voronoi media, Philox thermal field)",
+           "config": {"workload": "cfg1: 256x256x4 cells of 2 nm, two coupled 4 nm layers, T=300 K, "
+                                  "Heun dt=50 fs, periodic FFT demag, FP64 plain form",
+                      "grid": [med.nx, med.ny, med.n_lay], "cells": med.cells, "grains": med.G,
+                      "grain_updates_per_s": med.G * a.steps / el64},
+           "roofline": roof,
+           "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}}
+
+    if not a.no_fp32:
+        x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
+        x32.step(a.warmup, philox_seed=SEED)
+        el32, _ = timed(x32, a.steps, t0)
+        v32 = med.cells * a.steps / el32
+        # accuracy of the precision form: same start, same noise, same number of steps
+        ref, _ = build_ctx(med, abi.MRMG_FP64, tensor=N)
+        ref.step(a.warmup + a.steps, philox_seed=SEED)
+        dmax, drms = 0.0, 0.0
+        for k in range(med.n_lay):
+            d = np.abs(ref.get_state(k)[0] - x32.get_state(k)[0])
+            dmax = max(dmax, float(d.max()))
+            drms += float((d ** 2).sum())
+        drms = (drms / (3 * med.G)) ** 0.5
+        log(f"fp32: {el32 * 1e3 / a.steps:.4f} ms/step, {v32:.4e} cell-updates/s, x{v32 / v64:.2f}, "
+            f"max|dm| {dmax:.2e} rms {drms:.2e} after {a.warmup + a.steps} steps")
+        out["fp32_form"] = {"value": v32, "unit": "cell-updates/s", "ms_per_step": el32 * 1e3 / a.steps,
+                            "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64": dmax, "rms_dm_vs_fp64": drms,
+                            "steps_compared": a.warmup + a.steps}
+
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(med, N)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,199 @@
+/*
+ * mrmg.h - C ABI of the MI355X (gfx950) granular-media time-stepping path.
+ *
+ * The library replaces the time loop of mrm's `llg_m` module together with the
+ * field assembly of `field_m` and the FFT demag convolution of `demag_m`/`fftw_m`:
+ *
+ *   llg.f90:111-318   llg()        time loop, termination          -> mrmg_step (n_steps per call)
+ *   llg.f90:320-388   get_fields() field + thermal draw + dt       -> stage-1 kernel
+ *   llg.f90:1195-1271 push_m()     two-stage Heun control          -> mrmg_step
+ *   llg.f90:1066-1155 push_llg()   Heun / Boris / quaternion push  -> grain stage kernels
+ *   field.f90:160-273 get_field()  H = Happ + Hdemag + Hk + Hex    -> grain stage kernels
+ *   field.f90:397-436 get_thermal()                                -> injected normals or Philox
+ *   field.f90:438-517 get_magnetostatic() cell -> grain average    -> grain stage kernels
+ *   data.f90:322-390  get_mag()    grain -> cell scatter           -> row r2c kernel
+ *   demag.f90:33-135  get_demag()  r2c, k-space product, c2r       -> row/column FFT kernels
+ *   demag.f90:200-286 init_demag() tensor FFT                      -> mrmg_set_tensor
+ *   tensor.f90:82-216 calculate_tensor() (plane_to_plane images)   -> mrmg_build_tensor
+ *
+ * Conventions follow the reference's one C-ABI precedent (src/FP/fp_main.cpp:600-660,
+ * src/FP/fp.f90:12-44): an opaque handle passed by value, a factory that owns it, no
+ * exceptions across the boundary.  Every entry point returns an int status (0 = ok); a
+ * non-zero status leaves a message readable with mrmg_last_error().
+ *
+ * Array arguments use the reference's own Fortran layouts and 1-based index values, so
+ * the Fortran caller passes its `data_m`/`field_m` arrays unchanged:
+ *   (3,ng) arrays are x,y,z-fastest; grid2grain is (nx,ny) x-fastest; grain2grid is
+ *   (2,ncell) with 1-based cell coordinates; grain2gridindex is the (0:ng) CSR offset
+ *   array; hxgrain/uhxgrain/lhxgrain hold 1-based grain numbers.
+ * Layer 1 of the reference (the top layer) is layer index 0 here.
+ * Units are the reference's CGS units (cm, emu/cc, Oe, K, s).
+ */
+#ifndef MRMG_H
+#define MRMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define MRMG_OK               0
+#define MRMG_ERR_ARG          1  /* bad argument (null pointer, size out of range)      */
+#define MRMG_ERR_HIP          2  /* HIP runtime error (no device, launch failure, ...)   */
+#define MRMG_ERR_STATE        3  /* call order violated (e.g. step before set_layer)     */
+#define MRMG_ERR_UNSUPPORTED  4  /* configuration outside what the kernels implement     */
+#define MRMG_ERR_NOMEM        5  /* device allocation failed                             */
+
+/* ---- precision forms -------------------------------------------------------- */
+#define MRMG_FP64 0   /* plain form: every stage in FP64 (reference built with FFTD)      */
+#define MRMG_FP32 1   /* precision form: FFT, convolution and push in FP32, |m|=1 renorm */
+
+/* ---- thermal noise sources --------------------------------------------------- */
+#define MRMG_NOISE_NONE   0  /* no thermal field (thermal_p = .false.)                    */
+#define MRMG_NOISE_INJECT 1  /* caller-supplied standard normals, (3, G, n_steps)         */
+#define MRMG_NOISE_PHILOX 2  /* on-device Philox4x32-10 keyed by seed, counter = step     */
+
+/* ---- integrator types (llg_type, llg.f90:179-199) ------------------------------ */
+#define MRMG_TYPE_HEUN       (-3)  /* Heun predictor-corrector, llg.f90:1122-1139          */
+#define MRMG_TYPE_BORIS        2   /* semi-implicit rotation, llg.f90:1109-1118            */
+#define MRMG_TYPE_BORIS_MID  (-2)  /* Boris, mid-point two-stage form                       */
+#define MRMG_TYPE_QUAT         1   /* analytic rotation with quaternions, llg.f90:1093-1107 */
+#define MRMG_TYPE_QUAT_MID   (-1)
+
+typedef struct mrmg_ctx mrmg_ctx;
+
+/* Geometry: set_geometry (data.f90:407-413) + start_demag (demag.f90:184-198). */
+typedef struct {
+    int nx, ny, n_lay;        /* FFT grid = media grid (the moving window is not used) */
+    int precision;            /* MRMG_FP64 or MRMG_FP32                                 */
+    int demag;                /* demag_p: 0 skips the magnetostatic field entirely      */
+    double dx, dy;            /* cell size (cm)                                         */
+    const double *dz;         /* [n_lay] layer thickness (cm), layer index 0 = top      */
+    const double *il_space;   /* [n_lay] spacing to the layer above (cm)                */
+} mrmg_geom;
+
+/* One layer's grains: the file_grain (data.f90:219-320) + file_field (field.f90:898-1008)
+ * payload plus the per-layer llg/thermal parameters (data.f90:91-104). */
+typedef struct {
+    int ng;                          /* number of grains                                  */
+    const int *grid2grain;           /* (nx,ny): 1-based grain number, 0 = non-magnetic   */
+    const int *grain2gridindex;      /* (0:ng) CSR offsets into grain2grid                */
+    const int *grain2grid;           /* (2,ncell): 1-based (i,j) of each grain's cells    */
+    const double *grain_vol;         /* (ng) grain volume (cm^3)                           */
+    const double *ms;                /* (ng) Ms (emu/cc)                                   */
+    const double *hk;                /* (2,ng) Hk1, Hk2 (Oe)                               */
+    const double *uni_axis;          /* (3,ng) uniaxial easy axis, or NULL                 */
+    const double *cubic_axis;        /* (3,3,ng) cubic axes, or NULL                       */
+    /* grain-boundary exchange, field.f90:326-363 (hx in Oe, already / (Ms_i A_i)) */
+    const int *hxgrainindex;  const int *hxgrain;  const double *hx;   /* intralayer, or NULL */
+    const int *uhxgrainindex; const int *uhxgrain; const double *uhx;  /* to layer index k-1  */
+    const int *lhxgrainindex; const int *lhxgrain; const double *lhx;  /* to layer index k+1  */
+    /* temperature scaling (temp_scaling.f90:365-431); NULL means 1 */
+    const double *ms_scale, *alpha_scale, *hk_scale;
+    const double *temp;              /* (ng) grain temperature (K); NULL means 0           */
+    const double *h_app;             /* (3,ng) static per-grain applied field, or NULL     */
+    double alpha;                    /* Gilbert damping of the push (llg namelist alpha)   */
+    double gamma;                    /* gyromagnetic ratio (rad/Oe/s)                      */
+    double thm_alpha;                /* alpha used by get_thermal (data_s%thm_alpha)       */
+    double floor_ms;                 /* lower bound of |m| in the thermal variance         */
+    int thermal;                     /* thermal_p                                          */
+} mrmg_layer;
+
+/* Uniform, time-ramped applied field of one layer: the single uniform field branch of
+ * get_appl (applied.f90:863-883) with the linear ramp of set_fld (applied.f90:210-320):
+ *   h(t) = ((s_end - s_start) * clamp((t - t_start)/(t_end - t_start), 0, 1) + s_start)
+ *          * scale * dir                                                               */
+typedef struct {
+    double dir[3];
+    double t_start, t_end, s_start, s_end, scale;
+} mrmg_ufield;
+
+typedef struct {
+    int type;                 /* MRMG_TYPE_*                                            */
+    int n_steps;              /* Heun steps to run in this call                         */
+    double t0;                /* simulation time at the first step (s)                  */
+    double dt;                /* fixed time step (fixed_dt_p, llg.f90:1484-1486)         */
+    int noise;                /* MRMG_NOISE_*                                           */
+    int pad_;
+    uint64_t seed;            /* Philox key                                             */
+    uint64_t step0;           /* Philox counter of the first step                       */
+} mrmg_step_params;
+
+/* Diagnostics of the last step (llg.f90:1209-1257, 217-237). */
+typedef struct {
+    double t;                 /* time after the last step                               */
+    double dt;
+    int steps;                /* steps completed in this call                           */
+    int n_lay;
+    double errmx;             /* max |m x H|/Ms over all grains (sqrt of errmx_l)        */
+    double errmx1;            /* max |dm|^2                                              */
+    double mmag;              /* sum over layers of volume-weighted <|m| ms_scale>       */
+    double mave[8][3];        /* per-layer volume-weighted <m> (up to 8 layers)          */
+} mrmg_diag;
+
+/* ---- lifecycle ----------------------------------------------------------------- */
+const char *mrmg_last_error(void);
+int mrmg_device_count(int *n);
+int mrmg_create(const mrmg_geom *g, mrmg_ctx **out);
+int mrmg_destroy(mrmg_ctx *ctx);
+
+/* Demag tensor, real space, the (nx,ny,9,n_lay,n_lay) FP64 array of calculate_tensor
+ * (tensor.f90:82-216) already divided by dx*dy*dz(ko)*nx*ny; component c = 3(a-1)+b maps
+ * source component b to field component a (demag.f90:117-121).  Transformed on the
+ * device in FP64 (demag.f90:259-267). */
+int mrmg_set_tensor(mrmg_ctx *ctx, const double *n_real);
+/* Build the same tensor on the device in FP64 (tensor.f90:136-183 without SUL/head
+ * images, plane_to_plane with num_images periodic images in x and y) and transform it.
+ * When n_real_out is not NULL, the real-space tensor is also copied back to it. */
+int mrmg_build_tensor(mrmg_ctx *ctx, int num_images, double *n_real_out);
+
+int mrmg_set_layer(mrmg_ctx *ctx, int k, const mrmg_layer *layer);
+int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);
+
+/* Magnetisation state of layer k: m (3,ng) unit vectors (the `in` slot of data_s%m). */
+int mrmg_set_state(mrmg_ctx *ctx, int k, const double *m);
+/* m (3,ng) and the last field h (3,ng) (either may be NULL). */
+int mrmg_get_state(mrmg_ctx *ctx, int k, double *m, double *h);
+
+/* Run p->n_steps fixed-dt steps of the llg loop.  noise: MRMG_NOISE_INJECT only, host
+ * array of standard normals laid out (3, G, n_steps), G = total grains over all layers in
+ * layer order, in the order get_thermal consumes them (field.f90:405-431).  diag may be
+ * NULL.  The call returns after the device work is complete. */
+int mrmg_step(mrmg_ctx *ctx, const mrmg_step_params *p, const double *noise, mrmg_diag *diag);
+
+/* The normals the Philox source draws for steps step0 .. step0+n_steps-1, same (3,G,n)
+ * layout as injected noise, so a CPU path can be fed the identical thermal field. */
+int mrmg_philox_normals(mrmg_ctx *ctx, uint64_t seed, uint64_t step0, int n_steps, double *out);
+
+/* Demag field of the current state on the cell grid: H (nx,ny,n_lay,3) FP64 (component
+ * slowest) and its grain average (3,G); either may be NULL.  Used by tests. */
+int mrmg_demag_field(mrmg_ctx *ctx, double *h_cell, double *h_grain);
+
+/* Device bytes held by the context. */
+int mrmg_footprint(mrmg_ctx *ctx, size_t *bytes);
+
+/* ---- instrumentation (no counterpart in the reference: its only timers are cpu_time
+ * print-outs, demag.f90:280-283, main.f90:31-34) ------------------------------------- */
+typedef struct {
+    char name[32];          /* kernel class: row_fwd, col, row_inv, grain1, grain2        */
+    int launches;           /* launches timed since profiling was enabled                 */
+    double total_ms;        /* sum of HIP-event durations on the context's stream         */
+} mrmg_kstat;
+
+/* enable != 0 brackets every kernel launch of mrmg_step with HIP events on the
+ * context's stream and accumulates per-class durations (resets the counters). */
+int mrmg_profile(mrmg_ctx *ctx, int enable);
+/* Copies up to max entries; *n receives the number of kernel classes. */
+int mrmg_kernel_stats(mrmg_ctx *ctx, mrmg_kstat *out, int max, int *n);
+/* Device-to-device copy of `bytes` bytes on the context's stream with the library's own
+ * streaming copy kernel, `reps` times; *ms receives the average duration.  Used as the
+ * known-size reference when scaling HBM traffic counters. */
+int mrmg_copy_probe(mrmg_ctx *ctx, size_t bytes, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRMG_H */

@@ -1,0 +1,65 @@
+// kernels.h - gfx950 kernels of the granular-media step (FP64 plain form, FP32 precision form).
+//
+// Device data layout (G = grains over all layers in layer order, P = nx*ny cells per layer):
+//   grain vectors      component-major, v[c*G + g]                 (coalesced per component)
+//   cell2grain         [k][y][x] -> global grain or -1               (get_mag scatter, data.f90:344-353)
+//   spectra S1, S2     [k][c][y][kx], kx = 0..nx/2                   (x = halved axis, fftw.f90:66-84)
+//   Hcell              [c][k][y][x]                                   (rx2d, ry2d, rz2d of demag.f90)
+//   Tspec              [kx][ko][ks][slot][ky]                         (n2d of demag.f90:255 regrouped so
+//                                                                     one column workgroup streams one
+//                                                                     contiguous block)
+#pragma once
+#include "fft_lds.h"
+
+#define MRMG_MAX_LAYERS 16
+
+struct LayerParams {
+    double alpha, gamma;
+    double uf_dir[3], uf_t0, uf_t1, uf_s0, uf_s1, uf_scale;
+    int has_uf, has_uni, has_cub, thermal;
+    int off, ng;
+};
+
+// Per-grain static data of the field assembly and push.
+template <typename T> struct GrainData {
+    int G;
+    const int* layer_of;
+    const T* ms;          // Ms
+    const T* msf;         // Ms*ms_scale (scatter factor, data.f90:349)
+    const T* ms_scale;
+    const T* alpha_scale;
+    const T* hk_scale;
+    const T* hk1;
+    const T* hk2;
+    const T* uni;         // [3][G]
+    const T* cub;         // [9][G], (r + 3c) = cubic_axis(r,c,n)
+    const T* happ;        // [3][G] static per-grain applied field, or null
+    const T* sigma;       // thermal std sqrt(x)*tsigma (field.f90:407-423)
+    const T* vol;
+    const int* gc_off;    // grain -> cells CSR (grain2gridindex)
+    const int* gc_idx;    //   global cell index k*P + j*nx + i
+    const int* xi_off; const int* xi_nb; const T* xi_w;   // intralayer exchange CSR
+    const int* xu_off; const int* xu_nb; const T* xu_w;   // to the layer above
+    const int* xl_off; const int* xl_nb; const T* xl_w;   // to the layer below
+    const LayerParams* lp;
+};
+
+// The grain kernel modes
+enum GrainMode {
+    GM_HEUN1 = 0,   // Heun predictor: field at ma, thermal draw, write mb, dm, hthm
+    GM_HEUN2 = 1,   // Heun corrector: field at mb, write ma = norm(ma + (d + dm)/2), h
+    GM_ROT1 = 2,    // rotational push, single stage or mid-point first half: ma -> mb
+    GM_ROT2 = 3,    // rotational mid-point second half: field at mb, push from ma -> ma
+};
+
+struct StepArgs {
+    double t;          // time of the field evaluation of this stage
+    double dt;         // dt0 of push_llg (full step)
+    double dts;        // dt*dt_s, the push's time step
+    double sqrt_dt;    // sqrt(dt0)
+    int type;          // llg_type
+    int noise;         // 0 none, 1 inject (normals[3][G] doubles), 2 philox
+    unsigned long long seed, step;
+    const double* normals;   // injected normals of this step, [G][3]
+    int diag;          // write diagnostics partials
+};

@@ -1,0 +1,1756 @@
+// mrmg.hip - MI355X (gfx950) implementation of mrm's granular-media time step behind the
+// C ABI of include/mrmg.h.  Reference behaviour (paths relative to the reference's src/):
+//   data.f90:322-390   get_mag      -> k_row_fwd   (scatter fused into the row r2c FFT)
+//   demag.f90:33-135   get_demag    -> k_row_fwd, k_col (column FFT + tensor product +
+//                                       inverse column FFT in one LDS pass), k_row_inv
+//   field.f90:160-273  get_field    -> k_grain     (gather, anisotropy, exchange, applied)
+//   field.f90:397-436  get_thermal  -> k_grain     (injected normals or Philox4x32-10)
+//   llg.f90:1066-1155  push_llg     -> k_grain     (Heun / Boris / quaternion)
+//   llg.f90:1195-1271  push_m       -> mrmg_step   (stage sequencing)
+//   tensor_element.f90:13-102, 219-261 -> k_plane_to_plane (FP64 tensor build)
+//   demag.f90:259-267  init_demag   -> k_tensor_rows / k_tensor_cols / k_tensor_pack
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mrmg.h"
+#include "kernels.h"
+
+// ------------------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(MRMG_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x,             \
+                           hipGetErrorString(e_));                                            \
+    } while (0)
+
+#define RCHK(x)                        \
+    do {                               \
+        int r_ = (x);                  \
+        if (r_ != MRMG_OK) return r_;  \
+    } while (0)
+
+// ------------------------------------------------------------------------- device helpers
+template <typename T>
+__device__ __forceinline__ void cross3(const T* a, const T* b, T* t)
+{
+    t[0] = a[1] * b[2] - a[2] * b[1];
+    t[1] = a[2] * b[0] - a[0] * b[2];
+    t[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <typename T>
+__device__ __forceinline__ T dot3(const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter (grain, step lo, step hi, 0),
+// key = seed.  Box-Muller on the first two pairs gives the grain's three normals.
+__device__ __forceinline__ void philox4x32_10(unsigned int c[4], unsigned int k0, unsigned int k1)
+{
+    const unsigned int M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        unsigned int hi0 = __umulhi(M0, c[0]), lo0 = M0 * c[0];
+        unsigned int hi1 = __umulhi(M1, c[2]), lo1 = M1 * c[2];
+        unsigned int n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += W0; k1 += W1;
+    }
+}
+
+__device__ __forceinline__ void philox_normals3(unsigned long long seed, unsigned long long step, int g, double* n)
+{
+    unsigned int c[4] = {(unsigned int)g, (unsigned int)(step & 0xffffffffull), (unsigned int)(step >> 32), 0u};
+    philox4x32_10(c, (unsigned int)(seed & 0xffffffffull), (unsigned int)(seed >> 32));
+    // u in (0,1]: (x + 1) * 2^-32 ; angle from the second word
+    const double inv32 = 2.3283064365386963e-10;
+    double u1 = ((double)c[0] + 1.0) * inv32, a1 = (double)c[1] * inv32 * 6.283185307179586;
+    double u2 = ((double)c[2] + 1.0) * inv32, a2 = (double)c[3] * inv32 * 6.283185307179586;
+    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u2));
+    double s1, c1, s2, c2;
+    sincos(a1, &s1, &c1);
+    sincos(a2, &s2, &c2);
+    n[0] = r1 * c1;
+    n[1] = r1 * s1;
+    n[2] = r2 * c2;
+}
+
+__global__ void k_philox_dump(unsigned long long seed, unsigned long long step0, int nsteps, int G, double* out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nsteps * G) return;
+    int it = (int)(i / G), g = (int)(i - (long long)it * G);
+    double n[3];
+    philox_normals3(seed, step0 + it, g, n);
+    out[3 * i] = n[0]; out[3 * i + 1] = n[1]; out[3 * i + 2] = n[2];
+}
+
+// XCD-aware remap: hardware places block b on XCD b % 8; give the blocks of one XCD a
+// contiguous range of logical ids so neighbouring columns share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb8)
+{
+    const int per = nb8 >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// ------------------------------------------------------------------------- demag kernels
+// get_mag + row r2c: one workgroup per (row pair, layer).  Two rows of each component are
+// packed as z = M(y0) + i M(y1), transformed with one complex FFT and separated.
+template <typename T, int MAXB>
+__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ m, int G, const T* __restrict__ msf,
+                                                 const int* __restrict__ cell2grain, C2<T>* __restrict__ S,
+                                                 int nx, int ny, int nxh, FFTPlan px, const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* re = reinterpret_cast<T*>(smem_raw);
+    T* im = re + 3 * nx;
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
+    const bool has1 = y1 < ny;
+    const int* c0 = cell2grain + ((size_t)k * ny + y0) * nx;
+    const int* c1 = c0 + nx;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        const int g0 = c0[x];
+        const int g1 = has1 ? c1[x] : -1;
+        const T f0 = g0 >= 0 ? msf[g0] : (T)0, f1 = g1 >= 0 ? msf[g1] : (T)0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            re[c * nx + x] = g0 >= 0 ? f0 * m[(size_t)c * G + g0] : (T)0;
+            im[c * nx + x] = g1 >= 0 ? f1 * m[(size_t)c * G + g1] : (T)0;
+        }
+    }
+    __syncthreads();
+    lds_fft<T, MAXB, false>(re, im, 3, px, twx);
+    for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
+        const int c = idx / nxh, kx = idx - c * nxh;
+        const int kn = kx == 0 ? 0 : nx - kx;
+        const T zr = re[c * nx + kx], zi = im[c * nx + kx], nr = re[c * nx + kn], ni = im[c * nx + kn];
+        C2<T> A = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
+        C2<T> B = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
+        C2<T>* row = S + (((size_t)k * 3 + c) * ny + y0) * nxh;
+        row[kx] = A;
+        if (has1) row[nxh + kx] = B;
+    }
+}
+
+// Column pass with the k-space product fused (demag.f90:117-121): one workgroup per
+// (kx, observation layer ko).  The 3*n_lay source columns of this kx are transformed
+// along y in LDS, multiplied by the tensor block of (kx, ko), and the three field
+// columns transformed back.
+template <typename T, int MAXB, bool SYM>
+__global__ void __launch_bounds__(256) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+                                             const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot,
+                                             int nblocks, int nb8, FFTPlan py, const C2<T>* __restrict__ twy)
+{
+    const int L = xcd_remap(blockIdx.x, nb8);
+    if (L >= nblocks) return;
+    const int kx = L % nxh, ko = L / nxh;
+    extern __shared__ unsigned char smem_raw[];
+    const int ncol = 3 * nl;
+    T* ire = reinterpret_cast<T*>(smem_raw);
+    T* iim = ire + ncol * ny;
+    T* ore = iim + ncol * ny;
+    T* oim = ore + 3 * ny;
+    for (int idx = threadIdx.x; idx < ncol * ny; idx += blockDim.x) {
+        const int col = idx / ny, y = idx - col * ny;
+        const C2<T> v = S1[((size_t)col * ny + y) * nxh + kx];
+        ire[idx] = v.x;
+        iim[idx] = v.y;
+    }
+    __syncthreads();
+    lds_fft<T, MAXB, false>(ire, iim, ncol, py, twy);
+    const C2<T>* Tk = Tspec + ((size_t)kx * nl + ko) * nl * nslot * ny;
+    for (int ky = threadIdx.x; ky < ny; ky += blockDim.x) {
+        C2<T> h0 = {0, 0}, h1 = {0, 0}, h2 = {0, 0};
+        for (int ks = 0; ks < nl; ks++) {
+            const C2<T> mx = {ire[(3 * ks + 0) * ny + ky], iim[(3 * ks + 0) * ny + ky]};
+            const C2<T> my = {ire[(3 * ks + 1) * ny + ky], iim[(3 * ks + 1) * ny + ky]};
+            const C2<T> mz = {ire[(3 * ks + 2) * ny + ky], iim[(3 * ks + 2) * ny + ky]};
+            const C2<T>* t = Tk + (size_t)ks * nslot * ny + ky;
+            if (SYM) {  // slots xx yy zz xy xz yz
+                const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
+                C2<T> a;
+                a = cmul(txx, mx); h0.x += a.x; h0.y += a.y;
+                a = cmul(txy, my); h0.x += a.x; h0.y += a.y;
+                a = cmul(txz, mz); h0.x += a.x; h0.y += a.y;
+                a = cmul(txy, mx); h1.x += a.x; h1.y += a.y;
+                a = cmul(tyy, my); h1.x += a.x; h1.y += a.y;
+                a = cmul(tyz, mz); h1.x += a.x; h1.y += a.y;
+                a = cmul(txz, mx); h2.x += a.x; h2.y += a.y;
+                a = cmul(tyz, my); h2.x += a.x; h2.y += a.y;
+                a = cmul(tzz, mz); h2.x += a.x; h2.y += a.y;
+            } else {    // slot 3a+b: field component a from source component b
+                C2<T> a;
+                a = cmul(t[0 * ny], mx); h0.x += a.x; h0.y += a.y;
+                a = cmul(t[1 * ny], my); h0.x += a.x; h0.y += a.y;
+                a = cmul(t[2 * ny], mz); h0.x += a.x; h0.y += a.y;
+                a = cmul(t[3 * ny], mx); h1.x += a.x; h1.y += a.y;
+                a = cmul(t[4 * ny], my); h1.x += a.x; h1.y += a.y;
+                a = cmul(t[5 * ny], mz); h1.x += a.x; h1.y += a.y;
+                a = cmul(t[6 * ny], mx); h2.x += a.x; h2.y += a.y;
+                a = cmul(t[7 * ny], my); h2.x += a.x; h2.y += a.y;
+                a = cmul(t[8 * ny], mz); h2.x += a.x; h2.y += a.y;
+            }
+        }
+        ore[ky] = h0.x; oim[ky] = h0.y;
+        ore[ny + ky] = h1.x; oim[ny + ky] = h1.y;
+        ore[2 * ny + ky] = h2.x; oim[2 * ny + ky] = h2.y;
+    }
+    __syncthreads();
+    lds_fft<T, MAXB, true>(ore, oim, 3, py, twy);
+    for (int idx = threadIdx.x; idx < 3 * ny; idx += blockDim.x) {
+        const int a = idx / ny, y = idx - a * ny;
+        S2[(((size_t)ko * 3 + a) * ny + y) * nxh + kx] = C2<T>{ore[idx], oim[idx]};
+    }
+}
+
+// Row c2r: FFTW's 1D c2r semantics (imaginary parts of the DC and Nyquist bins ignored).
+template <typename T, int MAXB>
+__global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T* __restrict__ Hcell,
+                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+                                                 const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* re = reinterpret_cast<T*>(smem_raw);
+    T* im = re + 3 * nx;
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
+    const bool has1 = y1 < ny;
+    for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
+        const int c = idx / nxh, kx = idx - c * nxh;
+        const C2<T>* row = S2 + (((size_t)k * 3 + c) * ny + y0) * nxh;
+        C2<T> A = row[kx];
+        C2<T> B = has1 ? row[nxh + kx] : C2<T>{0, 0};
+        if (kx == 0 || 2 * kx == nx) { A.y = 0; B.y = 0; }
+        re[c * nx + kx] = A.x - B.y;
+        im[c * nx + kx] = A.y + B.x;
+        if (kx > 0 && nx - kx != kx) {
+            re[c * nx + nx - kx] = A.x + B.y;
+            im[c * nx + nx - kx] = B.x - A.y;
+        }
+    }
+    __syncthreads();
+    lds_fft<T, MAXB, true>(re, im, 3, px, twx);
+    const size_t plane = (size_t)nx * ny;
+    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+        const int c = idx / nx, x = idx - c * nx;
+        T* hc = Hcell + ((size_t)c * nl + k) * plane;
+        hc[(size_t)y0 * nx + x] = re[idx];
+        if (has1) hc[(size_t)y1 * nx + x] = im[idx];
+    }
+}
+
+// ------------------------------------------------------------------------- grain kernel
+template <typename T>
+__device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int k,
+                                         const T* __restrict__ mf, const T* __restrict__ Hcell, size_t CP,
+                                         double t, const T* m, T* h)
+{
+    // get_applied: per-grain static average + uniform ramped field (applied.f90:871-883)
+    T happ[3] = {0, 0, 0};
+    if (gd.happ) {
+        happ[0] = gd.happ[g]; happ[1] = gd.happ[gd.G + g]; happ[2] = gd.happ[2 * gd.G + g];
+    }
+    if (L.has_uf) {
+        double c = (t - L.uf_t0) / (L.uf_t1 - L.uf_t0);
+        c = c < 1.0 ? c : 1.0;
+        c = c > 0.0 ? c : 0.0;
+        const double sc = ((L.uf_s1 - L.uf_s0) * c + L.uf_s0) * L.uf_scale;
+#pragma unroll
+        for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)(sc * L.uf_dir[i]);
+    }
+    // get_magnetostatic gather (field.f90:463-468)
+    T hd[3] = {0, 0, 0};
+    if (Hcell) {
+        const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
+        for (int q = b0; q < b1; q++) {
+            const int ci = gd.gc_idx[q];
+            hd[0] += Hcell[ci];
+            hd[1] += Hcell[CP + ci];
+            hd[2] += Hcell[2 * CP + ci];
+        }
+        const T cnt = (T)(b1 - b0);
+        hd[0] = hd[0] / cnt; hd[1] = hd[1] / cnt; hd[2] = hd[2] / cnt;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) h[i] = happ[i] + hd[i];
+    const T hks = gd.hk_scale[g];
+    // get_cubic_hk (field.f90:377-395)
+    if (L.has_cub && hks > (T)0) {
+        const T* A = gd.cub;
+        const int G = gd.G;
+        T d1[3], d2[3], d3[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            d1[c] = A[(3 * c + 0) * G + g] * m[0] + A[(3 * c + 1) * G + g] * m[1] + A[(3 * c + 2) * G + g] * m[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) d2[c] = d1[c] * d1[c];
+        const T k1 = gd.hk1[g], k2 = gd.hk2[g];
+        d3[0] = d1[0] * (k1 * (d2[1] + d2[2]) + k2 * d2[1] * d2[2]);
+        d3[1] = d1[1] * (k1 * (d2[0] + d2[2]) + k2 * d2[0] * d2[2]);
+        d3[2] = d1[2] * (k1 * (d2[0] + d2[1]) + k2 * d2[0] * d2[1]);
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            h[r] = h[r] - (A[(r + 0) * G + g] * d3[0] + A[(r + 3) * G + g] * d3[1] + A[(r + 6) * G + g] * d3[2]) * hks;
+    }
+    // get_uni_hk (field.f90:366-375)
+    if (L.has_uni && hks > (T)0) {
+        const T u[3] = {gd.uni[g], gd.uni[gd.G + g], gd.uni[2 * gd.G + g]};
+        const T p = dot3(m, u), yy = (T)1 - p * p;
+        const T s = p * (gd.hk1[g] + gd.hk2[g] * yy) * hks;
+#pragma unroll
+        for (int i = 0; i < 3; i++) h[i] = h[i] + s * u[i];
+    }
+    // exchange CSRs (field.f90:326-363)
+    const int G = gd.G;
+    if (gd.xi_off) {
+        T s[3] = {0, 0, 0};
+        for (int j = gd.xi_off[g]; j < gd.xi_off[g + 1]; j++) {
+            const int nb = gd.xi_nb[j];
+            const T w = gd.xi_w[j] * gd.ms_scale[nb];
+            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
+        }
+        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
+    }
+    if (gd.xu_off) {
+        T s[3] = {0, 0, 0};
+        for (int j = gd.xu_off[g]; j < gd.xu_off[g + 1]; j++) {
+            const int nb = gd.xu_nb[j];
+            const T w = gd.xu_w[j] * gd.ms_scale[nb];
+            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
+        }
+        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
+    }
+    if (gd.xl_off) {
+        T s[3] = {0, 0, 0};
+        for (int j = gd.xl_off[g]; j < gd.xl_off[g + 1]; j++) {
+            const int nb = gd.xl_nb[j];
+            const T w = gd.xl_w[j] * gd.ms_scale[nb];
+            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
+        }
+        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
+    }
+    (void)k;
+}
+
+// Rotational pushes of push_llg (llg.f90:1086-1121): Boris (|type|==2) or analytic
+// rotation with quaternions (|type|==1).  Returns d.
+template <typename T>
+__device__ __forceinline__ void rot_push(int type, T scale1, T scale2, const T* m, const T* H, T* d)
+{
+    if (type == 2 || type == -2) {
+        T o[3], tt[3], c3[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] = scale1 * H[c] - scale2 * m[c];
+        cross3(m, o, tt);
+#pragma unroll
+        for (int c = 0; c < 3; c++) tt[c] = m[c] + tt[c];
+        const T y = (T)2 / ((T)1 + dot3(o, o));
+        cross3(tt, o, c3);
+#pragma unroll
+        for (int c = 0; c < 3; c++) d[c] = y * c3[c];
+    } else {
+        T y = dot3(H, H);
+        if (y > (T)0) {
+            y = sqrt(y);
+            const T scale11 = (T)2 * scale1 * y / ((T)1 + scale2 * scale2);
+            T o[3], u0[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) o[c] = H[c] / y;
+            T dp = dot3(o, m);
+            dp = dp < (T)1 ? dp : (T)1;
+            dp = dp > (T)-1 ? dp : (T)-1;
+            const T th0 = acos(dp);
+            cross3(m, o, u0);
+            // rotate_2(theta0, u0 (normalised), theta1, u1 = o, m)  (transform.f90:97-115)
+            T theta0 = th0 - (T)2 * atan(exp(scale11 * scale2) * tan((T)0.5 * th0));
+            T theta1 = fabs(scale11);
+            T st1 = (T)0.5 * theta1, st0 = (T)0.5 * theta0;
+            T t0[3] = {u0[0], u0[1], u0[2]};
+            T ct0 = sqrt(dot3(t0, t0));
+            if (ct0 != (T)0) {
+                t0[0] = t0[0] / ct0; t0[1] = t0[1] / ct0; t0[2] = t0[2] / ct0;
+            } else {
+                t0[0] = 0; t0[1] = 0; t0[2] = 1; st0 = 0;
+            }
+            if (st0 == (T)0 && st0 == st1) {
+                d[0] = d[1] = d[2] = 0;
+                return;
+            }
+            const T ct1 = cos(st1);
+            st1 = sin(st1);
+            ct0 = cos(st0);
+            st0 = sin(st0);
+            const T* t1 = o;
+            T w[4];
+            w[0] = ct0 * ct1 - dot3(t0, t1) * st0 * st1;
+            w[1] = t0[0] * ct1 * st0 + (t1[0] * ct0 + (t1[1] * t0[2] - t0[1] * t1[2]) * st0) * st1;
+            w[2] = t0[1] * ct1 * st0 + (t1[1] * ct0 + (t1[2] * t0[0] - t0[2] * t1[0]) * st0) * st1;
+            w[3] = t0[2] * ct1 * st0 + (t1[2] * ct0 + (t1[0] * t0[1] - t0[0] * t1[1]) * st0) * st1;
+            const T w1 = w[0] * w[0], w2 = w[1] * w[1], w3 = w[2] * w[2], w4 = w[3] * w[3];
+            T r[3];
+            r[0] = (T)2 * (m[2] * (w[0] * w[2] + w[1] * w[3]) + m[1] * (w[1] * w[2] - w[0] * w[3])) + m[0] * (w1 + w2 - w3 - w4);
+            r[1] = (T)2 * (m[0] * (w[0] * w[3] + w[1] * w[2]) + m[2] * (w[2] * w[3] - w[0] * w[1])) + m[1] * (w1 + w3 - w2 - w4);
+            r[2] = (T)2 * (m[1] * (w[0] * w[1] + w[2] * w[3]) + m[0] * (w[1] * w[3] - w[0] * w[2])) + m[2] * (w1 + w4 - w3 - w2);
+#pragma unroll
+            for (int c = 0; c < 3; c++) d[c] = r[c] - m[c];
+        } else {
+            d[0] = d[1] = d[2] = 0;
+        }
+    }
+}
+
+// Diagnostics partial layout per block: [0] max errmx_l, [1] max errmx1_l,
+// [2 + k] sum |m| ms_scale vol for layer k, [2+NL+3k+c] sum vol m_c for layer k.
+#define DIAG_STRIDE (2 + 4 * MRMG_MAX_LAYERS)
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ ma, T* __restrict__ mb,
+                                               T* __restrict__ dm, T* __restrict__ hthm, T* __restrict__ hout,
+                                               const T* __restrict__ Hcell, size_t CP, StepArgs a,
+                                               double* __restrict__ diag_part)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = gd.G;
+    __shared__ double red[DIAG_STRIDE];
+    if (a.diag) {
+        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
+        __syncthreads();
+    }
+    if (g < G) {
+        const int k = gd.layer_of[g];
+        const LayerParams& L = gd.lp[k];
+        // the magnetisation the field is evaluated on
+        const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
+        T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
+        T h[3];
+        field_at<T>(gd, L, g, k, mf, Hcell, CP, a.t, mfield, h);
+        // thermal draw (get_thermal) happens with the first field of a step
+        T th[3];
+        if (MODE == GM_HEUN1 || (MODE == GM_ROT1)) {
+            th[0] = th[1] = th[2] = 0;
+            if (a.noise != 0 && L.thermal) {
+                double n[3];
+                if (a.noise == 1) {
+                    n[0] = a.normals[3 * (size_t)g]; n[1] = a.normals[3 * (size_t)g + 1]; n[2] = a.normals[3 * (size_t)g + 2];
+                } else {
+                    philox_normals3(a.seed, a.step, g, n);
+                }
+                const T sg = gd.sigma[g];
+#pragma unroll
+                for (int c = 0; c < 3; c++) th[c] = sg * (T)n[c];
+            }
+            hthm[g] = th[0]; hthm[G + g] = th[1]; hthm[2 * G + g] = th[2];
+        } else {
+            th[0] = hthm[g]; th[1] = hthm[G + g]; th[2] = hthm[2 * G + g];
+        }
+        if (MODE != GM_HEUN1) { hout[g] = h[0]; hout[G + g] = h[1]; hout[2 * G + g] = h[2]; }
+        const T sq = (T)a.sqrt_dt;
+        T H[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
+        const T msc = gd.ms_scale[g];
+        T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
+        T mo[3], d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
+        const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
+        const T scale2 = gd.alpha_scale[g] * (T)L.alpha;
+        if (msc > (T)0) {
+            if (MODE == GM_HEUN1 || MODE == GM_HEUN2) {
+                const T scale11 = (T)2 * scale1 / ((T)1 + scale2 * scale2);
+                const T* m = mfield;   // stage 1: m_in, stage 2: predicted m
+                T mt[3];
+                cross3(m, H, tq);
+                cross3(m, tq, mt);
+#pragma unroll
+                for (int c = 0; c < 3; c++) d[c] = scale11 * (tq[c] + scale2 * mt[c]);
+                if (MODE == GM_HEUN2) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) d[c] = (T)0.5 * (d[c] + dm[(size_t)c * G + g]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) dm[(size_t)c * G + g] = d[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; c++) mo[c] = mi[c] + d[c];
+                const T y = dot3(mo, mo);
+                if (y > (T)0) {
+                    const T r = sqrt(y);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) mo[c] = mo[c] / r;
+                }
+            } else {
+                // rotational types push from m_in (llg.f90:1085)
+                cross3(mi, H, tq);
+                rot_push<T>(a.type, scale1, scale2, mi, H, d);
+#pragma unroll
+                for (int c = 0; c < 3; c++) mo[c] = mi[c] + d[c];
+            }
+        } else {
+            // ms_scale = 0: m along the (non-thermal) field (llg.f90:1140-1147)
+            const T y = sqrt(dot3(h, h));
+            if (y > (T)0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) mo[c] = h[c] / y;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) mo[c] = mi[c];
+            }
+        }
+        T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1) ? mb : ma;
+        dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
+        if (a.diag) {
+            const T msv = gd.ms[g];
+            const double e = msc > (T)0 ? (double)dot3(tq, tq) / ((double)msv * (double)msv) : 0.0;
+            const double e1 = msc > (T)0 ? (double)dot3(d, d) : 0.0;
+            const double vol = (double)gd.vol[g];
+            atomicMax(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)__double_as_longlong(e));
+            atomicMax(reinterpret_cast<unsigned long long*>(&red[1]), (unsigned long long)__double_as_longlong(e1));
+            atomicAdd(&red[2 + k], sqrt((double)dot3(mo, mo)) * (double)msc * vol);
+#pragma unroll
+            for (int c = 0; c < 3; c++) atomicAdd(&red[2 + MRMG_MAX_LAYERS + 3 * k + c], vol * (double)mo[c]);
+        }
+    }
+    if (a.diag) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)blockIdx.x * DIAG_STRIDE + i] = red[i];
+    }
+}
+
+// streaming copy (16-byte vectors), the known-size reference of the traffic counters
+__global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// ------------------------------------------------------------------------- tensor kernels
+// Newell's volume-averaged tensor of two blocks (tensor_element.f90:13-102): n -= t.
+__device__ __forceinline__ double dsignv(double a, double b) { return signbit(b) ? -fabs(a) : fabs(a); }
+__device__ __forceinline__ double dmaxv(double a, double b) { return a > b ? a : b; }
+
+__device__ double tg1(double x, double y, double z, double d)
+{
+    return x * y * z * atan(y * z / dsignv(dmaxv(1.e-300, fabs(x) * d), x))
+         + 0.5 * y * (z * z - x * x) * log(dmaxv(1.e-300, d - y))
+         + 0.5 * z * (y * y - x * x) * log(dmaxv(1.e-300, d - z))
+         + (y * y + z * z - 2.0 * x * x) * d / 6.0;
+}
+__device__ double tg2(double z, double x, double y, double d)
+{
+    return -x * y * z * log(dmaxv(1.e-300, d + z))
+         + y * (y * y - 3.0 * z * z) * log(dmaxv(1.e-300, d + x)) / 6.0
+         + x * (x * x - 3.0 * z * z) * log(dmaxv(1.e-300, d + y)) / 6.0
+         + 0.5 * x * x * z * atan(y * z / dsignv(dmaxv(1.e-300, fabs(x) * d), x))
+         + 0.5 * y * y * z * atan(x * z / dsignv(dmaxv(1.e-300, fabs(y) * d), y))
+         + z * z * z * atan(x * y / dsignv(dmaxv(1.e-300, fabs(z) * d), z)) / 6.0 + x * y * d / 3.0;
+}
+
+__device__ void van(double rx, double ry, double rz, double xs, double ys, double zs, double xo, double zo, double* n)
+{
+    const double dxo[2] = {-0.5 * xo, 0.5 * xo}, dyo[2] = {-0.5 * xo, 0.5 * xo}, dzo[2] = {-0.5 * zo, 0.5 * zo};
+    const double dxs[2] = {-0.5 * xs, 0.5 * xs}, dys[2] = {-0.5 * ys, 0.5 * ys}, dzs[2] = {-0.5 * zs, 0.5 * zs};
+    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i6 = 0; i6 < 2; i6++)
+    for (int i5 = 0; i5 < 2; i5++)
+    for (int i4 = 0; i4 < 2; i4++)
+    for (int i3 = 0; i3 < 2; i3++) {
+        const double dx = -rx + dxo[i3] - dxs[i6];
+        for (int i2 = 0; i2 < 2; i2++) {
+            const double dy = -ry + dyo[i2] - dys[i5];
+            for (int i1 = 0; i1 < 2; i1++) {
+                const double dz = -rz + dzo[i1] - dzs[i4];
+                const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                if (d != 0.0) {
+                    const double s = ((i1 + i2 + i3 + i4 + i5 + i6 + 5) & 1) ? -1.0 : 1.0;
+                    double x;
+                    t[0] = t[0] + s * tg1(dx, dy, dz, d);
+                    t[4] = t[4] + s * tg1(dy, dz, dx, d);
+                    t[8] = t[8] + s * tg1(dz, dx, dy, d);
+                    x = s * tg2(dx, dy, dz, d); t[5] = t[5] + x; t[7] = t[7] + x;
+                    x = s * tg2(dy, dz, dx, d); t[2] = t[2] + x; t[6] = t[6] + x;
+                    x = s * tg2(dz, dx, dy, d); t[1] = t[1] + x; t[3] = t[3] + x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) n[i] = n[i] - t[i];
+}
+
+// plane_to_plane for one layer pair: one thread per output cell; the offsets (i,j) that
+// alias onto it (i = +-nx/2, j = +-ny/2 for even sizes) are summed in the reference's
+// j-outer, i-inner order (tensor_element.f90:233-259).  Writes the normalised block.
+__global__ void k_plane_to_plane(int nx, int ny, double dx, double dy, double dzs, double dzo, double rz,
+                                 int nimg, double norm, double* __restrict__ blk)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nx * ny) return;
+    const int oi = p % nx, oj = p / nx;
+    int is[2], js[2], ni = 0, nj = 0;
+    if (nx % 2 == 0 && oi == nx / 2) { is[0] = -nx / 2; is[1] = nx / 2; ni = 2; }
+    else { is[0] = (oi <= nx / 2) ? -oi : nx - oi; ni = 1; }
+    if (ny % 2 == 0 && oj == ny / 2) { js[0] = -ny / 2; js[1] = ny / 2; nj = 2; }
+    else { js[0] = (oj <= ny / 2) ? -oj : ny - oj; nj = 1; }
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nj; b++) {
+        const int j = js[b];
+        const int jq = j / ((ny + 1) / 2);
+        const int yp0 = -nimg * (1 - (jq > 0 ? jq : 0)), yp1 = nimg * (1 + (jq < 0 ? jq : 0));
+        for (int a = 0; a < ni; a++) {
+            const int i = is[a];
+            const int iq = i / ((nx + 1) / 2);
+            const int xp0 = -nimg * (1 - (iq > 0 ? iq : 0)), xp1 = nimg * (1 + (iq < 0 ? iq : 0));
+            double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int nyi = yp0; nyi <= yp1; nyi++)
+                for (int nxi = xp0; nxi <= xp1; nxi++)
+                    van(i * dx + nxi * dx * nx, j * dy + nyi * dy * ny, rz, dx, dy, dzs, dx, dzo, t);
+#pragma unroll
+            for (int c = 0; c < 9; c++) acc[c] = acc[c] + t[c];
+        }
+    }
+    const size_t plane = (size_t)nx * ny;
+#pragma unroll
+    for (int c = 0; c < 9; c++) blk[p + plane * c] = acc[c] / norm;
+}
+
+// Tensor FFT in FP64, rows: one workgroup per (row pair, plane) of P real planes.
+template <int MAXB>
+__global__ void __launch_bounds__(256) k_tensor_rows(const double* __restrict__ R, C2<double>* __restrict__ TC,
+                                                     int nx, int ny, int nxh, FFTPlan px,
+                                                     const C2<double>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double* re = reinterpret_cast<double*>(smem_raw);
+    double* im = re + nx;
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+    const size_t pl = blockIdx.y;
+    const bool has1 = y1 < ny;
+    const double* r = R + pl * nx * ny;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        re[x] = r[(size_t)y0 * nx + x];
+        im[x] = has1 ? r[(size_t)y1 * nx + x] : 0.0;
+    }
+    __syncthreads();
+    lds_fft<double, MAXB, false>(re, im, 1, px, twx);
+    C2<double>* out = TC + pl * nxh * ny;
+    for (int kx = threadIdx.x; kx < nxh; kx += blockDim.x) {
+        const int kn = kx == 0 ? 0 : nx - kx;
+        const double zr = re[kx], zi = im[kx], nr = re[kn], ni = im[kn];
+        out[(size_t)y0 * nxh + kx] = {(zr + nr) * 0.5, (zi - ni) * 0.5};
+        if (has1) out[(size_t)y1 * nxh + kx] = {(zi + ni) * 0.5, (nr - zr) * 0.5};
+    }
+}
+
+// Tensor FFT in FP64, columns in place: one workgroup per (kx, plane).
+template <int MAXB>
+__global__ void __launch_bounds__(256) k_tensor_cols(C2<double>* __restrict__ TC, int ny, int nxh, FFTPlan py,
+                                                     const C2<double>* __restrict__ twy)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double* re = reinterpret_cast<double*>(smem_raw);
+    double* im = re + ny;
+    const int kx = blockIdx.x;
+    C2<double>* col = TC + (size_t)blockIdx.y * nxh * ny + kx;
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+        const C2<double> v = col[(size_t)y * nxh];
+        re[y] = v.x;
+        im[y] = v.y;
+    }
+    __syncthreads();
+    lds_fft<double, MAXB, false>(re, im, 1, py, twy);
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) col[(size_t)y * nxh] = {re[y], im[y]};
+}
+
+// Regroup the tensor spectra into Tspec[kx][ko][ks][slot][ky] and round to T.
+template <typename T>
+__global__ void k_tensor_pack(const C2<double>* __restrict__ TC, C2<T>* __restrict__ Tspec, int ny, int nxh,
+                              int nl, int nslot, const int* __restrict__ slot2c)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)nxh * nl * nl * nslot * ny;
+    if (i >= total) return;
+    size_t r = i;
+    const int ky = r % ny; r /= ny;
+    const int s = r % nslot; r /= nslot;
+    const int ks = r % nl; r /= nl;
+    const int ko = r % nl; r /= nl;
+    const int kx = (int)r;
+    const int c = slot2c[s];
+    // plane (c, ko, ks) of the (nx,ny,9,nl,nl) tensor
+    const size_t plane = (size_t)c + 9 * ((size_t)ko + (size_t)nl * ks);
+    const C2<double> v = TC[plane * nxh * ny + (size_t)ky * nxh + kx];
+    Tspec[i] = C2<T>{(T)v.x, (T)v.y};
+}
+
+// ------------------------------------------------------------------------- host side
+static bool factorize(int n, FFTPlan& p)
+{
+    p.n = n;
+    p.nfac = 0;
+    int m = n;
+    while (m % 4 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 4; m /= 4; }
+    while (m % 2 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 2; m /= 2; }
+    while (m % 3 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 3; m /= 3; }
+    while (m % 5 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 5; m /= 5; }
+    return m == 1;
+}
+
+static int min_radix(const FFTPlan& p)
+{
+    int r = 5;
+    for (int i = 0; i < p.nfac; i++) r = std::min(r, p.fac[i]);
+    return p.nfac ? r : 4;
+}
+
+template <typename T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void swap_with(DBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count)
+    {
+        release();
+        if (count == 0) return MRMG_OK;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return set_err(MRMG_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        }
+        n = count;
+        return MRMG_OK;
+    }
+    int upload(const T* h, size_t count, hipStream_t s)
+    {
+        RCHK(alloc(count));
+        if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return MRMG_OK;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct HostLayer {
+    bool set = false;
+    int ng = 0, ncell = 0;
+    std::vector<int> grid2grain, g2gi, g2g;
+    std::vector<double> vol, ms, hk, uni, cub, ms_scale, alpha_scale, hk_scale, temp, happ;
+    std::vector<int> hxi, hxg, uhxi, uhxg, lhxi, lhxg;
+    std::vector<double> hx, uhx, lhx;
+    double alpha = 1, gamma = 1.76e7, thm_alpha = 1, floor_ms = 0.01;
+    int thermal = 0;
+    bool has_uni = false, has_cub = false, has_happ = false, has_hx = false, has_uhx = false, has_lhx = false;
+    mrmg_ufield uf{};
+    bool has_uf = false;
+    std::vector<double> m;   // (3,ng) host copy used before the device state exists
+};
+
+struct EngineBase {
+    virtual ~EngineBase() {}
+    virtual int build(std::vector<HostLayer>& L) = 0;
+    virtual int set_tensor_planes(const double* d_real) = 0;
+    virtual int set_state(int k, const double* m) = 0;
+    virtual int get_state(int k, double* m, double* h) = 0;
+    virtual int update_ufield(std::vector<HostLayer>& L) = 0;
+    virtual int step(const mrmg_step_params* p, const double* noise, mrmg_diag* diag) = 0;
+    virtual int demag_field(double* h_cell, double* h_grain) = 0;
+    virtual size_t footprint() const = 0;
+    virtual bool has_tensor() const = 0;
+    virtual int profile(int enable) = 0;
+    virtual int kernel_stats(mrmg_kstat* out, int max, int* n) = 0;
+};
+
+struct mrmg_ctx {
+    int nx = 0, ny = 0, nl = 0, nxh = 0, precision = 0, demag = 1;
+    double dx = 0, dy = 0;
+    std::vector<double> dz, il;
+    hipStream_t stream = nullptr;
+    FFTPlan px{}, py{};
+    std::vector<HostLayer> layers;
+    bool dirty = true;
+    std::unique_ptr<EngineBase> eng;
+};
+
+static std::vector<C2<double>> twiddle_table(int n)
+{
+    std::vector<C2<double>> w(n);
+    for (int k = 0; k < n; k++) {
+        // exp(-2 pi i k/n), angle reduced by symmetry before the libm call
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+        w[k] = {(double)cosl(a), (double)(-sinl(a))};
+    }
+    return w;
+}
+
+// pick MAXB so that MAXB * threads >= largest butterfly count of a pass
+static int pick_maxb(int batch, const FFTPlan& p, int threads)
+{
+    const int total = batch * p.n / min_radix(p);
+    const int need = (total + threads - 1) / threads;
+    if (need <= 1) return 1;
+    if (need <= 2) return 2;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    if (need <= 16) return 16;
+    return -1;
+}
+
+#define LAUNCH_MAXB(MB, KERNEL, ...)                                                                  \
+    do {                                                                                              \
+        switch (MB) {                                                                                 \
+        case 1: KERNEL<1>(__VA_ARGS__); break;                                                        \
+        case 2: KERNEL<2>(__VA_ARGS__); break;                                                        \
+        case 4: KERNEL<4>(__VA_ARGS__); break;                                                        \
+        case 8: KERNEL<8>(__VA_ARGS__); break;                                                        \
+        default: KERNEL<16>(__VA_ARGS__); break;                                                      \
+        }                                                                                             \
+    } while (0)
+
+template <typename T>
+struct Engine : EngineBase {
+    mrmg_ctx* c;
+    int G = 0, Ctot = 0, nslot = 0;
+    std::vector<int> off;
+    // grain data
+    DBuf<int> layer_of, gc_off, gc_idx, cell2grain;
+    DBuf<int> xi_off, xi_nb, xu_off, xu_nb, xl_off, xl_nb;
+    DBuf<T> xi_w, xu_w, xl_w;
+    DBuf<T> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, vol;
+    DBuf<LayerParams> lp;
+    // state
+    DBuf<T> ma, mb, dm, hthm, h;
+    // demag
+    DBuf<C2<T>> S1, S2, Tspec, twx, twy;
+    DBuf<T> Hcell;
+    DBuf<int> slot2c;
+    DBuf<double> noise_buf, diag_part;
+    // instrumentation: event pairs of the current mrmg_step call, per-class totals
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_class;
+    size_t ev_used = 0;
+    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool tensor_ok = false, has_happ = false, has_xi = false, has_xu = false, has_xl = false;
+    int maxb_row = 1, maxb_col = 1;
+    size_t lds_row = 0, lds_col = 0;
+    std::vector<LayerParams> hlp;
+
+    explicit Engine(mrmg_ctx* ctx) : c(ctx) {}
+    ~Engine() override
+    {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+
+    // kernel classes of the profile
+    enum { K_ROW_FWD = 0, K_COL = 1, K_ROW_INV = 2, K_GRAIN1 = 3, K_GRAIN2 = 4, K_NCLASS = 5 };
+    int ev_begin(int cls)
+    {
+        if (!prof) return MRMG_OK;
+        if (ev_used + 2 > ev_pool.size()) {
+            for (int i = 0; i < 2; i++) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                ev_pool.push_back(e);
+            }
+            ev_class.resize(ev_pool.size() / 2);
+        }
+        ev_class[ev_used / 2] = cls;
+        HIPCHK(hipEventRecord(ev_pool[ev_used], c->stream));
+        return MRMG_OK;
+    }
+    int ev_end()
+    {
+        if (!prof) return MRMG_OK;
+        HIPCHK(hipEventRecord(ev_pool[ev_used + 1], c->stream));
+        ev_used += 2;
+        return MRMG_OK;
+    }
+    // after the stream is synchronised: fold the recorded pairs into the totals
+    int ev_collect()
+    {
+        for (size_t i = 0; i < ev_used; i += 2) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]));
+            kt_ms[ev_class[i / 2]] += ms;
+            kt_n[ev_class[i / 2]] += 1;
+        }
+        ev_used = 0;
+        return MRMG_OK;
+    }
+
+    size_t footprint() const override
+    {
+        size_t b = 0;
+        b += layer_of.bytes() + gc_off.bytes() + gc_idx.bytes() + cell2grain.bytes();
+        b += xi_off.bytes() + xi_nb.bytes() + xu_off.bytes() + xu_nb.bytes() + xl_off.bytes() + xl_nb.bytes();
+        b += xi_w.bytes() + xu_w.bytes() + xl_w.bytes();
+        b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
+             hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + vol.bytes() + lp.bytes();
+        b += ma.bytes() + mb.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
+        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + slot2c.bytes();
+        b += noise_buf.bytes() + diag_part.bytes();
+        return b;
+    }
+    bool has_tensor() const override { return tensor_ok; }
+
+    int init_fft()
+    {
+        const int nx = c->nx, ny = c->ny, nxh = c->nxh, nl = c->nl;
+        auto wx = twiddle_table(nx), wy = twiddle_table(ny);
+        std::vector<C2<T>> tx(nx), ty(ny);
+        for (int i = 0; i < nx; i++) tx[i] = {(T)wx[i].x, (T)wx[i].y};
+        for (int i = 0; i < ny; i++) ty[i] = {(T)wy[i].x, (T)wy[i].y};
+        RCHK(twx.upload(tx.data(), nx, c->stream));
+        RCHK(twy.upload(ty.data(), ny, c->stream));
+        maxb_row = pick_maxb(3, c->px, 256);
+        maxb_col = pick_maxb(3 * nl, c->py, 256);
+        if (maxb_row < 0 || maxb_col < 0)
+            return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs", nx, ny, nl);
+        lds_row = sizeof(T) * 2 * 3 * nx;
+        lds_col = sizeof(T) * 2 * (3 * nl + 3) * ny;
+        if (lds_row > 160 * 1024 || lds_col > 160 * 1024)
+            return set_err(MRMG_ERR_UNSUPPORTED, "LDS need %zu/%zu bytes exceeds 160 KiB (ny=%d, n_lay=%d)", lds_row,
+                           lds_col, ny, nl);
+        RCHK(lds_attr());
+        RCHK(S1.alloc((size_t)nl * 3 * ny * nxh));
+        RCHK(S2.alloc((size_t)nl * 3 * ny * nxh));
+        RCHK(Hcell.alloc((size_t)3 * nl * nx * ny));
+        HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
+        return MRMG_OK;
+    }
+
+    template <int MB> int lds_attr_mb()
+    {
+        const int r = (int)lds_row, cl = (int)lds_col;
+        HIPCHK(hipFuncSetAttribute((const void*)k_row_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        return MRMG_OK;
+    }
+    // dynamic LDS above 64 KiB must be requested per kernel
+    int lds_attr()
+    {
+        if (lds_row <= 64 * 1024 && lds_col <= 64 * 1024) return MRMG_OK;
+        RCHK(lds_attr_mb<1>()); RCHK(lds_attr_mb<2>()); RCHK(lds_attr_mb<4>());
+        RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
+        return MRMG_OK;
+    }
+
+    int build(std::vector<HostLayer>& L) override
+    {
+        const int nl = c->nl, nx = c->nx, ny = c->ny;
+        const size_t P = (size_t)nx * ny;
+        off.assign(nl + 1, 0);
+        for (int k = 0; k < nl; k++) off[k + 1] = off[k] + L[k].ng;
+        G = off[nl];
+        Ctot = 0;
+        for (int k = 0; k < nl; k++) Ctot += L[k].ncell;
+        std::vector<int> h_layer(G), h_gco(G + 1), h_gci(Ctot), h_c2g(nl * P, -1);
+        std::vector<double> h_ms(G), h_msf(G), h_mss(G), h_as(G), h_hks(G), h_hk1(G), h_hk2(G), h_uni(3 * (size_t)G, 0.0),
+            h_cub(9 * (size_t)G, 0.0), h_happ(3 * (size_t)G, 0.0), h_sig(G, 0.0), h_vol(G);
+        std::vector<double> h_m(3 * (size_t)G);
+        has_happ = has_xi = has_xu = has_xl = false;
+        hlp.assign(nl, LayerParams{});
+        int q = 0;
+        h_gco[0] = 0;
+        for (int k = 0; k < nl; k++) {
+            const HostLayer& H = L[k];
+            for (int n = 0; n < H.ng; n++) {
+                const int g = off[k] + n;
+                h_layer[g] = k;
+                for (int e = H.g2gi[n]; e < H.g2gi[n + 1]; e++) {
+                    const int i = H.g2g[2 * e] - 1, j = H.g2g[2 * e + 1] - 1;
+                    if (i < 0 || i >= nx || j < 0 || j >= ny)
+                        return set_err(MRMG_ERR_ARG, "layer %d grain %d: cell (%d,%d) outside grid", k, n + 1, i + 1, j + 1);
+                    h_gci[q++] = (int)(k * P + (size_t)j * nx + i);
+                }
+                h_gco[g + 1] = q;
+                if (H.g2gi[n + 1] == H.g2gi[n]) return set_err(MRMG_ERR_ARG, "layer %d grain %d has no cells", k, n + 1);
+                h_ms[g] = H.ms[n];
+                h_mss[g] = H.ms_scale[n];
+                h_msf[g] = H.ms[n] * H.ms_scale[n];
+                h_as[g] = H.alpha_scale[n];
+                h_hks[g] = H.hk_scale[n];
+                h_hk1[g] = H.hk[2 * n];
+                h_hk2[g] = H.hk[2 * n + 1];
+                h_vol[g] = H.vol[n];
+                if (H.has_uni) for (int cc = 0; cc < 3; cc++) h_uni[(size_t)cc * G + g] = H.uni[3 * n + cc];
+                if (H.has_cub) for (int cc = 0; cc < 9; cc++) h_cub[(size_t)cc * G + g] = H.cub[9 * n + cc];
+                if (H.has_happ) for (int cc = 0; cc < 3; cc++) h_happ[(size_t)cc * G + g] = H.happ[3 * n + cc];
+                // get_thermal variance, LLG branch with |m| = 1 (field.f90:407-423)
+                if (H.thermal) {
+                    const double tsigma = sqrt(2.0 * 1.380662e-16 * H.thm_alpha / H.gamma);
+                    double x;
+                    if (H.ms_scale[n] > 0.0)
+                        x = H.alpha_scale[n] * H.temp[n] / (H.ms[n] * H.vol[n] * std::max(H.floor_ms, H.ms_scale[n]));
+                    else
+                        x = 0.0;
+                    h_sig[g] = sqrt(x) * tsigma;
+                }
+                for (int cc = 0; cc < 3; cc++) h_m[(size_t)cc * G + g] = H.m[3 * n + cc];
+            }
+            for (size_t p = 0; p < P; p++) {
+                const int gr = H.grid2grain[p];
+                if (gr < 0 || gr > H.ng) return set_err(MRMG_ERR_ARG, "layer %d grid2grain value %d out of range", k, gr);
+                h_c2g[k * P + p] = gr > 0 ? off[k] + gr - 1 : -1;
+            }
+            has_happ |= H.has_happ;
+            has_xi |= H.has_hx;
+            has_xu |= H.has_uhx;
+            has_xl |= H.has_lhx;
+            LayerParams& lpk = hlp[k];
+            lpk.alpha = H.alpha;
+            lpk.gamma = H.gamma;
+            lpk.has_uni = H.has_uni;
+            lpk.has_cub = H.has_cub;
+            lpk.thermal = H.thermal;
+            lpk.off = off[k];
+            lpk.ng = H.ng;
+        }
+        RCHK(fill_ufield(L));
+        // exchange CSRs in global numbering
+        auto csr = [&](int which, DBuf<int>& doff, DBuf<int>& dnb, DBuf<T>& dw) -> int {
+            std::vector<int> o(G + 1, 0), nb;
+            std::vector<T> w;
+            for (int k = 0; k < nl; k++) {
+                const HostLayer& H = L[k];
+                const std::vector<int>* ix = which == 0 ? &H.hxi : which == 1 ? &H.uhxi : &H.lhxi;
+                const std::vector<int>* gx = which == 0 ? &H.hxg : which == 1 ? &H.uhxg : &H.lhxg;
+                const std::vector<double>* vx = which == 0 ? &H.hx : which == 1 ? &H.uhx : &H.lhx;
+                const bool has = which == 0 ? H.has_hx : which == 1 ? H.has_uhx : H.has_lhx;
+                const int tk = which == 0 ? k : which == 1 ? k - 1 : k + 1;
+                for (int n = 0; n < H.ng; n++) {
+                    const int g = off[k] + n;
+                    if (has) {
+                        if (tk < 0 || tk >= nl) return set_err(MRMG_ERR_ARG, "layer %d: interlayer CSR to missing layer", k);
+                        for (int e = (*ix)[n]; e < (*ix)[n + 1]; e++) {
+                            const int gg = (*gx)[e];
+                            if (gg < 1 || gg > L[tk].ng) return set_err(MRMG_ERR_ARG, "layer %d: exchange neighbour %d out of range", k, gg);
+                            nb.push_back(off[tk] + gg - 1);
+                            w.push_back((T)(*vx)[e]);
+                        }
+                    }
+                    o[g + 1] = (int)nb.size();
+                }
+            }
+            RCHK(doff.upload(o.data(), o.size(), c->stream));
+            RCHK(dnb.upload(nb.data(), nb.size(), c->stream));
+            RCHK(dw.upload(w.data(), w.size(), c->stream));
+            return MRMG_OK;
+        };
+        if (has_xi) RCHK(csr(0, xi_off, xi_nb, xi_w)); else { xi_off.release(); xi_nb.release(); xi_w.release(); }
+        if (has_xu) RCHK(csr(1, xu_off, xu_nb, xu_w)); else { xu_off.release(); xu_nb.release(); xu_w.release(); }
+        if (has_xl) RCHK(csr(2, xl_off, xl_nb, xl_w)); else { xl_off.release(); xl_nb.release(); xl_w.release(); }
+
+        auto upT = [&](DBuf<T>& d, const std::vector<double>& v) -> int {
+            std::vector<T> t(v.size());
+            for (size_t i = 0; i < v.size(); i++) t[i] = (T)v[i];
+            return d.upload(t.data(), t.size(), c->stream);
+        };
+        RCHK(layer_of.upload(h_layer.data(), G, c->stream));
+        RCHK(gc_off.upload(h_gco.data(), G + 1, c->stream));
+        RCHK(gc_idx.upload(h_gci.data(), Ctot, c->stream));
+        RCHK(cell2grain.upload(h_c2g.data(), h_c2g.size(), c->stream));
+        RCHK(upT(ms, h_ms)); RCHK(upT(msf, h_msf)); RCHK(upT(ms_scale, h_mss)); RCHK(upT(alpha_scale, h_as));
+        RCHK(upT(hk_scale, h_hks)); RCHK(upT(hk1, h_hk1)); RCHK(upT(hk2, h_hk2)); RCHK(upT(uni, h_uni));
+        RCHK(upT(cub, h_cub)); RCHK(upT(sigma, h_sig)); RCHK(upT(vol, h_vol));
+        if (has_happ) RCHK(upT(happ, h_happ)); else happ.release();
+        RCHK(upT(ma, h_m));
+        RCHK(mb.alloc(3 * (size_t)G)); RCHK(dm.alloc(3 * (size_t)G)); RCHK(hthm.alloc(3 * (size_t)G)); RCHK(h.alloc(3 * (size_t)G));
+        HIPCHK(hipMemsetAsync(mb.p, 0, mb.bytes(), c->stream));
+        HIPCHK(hipMemsetAsync(dm.p, 0, dm.bytes(), c->stream));
+        HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
+        HIPCHK(hipMemsetAsync(h.p, 0, h.bytes(), c->stream));
+        RCHK(diag_part.alloc((size_t)((G + 255) / 256) * DIAG_STRIDE));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return MRMG_OK;
+    }
+
+    int fill_ufield(std::vector<HostLayer>& L)
+    {
+        for (int k = 0; k < c->nl; k++) {
+            LayerParams& p = hlp[k];
+            p.has_uf = L[k].has_uf;
+            mrmg_ufield u = L[k].uf;
+            // set_fld normalisation (applied.f90:307-310)
+            if (u.s_start == u.s_end) { u.scale = u.scale * u.s_start; u.s_end = 1.0; u.s_start = 1.0; }
+            for (int i = 0; i < 3; i++) p.uf_dir[i] = u.dir[i];
+            p.uf_t0 = u.t_start; p.uf_t1 = u.t_end; p.uf_s0 = u.s_start; p.uf_s1 = u.s_end; p.uf_scale = u.scale;
+        }
+        RCHK(lp.upload(hlp.data(), hlp.size(), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return MRMG_OK;
+    }
+
+    int update_ufield(std::vector<HostLayer>& L) override { return fill_ufield(L); }
+
+    int set_tensor_planes(const double* d_real) override
+    {
+        // d_real: device (nx,ny,9,nl,nl) FP64
+        const int nx = c->nx, ny = c->ny, nxh = c->nxh, nl = c->nl;
+        const int P = 9 * nl * nl;
+        DBuf<C2<double>> TC, tdx, tdy;
+        RCHK(TC.alloc((size_t)P * nxh * ny));
+        auto wx = twiddle_table(nx), wy = twiddle_table(ny);
+        RCHK(tdx.upload(wx.data(), nx, c->stream));
+        RCHK(tdy.upload(wy.data(), ny, c->stream));
+        const int mbr = pick_maxb(1, c->px, 256), mbc = pick_maxb(1, c->py, 256);
+        if (mbr < 0 || mbc < 0) return set_err(MRMG_ERR_UNSUPPORTED, "tensor FFT too large");
+        {
+            dim3 grid((ny + 1) / 2, P), blk(256);
+            size_t lds = sizeof(double) * 2 * nx;
+#define KTR(MB, ...) hipLaunchKernelGGL(k_tensor_rows<MB>, grid, blk, lds, c->stream, __VA_ARGS__)
+            switch (mbr) {
+            case 1: KTR(1, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            case 2: KTR(2, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            case 4: KTR(4, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            case 8: KTR(8, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            default: KTR(16, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            }
+#undef KTR
+            HIPCHK(hipGetLastError());
+        }
+        {
+            dim3 grid(nxh, P), blk(256);
+            size_t lds = sizeof(double) * 2 * ny;
+#define KTC(MB) hipLaunchKernelGGL(k_tensor_cols<MB>, grid, blk, lds, c->stream, TC.p, ny, nxh, c->py, tdy.p)
+            switch (mbc) {
+            case 1: KTC(1); break;
+            case 2: KTC(2); break;
+            case 4: KTC(4); break;
+            case 8: KTC(8); break;
+            default: KTC(16); break;
+            }
+#undef KTC
+            HIPCHK(hipGetLastError());
+        }
+        // symmetry decides 6 or 9 slots: checked on the real tensor by the caller
+        const size_t total = (size_t)nxh * nl * nl * nslot * ny;
+        RCHK(Tspec.alloc(total));
+        hipLaunchKernelGGL(k_tensor_pack<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, TC.p,
+                           Tspec.p, ny, nxh, nl, nslot, slot2c.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        tensor_ok = true;
+        return MRMG_OK;
+    }
+
+    int set_slots(bool sym)
+    {
+        nslot = sym ? 6 : 9;
+        const int s6[6] = {0, 4, 8, 1, 2, 5};   // xx yy zz xy xz yz as c = 3a+b
+        int s9[9];
+        for (int i = 0; i < 9; i++) s9[i] = i;
+        return slot2c.upload(sym ? s6 : s9, nslot, c->stream);
+    }
+
+    int set_state(int k, const double* m) override
+    {
+        const int ng = c->layers[k].ng;
+        std::vector<T> t(3 * (size_t)ng);
+        for (int n = 0; n < ng; n++)
+            for (int cc = 0; cc < 3; cc++) t[(size_t)cc * ng + n] = (T)m[3 * n + cc];
+        for (int cc = 0; cc < 3; cc++)
+            HIPCHK(hipMemcpyAsync(ma.p + (size_t)cc * G + off[k], t.data() + (size_t)cc * ng, sizeof(T) * ng,
+                                  hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return MRMG_OK;
+    }
+
+    int get_state(int k, double* m, double* hh) override
+    {
+        const int ng = c->layers[k].ng;
+        std::vector<T> t(3 * (size_t)ng);
+        for (int pass = 0; pass < 2; pass++) {
+            double* out = pass == 0 ? m : hh;
+            T* src = pass == 0 ? ma.p : h.p;
+            if (!out) continue;
+            for (int cc = 0; cc < 3; cc++)
+                HIPCHK(hipMemcpyAsync(t.data() + (size_t)cc * ng, src + (size_t)cc * G + off[k], sizeof(T) * ng,
+                                      hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int n = 0; n < ng; n++)
+                for (int cc = 0; cc < 3; cc++) out[3 * n + cc] = (double)t[(size_t)cc * ng + n];
+        }
+        return MRMG_OK;
+    }
+
+    GrainData<T> gdata() const
+    {
+        GrainData<T> d;
+        d.G = G;
+        d.layer_of = layer_of.p;
+        d.ms = ms.p; d.msf = msf.p; d.ms_scale = ms_scale.p; d.alpha_scale = alpha_scale.p; d.hk_scale = hk_scale.p;
+        d.hk1 = hk1.p; d.hk2 = hk2.p; d.uni = uni.p; d.cub = cub.p; d.happ = has_happ ? happ.p : nullptr;
+        d.sigma = sigma.p; d.vol = vol.p; d.gc_off = gc_off.p; d.gc_idx = gc_idx.p;
+        d.xi_off = has_xi ? xi_off.p : nullptr; d.xi_nb = xi_nb.p; d.xi_w = xi_w.p;
+        d.xu_off = has_xu ? xu_off.p : nullptr; d.xu_nb = xu_nb.p; d.xu_w = xu_w.p;
+        d.xl_off = has_xl ? xl_off.p : nullptr; d.xl_nb = xl_nb.p; d.xl_w = xl_w.p;
+        d.lp = lp.p;
+        return d;
+    }
+
+    bool demag_on() const { return c->demag && tensor_ok; }
+
+    // get_demag: scatter + row r2c, column FFT * tensor, row c2r; m = device grain vectors
+    int demag(const T* m)
+    {
+        const int nx = c->nx, ny = c->ny, nl = c->nl, nxh = c->nxh;
+        dim3 grow((ny + 1) / 2, nl), blk(256);
+        RCHK(ev_begin(K_ROW_FWD));
+        LAUNCH_MAXB(maxb_row, launch_row_fwd, grow, blk, m);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        const int nblocks = nxh * nl, nb8 = (nblocks + 7) / 8 * 8;
+        RCHK(ev_begin(K_COL));
+        if (nslot == 6) LAUNCH_MAXB(maxb_col, launch_col_sym, nblocks, nb8);
+        else LAUNCH_MAXB(maxb_col, launch_col_gen, nblocks, nb8);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        RCHK(ev_begin(K_ROW_INV));
+        LAUNCH_MAXB(maxb_row, launch_row_inv, grow, blk);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        (void)nx;
+        return MRMG_OK;
+    }
+    template <int MB> void launch_row_fwd(dim3 g, dim3 b, const T* m)
+    {
+        hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, m, G, msf.p, cell2grain.p, S1.p, c->nx, c->ny,
+                           c->nxh, c->px, twx.p);
+    }
+    template <int MB> void launch_col_sym(int nblocks, int nb8)
+    {
+        hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(256), lds_col, c->stream, S1.p, S2.p, Tspec.p, c->ny,
+                           c->nl, c->nxh, nslot, nblocks, nb8, c->py, twy.p);
+    }
+    template <int MB> void launch_col_gen(int nblocks, int nb8)
+    {
+        hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(256), lds_col, c->stream, S1.p, S2.p, Tspec.p, c->ny,
+                           c->nl, c->nxh, nslot, nblocks, nb8, c->py, twy.p);
+    }
+    template <int MB> void launch_row_inv(dim3 g, dim3 b)
+    {
+        hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
+                           c->px, twx.p);
+    }
+
+    template <int MODE> int grain(const StepArgs& a)
+    {
+        const size_t CP = (size_t)c->nl * c->nx * c->ny;
+        const T* hc = demag_on() ? Hcell.p : nullptr;
+        RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1) ? K_GRAIN1 : K_GRAIN2));
+        hipLaunchKernelGGL((k_grain<T, MODE>), dim3((G + 255) / 256), dim3(256), 0, c->stream, gdata(), ma.p, mb.p,
+                           dm.p, hthm.p, h.p, hc, CP, a, diag_part.p);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        return MRMG_OK;
+    }
+
+    int step(const mrmg_step_params* p, const double* noise, mrmg_diag* diag) override
+    {
+        const int type = p->type;
+        if (!(type == -3 || type == 2 || type == -2 || type == 1 || type == -1))
+            return set_err(MRMG_ERR_UNSUPPORTED, "llg_type %d not implemented", type);
+        if (p->dt <= 0.0) return set_err(MRMG_ERR_ARG, "dt must be > 0");
+        if (p->noise == MRMG_NOISE_INJECT && !noise) return set_err(MRMG_ERR_ARG, "noise array is NULL");
+        // llg() start: init_applied zeroes hthm (field.f90:703)
+        HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
+        const int chunk = 64;
+        const size_t per = 3 * (size_t)G;
+        if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per * std::min(chunk, std::max(p->n_steps, 1))));
+        double t = p->t0;
+        const double dt = p->dt;
+        StepArgs a{};
+        a.dt = dt;
+        a.sqrt_dt = sqrt(dt);
+        a.type = type;
+        a.noise = p->noise;
+        a.seed = p->seed;
+        for (int it = 0; it < p->n_steps; it++) {
+            const bool last = it == p->n_steps - 1;
+            a.step = p->step0 + it;
+            a.normals = nullptr;
+            if (p->noise == MRMG_NOISE_INJECT) {
+                const int ci = it % chunk;
+                if (ci == 0) {
+                    const int n = std::min(chunk, p->n_steps - it);
+                    HIPCHK(hipMemcpyAsync(noise_buf.p, noise + per * it, sizeof(double) * per * n,
+                                          hipMemcpyHostToDevice, c->stream));
+                }
+                a.normals = noise_buf.p + per * ci;
+            }
+            if (type == -3) {
+                // stage 1 (get_fields at t, push iter=2)
+                if (demag_on()) RCHK(demag(ma.p));
+                a.t = t; a.dts = dt; a.diag = 0;
+                RCHK(grain<GM_HEUN1>(a));
+                // stage 2 (get_field at t+dt on the predicted m, push iter=1)
+                if (demag_on()) RCHK(demag(mb.p));
+                a.t = t + dt; a.diag = last;
+                RCHK(grain<GM_HEUN2>(a));
+            } else if (type < 0) {
+                if (demag_on()) RCHK(demag(ma.p));
+                a.t = t; a.dts = dt * 0.5; a.diag = 0;
+                RCHK(grain<GM_ROT1>(a));
+                if (demag_on()) RCHK(demag(mb.p));
+                a.t = t + dt * 0.5; a.dts = dt; a.diag = last;
+                RCHK(grain<GM_ROT2>(a));
+            } else {
+                if (demag_on()) RCHK(demag(ma.p));
+                a.t = t; a.dts = dt; a.diag = last;
+                RCHK(grain<GM_ROT1>(a));
+                ma.swap_with(mb);
+            }
+            t = t + dt;
+            if (prof && ev_used >= 512) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                RCHK(ev_collect());
+            }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (prof) RCHK(ev_collect());
+        if (diag) RCHK(read_diag(diag, t, dt, p->n_steps));
+        return MRMG_OK;
+    }
+
+    int profile(int enable) override
+    {
+        prof = enable != 0;
+        ev_used = 0;
+        for (int i = 0; i < 8; i++) { kt_ms[i] = 0; kt_n[i] = 0; }
+        return MRMG_OK;
+    }
+    int kernel_stats(mrmg_kstat* out, int max, int* n) override
+    {
+        static const char* names[K_NCLASS] = {"row_fwd", "col", "row_inv", "grain1", "grain2"};
+        *n = K_NCLASS;
+        for (int i = 0; i < K_NCLASS && i < max; i++) {
+            memset(out[i].name, 0, sizeof(out[i].name));
+            strncpy(out[i].name, names[i], sizeof(out[i].name) - 1);
+            out[i].launches = kt_n[i];
+            out[i].total_ms = kt_ms[i];
+        }
+        return MRMG_OK;
+    }
+
+    int read_diag(mrmg_diag* d, double t, double dt, int steps)
+    {
+        memset(d, 0, sizeof(*d));
+        d->t = t; d->dt = dt; d->steps = steps; d->n_lay = c->nl;
+        if (steps <= 0) return MRMG_OK;
+        const int nb = (G + 255) / 256;
+        std::vector<double> part((size_t)nb * DIAG_STRIDE);
+        HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double e = 0, e1 = 0;
+        std::vector<double> mm(c->nl, 0.0), mv(3 * c->nl, 0.0);
+        for (int b = 0; b < nb; b++) {
+            const double* q = &part[(size_t)b * DIAG_STRIDE];
+            e = std::max(e, q[0]);
+            e1 = std::max(e1, q[1]);
+            for (int k = 0; k < c->nl; k++) {
+                mm[k] += q[2 + k];
+                for (int cc = 0; cc < 3; cc++) mv[3 * k + cc] += q[2 + MRMG_MAX_LAYERS + 3 * k + cc];
+            }
+        }
+        d->errmx = sqrt(e);
+        d->errmx1 = e1;
+        for (int k = 0; k < c->nl; k++) {
+            double sv = 0;
+            for (int n = 0; n < c->layers[k].ng; n++) sv += c->layers[k].vol[n];
+            d->mmag += mm[k] / sv;
+            if (k < 8)
+                for (int cc = 0; cc < 3; cc++) d->mave[k][cc] = mv[3 * k + cc] / sv;
+        }
+        return MRMG_OK;
+    }
+
+    int demag_field(double* h_cell, double* h_grain) override
+    {
+        const size_t CP = (size_t)c->nl * c->nx * c->ny;
+        if (demag_on()) {
+            RCHK(demag(ma.p));
+        } else {
+            HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (h_cell) {
+            std::vector<T> t(3 * CP);
+            HIPCHK(hipMemcpy(t.data(), Hcell.p, t.size() * sizeof(T), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < t.size(); i++) h_cell[i] = (double)t[i];
+        }
+        if (h_grain) {
+            std::vector<T> t(3 * CP);
+            HIPCHK(hipMemcpy(t.data(), Hcell.p, t.size() * sizeof(T), hipMemcpyDeviceToHost));
+            int g = 0;
+            for (int k = 0; k < c->nl; k++) {
+                const HostLayer& H = c->layers[k];
+                for (int n = 0; n < H.ng; n++, g++) {
+                    double y[3] = {0, 0, 0};
+                    for (int e = H.g2gi[n]; e < H.g2gi[n + 1]; e++) {
+                        const size_t ci = k * (size_t)c->nx * c->ny + (size_t)(H.g2g[2 * e + 1] - 1) * c->nx + (H.g2g[2 * e] - 1);
+                        for (int cc = 0; cc < 3; cc++) y[cc] += (double)t[cc * CP + ci];
+                    }
+                    const int cnt = H.g2gi[n + 1] - H.g2gi[n];
+                    for (int cc = 0; cc < 3; cc++) h_grain[3 * g + cc] = y[cc] / cnt;
+                }
+            }
+        }
+        return MRMG_OK;
+    }
+};
+
+// ------------------------------------------------------------------------- C ABI
+// (the definitions take C linkage from their declarations in mrmg.h)
+
+const char* mrmg_last_error(void) { return g_err.c_str(); }
+
+int mrmg_device_count(int* n)
+{
+    if (!n) return set_err(MRMG_ERR_ARG, "null argument");
+    HIPCHK(hipGetDeviceCount(n));
+    return MRMG_OK;
+}
+
+int mrmg_create(const mrmg_geom* g, mrmg_ctx** out)
+{
+    if (!g || !out) return set_err(MRMG_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (g->nx < 2 || g->ny < 1 || g->n_lay < 1 || g->n_lay > MRMG_MAX_LAYERS)
+        return set_err(MRMG_ERR_ARG, "bad grid %d x %d x %d (n_lay <= %d)", g->nx, g->ny, g->n_lay, MRMG_MAX_LAYERS);
+    if (g->precision != MRMG_FP64 && g->precision != MRMG_FP32) return set_err(MRMG_ERR_ARG, "bad precision");
+    if (!g->dz) return set_err(MRMG_ERR_ARG, "dz is NULL");
+    std::unique_ptr<mrmg_ctx> c(new mrmg_ctx());
+    c->nx = g->nx; c->ny = g->ny; c->nl = g->n_lay; c->nxh = g->nx / 2 + 1;
+    c->precision = g->precision; c->demag = g->demag;
+    c->dx = g->dx; c->dy = g->dy;
+    c->dz.assign(g->dz, g->dz + g->n_lay);
+    c->il.assign(g->n_lay, 0.0);
+    if (g->il_space) c->il.assign(g->il_space, g->il_space + g->n_lay);
+    if (!factorize(c->nx, c->px) || !factorize(c->ny, c->py))
+        return set_err(MRMG_ERR_UNSUPPORTED, "nx=%d, ny=%d must factor into 2, 3, 5", g->nx, g->ny);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return set_err(MRMG_ERR_HIP, "no HIP device");
+    HIPCHK(hipStreamCreate(&c->stream));
+    c->layers.resize(c->nl);
+    if (c->precision == MRMG_FP64) c->eng.reset(new Engine<double>(c.get()));
+    else c->eng.reset(new Engine<float>(c.get()));
+    int rc = c->precision == MRMG_FP64 ? static_cast<Engine<double>*>(c->eng.get())->init_fft()
+                                       : static_cast<Engine<float>*>(c->eng.get())->init_fft();
+    if (rc != MRMG_OK) {
+        c->eng.reset();
+        (void)hipStreamDestroy(c->stream);
+        return rc;
+    }
+    *out = c.release();
+    return MRMG_OK;
+}
+
+int mrmg_destroy(mrmg_ctx* c)
+{
+    if (!c) return MRMG_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->eng.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MRMG_OK;
+}
+
+static int ensure_built(mrmg_ctx* c)
+{
+    if (!c->dirty) return MRMG_OK;
+    for (int k = 0; k < c->nl; k++)
+        if (!c->layers[k].set) return set_err(MRMG_ERR_STATE, "layer %d not set (mrmg_set_layer)", k);
+    RCHK(c->eng->build(c->layers));
+    c->dirty = false;
+    return MRMG_OK;
+}
+
+static int tensor_sym(const mrmg_ctx* c, const double* n)
+{
+    const size_t plane = (size_t)c->nx * c->ny;
+    for (int pr = 0; pr < c->nl * c->nl; pr++)
+        for (int a = 0; a < 3; a++)
+            for (int b = a + 1; b < 3; b++) {
+                const double* p = n + plane * (9 * (size_t)pr + 3 * a + b);
+                const double* q = n + plane * (9 * (size_t)pr + 3 * b + a);
+                if (memcmp(p, q, plane * sizeof(double)) != 0) return 0;
+            }
+    return 1;
+}
+
+template <typename T>
+static int set_tensor_T(mrmg_ctx* c, const double* d_real, bool sym)
+{
+    Engine<T>* e = static_cast<Engine<T>*>(c->eng.get());
+    RCHK(e->set_slots(sym));
+    return e->set_tensor_planes(d_real);
+}
+
+int mrmg_set_tensor(mrmg_ctx* c, const double* n_real)
+{
+    if (!c || !n_real) return set_err(MRMG_ERR_ARG, "null argument");
+    const size_t count = (size_t)c->nx * c->ny * 9 * c->nl * c->nl;
+    const bool sym = tensor_sym(c, n_real);
+    DBuf<double> d;
+    RCHK(d.upload(n_real, count, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return c->precision == MRMG_FP64 ? set_tensor_T<double>(c, d.p, sym) : set_tensor_T<float>(c, d.p, sym);
+}
+
+int mrmg_build_tensor(mrmg_ctx* c, int num_images, double* n_real_out)
+{
+    if (!c) return set_err(MRMG_ERR_ARG, "null argument");
+    if (num_images < 0) return set_err(MRMG_ERR_ARG, "num_images < 0");
+    const int nx = c->nx, ny = c->ny, nl = c->nl;
+    const size_t plane = (size_t)nx * ny, count = plane * 9 * nl * nl;
+    DBuf<double> d;
+    RCHK(d.alloc(count));
+    // calculate_tensor layer loop (tensor.f90:148-200), space_to_sul = 40e-7 cancels in rz
+    const double space_to_sul = 40.e-7;
+    double zsb = space_to_sul;
+    for (int ks = nl - 1; ks >= 0; ks--) {
+        const double zst = zsb + c->dz[ks];
+        double zob = space_to_sul;
+        for (int ko = nl - 1; ko >= 0; ko--) {
+            const double zot = zob + c->dz[ko];
+            const double rz = 0.5 * (zst - zot + zsb - zob);
+            const double norm = c->dx * c->dy * c->dz[ko] * nx * ny;
+            double* blk = d.p + plane * 9 * ((size_t)ko + (size_t)nl * ks);
+            hipLaunchKernelGGL(k_plane_to_plane, dim3((unsigned)((plane + 127) / 128)), dim3(128), 0, c->stream, nx, ny,
+                               c->dx, c->dy, c->dz[ks], c->dz[ko], rz, num_images, norm, blk);
+            HIPCHK(hipGetLastError());
+            zob = zot + c->il[ko];
+        }
+        zsb = zst + c->il[ks];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> host(count);
+    HIPCHK(hipMemcpy(host.data(), d.p, count * sizeof(double), hipMemcpyDeviceToHost));
+    if (n_real_out) memcpy(n_real_out, host.data(), count * sizeof(double));
+    const bool sym = tensor_sym(c, host.data());
+    return c->precision == MRMG_FP64 ? set_tensor_T<double>(c, d.p, sym) : set_tensor_T<float>(c, d.p, sym);
+}
+
+template <typename V> static void copy_vec(std::vector<V>& dst, const V* src, size_t n)
+{
+    if (src) dst.assign(src, src + n);
+    else dst.clear();
+}
+
+int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
+{
+    if (!c || !in) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (in->ng < 1) return set_err(MRMG_ERR_ARG, "ng must be >= 1");
+    if (!in->grid2grain || !in->grain2gridindex || !in->grain2grid || !in->grain_vol || !in->ms || !in->hk)
+        return set_err(MRMG_ERR_ARG, "layer %d: required array is NULL", k);
+    // keep the device state of the other layers across the rebuild
+    if (!c->dirty) {
+        for (int q = 0; q < c->nl; q++) {
+            HostLayer& H = c->layers[q];
+            H.m.resize(3 * (size_t)H.ng);
+            RCHK(c->eng->get_state(q, H.m.data(), nullptr));
+        }
+    }
+    HostLayer H;
+    H.set = true;
+    const int ng = in->ng;
+    H.ng = ng;
+    const size_t P = (size_t)c->nx * c->ny;
+    copy_vec(H.grid2grain, in->grid2grain, P);
+    copy_vec(H.g2gi, in->grain2gridindex, ng + 1);
+    if (H.g2gi[0] != 0) return set_err(MRMG_ERR_ARG, "grain2gridindex(0) must be 0");
+    for (int n = 0; n < ng; n++)
+        if (H.g2gi[n + 1] < H.g2gi[n]) return set_err(MRMG_ERR_ARG, "grain2gridindex not monotone");
+    H.ncell = H.g2gi[ng];
+    if ((size_t)H.ncell > P) return set_err(MRMG_ERR_ARG, "more grain cells than grid cells");
+    copy_vec(H.g2g, in->grain2grid, 2 * (size_t)H.ncell);
+    copy_vec(H.vol, in->grain_vol, ng);
+    copy_vec(H.ms, in->ms, ng);
+    copy_vec(H.hk, in->hk, 2 * (size_t)ng);
+    H.has_uni = in->uni_axis != nullptr;
+    if (H.has_uni) copy_vec(H.uni, in->uni_axis, 3 * (size_t)ng);
+    H.has_cub = in->cubic_axis != nullptr;
+    if (H.has_cub) copy_vec(H.cub, in->cubic_axis, 9 * (size_t)ng);
+    auto csr = [&](const int* idx, const int* gr, const double* w, std::vector<int>& vi, std::vector<int>& vg,
+                   std::vector<double>& vw, bool& has) -> int {
+        has = idx != nullptr;
+        if (!has) return MRMG_OK;
+        if (!gr || !w) return set_err(MRMG_ERR_ARG, "layer %d: exchange CSR incomplete", k);
+        copy_vec(vi, idx, ng + 1);
+        if (vi[0] != 0) return set_err(MRMG_ERR_ARG, "layer %d: exchange CSR offset(0) must be 0", k);
+        for (int n = 0; n < ng; n++)
+            if (vi[n + 1] < vi[n]) return set_err(MRMG_ERR_ARG, "layer %d: exchange CSR not monotone", k);
+        copy_vec(vg, gr, vi[ng]);
+        copy_vec(vw, w, vi[ng]);
+        return MRMG_OK;
+    };
+    RCHK(csr(in->hxgrainindex, in->hxgrain, in->hx, H.hxi, H.hxg, H.hx, H.has_hx));
+    RCHK(csr(in->uhxgrainindex, in->uhxgrain, in->uhx, H.uhxi, H.uhxg, H.uhx, H.has_uhx));
+    RCHK(csr(in->lhxgrainindex, in->lhxgrain, in->lhx, H.lhxi, H.lhxg, H.lhx, H.has_lhx));
+    H.ms_scale.assign(ng, 1.0); if (in->ms_scale) copy_vec(H.ms_scale, in->ms_scale, ng);
+    H.alpha_scale.assign(ng, 1.0); if (in->alpha_scale) copy_vec(H.alpha_scale, in->alpha_scale, ng);
+    H.hk_scale.assign(ng, 1.0); if (in->hk_scale) copy_vec(H.hk_scale, in->hk_scale, ng);
+    H.temp.assign(ng, 0.0); if (in->temp) copy_vec(H.temp, in->temp, ng);
+    H.has_happ = in->h_app != nullptr;
+    if (H.has_happ) copy_vec(H.happ, in->h_app, 3 * (size_t)ng);
+    H.alpha = in->alpha; H.gamma = in->gamma; H.thm_alpha = in->thm_alpha; H.floor_ms = in->floor_ms;
+    H.thermal = in->thermal;
+    // init_data_one: m = +z (data.f90:201-202)
+    H.m.assign(3 * (size_t)ng, 0.0);
+    for (int n = 0; n < ng; n++) H.m[3 * n + 2] = 1.0;
+    H.uf = c->layers[k].uf;
+    H.has_uf = c->layers[k].has_uf;
+    c->layers[k] = std::move(H);
+    c->dirty = true;
+    return MRMG_OK;
+}
+
+int mrmg_set_ufield(mrmg_ctx* c, int k, const mrmg_ufield* uf)
+{
+    if (!c || !uf) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (uf->t_end == uf->t_start) return set_err(MRMG_ERR_ARG, "t_end == t_start");
+    c->layers[k].uf = *uf;
+    c->layers[k].has_uf = true;
+    if (!c->dirty) return c->eng->update_ufield(c->layers);
+    return MRMG_OK;
+}
+
+int mrmg_set_state(mrmg_ctx* c, int k, const double* m)
+{
+    if (!c || !m) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (!c->layers[k].set) return set_err(MRMG_ERR_STATE, "layer %d not set", k);
+    if (c->dirty) {
+        c->layers[k].m.assign(m, m + 3 * (size_t)c->layers[k].ng);
+        return MRMG_OK;
+    }
+    return c->eng->set_state(k, m);
+}
+
+int mrmg_get_state(mrmg_ctx* c, int k, double* m, double* h)
+{
+    if (!c) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (!c->layers[k].set) return set_err(MRMG_ERR_STATE, "layer %d not set", k);
+    if (c->dirty) {
+        const HostLayer& H = c->layers[k];
+        if (m) memcpy(m, H.m.data(), sizeof(double) * 3 * H.ng);
+        if (h) memset(h, 0, sizeof(double) * 3 * H.ng);
+        return MRMG_OK;
+    }
+    return c->eng->get_state(k, m, h);
+}
+
+int mrmg_step(mrmg_ctx* c, const mrmg_step_params* p, const double* noise, mrmg_diag* diag)
+{
+    if (!c || !p) return set_err(MRMG_ERR_ARG, "null argument");
+    if (p->n_steps < 0) return set_err(MRMG_ERR_ARG, "n_steps < 0");
+    if (c->demag && !c->eng->has_tensor())
+        return set_err(MRMG_ERR_STATE, "demag enabled but no tensor (mrmg_set_tensor / mrmg_build_tensor)");
+    RCHK(ensure_built(c));
+    return c->eng->step(p, noise, diag);
+}
+
+int mrmg_philox_normals(mrmg_ctx* c, uint64_t seed, uint64_t step0, int n_steps, double* out)
+{
+    if (!c || !out) return set_err(MRMG_ERR_ARG, "null argument");
+    RCHK(ensure_built(c));
+    int G = 0;
+    for (auto& L : c->layers) G += L.ng;
+    const size_t n = (size_t)n_steps * G;
+    DBuf<double> d;
+    RCHK(d.alloc(3 * n));
+    hipLaunchKernelGGL(k_philox_dump, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (unsigned long long)seed, (unsigned long long)step0, n_steps, G, d.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, d.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    return MRMG_OK;
+}
+
+int mrmg_demag_field(mrmg_ctx* c, double* h_cell, double* h_grain)
+{
+    if (!c) return set_err(MRMG_ERR_ARG, "null argument");
+    if (c->demag && !c->eng->has_tensor()) return set_err(MRMG_ERR_STATE, "no tensor");
+    RCHK(ensure_built(c));
+    return c->eng->demag_field(h_cell, h_grain);
+}
+
+int mrmg_profile(mrmg_ctx* c, int enable)
+{
+    if (!c) return set_err(MRMG_ERR_ARG, "null argument");
+    return c->eng->profile(enable);
+}
+
+int mrmg_kernel_stats(mrmg_ctx* c, mrmg_kstat* out, int max, int* n)
+{
+    if (!c || !out || !n) return set_err(MRMG_ERR_ARG, "null argument");
+    return c->eng->kernel_stats(out, max, n);
+}
+
+int mrmg_copy_probe(mrmg_ctx* c, size_t bytes, int reps, double* ms)
+{
+    if (!c || !ms || reps < 1) return set_err(MRMG_ERR_ARG, "bad argument");
+    const size_t n = bytes / 16;
+    if (n == 0) return set_err(MRMG_ERR_ARG, "bytes < 16");
+    DBuf<uint4> a, b;
+    RCHK(a.alloc(n));
+    RCHK(b.alloc(n));
+    HIPCHK(hipMemsetAsync(a.p, 1, n * 16, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, a.p, b.p, n);   // warm
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, a.p, b.p, n);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return MRMG_OK;
+}
+
+int mrmg_footprint(mrmg_ctx* c, size_t* bytes)
+{
+    if (!c || !bytes) return set_err(MRMG_ERR_ARG, "null argument");
+    *bytes = c->eng->footprint();
+    return MRMG_OK;
+}
+

@@ -1,0 +1,160 @@
+"""Host-side mirror of the reference's llg_m interface over the C ABI of libmrmg.so.
+
+The reference's driver calls set_llg(...) to configure the integrator and llg(t) to run
+the time loop of an `LLG` command (llg.f90:111-318, 1452-1487; cli.f90:889-896).  Here
+the loop body runs on the GPU through mrmg_step; this module keeps the loop's host-side
+bookkeeping: the step count of a fixed-dt run from del_t / t_max / min_it (the
+termination test of llg.f90:277-281 with the convergence tests disabled, LLG_ERR 0), the
+simulation time t, and the per-layer thermal streams.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class Llg:
+    """One simulation context (a grid, its layers, tensor and state) on the GPU."""
+
+    def __init__(self, nx, ny, n_lay, dx, dy, dz, il_space=None, precision=abi.MRMG_FP64, demag=True):
+        self.lib = abi.load()
+        self.geom = abi.make_geom(nx, ny, n_lay, dx, dy, dz, il_space, precision, demag)
+        h = C.c_void_p()
+        abi.check(self.lib, self.lib.mrmg_create(C.byref(self.geom), C.byref(h)))
+        self.ctx = h
+        self.nx, self.ny, self.n_lay = nx, ny, n_lay
+        self.ngs = [0] * n_lay
+        self._keep = {}
+        # set_llg defaults (llg.f90:8-13, input.f90:386-412)
+        self.dt = 1e-14
+        self.type = abi.MRMG_TYPE_HEUN
+        self.del_t = 1e22
+        self.t_max = 1e22
+        self.t_min = -1e22
+        self.min_it = 10
+        self.step_index = 0
+
+    # -- setup ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mrmg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def set_tensor(self, n_real):
+        n_real = np.ascontiguousarray(n_real, dtype=np.float64)
+        abi.check(self.lib, self.lib.mrmg_set_tensor(self.ctx, abi.dptr(n_real)))
+
+    def build_tensor(self, num_images=2, return_real=False):
+        out = None
+        if return_real:
+            out = np.zeros(self.nx * self.ny * 9 * self.n_lay * self.n_lay)
+        abi.check(self.lib, self.lib.mrmg_build_tensor(self.ctx, num_images, abi.dptr(out)))
+        return out
+
+    def set_layer(self, k, la):
+        s = la.struct()
+        self._keep[("layer", k)] = la
+        abi.check(self.lib, self.lib.mrmg_set_layer(self.ctx, k, C.byref(s)))
+        self.ngs[k] = la.ng
+
+    def set_media(self, media):
+        for k, la in enumerate(media.layers):
+            self.set_layer(k, la)
+
+    def set_ufield(self, k, uf):
+        abi.check(self.lib, self.lib.mrmg_set_ufield(self.ctx, k, C.byref(uf)))
+
+    def set_state(self, k, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        abi.check(self.lib, self.lib.mrmg_set_state(self.ctx, k, abi.dptr(m)))
+
+    def get_state(self, k):
+        m = np.zeros((self.ngs[k], 3))
+        h = np.zeros((self.ngs[k], 3))
+        abi.check(self.lib, self.lib.mrmg_get_state(self.ctx, k, abi.dptr(m), abi.dptr(h)))
+        return m, h
+
+    def profile(self, enable=True):
+        abi.check(self.lib, self.lib.mrmg_profile(self.ctx, int(bool(enable))))
+
+    def kernel_stats(self):
+        arr = (abi.KStat * 16)()
+        n = C.c_int(0)
+        abi.check(self.lib, self.lib.mrmg_kernel_stats(self.ctx, arr, 16, C.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+    def copy_probe(self, nbytes, reps=20):
+        ms = C.c_double()
+        abi.check(self.lib, self.lib.mrmg_copy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
+    def footprint(self):
+        b = C.c_size_t()
+        abi.check(self.lib, self.lib.mrmg_footprint(self.ctx, C.byref(b)))
+        return b.value
+
+    # -- integrator ----------------------------------------------------------
+    def set_llg(self, dt=None, llg_type=None, del_t=None, t_max=None, t_min=None, min_it=None):
+        """set_llg (llg.f90:1452-1487) restricted to the fixed-dt form."""
+        if dt is not None:
+            self.dt = float(dt)
+        if llg_type is not None:
+            self.type = int(llg_type)
+        if del_t is not None:
+            self.del_t = float(del_t)
+        if t_max is not None:
+            self.t_max = float(t_max)
+        if t_min is not None:
+            self.t_min = float(t_min)
+        if min_it is not None:
+            self.min_it = max(1, int(min_it))
+
+    def steps_for(self, t):
+        """Steps an `LLG` command takes from time t with fixed dt (llg.f90:268-281)."""
+        it, del_t = 0, 0.0
+        while True:
+            t = t + self.dt
+            del_t = del_t + self.dt
+            it += 1
+            if not ((del_t < self.del_t and t < max(self.t_max, self.t_min)) or it < self.min_it):
+                return it
+
+    def step(self, n_steps, t0=0.0, noise=None, philox_seed=None, diag=True):
+        """Run n_steps fixed-dt steps from time t0.  noise: (n_steps, G, 3) normals to inject,
+        or philox_seed to draw them on the device."""
+        p = abi.StepParams(type=self.type, n_steps=int(n_steps), t0=float(t0), dt=self.dt,
+                           noise=abi.MRMG_NOISE_NONE, seed=0, step0=self.step_index)
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            assert nz.size == 3 * sum(self.ngs) * n_steps, "noise must be (n_steps, G, 3)"
+            p.noise = abi.MRMG_NOISE_INJECT
+        elif philox_seed is not None:
+            p.noise = abi.MRMG_NOISE_PHILOX
+            p.seed = int(philox_seed)
+        d = abi.Diag() if diag else None
+        abi.check(self.lib, self.lib.mrmg_step(self.ctx, C.byref(p), abi.dptr(nz), C.byref(d) if d else None))
+        self.step_index += int(n_steps)
+        return d
+
+    def llg(self, t, noise=None, philox_seed=None):
+        """One `LLG` command: integrate from t until the termination test; returns (t, diag)."""
+        n = self.steps_for(t)
+        d = self.step(n, t0=t, noise=noise, philox_seed=philox_seed)
+        return d.t, d
+
+    def philox_normals(self, seed, step0, n_steps):
+        out = np.zeros((n_steps, sum(self.ngs), 3))
+        abi.check(self.lib, self.lib.mrmg_philox_normals(self.ctx, int(seed), int(step0), int(n_steps),
+                                                         abi.dptr(out)))
+        return out
+
+    def demag_field(self):
+        hc = np.zeros((3, self.n_lay, self.ny, self.nx))
+        hg = np.zeros((sum(self.ngs), 3))
+        abi.check(self.lib, self.lib.mrmg_demag_field(self.ctx, abi.dptr(hc), abi.dptr(hg)))
+        return hc, hg

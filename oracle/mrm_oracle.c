@@ -1,0 +1,1007 @@
+/*
+ * mrm_oracle.c -- CPU restatement (FP64) of mrm's granular-media time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * mrm_amd/csrc: only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it.  The product path never links or calls it.
+ *
+ * It restates, in plain C with the same operation order where that matters for
+ * rounding, the reference Fortran (paths relative to the reference's src/):
+ *   random.f90:166-249          MT19937 with 69069 seeding, polar normals     (ora_mt_*)
+ *   tensor_element.f90:13-102   volume_averaged_n (Newell g1/g2 sums)          (ora_volume_averaged_n)
+ *   tensor_element.f90:219-261  plane_to_plane (periodic images, even-n fix)   (ora_plane_to_plane)
+ *   tensor.f90:82-216           calculate_tensor (layer pairs, SUL/head images) (ora_calculate_tensor)
+ *   fftw.f90:37-93              2D r2c / c2r, unnormalised, x = halved axis     (ora_rfft2 / ora_irfft2)
+ *   demag.f90:33-135, 200-286   tensor FFT, k-space product, inverse FFT        (ora_demag)
+ *   data.f90:322-390            grain -> cell scatter of Ms*ms_scale*m          (ora_demag)
+ *   field.f90:160-273, 326-395  field assembly, anisotropy, CSR exchange        (field_grain)
+ *   field.f90:397-436           thermal field (sigma * injected normals)        (thermal)
+ *   field.f90:438-517           cell -> grain average of the demag field        (get_field)
+ *   applied.f90:863-883, 210-320 uniform field with a linear ramp               (ufield_at)
+ *   llg.f90:320-388             get_fields: dt control                          (ora_step)
+ *   llg.f90:1066-1155           push_llg: Heun / Boris / quaternion             (push_llg)
+ *   llg.f90:1195-1271           push_m: stage control, rejection, next dt       (push_m)
+ *   transform.f90:16-20,65-115  cross, rotate_q, rotate_2                       (cross3 / rotate_2)
+ *
+ * Parity pins (see tests/): MT19937 against numpy's MT19937 and the flang-built
+ * reference in oracle/_ref; the tensor against oracle/_ref and the closed forms of a
+ * cube; the FFT against numpy.fft; the demag product against a brute-force circular
+ * sum; the pushes against the analytic macrospin solution (2nd-order convergence for
+ * Heun); exchange against reciprocity; thermal sigma against the fluctuation-dissipation
+ * value.  The adaptive-dt rejection path (push_m with rot_ratio > 1) is pinned only by
+ * invariants: parity unpinned for its exact retry sequence.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include "../include/mrmg.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double re, im; } cplx;
+
+/* ============================================================================
+ * MT19937, random.f90:9-249 (32-bit signed arithmetic of the Fortran restated
+ * with uint32_t, which gives the same bit patterns)
+ * ========================================================================== */
+typedef struct { uint32_t mt[624]; int mti; int iset; double gset; } ora_mt;
+
+void ora_mt_seed(ora_mt *s, int seed)  /* random_create, random.f90:166-183 */
+{
+    uint32_t iseed = (uint32_t)(4357 + abs(seed));
+    s->mt[0] = iseed;
+    for (int i = 1; i < 624; i++) s->mt[i] = 69069u * s->mt[i - 1];
+    s->iset = 0; s->mti = 624; s->gset = 0.0;
+}
+
+double ora_mt_uniform(ora_mt *s)  /* random_uniform, random.f90:185-213 */
+{
+    const uint32_t MATA = 0x9908b0dfu, UM = 0x80000000u, LM = 0x7fffffffu;
+    uint32_t j;
+    if (s->mti >= 624) {
+        int i;
+        for (i = 0; i < 624 - 397; i++) {
+            j = (s->mt[i] & UM) | (s->mt[i + 1] & LM);
+            s->mt[i] = s->mt[i + 397] ^ (j >> 1) ^ ((j & 1u) ? MATA : 0u);
+        }
+        for (; i < 623; i++) {
+            j = (s->mt[i] & UM) | (s->mt[i + 1] & LM);
+            s->mt[i] = s->mt[i + 397 - 624] ^ (j >> 1) ^ ((j & 1u) ? MATA : 0u);
+        }
+        j = (s->mt[623] & UM) | (s->mt[0] & LM);
+        s->mt[623] = s->mt[396] ^ (j >> 1) ^ ((j & 1u) ? MATA : 0u);
+        s->mti = 0;
+    }
+    j = s->mt[s->mti++];
+    j ^= j >> 11;
+    j ^= (j << 7) & 0x9d2c5680u;
+    j ^= (j << 15) & 0xefc60000u;
+    j ^= j >> 18;
+    return (double)j / 4294967296.0;
+}
+
+static double polar_draw(ora_mt *s)  /* random_normal, random.f90:215-229 */
+{
+    double r, rsq, v1, v2, f;
+    if (s->iset != 0) { r = s->gset; s->iset = 0; return r; }
+    do {
+        v1 = 2.0 * ora_mt_uniform(s) - 1.0;
+        v2 = 2.0 * ora_mt_uniform(s) - 1.0;
+        rsq = v1 * v1 + v2 * v2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    f = sqrt(-2.0 * log(rsq) / rsq);
+    s->gset = v1 * f; r = v2 * f; s->iset = 1;
+    return r;
+}
+double ora_mt_normal(ora_mt *s) { return polar_draw(s); }
+void ora_mt_normal3d(ora_mt *s, double *r3)  /* random_normal3d, random.f90:231-249 */
+{
+    for (int i = 0; i < 3; i++) r3[i] = polar_draw(s);
+}
+size_t ora_mt_size(void) { return sizeof(ora_mt); }
+void ora_mt_fill_uniform(ora_mt *s, int n, double *out) { for (int i = 0; i < n; i++) out[i] = ora_mt_uniform(s); }
+void ora_mt_fill_normal3d(ora_mt *s, int n, double *out) { for (int i = 0; i < n; i++) ora_mt_normal3d(s, out + 3 * i); }
+
+/* ============================================================================
+ * Newell tensor, tensor_element.f90:13-102
+ * n is the Fortran (3,3) array, n[(a-1)+3(b-1)] = n(a,b)
+ * ========================================================================== */
+static double fsign(double a, double b) { return signbit(b) ? -fabs(a) : fabs(a); }
+static double fmax1(double a, double b) { return a > b ? a : b; }
+
+static double g1(double x, double y, double z, double d)
+{
+    double r = x * y * z * atan(y * z / fsign(fmax1(1.e-300, fabs(x) * d), x))
+             + 0.5 * y * (z * z - x * x) * log(fmax1(1.e-300, d - y))
+             + 0.5 * z * (y * y - x * x) * log(fmax1(1.e-300, d - z))
+             + (y * y + z * z - 2.0 * x * x) * d / 6.0;
+    return r;
+}
+static double g2(double z, double x, double y, double d)
+{
+    double r = -x * y * z * log(fmax1(1.e-300, d + z))
+             + y * (y * y - 3.0 * z * z) * log(fmax1(1.e-300, d + x)) / 6.0
+             + x * (x * x - 3.0 * z * z) * log(fmax1(1.e-300, d + y)) / 6.0
+             + 0.5 * x * x * z * atan(y * z / fsign(fmax1(1.e-300, fabs(x) * d), x))
+             + 0.5 * y * y * z * atan(x * z / fsign(fmax1(1.e-300, fabs(y) * d), y))
+             + z * z * z * atan(x * y / fsign(fmax1(1.e-300, fabs(z) * d), z)) / 6.0 + x * y * d / 3.0;
+    return r;
+}
+
+double ora_volume_averaged_n(double rx, double ry, double rz, double xs, double ys, double zs,
+                             double xo, double yo, double zo, double *n)
+{
+    double dxo[2] = {-0.5 * xo, 0.5 * xo};
+    double dyo[2] = {-0.5 * xo, 0.5 * xo};   /* the reference uses xo here (tensor_element.f90:31) */
+    double dzo[2] = {-0.5 * zo, 0.5 * zo};
+    double dxs[2] = {-0.5 * xs, 0.5 * xs}, dys[2] = {-0.5 * ys, 0.5 * ys}, dzs[2] = {-0.5 * zs, 0.5 * zs};
+    double vol = 1.0 / (xo * yo * zo);
+    double t[9] = {0};
+    (void)yo;
+    for (int i6 = 1; i6 <= 2; i6++)
+    for (int i5 = 1; i5 <= 2; i5++)
+    for (int i4 = 1; i4 <= 2; i4++)
+    for (int i3 = 1; i3 <= 2; i3++) {
+        double dx = -rx + dxo[i3 - 1] - dxs[i6 - 1];
+        for (int i2 = 1; i2 <= 2; i2++) {
+            double dy = -ry + dyo[i2 - 1] - dys[i5 - 1];
+            for (int i1 = 1; i1 <= 2; i1++) {
+                double dz = -rz + dzo[i1 - 1] - dzs[i4 - 1];
+                double d = sqrt(dx * dx + dy * dy + dz * dz);
+                if (d != 0.0) {
+                    int e = i1 + i2 + i3 + i4 + i5 + i6 - 1;
+                    double s = (e & 1) ? -1.0 : 1.0;
+                    double x;
+                    t[0] = t[0] + s * g1(dx, dy, dz, d);
+                    t[4] = t[4] + s * g1(dy, dz, dx, d);
+                    t[8] = t[8] + s * g1(dz, dx, dy, d);
+                    x = s * g2(dx, dy, dz, d); t[5] = t[5] + x; t[7] = t[7] + x;  /* t(3,2), t(2,3) */
+                    x = s * g2(dy, dz, dx, d); t[2] = t[2] + x; t[6] = t[6] + x;  /* t(3,1), t(1,3) */
+                    x = s * g2(dz, dx, dy, d); t[1] = t[1] + x; t[3] = t[3] + x;  /* t(2,1), t(1,2) */
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 9; i++) n[i] = n[i] - t[i];
+    return vol;
+}
+
+/* plane_to_plane, tensor_element.f90:219-261; out = Fortran (nx,ny,3,3) */
+void ora_plane_to_plane(int nx, int ny, double dx, double dy, double dz, double dzo, double rz,
+                        int n_imgxi, int n_imgyi, double *out)
+{
+    int n_imgx = n_imgxi > 0 ? n_imgxi : 0;
+    int n_imgy = n_imgxi > 0 ? n_imgxi : 0;  /* the reference takes n_imgxi here too (line 230) */
+    (void)n_imgyi;
+    size_t plane = (size_t)nx * ny;
+    memset(out, 0, sizeof(double) * plane * 9);
+    int nj = ny / 2 * 2 + 1, ni = nx / 2 * 2 + 1;
+    /* the (i,j) offsets that alias the same cell are accumulated in this order serially;
+       parallelise over disjoint output cells only */
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int jj = 0; jj < nj; jj++) {
+        int j = -ny / 2 + jj;
+        int yp0 = -n_imgy * (1 - (j / ((ny + 1) / 2) > 0 ? j / ((ny + 1) / 2) : 0));
+        int yp1 = n_imgy * (1 + (j / ((ny + 1) / 2) < 0 ? j / ((ny + 1) / 2) : 0));
+        int oj = ((ny - j) % ny + ny) % ny;
+        /* j = +ny/2 and j = -ny/2 alias the same row for even ny: handled by the
+           serial pass below to keep the reference's accumulation order */
+        if (ny % 2 == 0 && j == ny / 2) continue;
+        for (int ii = 0; ii < ni; ii++) {
+            int i = -nx / 2 + ii;
+            int xp0 = -n_imgx * (1 - (i / ((nx + 1) / 2) > 0 ? i / ((nx + 1) / 2) : 0));
+            int xp1 = n_imgx * (1 + (i / ((nx + 1) / 2) < 0 ? i / ((nx + 1) / 2) : 0));
+            double t[9] = {0};
+            for (int nyi = yp0; nyi <= yp1; nyi++)
+                for (int nxi = xp0; nxi <= xp1; nxi++)
+                    ora_volume_averaged_n(i * dx + nxi * dx * nx, j * dy + nyi * dy * ny, rz,
+                                          dx, dy, dz, dx, dy, dzo, t);
+            int oi = ((nx - i) % nx + nx) % nx;
+            for (int c = 0; c < 9; c++) out[oi + (size_t)nx * oj + plane * c] += t[c];
+        }
+    }
+    if (ny % 2 == 0) {
+        int j = ny / 2;
+        int yp0 = -n_imgy * (1 - (j / ((ny + 1) / 2) > 0 ? j / ((ny + 1) / 2) : 0));
+        int yp1 = n_imgy * (1 + (j / ((ny + 1) / 2) < 0 ? j / ((ny + 1) / 2) : 0));
+        int oj = ((ny - j) % ny + ny) % ny;
+        for (int ii = 0; ii < ni; ii++) {
+            int i = -nx / 2 + ii;
+            int xp0 = -n_imgx * (1 - (i / ((nx + 1) / 2) > 0 ? i / ((nx + 1) / 2) : 0));
+            int xp1 = n_imgx * (1 + (i / ((nx + 1) / 2) < 0 ? i / ((nx + 1) / 2) : 0));
+            double t[9] = {0};
+            for (int nyi = yp0; nyi <= yp1; nyi++)
+                for (int nxi = xp0; nxi <= xp1; nxi++)
+                    ora_volume_averaged_n(i * dx + nxi * dx * nx, j * dy + nyi * dy * ny, rz,
+                                          dx, dy, dz, dx, dy, dzo, t);
+            int oi = ((nx - i) % nx + nx) % nx;
+            for (int c = 0; c < 9; c++) out[oi + (size_t)nx * oj + plane * c] += t[c];
+        }
+    }
+}
+
+/* calculate_tensor, tensor.f90:82-216 (the mrm_N.dat cache is not used).
+ * n2d = Fortran (nx,ny,9,n_lay,n_lay); sul_mu/head_mu <= 1 disable those images. */
+void ora_calculate_tensor(int nx, int ny, int n_lay, double dx, double dy, const double *dz,
+                          const double *il_space, int imx, int imy, double sul_mu, double head_mu,
+                          double fly_height, double space_to_sul, double *n2d)
+{
+    size_t plane = (size_t)nx * ny;
+    double *nn = (double *)malloc(sizeof(double) * plane * 9);
+    double *hdz = (double *)malloc(sizeof(double) * n_lay);
+    int sul_p = sul_mu > 1.0, head_p = head_mu > 1.0;
+    double mus = 0.0, mush = 0.0;
+    if (sul_p) mus = fmax1(0.0, (fmax1(1.0, sul_mu) - 1.0) / (fmax1(1.0, sul_mu) + 1.0));
+    if (head_p) mush = fmax1(0.0, (fmax1(1.0, head_mu) - 1.0) / (fmax1(1.0, head_mu) + 1.0));
+    memset(n2d, 0, sizeof(double) * plane * 9 * n_lay * n_lay);
+    hdz[0] = fly_height;
+    for (int ks = 1; ks < n_lay; ks++) hdz[ks] = hdz[ks - 1] + dz[ks - 1] + il_space[ks];
+    for (int ks = 0; ks < n_lay; ks++) hdz[ks] = hdz[ks] + 0.5 * dz[ks];
+    double zsb = space_to_sul;
+    for (int ks = n_lay - 1; ks >= 0; ks--) {
+        double zst = zsb + dz[ks];
+        double zob = space_to_sul;
+        for (int ko = n_lay - 1; ko >= 0; ko--) {
+            double zot = zob + dz[ko];
+            double *blk = n2d + plane * 9 * ((size_t)ko + (size_t)n_lay * ks);
+            /* n2d(:,:,1:3,ko,ks)=nn(:,:,:,1) etc: component 3(b'-1)+a' <- nn(:,:,a',b') */
+            ora_plane_to_plane(nx, ny, dx, dy, dz[ks], dz[ko], 0.5 * (zst - zot + zsb - zob), imx, imy, nn);
+            for (int c = 0; c < 9; c++) memcpy(blk + plane * c, nn + plane * c, sizeof(double) * plane);
+            if (sul_p) {
+                ora_plane_to_plane(nx, ny, dx, dy, dz[ks], dz[ko], 0.5 * (-zst - zot - zsb - zob), imx, imy, nn);
+                for (int b = 0; b < 3; b++) for (int a = 0; a < 3; a++) {
+                    double f = (a < 2) ? -mus : mus;   /* nn(:,:,1:2,:) flips sign */
+                    double *src = nn + plane * (a + 3 * b), *dst = blk + plane * (a + 3 * b);
+                    for (size_t p = 0; p < plane; p++) dst[p] = dst[p] + f * src[p];
+                }
+            }
+            if (head_p) {
+                ora_plane_to_plane(nx, ny, dx, dy, dz[ks], dz[ko], hdz[ks] + hdz[ko], imx, imy, nn);
+                for (int b = 0; b < 3; b++) for (int a = 0; a < 3; a++) {
+                    double f = (a < 2) ? -mush : mush;
+                    double *src = nn + plane * (a + 3 * b), *dst = blk + plane * (a + 3 * b);
+                    for (size_t p = 0; p < plane; p++) dst[p] = dst[p] + f * src[p];
+                }
+            }
+            double norm = dx * dy * dz[ko] * nx * ny;
+            for (size_t p = 0; p < plane * 9; p++) blk[p] = blk[p] / norm;
+            zob = zot + il_space[ko];
+        }
+        zsb = zst + il_space[ks];
+    }
+    free(nn); free(hdz);
+}
+
+/* ============================================================================
+ * FFT: unnormalised complex DFT by recursive mixed radix (2,3,4,5 and a direct
+ * DFT for other primes); 2D real transforms with FFTW's r2c/c2r semantics for a
+ * Fortran (nx,ny) array transformed as fftw_plan_dft_r2c_2d(ny,nx) (fftw.f90:66-84)
+ * ========================================================================== */
+/* W[k] = exp(sign*2*pi*i*k/N), k = 0..N-1; every sub-length n | N uses W[e*(N/n)] */
+static cplx *twiddles(int N, int sign)
+{
+    cplx *W = (cplx *)malloc(sizeof(cplx) * N);
+    for (int k = 0; k < N; k++) {
+        double a = 2.0 * M_PI * (double)k / N;
+        W[k].re = cos(a); W[k].im = sign * sin(a);
+    }
+    return W;
+}
+
+/* out[k] = sum_j in[j*stride] w_n^{jk}; w_n^e = W[e*ts], ts = N/n; scratch >= n entries */
+static void fft_rec(const cplx *in, int stride, cplx *out, int n, int ts, const cplx *W, cplx *scratch)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int r = 0;
+    if (n % 4 == 0) r = 4; else if (n % 2 == 0) r = 2; else if (n % 3 == 0) r = 3; else if (n % 5 == 0) r = 5;
+    if (r == 0) {  /* other prime: direct DFT */
+        for (int j = 0; j < n; j++) scratch[j] = in[(size_t)j * stride];
+        for (int k = 0; k < n; k++) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < n; j++) {
+                cplx w = W[(size_t)((long)j * k % n) * ts];
+                sr += scratch[j].re * w.re - scratch[j].im * w.im;
+                si += scratch[j].re * w.im + scratch[j].im * w.re;
+            }
+            out[k].re = sr; out[k].im = si;
+        }
+        return;
+    }
+    int m = n / r;
+    for (int q = 0; q < r; q++) fft_rec(in + (size_t)q * stride, stride * r, out + q * m, m, ts * r, W, scratch);
+    memcpy(scratch, out, sizeof(cplx) * n);
+    for (int k = 0; k < m; k++) {
+        cplx v[5];
+        v[0] = scratch[k];
+        for (int q = 1; q < r; q++) {
+            cplx w = W[(size_t)q * k * ts], z = scratch[q * m + k];
+            v[q].re = z.re * w.re - z.im * w.im; v[q].im = z.re * w.im + z.im * w.re;
+        }
+        for (int p = 0; p < r; p++) {
+            double sr = v[0].re, si = v[0].im;
+            for (int q = 1; q < r; q++) {
+                int e = (p * q) % r;
+                cplx w = W[(size_t)e * m * ts];
+                sr += v[q].re * w.re - v[q].im * w.im;
+                si += v[q].re * w.im + v[q].im * w.re;
+            }
+            out[k + p * m].re = sr; out[k + p * m].im = si;
+        }
+    }
+}
+
+static void fft_tab(cplx *x, int n, int stride, const cplx *W)
+{
+    cplx *in = (cplx *)malloc(sizeof(cplx) * n), *out = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *scr = (cplx *)malloc(sizeof(cplx) * n);
+    for (int j = 0; j < n; j++) in[j] = x[(size_t)j * stride];
+    fft_rec(in, 1, out, n, 1, W, scr);
+    for (int j = 0; j < n; j++) x[(size_t)j * stride] = out[j];
+    free(in); free(out); free(scr);
+}
+
+void ora_fft(cplx *x, int n, int stride, int sign)
+{
+    cplx *W = twiddles(n, sign);
+    fft_tab(x, n, stride, W);
+    free(W);
+}
+
+/* r: Fortran (nx,ny) real; c: Fortran (nx/2+1,ny) complex */
+void ora_rfft2(const double *r, cplx *c, int nx, int ny)
+{
+    int nxh = nx / 2 + 1;
+    cplx *Wx = twiddles(nx, -1), *Wy = twiddles(ny, -1);
+    #pragma omp parallel for
+    for (int j = 0; j < ny; j++) {
+        cplx *row = (cplx *)malloc(sizeof(cplx) * nx);
+        for (int i = 0; i < nx; i++) { row[i].re = r[i + (size_t)nx * j]; row[i].im = 0.0; }
+        fft_tab(row, nx, 1, Wx);
+        for (int k = 0; k < nxh; k++) c[k + (size_t)nxh * j] = row[k];
+        free(row);
+    }
+    #pragma omp parallel for
+    for (int k = 0; k < nxh; k++) fft_tab(c + k, ny, nxh, Wy);
+    free(Wx); free(Wy);
+}
+
+/* unnormalised inverse; c is destroyed (FFTW_DESTROY_INPUT) */
+void ora_irfft2(cplx *c, double *r, int nx, int ny)
+{
+    int nxh = nx / 2 + 1;
+    cplx *Wx = twiddles(nx, +1), *Wy = twiddles(ny, +1);
+    #pragma omp parallel for
+    for (int k = 0; k < nxh; k++) fft_tab(c + k, ny, nxh, Wy);
+    #pragma omp parallel for
+    for (int j = 0; j < ny; j++) {
+        cplx *row = (cplx *)malloc(sizeof(cplx) * nx);
+        const cplx *cj = c + (size_t)nxh * j;
+        row[0].re = cj[0].re; row[0].im = 0.0;
+        for (int k = 1; k < nxh; k++) {
+            row[k] = cj[k];
+            if (nx - k != k) { row[nx - k].re = cj[k].re; row[nx - k].im = -cj[k].im; }
+        }
+        if (nx % 2 == 0) { row[nx / 2].re = cj[nx / 2].re; row[nx / 2].im = 0.0; }
+        fft_tab(row, nx, 1, Wx);
+        for (int i = 0; i < nx; i++) r[i + (size_t)nx * j] = row[i].re;
+        free(row);
+    }
+    free(Wx); free(Wy);
+}
+
+/* ============================================================================
+ * Simulation state: the reference's per-layer data_s/field_ss/llg_s arrays
+ * ========================================================================== */
+typedef struct {
+    int ng, ncell;
+    int *g2gi, *g2g, *grid2grain;
+    double *vol, *ms, *hk, *uni, *cub;
+    int *hxi, *hxg, *uhxi, *uhxg, *lhxi, *lhxg;
+    double *hx, *uhx, *lhx;
+    double *ms_scale, *alpha_scale, *hk_scale, *temp, *happ_s;
+    double alpha, gamma, thm_alpha, floor_ms;
+    int thermal;
+    double *m;      /* (6,ng) double buffer, slot `in` */
+    double *mag;    /* (2,ng) */
+    double *h, *hthm, *dm, *h_app, *h_demag;
+    mrmg_ufield uf; int has_uf;
+    int errl, err1l;
+    double max_dm;
+} olayer;
+
+typedef struct ora_sim {
+    int nx, ny, nl, demag;
+    double dx, dy, *dz, *il;
+    olayer *L;
+    cplx *nh;      /* tensor spectra (nxh,ny,9,nl,nl) */
+    int have_tensor;
+    int in;        /* double-buffer toggle shared by all layers (0 or 3) */
+    /* llgp_s */
+    double dt, max_dmdt, max_dmdtnext, errmx, errmx1, mmag;
+    double *hcell; /* (nx,ny,nl) x3 */
+    double tsum_dummy;
+} ora_sim;
+
+static void *dupa(const void *p, size_t bytes)
+{
+    if (!p) return NULL;
+    void *q = malloc(bytes ? bytes : 1);
+    memcpy(q, p, bytes);
+    return q;
+}
+
+ora_sim *ora_create(const mrmg_geom *g)
+{
+    ora_sim *s = (ora_sim *)calloc(1, sizeof(ora_sim));
+    s->nx = g->nx; s->ny = g->ny; s->nl = g->n_lay; s->demag = g->demag;
+    s->dx = g->dx; s->dy = g->dy;
+    s->dz = (double *)dupa(g->dz, sizeof(double) * g->n_lay);
+    s->il = (double *)dupa(g->il_space, sizeof(double) * g->n_lay);
+    s->L = (olayer *)calloc(g->n_lay, sizeof(olayer));
+    s->dt = 1e-14; s->max_dmdt = 1.0; s->max_dmdtnext = 1.0;
+    s->hcell = (double *)calloc((size_t)3 * g->nx * g->ny * g->n_lay, sizeof(double));
+    return s;
+}
+
+static void free_layer(olayer *L)
+{
+    void *ps[] = {L->g2gi, L->g2g, L->grid2grain, L->vol, L->ms, L->hk, L->uni, L->cub, L->hxi, L->hxg,
+                  L->uhxi, L->uhxg, L->lhxi, L->lhxg, L->hx, L->uhx, L->lhx, L->ms_scale, L->alpha_scale,
+                  L->hk_scale, L->temp, L->happ_s, L->m, L->mag, L->h, L->hthm, L->dm, L->h_app, L->h_demag};
+    for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
+    memset(L, 0, sizeof(*L));
+}
+
+void ora_destroy(ora_sim *s)
+{
+    if (!s) return;
+    for (int k = 0; k < s->nl; k++) free_layer(&s->L[k]);
+    free(s->L); free(s->dz); free(s->il); free(s->nh); free(s->hcell); free(s);
+}
+
+static double *fill1(const double *p, int n)
+{
+    double *q = (double *)malloc(sizeof(double) * (n ? n : 1));
+    for (int i = 0; i < n; i++) q[i] = p ? p[i] : 1.0;
+    return q;
+}
+
+int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
+{
+    if (k < 0 || k >= s->nl) return MRMG_ERR_ARG;
+    olayer *L = &s->L[k];
+    free_layer(L);
+    int ng = in->ng;
+    L->ng = ng;
+    L->g2gi = (int *)dupa(in->grain2gridindex, sizeof(int) * (ng + 1));
+    L->ncell = L->g2gi[ng];
+    L->g2g = (int *)dupa(in->grain2grid, sizeof(int) * 2 * L->ncell);
+    L->grid2grain = (int *)dupa(in->grid2grain, sizeof(int) * s->nx * s->ny);
+    L->vol = (double *)dupa(in->grain_vol, sizeof(double) * ng);
+    L->ms = (double *)dupa(in->ms, sizeof(double) * ng);
+    L->hk = (double *)dupa(in->hk, sizeof(double) * 2 * ng);
+    L->uni = (double *)dupa(in->uni_axis, sizeof(double) * 3 * ng);
+    L->cub = (double *)dupa(in->cubic_axis, sizeof(double) * 9 * ng);
+    if (in->hxgrainindex) {
+        L->hxi = (int *)dupa(in->hxgrainindex, sizeof(int) * (ng + 1));
+        L->hxg = (int *)dupa(in->hxgrain, sizeof(int) * L->hxi[ng]);
+        L->hx = (double *)dupa(in->hx, sizeof(double) * L->hxi[ng]);
+    }
+    if (in->uhxgrainindex) {
+        L->uhxi = (int *)dupa(in->uhxgrainindex, sizeof(int) * (ng + 1));
+        L->uhxg = (int *)dupa(in->uhxgrain, sizeof(int) * L->uhxi[ng]);
+        L->uhx = (double *)dupa(in->uhx, sizeof(double) * L->uhxi[ng]);
+    }
+    if (in->lhxgrainindex) {
+        L->lhxi = (int *)dupa(in->lhxgrainindex, sizeof(int) * (ng + 1));
+        L->lhxg = (int *)dupa(in->lhxgrain, sizeof(int) * L->lhxi[ng]);
+        L->lhx = (double *)dupa(in->lhx, sizeof(double) * L->lhxi[ng]);
+    }
+    L->ms_scale = fill1(in->ms_scale, ng);
+    L->alpha_scale = fill1(in->alpha_scale, ng);
+    L->hk_scale = fill1(in->hk_scale, ng);
+    L->temp = (double *)calloc(ng ? ng : 1, sizeof(double));
+    if (in->temp) memcpy(L->temp, in->temp, sizeof(double) * ng);
+    L->happ_s = (double *)dupa(in->h_app, sizeof(double) * 3 * ng);
+    L->alpha = in->alpha; L->gamma = in->gamma; L->thm_alpha = in->thm_alpha; L->floor_ms = in->floor_ms;
+    L->thermal = in->thermal;
+    /* init_data_one, data.f90:201-202 */
+    L->m = (double *)calloc((size_t)6 * ng + 1, sizeof(double));
+    for (int n = 0; n < ng; n++) { L->m[6 * n + 2] = 1.0; L->m[6 * n + 5] = 1.0; }
+    L->mag = (double *)malloc(sizeof(double) * (2 * ng + 1));
+    for (int n = 0; n < 2 * ng; n++) L->mag[n] = 1.0;
+    L->h = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
+    L->hthm = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
+    L->dm = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
+    L->h_app = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
+    L->h_demag = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
+    L->uf.scale = 0.0; L->has_uf = 0;
+    L->max_dm = 3.1415925;
+    return MRMG_OK;
+}
+
+int ora_set_ufield(ora_sim *s, int k, const mrmg_ufield *uf)
+{
+    if (k < 0 || k >= s->nl) return MRMG_ERR_ARG;
+    mrmg_ufield u = *uf;
+    /* set_fld normalisation, applied.f90:307-310 */
+    if (u.s_start == u.s_end) { u.scale = u.scale * u.s_start; u.s_end = 1.0; u.s_start = 1.0; }
+    s->L[k].uf = u; s->L[k].has_uf = 1;
+    return MRMG_OK;
+}
+
+void ora_set_max_dm(ora_sim *s, int k, double max_dm_deg)
+{
+    s->L[k].max_dm = max_dm_deg * acos(-1.0) / 180.0;
+}
+
+int ora_set_state(ora_sim *s, int k, const double *m)
+{
+    olayer *L = &s->L[k];
+    for (int n = 0; n < L->ng; n++)
+        for (int c = 0; c < 3; c++) L->m[6 * n + s->in + c] = m[3 * n + c];
+    return MRMG_OK;
+}
+
+int ora_get_state(ora_sim *s, int k, double *m, double *h)
+{
+    olayer *L = &s->L[k];
+    for (int n = 0; n < L->ng; n++)
+        for (int c = 0; c < 3; c++) {
+            if (m) m[3 * n + c] = L->m[6 * n + s->in + c];
+            if (h) h[3 * n + c] = L->h[3 * n + c];
+        }
+    return MRMG_OK;
+}
+
+/* init_demag, demag.f90:259-267: FFT of every real tensor plane (FFTD: no rounding) */
+int ora_set_tensor(ora_sim *s, const double *n_real)
+{
+    int nx = s->nx, ny = s->ny, nl = s->nl, nxh = nx / 2 + 1;
+    size_t plane = (size_t)nx * ny, cplane = (size_t)nxh * ny;
+    free(s->nh);
+    s->nh = (cplx *)malloc(sizeof(cplx) * cplane * 9 * nl * nl);
+    for (size_t p = 0; p < (size_t)9 * nl * nl; p++) ora_rfft2(n_real + plane * p, s->nh + cplane * p, nx, ny);
+    s->have_tensor = 1;
+    return MRMG_OK;
+}
+
+/* get_mag (data.f90:344-356) + get_demag (demag.f90:113-127); hcell = (nx,ny,nl) per comp */
+static void ora_demag(ora_sim *s, double *hcell)
+{
+    int nx = s->nx, ny = s->ny, nl = s->nl, nxh = nx / 2 + 1, in = s->in;
+    size_t plane = (size_t)nx * ny, cplane = (size_t)nxh * ny;
+    double *mi = (double *)calloc(plane * nl * 3, sizeof(double));   /* xi2d,yi2d,zi2d */
+    cplx *mo = (cplx *)malloc(sizeof(cplx) * cplane * nl * 3);      /* xo2d,yo2d,zo2d */
+    cplx *ho = (cplx *)calloc(cplane * nl * 3, sizeof(cplx));        /* x2d,y2d,z2d   */
+    for (int k = 0; k < nl; k++) {
+        olayer *L = &s->L[k];
+        for (int n = 0; n < L->ng; n++) {
+            double mm[3];
+            for (int c = 0; c < 3; c++) mm[c] = L->ms[n] * L->ms_scale[n] * L->m[6 * n + in + c];
+            for (int q = L->g2gi[n]; q < L->g2gi[n + 1]; q++) {
+                int i = L->g2g[2 * q] - 1, j = L->g2g[2 * q + 1] - 1;
+                for (int c = 0; c < 3; c++) mi[(size_t)i + (size_t)nx * j + plane * (k + (size_t)nl * c)] = mm[c];
+            }
+        }
+    }
+    #pragma omp parallel for
+    for (int p = 0; p < 3 * nl; p++) ora_rfft2(mi + plane * p, mo + cplane * p, nx, ny);
+    /* k-space product in the reference's order: k1 = source outer, k2 = observation */
+    #pragma omp parallel for
+    for (int a = 0; a < 3; a++)
+        for (int k1 = 0; k1 < nl; k1++)
+            for (int k2 = 0; k2 < nl; k2++) {
+                cplx *dst = ho + cplane * (k2 + (size_t)nl * a);
+                const cplx *t1 = s->nh + cplane * (3 * a + 0 + 9 * ((size_t)k2 + (size_t)nl * k1));
+                const cplx *t2 = s->nh + cplane * (3 * a + 1 + 9 * ((size_t)k2 + (size_t)nl * k1));
+                const cplx *t3 = s->nh + cplane * (3 * a + 2 + 9 * ((size_t)k2 + (size_t)nl * k1));
+                const cplx *xo = mo + cplane * (k1 + (size_t)nl * 0);
+                const cplx *yo = mo + cplane * (k1 + (size_t)nl * 1);
+                const cplx *zo = mo + cplane * (k1 + (size_t)nl * 2);
+                for (size_t p = 0; p < cplane; p++) {
+                    double r = dst[p].re, i = dst[p].im;
+                    r = r + (t1[p].re * xo[p].re - t1[p].im * xo[p].im); i = i + (t1[p].re * xo[p].im + t1[p].im * xo[p].re);
+                    r = r + (t2[p].re * yo[p].re - t2[p].im * yo[p].im); i = i + (t2[p].re * yo[p].im + t2[p].im * yo[p].re);
+                    r = r + (t3[p].re * zo[p].re - t3[p].im * zo[p].im); i = i + (t3[p].re * zo[p].im + t3[p].im * zo[p].re);
+                    dst[p].re = r; dst[p].im = i;
+                }
+            }
+    #pragma omp parallel for
+    for (int p = 0; p < 3 * nl; p++) ora_irfft2(ho + cplane * p, hcell + plane * p, nx, ny);
+    free(mi); free(mo); free(ho);
+}
+
+static void cross3(const double *a, const double *b, double *t)
+{
+    t[0] = a[1] * b[2] - a[2] * b[1];
+    t[1] = a[2] * b[0] - a[0] * b[2];
+    t[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* get_appl uniform branch: scale(t) * uniform_h (applied.f90:871-883) */
+static void ufield_at(const olayer *L, double t, double *u)
+{
+    if (!L->has_uf) { u[0] = u[1] = u[2] = 0.0; return; }
+    const mrmg_ufield *f = &L->uf;
+    double c = (t - f->t_start) / (f->t_end - f->t_start);
+    c = c < 1.0 ? c : 1.0; c = c > 0.0 ? c : 0.0;
+    double sc = ((f->s_end - f->s_start) * c + f->s_start) * f->scale;
+    for (int i = 0; i < 3; i++) u[i] = sc * f->dir[i];
+}
+
+/* get_uni_hk + get_cubic_hk (field.f90:366-395) */
+static void hk_uni(const olayer *L, int n, const double *m, double *h)
+{
+    h[0] = h[1] = h[2] = 0.0;
+    if (!L->uni || L->hk_scale[n] <= 0.0) return;
+    const double *k = L->uni + 3 * n;
+    double hks = dot3(m, k), yy = 1.0 - hks * hks;
+    double s = hks * (L->hk[2 * n] + L->hk[2 * n + 1] * yy) * L->hk_scale[n];
+    for (int c = 0; c < 3; c++) h[c] = s * k[c];
+}
+static void hk_cubic(const olayer *L, int n, const double *m, double *h)
+{
+    h[0] = h[1] = h[2] = 0.0;
+    if (!L->cub || L->hk_scale[n] <= 0.0) return;
+    const double *A = L->cub + 9 * n;  /* A[(r-1)+3(c-1)] = cubic_axis(r,c,n) */
+    double d1[3], d2[3], d3[3];
+    for (int c = 0; c < 3; c++) d1[c] = A[0 + 3 * c] * m[0] + A[1 + 3 * c] * m[1] + A[2 + 3 * c] * m[2];
+    for (int c = 0; c < 3; c++) d2[c] = d1[c] * d1[c];
+    double k1 = L->hk[2 * n], k2 = L->hk[2 * n + 1];
+    d3[0] = d1[0] * (k1 * (d2[1] + d2[2]) + k2 * d2[1] * d2[2]);
+    d3[1] = d1[1] * (k1 * (d2[0] + d2[2]) + k2 * d2[0] * d2[2]);
+    d3[2] = d1[2] * (k1 * (d2[0] + d2[1]) + k2 * d2[0] * d2[1]);
+    for (int r = 0; r < 3; r++)
+        h[r] = -(A[r + 0] * d3[0] + A[r + 3] * d3[1] + A[r + 6] * d3[2]) * L->hk_scale[n];
+}
+
+/* get_field (field.f90:160-273) without the thermal part; t = time of the applied field */
+static void get_field(ora_sim *s, double t)
+{
+    int nx = s->nx, ny = s->ny, nl = s->nl, in = s->in;
+    size_t plane = (size_t)nx * ny;
+    /* get_applied: per-grain average of the (static) cell field + uniform field */
+    for (int k = 0; k < nl; k++) {
+        olayer *L = &s->L[k];
+        double u[3]; ufield_at(L, t, u);
+        for (int n = 0; n < L->ng; n++)
+            for (int c = 0; c < 3; c++) L->h_app[3 * n + c] = (L->happ_s ? L->happ_s[3 * n + c] : 0.0) + u[c];
+    }
+    /* get_magnetostatic */
+    if (s->demag && s->have_tensor) {
+        ora_demag(s, s->hcell);
+        for (int k = 0; k < nl; k++) {
+            olayer *L = &s->L[k];
+            for (int n = 0; n < L->ng; n++) {
+                double y[3] = {0, 0, 0};
+                for (int q = L->g2gi[n]; q < L->g2gi[n + 1]; q++) {
+                    int i = L->g2g[2 * q] - 1, j = L->g2g[2 * q + 1] - 1;
+                    for (int c = 0; c < 3; c++) y[c] = y[c] + s->hcell[(size_t)i + (size_t)nx * j + plane * (k + (size_t)nl * c)];
+                }
+                int cnt = L->g2gi[n + 1] - L->g2gi[n];
+                for (int c = 0; c < 3; c++) L->h_demag[3 * n + c] = y[c] / cnt;
+            }
+        }
+    }
+    for (int k = 0; k < nl; k++) {
+        olayer *L = &s->L[k];
+        for (int n = 0; n < L->ng; n++) {
+            const double *m = L->m + 6 * n + in;
+            double hc[3], hu[3], hxv[3] = {0, 0, 0};
+            hk_cubic(L, n, m, hc);
+            hk_uni(L, n, m, hu);
+            if (L->hxi) {
+                for (int j = L->hxi[n]; j < L->hxi[n + 1]; j++) {
+                    int nb = L->hxg[j] - 1;
+                    for (int c = 0; c < 3; c++) hxv[c] = hxv[c] + L->hx[j] * L->ms_scale[nb] * L->m[6 * nb + in + c];
+                }
+                for (int c = 0; c < 3; c++)
+                    L->h[3 * n + c] = L->h_app[3 * n + c] + L->h_demag[3 * n + c] + hc[c] + hu[c] + hxv[c];
+            } else {
+                for (int c = 0; c < 3; c++)
+                    L->h[3 * n + c] = L->h_app[3 * n + c] + L->h_demag[3 * n + c] + hc[c] + hu[c];
+            }
+        }
+        if (L->uhxi && k > 0) {  /* get_interlayer_hx with q = layer above */
+            olayer *Q = &s->L[k - 1];
+            for (int n = 0; n < L->ng; n++) {
+                double h[3] = {0, 0, 0};
+                for (int j = L->uhxi[n]; j < L->uhxi[n + 1]; j++) {
+                    int nb = L->uhxg[j] - 1;
+                    for (int c = 0; c < 3; c++) h[c] = h[c] + L->uhx[j] * Q->ms_scale[nb] * Q->m[6 * nb + in + c];
+                }
+                for (int c = 0; c < 3; c++) L->h[3 * n + c] = L->h[3 * n + c] + h[c];
+            }
+        }
+        if (L->lhxi && k < nl - 1) {
+            olayer *Q = &s->L[k + 1];
+            for (int n = 0; n < L->ng; n++) {
+                double h[3] = {0, 0, 0};
+                for (int j = L->lhxi[n]; j < L->lhxi[n + 1]; j++) {
+                    int nb = L->lhxg[j] - 1;
+                    for (int c = 0; c < 3; c++) h[c] = h[c] + L->lhx[j] * Q->ms_scale[nb] * Q->m[6 * nb + in + c];
+                }
+                for (int c = 0; c < 3; c++) L->h[3 * n + c] = L->h[3 * n + c] + h[c];
+            }
+        }
+    }
+}
+
+/* get_thermal (field.f90:397-436), LLG branch; normals = (3,G) for this step */
+static void thermal(ora_sim *s, const double *normals)
+{
+    int goff = 0;
+    for (int k = 0; k < s->nl; k++) {
+        olayer *L = &s->L[k];
+        if (L->thermal && normals) {
+            double tsigma = sqrt(2.0 * 1.380662e-16 * L->thm_alpha / L->gamma);
+            for (int n = 0; n < L->ng; n++) {
+                double x, magv = L->mag[2 * n + s->in / 3];
+                if (magv != 1.0) {
+                    double am = fabs(magv);
+                    x = L->alpha_scale[n] * L->temp[n] / (L->ms[n] * L->vol[n] * (L->floor_ms > am ? L->floor_ms : am));
+                } else if (L->ms_scale[n] > 0.0) {
+                    x = L->alpha_scale[n] * L->temp[n] / (L->ms[n] * L->vol[n] * (L->floor_ms > L->ms_scale[n] ? L->floor_ms : L->ms_scale[n]));
+                } else {
+                    x = 0.0;
+                }
+                x = sqrt(x) * tsigma;
+                for (int c = 0; c < 3; c++) L->hthm[3 * n + c] = x * normals[3 * (goff + n) + c];
+            }
+        }
+        goff += L->ng;
+    }
+}
+
+/* rotate_q and rotate_2, transform.f90:65-74, 97-115 */
+static void rotate_q(const double *w, const double *m, double *t)
+{
+    double w1 = w[0] * w[0], w2 = w[1] * w[1], w3 = w[2] * w[2], w4 = w[3] * w[3];
+    t[0] = 2.0 * (m[2] * (w[0] * w[2] + w[1] * w[3]) + m[1] * (w[1] * w[2] - w[0] * w[3])) + m[0] * (w1 + w2 - w3 - w4);
+    t[1] = 2.0 * (m[0] * (w[0] * w[3] + w[1] * w[2]) + m[2] * (w[2] * w[3] - w[0] * w[1])) + m[1] * (w1 + w3 - w2 - w4);
+    t[2] = 2.0 * (m[1] * (w[0] * w[1] + w[2] * w[3]) + m[0] * (w[1] * w[3] - w[0] * w[2])) + m[2] * (w1 + w4 - w3 - w2);
+}
+static void rotate_2(double theta0, const double *u0, double theta1, const double *u1, const double *m, double *t)
+{
+    double st1 = 0.5 * theta1, st0 = 0.5 * theta0, t0[3], t1[3], ct0, ct1;
+    t0[0] = u0[0]; t0[1] = u0[1]; t0[2] = u0[2];
+    ct0 = sqrt(dot3(t0, t0));
+    if (ct0 != 0.0) { t0[0] /= ct0; t0[1] /= ct0; t0[2] /= ct0; }
+    else { t0[0] = 0; t0[1] = 0; t0[2] = 1; st0 = 0.0; }
+    t1[0] = u1[0]; t1[1] = u1[1]; t1[2] = u1[2];
+    if (st0 == 0.0 && st0 == st1) { t[0] = m[0]; t[1] = m[1]; t[2] = m[2]; return; }
+    ct1 = cos(st1); st1 = sin(st1);
+    ct0 = cos(st0); st0 = sin(st0);
+    double w[4];
+    w[0] = ct0 * ct1 - dot3(t0, t1) * st0 * st1;
+    w[1] = t0[0] * ct1 * st0 + (t1[0] * ct0 + (t1[1] * t0[2] - t0[1] * t1[2]) * st0) * st1;
+    w[2] = t0[1] * ct1 * st0 + (t1[1] * ct0 + (t1[2] * t0[0] - t0[2] * t1[0]) * st0) * st1;
+    w[3] = t0[2] * ct1 * st0 + (t1[2] * ct0 + (t1[0] * t0[1] - t0[0] * t1[1]) * st0) * st1;
+    rotate_q(w, m, t);
+}
+
+/* push_llg (llg.f90:1066-1155) for one layer; returns errmx_l, errmx1_l, mmag1 */
+static void push_llg(ora_sim *s, int k, int type, int iter, double dt, double dt0,
+                     double *errmx_l, double *errmx1_l, double *mmag1)
+{
+    olayer *L = &s->L[k];
+    int id = s->in, od = 3 - s->in;
+    double scale1 = -0.5 * L->gamma * dt;
+    for (int n = 0; n < L->ng; n++) {
+        double *mi = L->m + 6 * n + id, *mo = L->m + 6 * n + od;
+        double h[3], m[3], t[3], d[3], y;
+        for (int c = 0; c < 3; c++) h[c] = L->hthm[3 * n + c] / sqrt(dt0) + L->h[3 * n + c];
+        if (L->ms_scale[n] > 0.0) {
+            for (int c = 0; c < 3; c++) m[c] = mi[c];
+            if (abs(type) < 3) {
+                cross3(m, h, t); y = dot3(t, t);
+                if (y > *errmx_l * L->ms[n] * L->ms[n]) *errmx_l = y / (L->ms[n] * L->ms[n]);
+                double scale2 = L->alpha_scale[n] * L->alpha;
+                if (abs(type) != 2) {
+                    y = dot3(h, h);
+                    if (y > 0.0) {
+                        double o[3], u0[3], r[3];
+                        y = sqrt(y);
+                        double scale11 = 2.0 * scale1 * y / (1.0 + scale2 * scale2);
+                        for (int c = 0; c < 3; c++) o[c] = h[c] / y;
+                        double dp = dot3(o, m); dp = dp < 1.0 ? dp : 1.0; dp = dp > -1.0 ? dp : -1.0;
+                        double th0 = acos(dp);
+                        cross3(m, o, u0);
+                        rotate_2(th0 - 2.0 * atan(exp(scale11 * scale2) * tan(0.5 * th0)), u0, fabs(scale11), o, m, r);
+                        for (int c = 0; c < 3; c++) d[c] = r[c] - m[c];
+                    } else {
+                        d[0] = d[1] = d[2] = 0.0;
+                    }
+                } else {
+                    double o[3], tt[3];
+                    for (int c = 0; c < 3; c++) o[c] = scale1 * h[c] - scale2 * m[c];
+                    cross3(m, o, tt);
+                    for (int c = 0; c < 3; c++) tt[c] = m[c] + tt[c];
+                    y = 2.0 / (1.0 + dot3(o, o));
+                    double c3[3]; cross3(tt, o, c3);
+                    for (int c = 0; c < 3; c++) d[c] = y * c3[c];
+                }
+                double dd = dot3(d, d); if (dd > *errmx1_l) *errmx1_l = dd;
+                for (int c = 0; c < 3; c++) mo[c] = m[c] + d[c];
+            } else {  /* Heun */
+                double scale2 = L->alpha_scale[n] * L->alpha;
+                double scale11 = 2.0 * scale1 / (1.0 + scale2 * scale2);
+                if (iter == 1) for (int c = 0; c < 3; c++) m[c] = mo[c];
+                double mt[3];
+                cross3(m, h, t); cross3(m, t, mt);
+                for (int c = 0; c < 3; c++) d[c] = scale11 * (t[c] + scale2 * mt[c]);
+                if (iter == 1) for (int c = 0; c < 3; c++) d[c] = 0.5 * (d[c] + L->dm[3 * n + c]);
+                else for (int c = 0; c < 3; c++) L->dm[3 * n + c] = d[c];
+                y = dot3(t, t);
+                if (y > *errmx_l * L->ms[n] * L->ms[n]) *errmx_l = y / (L->ms[n] * L->ms[n]);
+                double dd = dot3(d, d); if (dd > *errmx1_l) *errmx1_l = dd;
+                for (int c = 0; c < 3; c++) mo[c] = mi[c] + d[c];
+                y = dot3(mo, mo);
+                if (y > 0.0) { double r = sqrt(y); for (int c = 0; c < 3; c++) mo[c] = mo[c] / r; }
+            }
+        } else {
+            double hh[3] = {L->h[3 * n], L->h[3 * n + 1], L->h[3 * n + 2]};
+            y = sqrt(dot3(hh, hh));
+            if (y > 0.0) for (int c = 0; c < 3; c++) mo[c] = hh[c] / y;
+            else for (int c = 0; c < 3; c++) mo[c] = mi[c];
+        }
+        *mmag1 = *mmag1 + sqrt(dot3(mo, mo)) * L->ms_scale[n] * L->vol[n];
+    }
+}
+
+/* push_m, llg.f90:1195-1271 */
+static void push_m(ora_sim *s, double t0, int type, int fixed)
+{
+    int too_far = 1;
+    while (too_far) {
+        double dt = s->dt * s->max_dmdt;
+        double dt_s = 1.0; int iter = 1;
+        if (type < 0) { iter = 2; if (type > -3) dt_s = 0.5; }
+        while (iter > 0) {
+            s->errmx = 0.0; s->errmx1 = 0.0; s->mmag = 0.0;
+            too_far = 0; s->max_dmdtnext = 1.e22;
+            for (int k = 0; k < s->nl; k++) { s->L[k].errl = 0; s->L[k].err1l = 0; }
+            for (int k = 0; k < s->nl; k++) {
+                olayer *L = &s->L[k];
+                double errmx_l = 0.0, errmx1_l = 0.0, mmag1 = 0.0, rot_ratio;
+                push_llg(s, k, type, iter, dt * dt_s, dt, &errmx_l, &errmx1_l, &mmag1);
+                if (iter == 1) {
+                    if (fixed) { rot_ratio = 1.0; too_far = 0; }
+                    else { rot_ratio = acos(1.0 - 0.5 * errmx1_l) / L->max_dm; too_far = rot_ratio > 1.0; }
+                    if (too_far) { s->max_dmdt = 0.9 * s->max_dmdt / rot_ratio; break; }
+                    double sv = 0.0; for (int n = 0; n < L->ng; n++) sv += L->vol[n];
+                    s->mmag = s->mmag + mmag1 / sv;
+                    if (rot_ratio < 2.0 / 3.0) {
+                        double c = s->max_dmdt / (rot_ratio + 1.e-200); c = c < 2.0 ? c : 2.0;
+                        s->max_dmdtnext = s->max_dmdtnext < c ? s->max_dmdtnext : c;
+                    } else {
+                        s->max_dmdtnext = s->max_dmdtnext < s->max_dmdt ? s->max_dmdtnext : s->max_dmdt;
+                    }
+                    double e = sqrt(errmx_l);
+                    s->errmx = s->errmx > e ? s->errmx : e;
+                    s->errmx1 = s->errmx1 > errmx1_l ? s->errmx1 : errmx1_l;
+                }
+            }
+            if (too_far) break;
+            iter = iter - 1;
+            if (iter > 0) {
+                s->in = 3 - s->in;
+                get_field(s, t0 + dt * dt_s);
+                dt_s = 1.0;
+                s->in = 3 - s->in;
+            }
+        }
+    }
+    s->in = 3 - s->in;
+    if (s->max_dmdtnext > 1.e20) s->max_dmdtnext = s->max_dmdt;
+}
+
+/* Per-layer volume-weighted mean of m (llg.f90:217-237) */
+static void mave_layers(ora_sim *s, mrmg_diag *d)
+{
+    for (int k = 0; k < s->nl && k < 8; k++) {
+        olayer *L = &s->L[k];
+        double ma[3] = {0, 0, 0}, x1 = 0.0;
+        for (int n = 0; n < L->ng; n++) {
+            double x3 = L->vol[n] / (x1 + L->vol[n]);
+            x1 = x1 + L->vol[n];
+            for (int c = 0; c < 3; c++) ma[c] = ma[c] + x3 * (L->m[6 * n + s->in + c] - ma[c]);
+        }
+        for (int c = 0; c < 3; c++) d->mave[k][c] = ma[c];
+    }
+}
+
+/* The llg() loop body (llg.f90:212-285) for p->n_steps steps.  Fixed dt when
+ * dt_frac == 0 (set_llg, llg.f90:1484-1486); otherwise get_fields' dt control and
+ * push_m's rejection/next-dt logic.  noise: (3,G,n_steps) standard normals or NULL. */
+int ora_step_ex(ora_sim *s, const mrmg_step_params *p, double dt_frac, double dt_max,
+                const double *noise, mrmg_diag *diag)
+{
+    int G = 0; for (int k = 0; k < s->nl; k++) G += s->L[k].ng;
+    int fixed = (dt_frac == 0.0);
+    double t = p->t0;
+    /* llg() start: init_applied zeroes hthm (field.f90:703); max_dmdt = 1 (llg.f90:206) */
+    for (int k = 0; k < s->nl; k++) memset(s->L[k].hthm, 0, sizeof(double) * 3 * s->L[k].ng);
+    s->dt = p->dt; s->max_dmdt = 1.0;
+    if (fixed) s->max_dmdtnext = 1.0;
+    for (int it = 0; it < p->n_steps; it++) {
+        /* get_fields(t), llg.f90:320-388 */
+        get_field(s, t);
+        if (p->noise != MRMG_NOISE_NONE && noise) thermal(s, noise + (size_t)3 * G * it);
+        double dti = 1.0 / sqrt(s->dt * s->max_dmdt);
+        if (dt_frac != 0.0) s->dt = 1.e22;
+        for (int k = 0; k < s->nl; k++) {
+            olayer *L = &s->L[k];
+            double hmag = 0.0;
+            for (int n = 0; n < L->ng; n++) {
+                double v[3];
+                for (int c = 0; c < 3; c++) v[c] = L->h[3 * n + c] + L->hthm[3 * n + c] * dti;
+                double q = dot3(v, v); hmag = hmag > q ? hmag : q;
+            }
+            hmag = sqrt(hmag);
+            double fact1 = (1.0 + L->alpha * L->alpha) / ((1.0 + L->alpha) * L->gamma) * dt_frac;
+            double fact0 = fact1 != 0.0 ? 0.0 : 1.0;
+            double cand = fact1 / (hmag > 1.0 ? hmag : 1.0) + fact0 * s->dt;
+            double nd = s->dt < dt_max ? s->dt : dt_max;
+            s->dt = nd < cand ? nd : cand;
+        }
+        push_m(s, t, p->type, fixed);
+        s->dt = s->dt * s->max_dmdt; s->max_dmdt = s->max_dmdtnext;
+        t = t + s->dt;
+    }
+    if (diag) {
+        memset(diag, 0, sizeof(*diag));
+        diag->t = t; diag->dt = s->dt; diag->steps = p->n_steps; diag->n_lay = s->nl;
+        diag->errmx = s->errmx; diag->errmx1 = s->errmx1; diag->mmag = s->mmag;
+        mave_layers(s, diag);
+    }
+    return MRMG_OK;
+}
+
+int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_diag *diag)
+{
+    return ora_step_ex(s, p, 0.0, 1.e22, noise, diag);
+}
+
+/* Demag field of the current state (cells and grain averages) */
+int ora_demag_field(ora_sim *s, double *h_cell, double *h_grain)
+{
+    int nx = s->nx, ny = s->ny, nl = s->nl;
+    size_t plane = (size_t)nx * ny;
+    double *hc = (double *)calloc(plane * nl * 3, sizeof(double));
+    if (s->demag && s->have_tensor) ora_demag(s, hc);
+    if (h_cell) memcpy(h_cell, hc, sizeof(double) * plane * nl * 3);
+    if (h_grain) {
+        int goff = 0;
+        for (int k = 0; k < nl; k++) {
+            olayer *L = &s->L[k];
+            for (int n = 0; n < L->ng; n++) {
+                double y[3] = {0, 0, 0};
+                for (int q = L->g2gi[n]; q < L->g2gi[n + 1]; q++) {
+                    int i = L->g2g[2 * q] - 1, j = L->g2g[2 * q + 1] - 1;
+                    for (int c = 0; c < 3; c++) y[c] = y[c] + hc[(size_t)i + (size_t)nx * j + plane * (k + (size_t)nl * c)];
+                }
+                int cnt = L->g2gi[n + 1] - L->g2gi[n];
+                for (int c = 0; c < 3; c++) h_grain[3 * (goff + n) + c] = y[c] / cnt;
+            }
+            goff += L->ng;
+        }
+    }
+    free(hc);
+    return MRMG_OK;
+}
+
+int ora_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

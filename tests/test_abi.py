@@ -1,0 +1,87 @@
+"""The C-ABI library: it loads, exports every entry point of include/mrmg.h, and the
+ctypes structures match the header's layouts as the host C compiler sees them."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from mrm_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mrmg.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mrmg_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = header_functions()
+    for n in ("mrmg_create", "mrmg_destroy", "mrmg_set_tensor", "mrmg_build_tensor", "mrmg_set_layer",
+              "mrmg_set_state", "mrmg_get_state", "mrmg_step", "mrmg_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_header_symbol():
+    if not os.path.exists(abi.LIB_PATH):
+        pytest.skip("libmrmg.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    missing = [n for n in header_functions() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_without_gpu():
+    """Loading needs no device; the error path of create reports instead of crashing."""
+    if not os.path.exists(abi.LIB_PATH):
+        pytest.skip("libmrmg.so not built")
+    lib = abi.load()
+    assert lib.mrmg_last_error() is not None
+    assert lib.mrmg_create(None, None) == 1   # MRMG_ERR_ARG, no device call made
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "mrmg.h"
+#define F(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("mrmg_geom %zu\nmrmg_layer %zu\nmrmg_ufield %zu\nmrmg_step_params %zu\nmrmg_diag %zu\n",
+         sizeof(mrmg_geom), sizeof(mrmg_layer), sizeof(mrmg_ufield), sizeof(mrmg_step_params), sizeof(mrmg_diag));
+  F(mrmg_geom, dx) F(mrmg_geom, dz) F(mrmg_geom, il_space)
+  F(mrmg_layer, grid2grain) F(mrmg_layer, hx) F(mrmg_layer, lhx) F(mrmg_layer, h_app) F(mrmg_layer, alpha)
+  F(mrmg_layer, floor_ms) F(mrmg_layer, thermal)
+  F(mrmg_ufield, t_start) F(mrmg_ufield, scale)
+  F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
+  F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave)
+  return 0;
+}
+"""
+
+
+def test_ctypes_layout_matches_header():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(src, "w").write(PROBE)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, src], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    want = {}
+    for line in out.splitlines():
+        k, v = line.split()
+        want[k] = int(v)
+    assert C.sizeof(abi.Geom) == want["mrmg_geom"]
+    assert C.sizeof(abi.Layer) == want["mrmg_layer"]
+    assert C.sizeof(abi.UField) == want["mrmg_ufield"]
+    assert C.sizeof(abi.StepParams) == want["mrmg_step_params"]
+    assert C.sizeof(abi.Diag) == want["mrmg_diag"]
+    types = {"mrmg_geom": abi.Geom, "mrmg_layer": abi.Layer, "mrmg_ufield": abi.UField,
+             "mrmg_step_params": abi.StepParams, "mrmg_diag": abi.Diag}
+    for k, v in want.items():
+        if "." in k:
+            t, f = k.split(".")
+            assert getattr(types[t], f).offset == v, k

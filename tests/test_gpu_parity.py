@@ -1,0 +1,289 @@
+"""GPU parity: the HIP path (through the C ABI of libmrmg.so) against the oracle on the
+same inputs.  FP64 tolerances are written per test; the precision form is held to
+max |dm| <= 1e-4 against the FP64 GPU path."""
+import sys
+
+import numpy as np
+import pytest
+
+from mrm_amd import abi, media
+from mrm_amd.llg import Llg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    import ctypes as C
+    lib = abi.load()
+    n = C.c_int(0)
+    if lib.mrmg_device_count(C.byref(n)) != 0 or n.value < 1:
+        pytest.fail("no HIP device visible: GPU tests need the MI355X")
+
+
+def pair(O, med, tensor=None, demag=True, precision=abi.MRMG_FP64, ufields=None, il=None):
+    """An oracle Sim and a GPU Llg over the same media."""
+    g = abi.make_geom(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, il, abi.MRMG_FP64, demag)
+    s = O.Sim(g)
+    gpu = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, il, precision=precision, demag=demag)
+    if demag:
+        if tensor is None:
+            tensor = O.calculate_tensor(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, il_space=il, num_images=2)
+        s.set_tensor(tensor)
+        gpu.set_tensor(tensor)
+    for k, L in enumerate(med.layers):
+        s.set_layer(k, L)
+        gpu.set_layer(k, L)
+    for k, u in enumerate(ufields or []):
+        s.set_ufield(k, u)
+        gpu.set_ufield(k, u)
+    return s, gpu
+
+
+def random_state(med, seed, tilt=None):
+    rng = np.random.default_rng(seed)
+    out = []
+    for L in med.layers:
+        if tilt is None:
+            m = rng.standard_normal((L.ng, 3))
+        else:
+            m = np.tile([0.0, 0.0, 1.0], (L.ng, 1)) + tilt * rng.standard_normal((L.ng, 3))
+        out.append(m / np.linalg.norm(m, axis=1, keepdims=True))
+    return out
+
+
+def max_dm(s, gpu, nl):
+    d = 0.0
+    for k in range(nl):
+        d = max(d, np.abs(s.get_state(k)[0] - gpu.get_state(k)[0]).max())
+    return d
+
+
+def test_device_and_footprint(oracle_mod):
+    med = media.config_media("cfg0")
+    s, gpu = pair(oracle_mod, med)
+    assert gpu.footprint() > 0
+
+
+@pytest.mark.parametrize("nx,ny,nl", [(64, 64, 1), (48, 45, 3), (40, 30, 2), (20, 27, 1)])
+def test_demag_field_parity(oracle_mod, nx, ny, nl):
+    """Scatter + r2c + tensor product + c2r + grain average: FP64, |dH| <= 1e-12 max|H|."""
+    say(f"demag parity {nx}x{ny}x{nl}")
+    med = media.make_media(nx, ny, dz=[2e-7 + 1e-7 * k for k in range(nl)], n_lay=nl, pitch=6e-7, seed=nx + ny)
+    il = [0.0] + [5e-8] * (nl - 1)
+    s, gpu = pair(oracle_mod, med, il=il)
+    for k, m in enumerate(random_state(med, 3)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    hc_o, hg_o = s.demag_field()
+    hc_g, hg_g = gpu.demag_field()
+    scale = np.abs(hc_o).max()
+    assert np.abs(hc_g - hc_o).max() <= 1e-12 * scale
+    assert np.abs(hg_g - hg_o).max() <= 1e-12 * scale
+
+
+def test_tensor_build_parity(oracle_mod):
+    """FP64 Newell tensor built on the GPU against the oracle.  The 64-term sums cancel
+    catastrophically, so one-ulp differences between the device and host log/atan grow
+    to ~1e-9 of the largest element in the far field: near elements (>= 1 % of the max)
+    agree to 1e-10 relative, all elements to 1e-8 of the max."""
+    say("tensor build parity 32x24x2")
+    nx, ny, nl = 32, 24, 2
+    dz = [2e-7, 3e-7]
+    gpu = Llg(nx, ny, nl, 2e-7, 2e-7, dz, il_space=[0.0, 1e-7])
+    got = gpu.build_tensor(num_images=2, return_real=True)
+    ref = oracle_mod.calculate_tensor(nx, ny, nl, 2e-7, 2e-7, dz, il_space=[0.0, 1e-7], num_images=2)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-8 * scale
+    near = np.abs(ref) >= 0.01 * scale
+    assert (np.abs(got - ref)[near] / np.abs(ref)[near]).max() <= 1e-10
+    # the symmetric components are bit-identical on the device as well
+    T = got.reshape(nl * nl, 9, ny * nx)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert np.array_equal(T[:, 3 * a + b], T[:, 3 * b + a])
+
+
+def test_heun_step_parity_cfg0(oracle_mod):
+    """Configuration 0 media, demag on, ramped tilted field, T = 0: Heun FP64 matches the
+    oracle to 1e-10 after one step and after 100."""
+    say("heun parity cfg0")
+    med = media.config_media("cfg0")
+    s, gpu = pair(oracle_mod, med, ufields=media.ramp_field(1))
+    for k, m in enumerate(random_state(med, 5, tilt=0.05)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    dt = 5e-14
+    s.step(1, dt, t0=0.0)
+    gpu.set_llg(dt=dt)
+    gpu.step(1, t0=0.0)
+    assert max_dm(s, gpu, 1) <= 1e-10
+    do = s.step(99, dt, t0=dt)
+    dg = gpu.step(99, t0=dt)
+    assert max_dm(s, gpu, 1) <= 1e-10
+    assert do.t == dg.t
+    assert abs(do.errmx - dg.errmx) <= 1e-8 * do.errmx
+    assert abs(do.mave[0][2] - dg.mave[0][2]) < 1e-10
+    # the stored field is the stage-2 field of the last step
+    ho = s.get_state(0)[1]
+    hg = gpu.get_state(0)[1]
+    assert np.abs(ho - hg).max() <= 1e-9 * np.abs(ho).max()
+
+
+@pytest.mark.parametrize("ltype", [2, -2, 1, -1])
+def test_rotational_push_parity(oracle_mod, ltype):
+    say(f"rotational push parity type {ltype}")
+    med = media.config_media("cfg0", nx=32, ny=32)
+    s, gpu = pair(oracle_mod, med, ufields=media.ramp_field(1))
+    for k, m in enumerate(random_state(med, 6, tilt=0.3)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    s.step(30, 5e-14, type=ltype)
+    gpu.set_llg(dt=5e-14, llg_type=ltype)
+    gpu.step(30)
+    assert max_dm(s, gpu, 1) <= 1e-10
+
+
+def test_thermal_injected_parity_multilayer(oracle_mod):
+    """Four sub-layers (two physical layers, interlayer CSR), T = 300 K, the reference's MT
+    streams (one per layer, seed 4357 + thermal_seed) injected into both paths."""
+    say("thermal multilayer parity 64x64x4")
+    med = media.config_media("cfg1", nx=64, ny=64, thm_alpha=1.0)
+    s, gpu = pair(oracle_mod, med, ufields=media.ramp_field(4))
+    n = 20
+    noise = oracle_mod.mt_noise([34, 35, 36, 37], med.ngs, [True] * 4, n)
+    s.step(n, 5e-14, noise=noise)
+    gpu.set_llg(dt=5e-14)
+    gpu.step(n, noise=noise)
+    assert max_dm(s, gpu, 4) <= 1e-10
+
+
+def test_anisotropy_variants_parity(oracle_mod):
+    """Cubic anisotropy, a static per-grain applied field, ms_scale = 0 and hk_scale <= 0
+    grains, no intralayer exchange in one layer."""
+    say("anisotropy variants parity")
+    med = media.make_media(32, 32, n_lay=2, dz=4e-7, pitch=6e-7, seed=8)
+    rng = np.random.default_rng(1)
+    for k, L in enumerate(med.layers):
+        ng = L.ng
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        cub = np.tile(q.T.reshape(1, 3, 3), (ng, 1, 1))   # [n][c][r] = axis c component r
+        mss = np.ones(ng)
+        mss[::17] = 0.0
+        hks = np.ones(ng)
+        hks[::13] = 0.0
+        kw = dict(ng=ng, grid2grain=L.grid2grain, grain2gridindex=L.grain2gridindex, grain2grid=L.grain2grid,
+                  grain_vol=L.grain_vol, ms=L.ms, hk=np.tile([4000.0, 1500.0], (ng, 1)), cubic_axis=cub,
+                  uni_axis=L.uni_axis if k == 0 else None, ms_scale=mss, hk_scale=hks,
+                  h_app=rng.standard_normal((ng, 3)) * 300.0, alpha=0.05)
+        if k == 0:
+            kw.update(hxgrainindex=L.hxgrainindex, hxgrain=L.hxgrain, hx=L.hx,
+                      lhxgrainindex=L.lhxgrainindex, lhxgrain=L.lhxgrain, lhx=L.lhx)
+        else:
+            kw.update(uhxgrainindex=L.uhxgrainindex, uhxgrain=L.uhxgrain, uhx=L.uhx)
+        med.layers[k] = abi.LayerArrays(**kw)
+    s, gpu = pair(oracle_mod, med, ufields=media.ramp_field(2))
+    for k, m in enumerate(random_state(med, 2)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    s.step(25, 5e-14)
+    gpu.set_llg(dt=5e-14)
+    gpu.step(25)
+    assert max_dm(s, gpu, 2) <= 1e-10
+
+
+def test_philox_noise_is_what_the_step_uses(oracle_mod):
+    """Running with on-device Philox equals injecting the normals mrmg_philox_normals
+    dumps (bit for bit), and those normals match the oracle fed with them."""
+    say("philox equivalence")
+    med = media.config_media("cfg1", nx=32, ny=32)
+    a = Llg(32, 32, 4, med.dx, med.dy, med.dz, demag=False)
+    b = Llg(32, 32, 4, med.dx, med.dy, med.dz, demag=False)
+    for x in (a, b):
+        x.set_media(med)
+        x.set_llg(dt=5e-14)
+    a.step(10, philox_seed=1234)
+    nz = b.philox_normals(1234, 0, 10)
+    b.step(10, noise=nz)
+    for k in range(4):
+        assert np.array_equal(a.get_state(k)[0], b.get_state(k)[0])
+    flat = nz.ravel()
+    assert abs(flat.mean()) < 0.02 and abs(flat.var() - 1) < 0.02
+
+
+def test_philox_normals_match_numpy_restatement():
+    from tests.philox_np import philox_normals
+    say("philox vs numpy")
+    med = media.config_media("cfg0", nx=16, ny=16)
+    a = Llg(16, 16, 1, med.dx, med.dy, med.dz, demag=False)
+    a.set_media(med)
+    got = a.philox_normals(99, 7, 3)
+    ref = philox_normals(99, 7, 3, sum(med.ngs))
+    assert np.abs(got - ref).max() < 1e-12
+
+
+def test_precision_form_vs_fp64():
+    """FP32 precision form against the FP64 GPU path after 2000 steps: max |dm| <= 1e-4."""
+    say("fp32 vs fp64 cfg0 2000 steps")
+    med = media.config_media("cfg0")
+    out = []
+    for prec in (abi.MRMG_FP64, abi.MRMG_FP32):
+        x = Llg(64, 64, 1, med.dx, med.dy, med.dz, precision=prec)
+        x.build_tensor(num_images=2)
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(1)):
+            x.set_ufield(k, u)
+        x.set_llg(dt=5e-14)
+        x.step(2000)
+        out.append(x.get_state(0)[0])
+    d = np.abs(out[0] - out[1])
+    say("fp32 max|dm|", d.max(), "rms", np.sqrt((d ** 2).mean()))
+    assert d.max() <= 1e-4
+    assert np.abs(np.linalg.norm(out[1], axis=1) - 1).max() < 1e-5
+
+
+def test_cfg1_full_size_parity(oracle_mod):
+    """Configuration 1 at full size (256x256x4), GPU-built tensor handed to both paths,
+    reference MT noise injected, 3 Heun steps at T = 300 K."""
+    say("cfg1 full size parity")
+    med = media.config_media("cfg1", thm_alpha=1.0)
+    gpu = Llg(256, 256, 4, med.dx, med.dy, med.dz)
+    N = gpu.build_tensor(num_images=2, return_real=True)
+    s, g2 = pair(oracle_mod, med, tensor=N, ufields=media.ramp_field(4))
+    noise = oracle_mod.mt_noise([34, 35, 36, 37], med.ngs, [True] * 4, 3)
+    s.step(3, 5e-14, noise=noise)
+    g2.set_llg(dt=5e-14)
+    g2.step(3, noise=noise)
+    assert max_dm(s, g2, 4) <= 1e-10
+
+
+def test_state_survives_layer_rebuild(oracle_mod):
+    """set_layer after steps keeps the other layers' device state (the host copy is synced)."""
+    med = media.config_media("cfg1", nx=32, ny=32)
+    x = Llg(32, 32, 4, med.dx, med.dy, med.dz, demag=False)
+    x.set_media(med)
+    x.set_llg(dt=5e-14)
+    x.step(5, philox_seed=3)
+    before = [x.get_state(k)[0] for k in range(4)]
+    x.set_layer(2, med.layers[2])
+    after = [x.get_state(k)[0] for k in range(4)]
+    for k in (0, 1, 3):
+        assert np.array_equal(before[k], after[k])
+    assert np.allclose(after[2], [0, 0, 1])
+
+
+def test_step_rejects_bad_arguments():
+    med = media.config_media("cfg0", nx=16, ny=16)
+    x = Llg(16, 16, 1, med.dx, med.dy, med.dz)
+    x.set_media(med)
+    with pytest.raises(abi.MrmgError):
+        x.step(1)          # demag on, no tensor
+    y = Llg(16, 16, 1, med.dx, med.dy, med.dz, demag=False)
+    y.set_media(med)
+    y.set_llg(dt=5e-14, llg_type=7)
+    with pytest.raises(abi.MrmgError):
+        y.step(1)
