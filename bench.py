@@ -25,6 +25,7 @@ import numpy as np  # noqa: E402
 
 from mrm_amd import abi, media  # noqa: E402
 from mrm_amd.llg import Llg  # noqa: E402
+from mrm_amd.tune import autotune  # noqa: E402
 
 METRIC = "LLG cell-updates/s FP32 vs FP64 at max|Δm|≤1e-4; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
@@ -118,6 +119,8 @@ def main():
     med = media.config_media("cfg1")
     log(f"workload cfg1: {med.nx}x{med.ny}x{med.n_lay} cells, {med.G} grains")
     x64, N = build_ctx(med, abi.MRMG_FP64)
+    shapes64 = autotune(x64)
+    log(f"fp64 launch shapes: {shapes64}")
     x64.step(a.warmup, philox_seed=SEED)
     t0 = a.warmup * DT
     el64, _ = timed(x64, a.steps, t0)
@@ -156,12 +159,15 @@ def main():
            "config": {"workload": "cfg1: 256x256x4 cells of 2 nm, two coupled 4 nm layers, T=300 K, "
                                   "Heun dt=50 fs, periodic FFT demag, FP64 plain form",
                       "grid": [med.nx, med.ny, med.n_lay], "cells": med.cells, "grains": med.G,
+                      "launch_shapes": shapes64,
                       "grain_updates_per_s": med.G * a.steps / el64},
            "roofline": roof,
            "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}}
 
     if not a.no_fp32:
         x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
+        shapes32 = autotune(x32)
+        log(f"fp32 launch shapes: {shapes32}")
         x32.step(a.warmup, philox_seed=SEED)
         el32, _ = timed(x32, a.steps, t0)
         v32 = med.cells * a.steps / el32
@@ -178,7 +184,7 @@ def main():
             f"max|dm| {dmax:.2e} rms {drms:.2e} after {a.warmup + a.steps} steps")
         out["fp32_form"] = {"value": v32, "unit": "cell-updates/s", "ms_per_step": el32 * 1e3 / a.steps,
                             "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64": dmax, "rms_dm_vs_fp64": drms,
-                            "steps_compared": a.warmup + a.steps}
+                            "steps_compared": a.warmup + a.steps, "launch_shapes": shapes32}
 
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(med, N)

@@ -186,6 +186,13 @@ typedef struct {
 /* enable != 0 brackets every kernel launch of mrmg_step with HIP events on the
  * context's stream and accumulates per-class durations (resets the counters). */
 int mrmg_profile(mrmg_ctx *ctx, int enable);
+/* Launch shapes of the kernels, chosen by the caller or by mrm_amd's autotuner (the role
+ * FFTW_EXHAUSTIVE planning plays for the reference, fftw.f90:25): "row_threads" (64, 128,
+ * 256), "col_threads" (256, 512), "col_groups" (observation-layer groups per column
+ * workgroup, 1..n_lay), "lanes_per_grain" (4, 8, 16).  Results do not depend on them
+ * except "lanes_per_grain", which changes the order of the gather sums (last bits). */
+int mrmg_tune(mrmg_ctx *ctx, const char *key, int value);
+int mrmg_get_tune(mrmg_ctx *ctx, const char *key, int *value);
 /* Copies up to max entries; *n receives the number of kernel classes. */
 int mrmg_kernel_stats(mrmg_ctx *ctx, mrmg_kstat *out, int max, int *n);
 /* Device-to-device copy of `bytes` bytes on the context's stream with the library's own

@@ -2,16 +2,24 @@
 //
 // Data sit in LDS as separate real and imaginary arrays (8-byte lanes for FP64, 4-byte
 // for FP32), B independent transforms of length N back to back.  Each pass is a
-// Stockham autosort radix-R step (R = 2, 3, 4, 5): every butterfly reads its R inputs
+// Stockham autosort radix-R step (R = 16, 8, 4, 2, 3, 5; a 256-point transform takes two
+// radix-16 passes): every butterfly reads its R inputs
 // and twiddles into registers, the workgroup synchronises, then writes the R outputs to
 // their self-sorted positions in the same arrays.  Twiddles w_N^e = exp(-2*pi*i*e/N)
-// come from a table generated on the host in FP64 (rounded once for FP32).
+// come from a table generated on the host in FP64 (rounded once for FP32) and staged in
+// LDS by the calling kernel.
 // Transforms are unnormalised, forward = exp(-i...), inverse = exp(+i...), the FFTW
 // convention the reference relies on (fftw.f90:37-93, demag.f90:259-267).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #define MRMG_MAX_FAC 24
+
+// LDS index padding: one spare slot per 32 elements, so the stride-R writes of a
+// Stockham pass spread over the LDS banks instead of piling onto two of them.
+#define LP(i) ((i) + ((i) >> 5))
+// elements of an LDS array holding n padded entries
+#define LPN(n) ((n) + ((n) >> 5) + 1)
 
 struct FFTPlan {
     int n;
@@ -102,9 +110,69 @@ template <typename T, bool INV> struct Dft<T, 5, INV> {
     }
 };
 
+// w_16^e = exp(-2 pi i e/16), e = 0..15 (cos, sin); the forward twiddle is (c, -s)
+template <typename T> struct W16 {
+    __device__ __forceinline__ static C2<T> get(int e, bool inv)
+    {
+        const T c[16] = {(T)1, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173,
+                         (T)0, (T)-0.38268343236508977173, (T)-0.70710678118654752440, (T)-0.92387953251128675613,
+                         (T)-1, (T)-0.92387953251128675613, (T)-0.70710678118654752440, (T)-0.38268343236508977173,
+                         (T)0, (T)0.38268343236508977173, (T)0.70710678118654752440, (T)0.92387953251128675613};
+        const T s[16] = {(T)0, (T)0.38268343236508977173, (T)0.70710678118654752440, (T)0.92387953251128675613,
+                         (T)1, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173,
+                         (T)0, (T)-0.38268343236508977173, (T)-0.70710678118654752440, (T)-0.92387953251128675613,
+                         (T)-1, (T)-0.92387953251128675613, (T)-0.70710678118654752440, (T)-0.38268343236508977173};
+        return C2<T>{c[e & 15], inv ? s[e & 15] : -s[e & 15]};
+    }
+};
+
+// Length 8 = 4 x 2 and 16 = 4 x 4 in registers: X[p + A r] = sum_q w_R^{qp} w_B^{qr} U_q[p],
+// U_q = DFT_A of v[q + B m] over m (R = A B, A = 4).
+template <typename T, bool INV> struct Dft<T, 8, INV> {
+    __device__ __forceinline__ static void run(C2<T>* v)
+    {
+        C2<T> u[2][4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            C2<T> t[4] = {v[q], v[q + 2], v[q + 4], v[q + 6]};
+            Dft<T, 4, INV>::run(t);
+#pragma unroll
+            for (int p = 0; p < 4; p++) u[q][p] = (q && p) ? cmul(t[p], W16<T>::get(2 * p, INV)) : t[p];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            C2<T> t[2] = {u[0][p], u[1][p]};
+            Dft<T, 2, INV>::run(t);
+            v[p] = t[0];
+            v[p + 4] = t[1];
+        }
+    }
+};
+
+template <typename T, bool INV> struct Dft<T, 16, INV> {
+    __device__ __forceinline__ static void run(C2<T>* v)
+    {
+        C2<T> u[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            C2<T> t[4] = {v[q], v[q + 4], v[q + 8], v[q + 12]};
+            Dft<T, 4, INV>::run(t);
+#pragma unroll
+            for (int p = 0; p < 4; p++) u[q][p] = (q && p) ? cmul(t[p], W16<T>::get(q * p, INV)) : t[p];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            C2<T> t[4] = {u[0][p], u[1][p], u[2][p], u[3][p]};
+            Dft<T, 4, INV>::run(t);
+#pragma unroll
+            for (int r = 0; r < 4; r++) v[p + 4 * r] = t[r];
+        }
+    }
+};
+
 // One Stockham pass over B transforms of length N with span Ns.
 template <typename T, int R, int MAXB, bool INV>
-__device__ __forceinline__ void fft_pass(T* re, T* im, int N, int Ns, int B, const C2<T>* __restrict__ tw)
+__device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, int B, const C2<T>* __restrict__ tw)
 {
     const int NR = N / R;
     const int total = B * NR;
@@ -119,11 +187,11 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int N, int Ns, int B, con
             const int b = idx / NR;
             const int j = idx - b * NR;
             const int k = j % Ns;
-            const int base = b * N;
+            const int base = off + b * N;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                v[q][r].x = re[base + j + r * NR];
-                v[q][r].y = im[base + j + r * NR];
+                v[q][r].x = re[LP(base + j + r * NR)];
+                v[q][r].y = im[LP(base + j + r * NR)];
             }
             if (Ns > 1) {
 #pragma unroll
@@ -143,8 +211,8 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int N, int Ns, int B, con
         if (dst[q] >= 0) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                re[dst[q] + r * Ns] = v[q][r].x;
-                im[dst[q] + r * Ns] = v[q][r].y;
+                re[LP(dst[q] + r * Ns)] = v[q][r].x;
+                im[LP(dst[q] + r * Ns)] = v[q][r].y;
             }
         }
     }
@@ -152,17 +220,30 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int N, int Ns, int B, con
 }
 
 // Full transform: every thread of the workgroup must call it (it synchronises).
-template <typename T, int MAXB, bool INV>
-__device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw)
+// Element i of the B*N array sits at LDS slot LP(i).
+// E bounds the complex elements one thread holds in a pass: B*N <= E*blockDim.x.
+// off: element index of the first transform within the padded LDS arrays.
+template <typename T, int E, bool INV>
+__device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw, int off = 0)
 {
     int Ns = 1;
     const int N = plan.n;
     for (int p = 0; p < plan.nfac; p++) {
         const int R = plan.fac[p];
-        if (R == 4) fft_pass<T, 4, MAXB, INV>(re, im, N, Ns, B, tw);
-        else if (R == 2) fft_pass<T, 2, MAXB, INV>(re, im, N, Ns, B, tw);
-        else if (R == 3) fft_pass<T, 3, MAXB, INV>(re, im, N, Ns, B, tw);
-        else fft_pass<T, 5, MAXB, INV>(re, im, N, Ns, B, tw);
+        if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV>(re, im, off, N, Ns, B, tw);
+        else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV>(re, im, off, N, Ns, B, tw);
+        else if (R == 4) fft_pass<T, 4, (E + 3) / 4, INV>(re, im, off, N, Ns, B, tw);
+        else if (R == 2) fft_pass<T, 2, (E + 1) / 2, INV>(re, im, off, N, Ns, B, tw);
+        else if (R == 3) fft_pass<T, 3, (E + 2) / 3, INV>(re, im, off, N, Ns, B, tw);
+        else fft_pass<T, 5, (E + 4) / 5, INV>(re, im, off, N, Ns, B, tw);
         Ns *= R;
     }
+}
+
+// Copy a twiddle table into LDS (every thread calls; synchronises).
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(C2<T>* dst, const C2<T>* __restrict__ src, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
 }

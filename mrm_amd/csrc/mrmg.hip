@@ -113,36 +113,34 @@ __device__ __forceinline__ int xcd_remap(int b, int nb8)
 }
 
 // ------------------------------------------------------------------------- demag kernels
-// get_mag + row r2c: one workgroup per (row pair, layer).  Two rows of each component are
-// packed as z = M(y0) + i M(y1), transformed with one complex FFT and separated.
-template <typename T, int MAXB>
-__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ m, int G, const T* __restrict__ msf,
-                                                 const int* __restrict__ cell2grain, C2<T>* __restrict__ S,
-                                                 int nx, int ny, int nxh, FFTPlan px, const C2<T>* __restrict__ twx)
+// Row r2c: one workgroup per (row pair, layer).  Two rows of each component of the cell
+// magnetisation (written by the grain kernels, get_mag data.f90:344-353) are packed as
+// z = M(y0) + i M(y1), transformed with one complex FFT and separated.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+                                                 const C2<T>* __restrict__ twx)
 {
     extern __shared__ unsigned char smem_raw[];
-    T* re = reinterpret_cast<T*>(smem_raw);
-    T* im = re + 3 * nx;
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(3 * nx);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
-    const int* c0 = cell2grain + ((size_t)k * ny + y0) * nx;
-    const int* c1 = c0 + nx;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        const int g0 = c0[x];
-        const int g1 = has1 ? c1[x] : -1;
-        const T f0 = g0 >= 0 ? msf[g0] : (T)0, f1 = g1 >= 0 ? msf[g1] : (T)0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            re[c * nx + x] = g0 >= 0 ? f0 * m[(size_t)c * G + g0] : (T)0;
-            im[c * nx + x] = g1 >= 0 ? f1 * m[(size_t)c * G + g1] : (T)0;
-        }
+    const size_t CP = (size_t)nl * ny * nx;
+    const T* r0 = Mcell + ((size_t)k * ny + y0) * nx;
+    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+        const int c = idx / nx, x = idx - c * nx;
+        re[LP(idx)] = r0[c * CP + x];
+        im[LP(idx)] = has1 ? r0[c * CP + nx + x] : (T)0;
     }
     __syncthreads();
-    lds_fft<T, MAXB, false>(re, im, 3, px, twx);
+    lds_fft<T, E, false>(re, im, 3, px, tw);
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
         const int c = idx / nxh, kx = idx - c * nxh;
         const int kn = kx == 0 ? 0 : nx - kx;
-        const T zr = re[c * nx + kx], zi = im[c * nx + kx], nr = re[c * nx + kn], ni = im[c * nx + kn];
+        const T zr = re[LP(c * nx + kx)], zi = im[LP(c * nx + kx)], nr = re[LP(c * nx + kn)], ni = im[LP(c * nx + kn)];
         C2<T> A = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
         C2<T> B = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
         C2<T>* row = S + (((size_t)k * 3 + c) * ny + y0) * nxh;
@@ -152,85 +150,114 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ m, int G,
 }
 
 // Column pass with the k-space product fused (demag.f90:117-121): one workgroup per
-// (kx, observation layer ko).  The 3*n_lay source columns of this kx are transformed
-// along y in LDS, multiplied by the tensor block of (kx, ko), and the three field
-// columns transformed back.
-template <typename T, int MAXB, bool SYM>
-__global__ void __launch_bounds__(256) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
-                                             const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot,
-                                             int nblocks, int nb8, FFTPlan py, const C2<T>* __restrict__ twy)
+// (kx, group of observation layers).  The 3*n_lay source columns of this kx are
+// transformed along y in LDS, the field columns of the group's layers are formed in place
+// (all products read into registers, barrier, then written over the group's own source
+// columns), and those columns are transformed back.  kg groups per kx keep all CUs busy.
+#define COL_MAX_THREADS 512
+template <typename T, int E, bool SYM>
+__global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+                                                     const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
+                                                     int nslot, int kg, int nb8, FFTPlan py,
+                                                     const C2<T>* __restrict__ twy)
 {
     const int L = xcd_remap(blockIdx.x, nb8);
-    if (L >= nblocks) return;
-    const int kx = L % nxh, ko = L / nxh;
+    const int kx = L / kg, grp = L - kx * kg;
+    if (kx >= nxh) return;
+    const int per = (nl + kg - 1) / kg;
+    const int ko0 = grp * per, ko1 = ko0 + per < nl ? ko0 + per : nl;
+    if (ko0 >= ko1) return;
     extern __shared__ unsigned char smem_raw[];
     const int ncol = 3 * nl;
-    T* ire = reinterpret_cast<T*>(smem_raw);
-    T* iim = ire + ncol * ny;
-    T* ore = iim + ncol * ny;
-    T* oim = ore + 3 * ny;
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    T* re = reinterpret_cast<T*>(tw + ny);
+    T* im = re + LPN(ncol * ny);
     for (int idx = threadIdx.x; idx < ncol * ny; idx += blockDim.x) {
         const int col = idx / ny, y = idx - col * ny;
         const C2<T> v = S1[((size_t)col * ny + y) * nxh + kx];
-        ire[idx] = v.x;
-        iim[idx] = v.y;
+        re[LP(idx)] = v.x;
+        im[LP(idx)] = v.y;
+    }
+    for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twy[i];
+    __syncthreads();
+    lds_fft<T, E, false>(re, im, ncol, py, tw);
+    // product: work item (ko, ky); up to 4 items per thread held in registers
+    const int items = (ko1 - ko0) * ny;
+    constexpr int MAXI = 4;
+    C2<T> hv[MAXI][3];
+#pragma unroll
+    for (int q = 0; q < MAXI; q++) {
+        const int it = threadIdx.x + q * blockDim.x;
+        if (it < items) {
+            const int ko = ko0 + it / ny, ky = it % ny;
+            const C2<T>* Tk = Tspec + (((size_t)kx * nl + ko) * nl) * nslot * ny + ky;
+            C2<T> h0 = {0, 0}, h1 = {0, 0}, h2 = {0, 0};
+            for (int ks = 0; ks < nl; ks++) {
+                const C2<T> mx = {re[LP((3 * ks + 0) * ny + ky)], im[LP((3 * ks + 0) * ny + ky)]};
+                const C2<T> my = {re[LP((3 * ks + 1) * ny + ky)], im[LP((3 * ks + 1) * ny + ky)]};
+                const C2<T> mz = {re[LP((3 * ks + 2) * ny + ky)], im[LP((3 * ks + 2) * ny + ky)]};
+                const C2<T>* t = Tk + (size_t)ks * nslot * ny;
+                C2<T> a;
+                if (SYM) {  // slots xx yy zz xy xz yz
+                    const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
+                    a = cmul(txx, mx); h0.x += a.x; h0.y += a.y;
+                    a = cmul(txy, my); h0.x += a.x; h0.y += a.y;
+                    a = cmul(txz, mz); h0.x += a.x; h0.y += a.y;
+                    a = cmul(txy, mx); h1.x += a.x; h1.y += a.y;
+                    a = cmul(tyy, my); h1.x += a.x; h1.y += a.y;
+                    a = cmul(tyz, mz); h1.x += a.x; h1.y += a.y;
+                    a = cmul(txz, mx); h2.x += a.x; h2.y += a.y;
+                    a = cmul(tyz, my); h2.x += a.x; h2.y += a.y;
+                    a = cmul(tzz, mz); h2.x += a.x; h2.y += a.y;
+                } else {    // slot 3a+b: field component a from source component b
+                    a = cmul(t[0 * ny], mx); h0.x += a.x; h0.y += a.y;
+                    a = cmul(t[1 * ny], my); h0.x += a.x; h0.y += a.y;
+                    a = cmul(t[2 * ny], mz); h0.x += a.x; h0.y += a.y;
+                    a = cmul(t[3 * ny], mx); h1.x += a.x; h1.y += a.y;
+                    a = cmul(t[4 * ny], my); h1.x += a.x; h1.y += a.y;
+                    a = cmul(t[5 * ny], mz); h1.x += a.x; h1.y += a.y;
+                    a = cmul(t[6 * ny], mx); h2.x += a.x; h2.y += a.y;
+                    a = cmul(t[7 * ny], my); h2.x += a.x; h2.y += a.y;
+                    a = cmul(t[8 * ny], mz); h2.x += a.x; h2.y += a.y;
+                }
+            }
+            hv[q][0] = h0; hv[q][1] = h1; hv[q][2] = h2;
+        }
     }
     __syncthreads();
-    lds_fft<T, MAXB, false>(ire, iim, ncol, py, twy);
-    const C2<T>* Tk = Tspec + ((size_t)kx * nl + ko) * nl * nslot * ny;
-    for (int ky = threadIdx.x; ky < ny; ky += blockDim.x) {
-        C2<T> h0 = {0, 0}, h1 = {0, 0}, h2 = {0, 0};
-        for (int ks = 0; ks < nl; ks++) {
-            const C2<T> mx = {ire[(3 * ks + 0) * ny + ky], iim[(3 * ks + 0) * ny + ky]};
-            const C2<T> my = {ire[(3 * ks + 1) * ny + ky], iim[(3 * ks + 1) * ny + ky]};
-            const C2<T> mz = {ire[(3 * ks + 2) * ny + ky], iim[(3 * ks + 2) * ny + ky]};
-            const C2<T>* t = Tk + (size_t)ks * nslot * ny + ky;
-            if (SYM) {  // slots xx yy zz xy xz yz
-                const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
-                C2<T> a;
-                a = cmul(txx, mx); h0.x += a.x; h0.y += a.y;
-                a = cmul(txy, my); h0.x += a.x; h0.y += a.y;
-                a = cmul(txz, mz); h0.x += a.x; h0.y += a.y;
-                a = cmul(txy, mx); h1.x += a.x; h1.y += a.y;
-                a = cmul(tyy, my); h1.x += a.x; h1.y += a.y;
-                a = cmul(tyz, mz); h1.x += a.x; h1.y += a.y;
-                a = cmul(txz, mx); h2.x += a.x; h2.y += a.y;
-                a = cmul(tyz, my); h2.x += a.x; h2.y += a.y;
-                a = cmul(tzz, mz); h2.x += a.x; h2.y += a.y;
-            } else {    // slot 3a+b: field component a from source component b
-                C2<T> a;
-                a = cmul(t[0 * ny], mx); h0.x += a.x; h0.y += a.y;
-                a = cmul(t[1 * ny], my); h0.x += a.x; h0.y += a.y;
-                a = cmul(t[2 * ny], mz); h0.x += a.x; h0.y += a.y;
-                a = cmul(t[3 * ny], mx); h1.x += a.x; h1.y += a.y;
-                a = cmul(t[4 * ny], my); h1.x += a.x; h1.y += a.y;
-                a = cmul(t[5 * ny], mz); h1.x += a.x; h1.y += a.y;
-                a = cmul(t[6 * ny], mx); h2.x += a.x; h2.y += a.y;
-                a = cmul(t[7 * ny], my); h2.x += a.x; h2.y += a.y;
-                a = cmul(t[8 * ny], mz); h2.x += a.x; h2.y += a.y;
+#pragma unroll
+    for (int q = 0; q < MAXI; q++) {
+        const int it = threadIdx.x + q * blockDim.x;
+        if (it < items) {
+            const int ko = ko0 + it / ny, ky = it % ny;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                re[LP((3 * ko + a) * ny + ky)] = hv[q][a].x;
+                im[LP((3 * ko + a) * ny + ky)] = hv[q][a].y;
             }
         }
-        ore[ky] = h0.x; oim[ky] = h0.y;
-        ore[ny + ky] = h1.x; oim[ny + ky] = h1.y;
-        ore[2 * ny + ky] = h2.x; oim[2 * ny + ky] = h2.y;
     }
     __syncthreads();
-    lds_fft<T, MAXB, true>(ore, oim, 3, py, twy);
-    for (int idx = threadIdx.x; idx < 3 * ny; idx += blockDim.x) {
-        const int a = idx / ny, y = idx - a * ny;
-        S2[(((size_t)ko * 3 + a) * ny + y) * nxh + kx] = C2<T>{ore[idx], oim[idx]};
+    const int e0 = 3 * ko0 * ny, ne = 3 * (ko1 - ko0) * ny;
+    lds_fft<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+        const int idx = e0 + i;
+        const int col = idx / ny, y = idx - col * ny;
+        S2[((size_t)col * ny + y) * nxh + kx] = C2<T>{re[LP(idx)], im[LP(idx)]};
     }
 }
 
 // Row c2r: FFTW's 1D c2r semantics (imaginary parts of the DC and Nyquist bins ignored).
-template <typename T, int MAXB>
+template <typename T, int E>
 __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T* __restrict__ Hcell,
                                                  int nx, int ny, int nl, int nxh, FFTPlan px,
                                                  const C2<T>* __restrict__ twx)
 {
     extern __shared__ unsigned char smem_raw[];
-    T* re = reinterpret_cast<T*>(smem_raw);
-    T* im = re + 3 * nx;
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(3 * nx);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
@@ -239,27 +266,56 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
         C2<T> A = row[kx];
         C2<T> B = has1 ? row[nxh + kx] : C2<T>{0, 0};
         if (kx == 0 || 2 * kx == nx) { A.y = 0; B.y = 0; }
-        re[c * nx + kx] = A.x - B.y;
-        im[c * nx + kx] = A.y + B.x;
+        re[LP(c * nx + kx)] = A.x - B.y;
+        im[LP(c * nx + kx)] = A.y + B.x;
         if (kx > 0 && nx - kx != kx) {
-            re[c * nx + nx - kx] = A.x + B.y;
-            im[c * nx + nx - kx] = B.x - A.y;
+            re[LP(c * nx + nx - kx)] = A.x + B.y;
+            im[LP(c * nx + nx - kx)] = B.x - A.y;
         }
     }
     __syncthreads();
-    lds_fft<T, MAXB, true>(re, im, 3, px, twx);
+    lds_fft<T, E, true>(re, im, 3, px, tw);
     const size_t plane = (size_t)nx * ny;
     for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
         const int c = idx / nx, x = idx - c * nx;
         T* hc = Hcell + ((size_t)c * nl + k) * plane;
-        hc[(size_t)y0 * nx + x] = re[idx];
-        if (has1) hc[(size_t)y1 * nx + x] = im[idx];
+        hc[(size_t)y0 * nx + x] = re[LP(idx)];
+        if (has1) hc[(size_t)y1 * nx + x] = im[LP(idx)];
     }
 }
 
 // ------------------------------------------------------------------------- grain kernel
-template <typename T>
-__device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int k,
+// LPG lanes cooperate on one grain: the cell gather and the exchange sums are split over
+// the lanes and reduced with a fixed xor-butterfly (deterministic); the rest of the grain
+// work is computed redundantly by every lane and written by lane 0.
+template <int LPG, typename T>
+__device__ __forceinline__ T lane_sum(T v)
+{
+#pragma unroll
+    for (int o = LPG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPG);
+    return v;
+}
+
+template <typename T, int LPG>
+__device__ __forceinline__ void csr_sum(const int* __restrict__ off, const int* __restrict__ nb_,
+                                        const T* __restrict__ w_, const T* __restrict__ ms_scale,
+                                        const T* __restrict__ mf, int G, int g, int lane, T* h)
+{
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int j = off[g] + lane; j < off[g + 1]; j += LPG) {
+        const int nb = nb_[j];
+        const T w = w_[j] * ms_scale[nb];
+        s0 += w * mf[nb];
+        s1 += w * mf[G + nb];
+        s2 += w * mf[2 * G + nb];
+    }
+    h[0] += lane_sum<LPG>(s0);
+    h[1] += lane_sum<LPG>(s1);
+    h[2] += lane_sum<LPG>(s2);
+}
+
+template <typename T, int LPG>
+__device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int lane,
                                          const T* __restrict__ mf, const T* __restrict__ Hcell, size_t CP,
                                          double t, const T* m, T* h)
 {
@@ -280,14 +336,17 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
     T hd[3] = {0, 0, 0};
     if (Hcell) {
         const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
-        for (int q = b0; q < b1; q++) {
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int q = b0 + lane; q < b1; q += LPG) {
             const int ci = gd.gc_idx[q];
-            hd[0] += Hcell[ci];
-            hd[1] += Hcell[CP + ci];
-            hd[2] += Hcell[2 * CP + ci];
+            s0 += Hcell[ci];
+            s1 += Hcell[CP + ci];
+            s2 += Hcell[2 * CP + ci];
         }
         const T cnt = (T)(b1 - b0);
-        hd[0] = hd[0] / cnt; hd[1] = hd[1] / cnt; hd[2] = hd[2] / cnt;
+        hd[0] = lane_sum<LPG>(s0) / cnt;
+        hd[1] = lane_sum<LPG>(s1) / cnt;
+        hd[2] = lane_sum<LPG>(s2) / cnt;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) h[i] = happ[i] + hd[i];
@@ -318,36 +377,36 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
 #pragma unroll
         for (int i = 0; i < 3; i++) h[i] = h[i] + s * u[i];
     }
-    // exchange CSRs (field.f90:326-363)
-    const int G = gd.G;
-    if (gd.xi_off) {
-        T s[3] = {0, 0, 0};
-        for (int j = gd.xi_off[g]; j < gd.xi_off[g + 1]; j++) {
-            const int nb = gd.xi_nb[j];
-            const T w = gd.xi_w[j] * gd.ms_scale[nb];
-            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
-        }
-        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
+    // exchange CSRs (field.f90:326-363): intralayer, layer above, layer below
+    if (gd.xi_off) csr_sum<T, LPG>(gd.xi_off, gd.xi_nb, gd.xi_w, gd.ms_scale, mf, gd.G, g, lane, h);
+    if (gd.xu_off) csr_sum<T, LPG>(gd.xu_off, gd.xu_nb, gd.xu_w, gd.ms_scale, mf, gd.G, g, lane, h);
+    if (gd.xl_off) csr_sum<T, LPG>(gd.xl_off, gd.xl_nb, gd.xl_w, gd.ms_scale, mf, gd.G, g, lane, h);
+}
+
+// get_mag (data.f90:344-353): M = Ms*ms_scale*m on every cell of grain g, lanes split the
+// grain's cells.  Non-magnetic cells are never written and stay 0.
+template <typename T, int LPG>
+__device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, T* __restrict__ Mcell, size_t CP, int g,
+                                              int lane, T f, const T* m)
+{
+    const T v0 = f * m[0], v1 = f * m[1], v2 = f * m[2];
+    for (int q = gd.gc_off[g] + lane; q < gd.gc_off[g + 1]; q += LPG) {
+        const int ci = gd.gc_idx[q];
+        Mcell[ci] = v0;
+        Mcell[CP + ci] = v1;
+        Mcell[2 * CP + ci] = v2;
     }
-    if (gd.xu_off) {
-        T s[3] = {0, 0, 0};
-        for (int j = gd.xu_off[g]; j < gd.xu_off[g + 1]; j++) {
-            const int nb = gd.xu_nb[j];
-            const T w = gd.xu_w[j] * gd.ms_scale[nb];
-            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
-        }
-        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
-    }
-    if (gd.xl_off) {
-        T s[3] = {0, 0, 0};
-        for (int j = gd.xl_off[g]; j < gd.xl_off[g + 1]; j++) {
-            const int nb = gd.xl_nb[j];
-            const T w = gd.xl_w[j] * gd.ms_scale[nb];
-            s[0] += w * mf[nb]; s[1] += w * mf[G + nb]; s[2] += w * mf[2 * G + nb];
-        }
-        h[0] += s[0]; h[1] += s[1]; h[2] += s[2];
-    }
-    (void)k;
+}
+
+template <typename T, int LPG>
+__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, const T* __restrict__ m, T* __restrict__ Mcell,
+                                                 size_t CP)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = tid / LPG, lane = tid % LPG;
+    if (g >= gd.G) return;
+    const T mm[3] = {m[g], m[gd.G + g], m[2 * gd.G + g]};
+    scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, gd.msf[g], mm);
 }
 
 // Rotational pushes of push_llg (llg.f90:1086-1121): Boris (|type|==2) or analytic
@@ -421,27 +480,37 @@ __device__ __forceinline__ void rot_push(int type, T scale1, T scale2, const T* 
 // [2 + k] sum |m| ms_scale vol for layer k, [2+NL+3k+c] sum vol m_c for layer k.
 #define DIAG_STRIDE (2 + 4 * MRMG_MAX_LAYERS)
 
-template <typename T, int MODE>
+template <typename T, int MODE, int LPG>
 __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ ma, T* __restrict__ mb,
                                                T* __restrict__ dm, T* __restrict__ hthm, T* __restrict__ hout,
-                                               const T* __restrict__ Hcell, size_t CP, StepArgs a,
-                                               double* __restrict__ diag_part)
+                                               const T* __restrict__ Hcell, T* __restrict__ Mcell, size_t CP,
+                                               StepArgs a, double* __restrict__ diag_part)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = tid / LPG, lane = tid % LPG;
+    const bool w0 = lane == 0;
     const int G = gd.G;
     __shared__ double red[DIAG_STRIDE];
     if (a.diag) {
         for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
         __syncthreads();
     }
+    T h[3] = {0, 0, 0};
     if (g < G) {
         const int k = gd.layer_of[g];
         const LayerParams& L = gd.lp[k];
         // the magnetisation the field is evaluated on
         const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
         T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
-        T h[3];
-        field_at<T>(gd, L, g, k, mf, Hcell, CP, a.t, mfield, h);
+        field_at<T, LPG>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h);
+    }
+    // after the lane reductions only lane 0 of each grain reads the state and writes it
+    T mo[3] = {0, 0, 0};
+    if (g < G && w0) {
+        const int k = gd.layer_of[g];
+        const LayerParams& L = gd.lp[k];
+        const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
+        T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
         // thermal draw (get_thermal) happens with the first field of a step
         T th[3];
         if (MODE == GM_HEUN1 || (MODE == GM_ROT1)) {
@@ -468,7 +537,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
         const T msc = gd.ms_scale[g];
         T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
-        T mo[3], d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
+        T d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
         const T scale2 = gd.alpha_scale[g] * (T)L.alpha;
         if (msc > (T)0) {
@@ -526,6 +595,14 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 3; c++) atomicAdd(&red[2 + MRMG_MAX_LAYERS + 3 * k + c], vol * (double)mo[c]);
         }
+    }
+    // the new magnetisation of the grain onto its cells, for the next field evaluation
+    if (Mcell && g < G) {
+        const int src = (threadIdx.x & 63) & ~(LPG - 1);
+        T mb3[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) mb3[c] = __shfl(mo[c], src);
+        scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, gd.msf[g], mb3);
     }
     if (a.diag) {
         __syncthreads();
@@ -631,51 +708,55 @@ __global__ void k_plane_to_plane(int nx, int ny, double dx, double dy, double dz
 }
 
 // Tensor FFT in FP64, rows: one workgroup per (row pair, plane) of P real planes.
-template <int MAXB>
+template <int E>
 __global__ void __launch_bounds__(256) k_tensor_rows(const double* __restrict__ R, C2<double>* __restrict__ TC,
                                                      int nx, int ny, int nxh, FFTPlan px,
                                                      const C2<double>* __restrict__ twx)
 {
     extern __shared__ unsigned char smem_raw[];
-    double* re = reinterpret_cast<double*>(smem_raw);
-    double* im = re + nx;
+    C2<double>* tw = reinterpret_cast<C2<double>*>(smem_raw);
+    double* re = reinterpret_cast<double*>(tw + nx);
+    double* im = re + LPN(nx);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
     const size_t pl = blockIdx.y;
     const bool has1 = y1 < ny;
     const double* r = R + pl * nx * ny;
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        re[x] = r[(size_t)y0 * nx + x];
-        im[x] = has1 ? r[(size_t)y1 * nx + x] : 0.0;
+        re[LP(x)] = r[(size_t)y0 * nx + x];
+        im[LP(x)] = has1 ? r[(size_t)y1 * nx + x] : 0.0;
     }
     __syncthreads();
-    lds_fft<double, MAXB, false>(re, im, 1, px, twx);
+    lds_fft<double, E, false>(re, im, 1, px, tw);
     C2<double>* out = TC + pl * nxh * ny;
     for (int kx = threadIdx.x; kx < nxh; kx += blockDim.x) {
         const int kn = kx == 0 ? 0 : nx - kx;
-        const double zr = re[kx], zi = im[kx], nr = re[kn], ni = im[kn];
+        const double zr = re[LP(kx)], zi = im[LP(kx)], nr = re[LP(kn)], ni = im[LP(kn)];
         out[(size_t)y0 * nxh + kx] = {(zr + nr) * 0.5, (zi - ni) * 0.5};
         if (has1) out[(size_t)y1 * nxh + kx] = {(zi + ni) * 0.5, (nr - zr) * 0.5};
     }
 }
 
 // Tensor FFT in FP64, columns in place: one workgroup per (kx, plane).
-template <int MAXB>
+template <int E>
 __global__ void __launch_bounds__(256) k_tensor_cols(C2<double>* __restrict__ TC, int ny, int nxh, FFTPlan py,
                                                      const C2<double>* __restrict__ twy)
 {
     extern __shared__ unsigned char smem_raw[];
-    double* re = reinterpret_cast<double*>(smem_raw);
-    double* im = re + ny;
+    C2<double>* tw = reinterpret_cast<C2<double>*>(smem_raw);
+    double* re = reinterpret_cast<double*>(tw + ny);
+    double* im = re + LPN(ny);
+    for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twy[i];
     const int kx = blockIdx.x;
     C2<double>* col = TC + (size_t)blockIdx.y * nxh * ny + kx;
     for (int y = threadIdx.x; y < ny; y += blockDim.x) {
         const C2<double> v = col[(size_t)y * nxh];
-        re[y] = v.x;
-        im[y] = v.y;
+        re[LP(y)] = v.x;
+        im[LP(y)] = v.y;
     }
     __syncthreads();
-    lds_fft<double, MAXB, false>(re, im, 1, py, twy);
-    for (int y = threadIdx.x; y < ny; y += blockDim.x) col[(size_t)y * nxh] = {re[y], im[y]};
+    lds_fft<double, E, false>(re, im, 1, py, tw);
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) col[(size_t)y * nxh] = {re[LP(y)], im[LP(y)]};
 }
 
 // Regroup the tensor spectra into Tspec[kx][ko][ks][slot][ky] and round to T.
@@ -705,18 +786,13 @@ static bool factorize(int n, FFTPlan& p)
     p.n = n;
     p.nfac = 0;
     int m = n;
+    while (m % 16 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 16; m /= 16; }
+    while (m % 8 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 8; m /= 8; }
     while (m % 4 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 4; m /= 4; }
     while (m % 2 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 3; m /= 3; }
     while (m % 5 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 5; m /= 5; }
     return m == 1;
-}
-
-static int min_radix(const FFTPlan& p)
-{
-    int r = 5;
-    for (int i = 0; i < p.nfac; i++) r = std::min(r, p.fac[i]);
-    return p.nfac ? r : 4;
 }
 
 template <typename T> struct DBuf {
@@ -784,6 +860,8 @@ struct EngineBase {
     virtual size_t footprint() const = 0;
     virtual bool has_tensor() const = 0;
     virtual int profile(int enable) = 0;
+    virtual int tune(const char* key, int v) = 0;
+    virtual int get_tune(const char* key, int* v) = 0;
     virtual int kernel_stats(mrmg_kstat* out, int max, int* n) = 0;
 };
 
@@ -809,27 +887,23 @@ static std::vector<C2<double>> twiddle_table(int n)
     return w;
 }
 
-// pick MAXB so that MAXB * threads >= largest butterfly count of a pass
-static int pick_maxb(int batch, const FFTPlan& p, int threads)
+// E: complex elements per thread of one pass, B*N <= E*threads, rounded up to 4..64
+static int pick_e(int batch, const FFTPlan& p, int threads)
 {
-    const int total = batch * p.n / min_radix(p);
-    const int need = (total + threads - 1) / threads;
-    if (need <= 1) return 1;
-    if (need <= 2) return 2;
-    if (need <= 4) return 4;
-    if (need <= 8) return 8;
-    if (need <= 16) return 16;
+    const int need = (batch * p.n + threads - 1) / threads;
+    for (int e = 4; e <= 64; e *= 2)
+        if (need <= e) return e;
     return -1;
 }
 
-#define LAUNCH_MAXB(MB, KERNEL, ...)                                                                  \
+#define LAUNCH_E(EV, KERNEL, ...)                                                                     \
     do {                                                                                              \
-        switch (MB) {                                                                                 \
-        case 1: KERNEL<1>(__VA_ARGS__); break;                                                        \
-        case 2: KERNEL<2>(__VA_ARGS__); break;                                                        \
+        switch (EV) {                                                                                 \
         case 4: KERNEL<4>(__VA_ARGS__); break;                                                        \
         case 8: KERNEL<8>(__VA_ARGS__); break;                                                        \
-        default: KERNEL<16>(__VA_ARGS__); break;                                                      \
+        case 16: KERNEL<16>(__VA_ARGS__); break;                                                      \
+        case 32: KERNEL<32>(__VA_ARGS__); break;                                                      \
+        default: KERNEL<64>(__VA_ARGS__); break;                                                      \
         }                                                                                             \
     } while (0)
 
@@ -848,8 +922,11 @@ struct Engine : EngineBase {
     DBuf<T> ma, mb, dm, hthm, h;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
-    DBuf<T> Hcell;
+    DBuf<T> Hcell, Mcell;
     DBuf<int> slot2c;
+    // launch shapes (mrmg_tune): threads of the row and column workgroups, observation-layer
+    // groups per column workgroup, lanes per grain in the grain kernels
+    int row_threads = 128, col_threads = 512, kg = 1, lpg = 8;
     DBuf<double> noise_buf, diag_part;
     // instrumentation: event pairs of the current mrmg_step call, per-class totals
     bool prof = false;
@@ -915,7 +992,8 @@ struct Engine : EngineBase {
         b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
              hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + vol.bytes() + lp.bytes();
         b += ma.bytes() + mb.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
-        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + slot2c.bytes();
+        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
+             slot2c.bytes();
         b += noise_buf.bytes() + diag_part.bytes();
         return b;
     }
@@ -930,12 +1008,11 @@ struct Engine : EngineBase {
         for (int i = 0; i < ny; i++) ty[i] = {(T)wy[i].x, (T)wy[i].y};
         RCHK(twx.upload(tx.data(), nx, c->stream));
         RCHK(twy.upload(ty.data(), ny, c->stream));
-        maxb_row = pick_maxb(3, c->px, 256);
-        maxb_col = pick_maxb(3 * nl, c->py, 256);
+        RCHK(shapes());
         if (maxb_row < 0 || maxb_col < 0)
             return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs", nx, ny, nl);
-        lds_row = sizeof(T) * 2 * 3 * nx;
-        lds_col = sizeof(T) * 2 * (3 * nl + 3) * ny;
+        lds_row = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN(3 * nx);
+        lds_col = sizeof(C2<T>) * ny + sizeof(T) * 2 * LPN(3 * nl * ny);
         if (lds_row > 160 * 1024 || lds_col > 160 * 1024)
             return set_err(MRMG_ERR_UNSUPPORTED, "LDS need %zu/%zu bytes exceeds 160 KiB (ny=%d, n_lay=%d)", lds_row,
                            lds_col, ny, nl);
@@ -944,6 +1021,59 @@ struct Engine : EngineBase {
         RCHK(S2.alloc((size_t)nl * 3 * ny * nxh));
         RCHK(Hcell.alloc((size_t)3 * nl * nx * ny));
         HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
+        RCHK(Mcell.alloc((size_t)3 * nl * nx * ny));
+        HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
+        kg = 1;
+        return MRMG_OK;
+    }
+
+    // derived launch parameters of the current shapes
+    int shapes()
+    {
+        const int nl = c->nl, ny = c->ny;
+        maxb_row = pick_e(3, c->px, row_threads);
+        maxb_col = pick_e(3 * nl, c->py, col_threads);
+        if (maxb_row < 0 || maxb_col < 0)
+            return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs (threads %d/%d)",
+                           c->nx, ny, nl, row_threads, col_threads);
+        const int per = (nl + kg - 1) / kg;
+        if (per * ny > 4 * col_threads)
+            return set_err(MRMG_ERR_UNSUPPORTED, "column workgroup of %d threads cannot hold %d products", col_threads,
+                           per * ny);
+        return MRMG_OK;
+    }
+    int tune(const char* key, int v) override
+    {
+        std::string k(key ? key : "");
+        int old;
+        if (k == "row_threads") {
+            if (v != 64 && v != 128 && v != 256) return set_err(MRMG_ERR_ARG, "row_threads must be 64, 128 or 256");
+            old = row_threads; row_threads = v;
+            if (shapes() != MRMG_OK) { row_threads = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "col_threads") {
+            if (v != 256 && v != 512) return set_err(MRMG_ERR_ARG, "col_threads must be 256 or 512");
+            old = col_threads; col_threads = v;
+            if (shapes() != MRMG_OK) { col_threads = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "col_groups") {
+            if (v < 1 || v > c->nl) return set_err(MRMG_ERR_ARG, "col_groups must be in 1..n_lay");
+            old = kg; kg = v;
+            if (shapes() != MRMG_OK) { kg = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "lanes_per_grain") {
+            if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 4, 8 or 16");
+            lpg = v;
+        } else {
+            return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
+        }
+        return MRMG_OK;
+    }
+    int get_tune(const char* key, int* v) override
+    {
+        std::string k(key ? key : "");
+        if (k == "row_threads") *v = row_threads;
+        else if (k == "col_threads") *v = col_threads;
+        else if (k == "col_groups") *v = kg;
+        else if (k == "lanes_per_grain") *v = lpg;
+        else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         return MRMG_OK;
     }
 
@@ -960,8 +1090,8 @@ struct Engine : EngineBase {
     int lds_attr()
     {
         if (lds_row <= 64 * 1024 && lds_col <= 64 * 1024) return MRMG_OK;
-        RCHK(lds_attr_mb<1>()); RCHK(lds_attr_mb<2>()); RCHK(lds_attr_mb<4>());
-        RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
+        RCHK(lds_attr_mb<4>()); RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
+        RCHK(lds_attr_mb<32>()); RCHK(lds_attr_mb<64>());
         return MRMG_OK;
     }
 
@@ -1090,7 +1220,8 @@ struct Engine : EngineBase {
         HIPCHK(hipMemsetAsync(dm.p, 0, dm.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(h.p, 0, h.bytes(), c->stream));
-        RCHK(diag_part.alloc((size_t)((G + 255) / 256) * DIAG_STRIDE));
+        RCHK(diag_part.alloc((size_t)(((size_t)G * 16 + 255) / 256) * DIAG_STRIDE));
+        HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return MRMG_OK;
     }
@@ -1123,32 +1254,32 @@ struct Engine : EngineBase {
         auto wx = twiddle_table(nx), wy = twiddle_table(ny);
         RCHK(tdx.upload(wx.data(), nx, c->stream));
         RCHK(tdy.upload(wy.data(), ny, c->stream));
-        const int mbr = pick_maxb(1, c->px, 256), mbc = pick_maxb(1, c->py, 256);
+        const int mbr = pick_e(1, c->px, 256), mbc = pick_e(1, c->py, 256);
         if (mbr < 0 || mbc < 0) return set_err(MRMG_ERR_UNSUPPORTED, "tensor FFT too large");
         {
             dim3 grid((ny + 1) / 2, P), blk(256);
-            size_t lds = sizeof(double) * 2 * nx;
+            size_t lds = sizeof(double) * (2 * nx + 2 * LPN(nx));
 #define KTR(MB, ...) hipLaunchKernelGGL(k_tensor_rows<MB>, grid, blk, lds, c->stream, __VA_ARGS__)
             switch (mbr) {
-            case 1: KTR(1, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
-            case 2: KTR(2, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
             case 4: KTR(4, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
             case 8: KTR(8, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
-            default: KTR(16, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            case 16: KTR(16, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            case 32: KTR(32, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
+            default: KTR(64, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
             }
 #undef KTR
             HIPCHK(hipGetLastError());
         }
         {
             dim3 grid(nxh, P), blk(256);
-            size_t lds = sizeof(double) * 2 * ny;
+            size_t lds = sizeof(double) * (2 * ny + 2 * LPN(ny));
 #define KTC(MB) hipLaunchKernelGGL(k_tensor_cols<MB>, grid, blk, lds, c->stream, TC.p, ny, nxh, c->py, tdy.p)
             switch (mbc) {
-            case 1: KTC(1); break;
-            case 2: KTC(2); break;
             case 4: KTC(4); break;
             case 8: KTC(8); break;
-            default: KTC(16); break;
+            case 16: KTC(16); break;
+            case 32: KTC(32); break;
+            default: KTC(64); break;
             }
 #undef KTC
             HIPCHK(hipGetLastError());
@@ -1221,42 +1352,53 @@ struct Engine : EngineBase {
 
     bool demag_on() const { return c->demag && tensor_ok; }
 
-    // get_demag: scatter + row r2c, column FFT * tensor, row c2r; m = device grain vectors
-    int demag(const T* m)
+    // get_mag for the state m (used where no grain kernel has written the cells yet)
+    int scatter(const T* m)
+    {
+        const size_t CP = (size_t)c->nl * c->nx * c->ny;
+        const size_t threads = (size_t)G * 8;
+        hipLaunchKernelGGL((k_scatter<T, 8>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                           gdata(), m, Mcell.p, CP);
+        HIPCHK(hipGetLastError());
+        return MRMG_OK;
+    }
+
+    // get_demag: row r2c of the cell magnetisation, column FFT * tensor, row c2r
+    int demag()
     {
         const int nx = c->nx, ny = c->ny, nl = c->nl, nxh = c->nxh;
-        dim3 grow((ny + 1) / 2, nl), blk(256);
+        dim3 grow((ny + 1) / 2, nl), blk(row_threads);
         RCHK(ev_begin(K_ROW_FWD));
-        LAUNCH_MAXB(maxb_row, launch_row_fwd, grow, blk, m);
+        LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
-        const int nblocks = nxh * nl, nb8 = (nblocks + 7) / 8 * 8;
+        const int nb8 = (nxh * kg + 7) / 8 * 8;
         RCHK(ev_begin(K_COL));
-        if (nslot == 6) LAUNCH_MAXB(maxb_col, launch_col_sym, nblocks, nb8);
-        else LAUNCH_MAXB(maxb_col, launch_col_gen, nblocks, nb8);
+        if (nslot == 6) LAUNCH_E(maxb_col, launch_col_sym, nb8);
+        else LAUNCH_E(maxb_col, launch_col_gen, nb8);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         RCHK(ev_begin(K_ROW_INV));
-        LAUNCH_MAXB(maxb_row, launch_row_inv, grow, blk);
+        LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         (void)nx;
         return MRMG_OK;
     }
-    template <int MB> void launch_row_fwd(dim3 g, dim3 b, const T* m)
+    template <int MB> void launch_row_fwd(dim3 g, dim3 b)
     {
-        hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, m, G, msf.p, cell2grain.p, S1.p, c->nx, c->ny,
-                           c->nxh, c->px, twx.p);
+        hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
+                           c->px, twx.p);
     }
-    template <int MB> void launch_col_sym(int nblocks, int nb8)
+    template <int MB> void launch_col_sym(int nb8)
     {
-        hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(256), lds_col, c->stream, S1.p, S2.p, Tspec.p, c->ny,
-                           c->nl, c->nxh, nslot, nblocks, nb8, c->py, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
+                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
-    template <int MB> void launch_col_gen(int nblocks, int nb8)
+    template <int MB> void launch_col_gen(int nb8)
     {
-        hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(256), lds_col, c->stream, S1.p, S2.p, Tspec.p, c->ny,
-                           c->nl, c->nxh, nslot, nblocks, nb8, c->py, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
+                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
     template <int MB> void launch_row_inv(dim3 g, dim3 b)
     {
@@ -1264,13 +1406,21 @@ struct Engine : EngineBase {
                            c->px, twx.p);
     }
 
-    template <int MODE> int grain(const StepArgs& a)
+    template <int MODE, int LPG> void launch_grain(const StepArgs& a)
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const T* hc = demag_on() ? Hcell.p : nullptr;
+        const size_t threads = (size_t)G * LPG;
+        hipLaunchKernelGGL((k_grain<T, MODE, LPG>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                           gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
+                           diag_part.p);
+    }
+    template <int MODE> int grain(const StepArgs& a)
+    {
         RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1) ? K_GRAIN1 : K_GRAIN2));
-        hipLaunchKernelGGL((k_grain<T, MODE>), dim3((G + 255) / 256), dim3(256), 0, c->stream, gdata(), ma.p, mb.p,
-                           dm.p, hthm.p, h.p, hc, CP, a, diag_part.p);
+        if (lpg == 4) launch_grain<MODE, 4>(a);
+        else if (lpg == 16) launch_grain<MODE, 16>(a);
+        else launch_grain<MODE, 8>(a);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         return MRMG_OK;
@@ -1285,6 +1435,8 @@ struct Engine : EngineBase {
         if (p->noise == MRMG_NOISE_INJECT && !noise) return set_err(MRMG_ERR_ARG, "noise array is NULL");
         // llg() start: init_applied zeroes hthm (field.f90:703)
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
+        // the cells must hold the current state before the first field evaluation
+        if (demag_on()) RCHK(scatter(ma.p));
         const int chunk = 64;
         const size_t per = 3 * (size_t)G;
         if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per * std::min(chunk, std::max(p->n_steps, 1))));
@@ -1311,22 +1463,22 @@ struct Engine : EngineBase {
             }
             if (type == -3) {
                 // stage 1 (get_fields at t, push iter=2)
-                if (demag_on()) RCHK(demag(ma.p));
+                if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt; a.diag = 0;
                 RCHK(grain<GM_HEUN1>(a));
                 // stage 2 (get_field at t+dt on the predicted m, push iter=1)
-                if (demag_on()) RCHK(demag(mb.p));
+                if (demag_on()) RCHK(demag());
                 a.t = t + dt; a.diag = last;
                 RCHK(grain<GM_HEUN2>(a));
             } else if (type < 0) {
-                if (demag_on()) RCHK(demag(ma.p));
+                if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt * 0.5; a.diag = 0;
                 RCHK(grain<GM_ROT1>(a));
-                if (demag_on()) RCHK(demag(mb.p));
+                if (demag_on()) RCHK(demag());
                 a.t = t + dt * 0.5; a.dts = dt; a.diag = last;
                 RCHK(grain<GM_ROT2>(a));
             } else {
-                if (demag_on()) RCHK(demag(ma.p));
+                if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt; a.diag = last;
                 RCHK(grain<GM_ROT1>(a));
                 ma.swap_with(mb);
@@ -1368,7 +1520,7 @@ struct Engine : EngineBase {
         memset(d, 0, sizeof(*d));
         d->t = t; d->dt = dt; d->steps = steps; d->n_lay = c->nl;
         if (steps <= 0) return MRMG_OK;
-        const int nb = (G + 255) / 256;
+        const int nb = (int)(((size_t)G * lpg + 255) / 256);
         std::vector<double> part((size_t)nb * DIAG_STRIDE);
         HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
         double e = 0, e1 = 0;
@@ -1398,7 +1550,8 @@ struct Engine : EngineBase {
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         if (demag_on()) {
-            RCHK(demag(ma.p));
+            RCHK(scatter(ma.p));
+            RCHK(demag());
         } else {
             HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
         }
@@ -1713,6 +1866,18 @@ int mrmg_profile(mrmg_ctx* c, int enable)
 {
     if (!c) return set_err(MRMG_ERR_ARG, "null argument");
     return c->eng->profile(enable);
+}
+
+int mrmg_tune(mrmg_ctx* c, const char* key, int value)
+{
+    if (!c || !key) return set_err(MRMG_ERR_ARG, "null argument");
+    return c->eng->tune(key, value);
+}
+
+int mrmg_get_tune(mrmg_ctx* c, const char* key, int* value)
+{
+    if (!c || !key || !value) return set_err(MRMG_ERR_ARG, "null argument");
+    return c->eng->get_tune(key, value);
 }
 
 int mrmg_kernel_stats(mrmg_ctx* c, mrmg_kstat* out, int max, int* n)

@@ -1,0 +1,138 @@
+! mrmg_m.f90 -- Fortran 2003 binding of the MI355X time-stepping library (include/mrmg.h).
+!
+! This is the host side a Fortran driver uses to hand its `data_m`/`field_m` arrays to the
+! GPU path.  The derived types are interoperable copies of the C structs; every entry
+! point returns an integer status (0 = ok), the Fortran form of the reference's
+! `logical ok_p` idiom (cli.f90:249), and mrmg_error() gives the message.
+!
+! The handle convention follows the reference's own C-ABI precedent for its C++ module
+! (src/FP/fp.f90:12-44): an opaque type(c_ptr) passed by value, scalars by reference.
+module mrmg_m
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+  public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag
+  public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_set_layer, &
+            mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_step, mrmg_error
+  public :: MRMG_FP64, MRMG_FP32, MRMG_NOISE_NONE, MRMG_NOISE_INJECT, MRMG_NOISE_PHILOX, MRMG_TYPE_HEUN
+
+  integer(c_int), parameter :: MRMG_FP64 = 0, MRMG_FP32 = 1
+  integer(c_int), parameter :: MRMG_NOISE_NONE = 0, MRMG_NOISE_INJECT = 1, MRMG_NOISE_PHILOX = 2
+  integer(c_int), parameter :: MRMG_TYPE_HEUN = -3
+
+  type, bind(C) :: mrmg_geom
+    integer(c_int) :: nx, ny, n_lay, precision, demag
+    real(c_double) :: dx, dy
+    type(c_ptr) :: dz, il_space
+  end type
+
+  type, bind(C) :: mrmg_layer
+    integer(c_int) :: ng
+    type(c_ptr) :: grid2grain, grain2gridindex, grain2grid
+    type(c_ptr) :: grain_vol, ms, hk, uni_axis, cubic_axis
+    type(c_ptr) :: hxgrainindex, hxgrain, hx
+    type(c_ptr) :: uhxgrainindex, uhxgrain, uhx
+    type(c_ptr) :: lhxgrainindex, lhxgrain, lhx
+    type(c_ptr) :: ms_scale, alpha_scale, hk_scale, temp, h_app
+    real(c_double) :: alpha, gamma, thm_alpha, floor_ms
+    integer(c_int) :: thermal
+  end type
+
+  type, bind(C) :: mrmg_ufield
+    real(c_double) :: dir(3)
+    real(c_double) :: t_start, t_end, s_start, s_end, scale
+  end type
+
+  type, bind(C) :: mrmg_step_params
+    integer(c_int) :: type, n_steps
+    real(c_double) :: t0, dt
+    integer(c_int) :: noise, pad_
+    integer(c_int64_t) :: seed, step0
+  end type
+
+  type, bind(C) :: mrmg_diag
+    real(c_double) :: t, dt
+    integer(c_int) :: steps, n_lay
+    real(c_double) :: errmx, errmx1, mmag
+    real(c_double) :: mave(3, 8)
+  end type
+
+  interface
+    integer(c_int) function mrmg_create(g, ctx) bind(C, name='mrmg_create')
+      import :: c_int, c_ptr, mrmg_geom
+      type(mrmg_geom), intent(in) :: g
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function mrmg_destroy(ctx) bind(C, name='mrmg_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    ! n_real: the (nx,ny,9,n_lay,n_lay) array of calculate_tensor (tensor.f90:82-216)
+    integer(c_int) function mrmg_set_tensor(ctx, n_real) bind(C, name='mrmg_set_tensor')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: n_real(*)
+    end function
+    integer(c_int) function mrmg_build_tensor(ctx, num_images, n_real_out) bind(C, name='mrmg_build_tensor')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: num_images
+      type(c_ptr), value :: n_real_out
+    end function
+    integer(c_int) function mrmg_set_layer(ctx, k, layer) bind(C, name='mrmg_set_layer')
+      import :: c_int, c_ptr, mrmg_layer
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      type(mrmg_layer), intent(in) :: layer
+    end function
+    integer(c_int) function mrmg_set_ufield(ctx, k, uf) bind(C, name='mrmg_set_ufield')
+      import :: c_int, c_ptr, mrmg_ufield
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      type(mrmg_ufield), intent(in) :: uf
+    end function
+    ! m: data_s%m(1+in:3+in,:) passed as a contiguous (3,ng) array
+    integer(c_int) function mrmg_set_state(ctx, k, m) bind(C, name='mrmg_set_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double), intent(in) :: m(*)
+    end function
+    integer(c_int) function mrmg_get_state(ctx, k, m, h) bind(C, name='mrmg_get_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: m(*), h(*)
+    end function
+    integer(c_int) function mrmg_step(ctx, p, noise, diag) bind(C, name='mrmg_step')
+      import :: c_int, c_ptr, mrmg_step_params, mrmg_diag
+      type(c_ptr), value :: ctx
+      type(mrmg_step_params), intent(in) :: p
+      type(c_ptr), value :: noise
+      type(mrmg_diag), intent(out) :: diag
+    end function
+    type(c_ptr) function mrmg_last_error() bind(C, name='mrmg_last_error')
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! the last error message as a Fortran string
+  function mrmg_error() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: n
+    p = mrmg_last_error()
+    call c_f_pointer(p, s, [1024])
+    n = 0
+    do while (n < 1024)
+      if (s(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function
+
+end module mrmg_m

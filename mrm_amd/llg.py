@@ -92,6 +92,14 @@ class Llg:
         abi.check(self.lib, self.lib.mrmg_copy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
         return ms.value
 
+    def tune(self, key, value):
+        abi.check(self.lib, self.lib.mrmg_tune(self.ctx, key.encode(), int(value)))
+
+    def get_tune(self, key):
+        v = C.c_int(0)
+        abi.check(self.lib, self.lib.mrmg_get_tune(self.ctx, key.encode(), C.byref(v)))
+        return v.value
+
     def footprint(self):
         b = C.c_size_t()
         abi.check(self.lib, self.lib.mrmg_footprint(self.ctx, C.byref(b)))

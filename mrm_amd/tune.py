@@ -1,0 +1,56 @@
+"""Launch-shape autotuner for a live simulation context.
+
+The reference asks FFTW to plan its transforms exhaustively before the first step
+(fftw.f90:25, FFTW_EXHAUSTIVE); this is the same idea for the HIP kernels: time the
+candidate launch shapes of each kernel class on the real workload with HIP events
+(mrmg_profile) and keep the fastest.  The simulation state, time and thermal counter
+are restored afterwards, so tuning does not change the trajectory.
+"""
+import numpy as np
+
+SPACE = {
+    "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
+    "col_threads": ([256, 512], ("col",)),
+    "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)
+    "lanes_per_grain": ([4, 8, 16], ("grain1", "grain2")),
+}
+
+
+def _cost(x, steps, classes, philox_seed):
+    x.profile(True)
+    x.step(steps, philox_seed=philox_seed, diag=False)
+    st = x.kernel_stats()
+    x.profile(False)
+    return sum(st[k][1] / max(st[k][0], 1) for k in classes)
+
+
+def autotune(x, steps=10, philox_seed=12345, verbose=False):
+    """Choose every tunable of context x (an mrm_amd.llg.Llg).  Returns {key: value}."""
+    saved = [x.get_state(k)[0] for k in range(x.n_lay)]
+    step_index = x.step_index
+    chosen = {}
+    for key, (values, classes) in SPACE.items():
+        if values is None:
+            values = list(range(1, min(x.n_lay, 4) + 1))
+        best, best_v = None, x.get_tune(key)
+        for v in values:
+            try:
+                x.tune(key, v)
+            except Exception:
+                continue
+            x.step(2, philox_seed=philox_seed, diag=False)   # warm the shape
+            c = _cost(x, steps, classes, philox_seed)
+            if verbose:
+                print(f"  tune {key}={v}: {c * 1e3:.2f} us", flush=True)
+            if best is None or c < best:
+                best, best_v = c, v
+        x.tune(key, best_v)
+        chosen[key] = best_v
+    for k, m in enumerate(saved):
+        x.set_state(k, m)
+    x.step_index = step_index
+    return chosen
+
+
+def state_equal(a, b):
+    return all(np.array_equal(p, q) for p, q in zip(a, b))

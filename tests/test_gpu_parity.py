@@ -88,10 +88,10 @@ def test_demag_field_parity(oracle_mod, nx, ny, nl):
 
 
 def test_tensor_build_parity(oracle_mod):
-    """FP64 Newell tensor built on the GPU against the oracle.  The 64-term sums cancel
-    catastrophically, so one-ulp differences between the device and host log/atan grow
-    to ~1e-9 of the largest element in the far field: near elements (>= 1 % of the max)
-    agree to 1e-10 relative, all elements to 1e-8 of the max."""
+    """FP64 Newell tensor built on the GPU against the oracle.  The 64-term g1/g2 sums
+    cancel catastrophically: the reference's own tensor is not even symmetric under
+    r -> -r beyond ~1e-8 relative (its noise floor).  The device tensor (different log/atan
+    ulps) must agree with the oracle to within 10x that floor, and to 1e-8 of the max."""
     say("tensor build parity 32x24x2")
     nx, ny, nl = 32, 24, 2
     dz = [2e-7, 3e-7]
@@ -100,8 +100,12 @@ def test_tensor_build_parity(oracle_mod):
     ref = oracle_mod.calculate_tensor(nx, ny, nl, 2e-7, 2e-7, dz, il_space=[0.0, 1e-7], num_images=2)
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-8 * scale
-    near = np.abs(ref) >= 0.01 * scale
-    assert (np.abs(got - ref)[near] / np.abs(ref)[near]).max() <= 1e-10
+    # noise floor: asymmetry of the even diagonal components under (i,j) -> (-i,-j)
+    R = ref.reshape(nl * nl, 9, ny, nx)
+    flip = R[:, :, (-np.arange(ny)) % ny][:, :, :, (-np.arange(nx)) % nx]
+    floor = max(np.abs(R[:, c] - flip[:, c]).max() for c in (0, 4, 8))
+    assert floor > 0
+    assert np.abs(got - ref).max() <= 10 * floor
     # the symmetric components are bit-identical on the device as well
     T = got.reshape(nl * nl, 9, ny * nx)
     for a, b in ((0, 1), (0, 2), (1, 2)):
@@ -206,13 +210,14 @@ def test_philox_noise_is_what_the_step_uses(oracle_mod):
     for x in (a, b):
         x.set_media(med)
         x.set_llg(dt=5e-14)
-    a.step(10, philox_seed=1234)
-    nz = b.philox_normals(1234, 0, 10)
-    b.step(10, noise=nz)
+    a.step(40, philox_seed=1234)
+    nz = b.philox_normals(1234, 0, 40)
+    b.step(40, noise=nz)
     for k in range(4):
         assert np.array_equal(a.get_state(k)[0], b.get_state(k)[0])
     flat = nz.ravel()
-    assert abs(flat.mean()) < 0.02 and abs(flat.var() - 1) < 0.02
+    # 40 x G x 3 = 30720 normals: 4 sigma bounds on the mean and variance
+    assert abs(flat.mean()) < 4 / np.sqrt(flat.size) and abs(flat.var() - 1) < 4 * np.sqrt(2 / flat.size)
 
 
 def test_philox_normals_match_numpy_restatement():
@@ -274,6 +279,42 @@ def test_state_survives_layer_rebuild(oracle_mod):
     for k in (0, 1, 3):
         assert np.array_equal(before[k], after[k])
     assert np.allclose(after[2], [0, 0, 1])
+
+
+def test_launch_shapes_do_not_change_results():
+    """Row/column launch shapes give bit-identical trajectories; lanes per grain only
+    reorder the gather sums (last bits).  The autotuner restores the state it steps."""
+    from mrm_amd.tune import autotune
+    say("launch shapes")
+    med = media.config_media("cfg1", nx=64, ny=64)
+    N = None
+    out = {}
+    for cfg in ({}, {"row_threads": 64}, {"row_threads": 256, "col_threads": 256}, {"col_groups": 2},
+                {"col_groups": 4, "col_threads": 256}, {"lanes_per_grain": 4}, {"lanes_per_grain": 16}):
+        x = Llg(64, 64, 4, med.dx, med.dy, med.dz)
+        if N is None:
+            N = x.build_tensor(num_images=1, return_real=True)
+        else:
+            x.set_tensor(N)
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(4)):
+            x.set_ufield(k, u)
+        for key, v in cfg.items():
+            x.tune(key, v)
+        x.set_llg(dt=5e-14)
+        x.step(15, philox_seed=9)
+        out[str(cfg)] = [x.get_state(k)[0] for k in range(4)]
+    base = out["{}"]
+    for key, ms in out.items():
+        d = max(np.abs(p - q).max() for p, q in zip(base, ms))
+        if "lanes" in key:
+            assert d <= 1e-12, key
+        else:
+            assert d == 0.0, key
+    before = [x.get_state(k)[0] for k in range(4)]
+    autotune(x, steps=3)
+    after = [x.get_state(k)[0] for k in range(4)]
+    assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
 def test_step_rejects_bad_arguments():
