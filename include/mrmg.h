@@ -113,25 +113,34 @@ typedef struct {
 
 typedef struct {
     int type;                 /* MRMG_TYPE_*                                            */
-    int n_steps;              /* Heun steps to run in this call                         */
+    int n_steps;              /* accepted steps to run in this call                     */
     double t0;                /* simulation time at the first step (s)                  */
-    double dt;                /* fixed time step (fixed_dt_p, llg.f90:1484-1486)         */
+    double dt;                /* llgp%dt: the fixed step, or the adaptive step carried
+                                 from the previous call (mrmg_diag.dt)                   */
     int noise;                /* MRMG_NOISE_*                                           */
     int pad_;
     uint64_t seed;            /* Philox key                                             */
     uint64_t step0;           /* Philox counter of the first step                       */
+    double dt_frac;           /* 0: fixed dt (fixed_dt_p, llg.f90:1484-1486); else the
+                                 gyro-period fraction of get_fields (llg.f90:384-386)    */
+    double dt_max;            /* upper bound of the adaptive dt (s)                     */
+    double max_dm;            /* max rotation per step (degrees) before push_m rejects
+                                 and retries the step (llg.f90:1229-1236)               */
 } mrmg_step_params;
 
 /* Diagnostics of the last step (llg.f90:1209-1257, 217-237). */
 typedef struct {
     double t;                 /* time after the last step                               */
-    double dt;
+    double dt;                /* llgp%dt after the last step (pass it to the next call)  */
     int steps;                /* steps completed in this call                           */
     int n_lay;
     double errmx;             /* max |m x H|/Ms over all grains (sqrt of errmx_l)        */
     double errmx1;            /* max |dm|^2                                              */
     double mmag;              /* sum over layers of volume-weighted <|m| ms_scale>       */
     double mave[8][3];        /* per-layer volume-weighted <m> (up to 8 layers)          */
+    int rejected;             /* steps push_m rejected and retried in this call          */
+    int pad_;
+    double max_dmdt;          /* llgp%max_dmdt after the call                             */
 } mrmg_diag;
 
 /* ---- lifecycle ----------------------------------------------------------------- */
@@ -158,7 +167,9 @@ int mrmg_set_state(mrmg_ctx *ctx, int k, const double *m);
 /* m (3,ng) and the last field h (3,ng) (either may be NULL). */
 int mrmg_get_state(mrmg_ctx *ctx, int k, double *m, double *h);
 
-/* Run p->n_steps fixed-dt steps of the llg loop.  noise: MRMG_NOISE_INJECT only, host
+/* Run p->n_steps steps of the llg loop (fixed dt when p->dt_frac == 0; otherwise the
+ * adaptive dt of get_fields and the rejection loop of push_m, Heun only; each adaptive
+ * step reads two small reductions back to the host).  noise: MRMG_NOISE_INJECT only, host
  * array of standard normals laid out (3, G, n_steps), G = total grains over all layers in
  * layer order, in the order get_thermal consumes them (field.f90:405-431).  diag may be
  * NULL.  The call returns after the device work is complete. */

@@ -54,13 +54,15 @@ class UField(C.Structure):
 
 class StepParams(C.Structure):
     _fields_ = [("type", C.c_int), ("n_steps", C.c_int), ("t0", C.c_double), ("dt", C.c_double),
-                ("noise", C.c_int), ("pad_", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint64)]
+                ("noise", C.c_int), ("pad_", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint64),
+                ("dt_frac", C.c_double), ("dt_max", C.c_double), ("max_dm", C.c_double)]
 
 
 class Diag(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("steps", C.c_int), ("n_lay", C.c_int),
                 ("errmx", C.c_double), ("errmx1", C.c_double), ("mmag", C.c_double),
-                ("mave", (C.c_double * 3) * 8)]
+                ("mave", (C.c_double * 3) * 8), ("rejected", C.c_int), ("pad_", C.c_int),
+                ("max_dmdt", C.c_double)]
 
 
 class KStat(C.Structure):

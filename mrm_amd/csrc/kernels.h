@@ -50,7 +50,18 @@ enum GrainMode {
     GM_HEUN2 = 1,   // Heun corrector: field at mb, write ma = norm(ma + (d + dm)/2), h
     GM_ROT1 = 2,    // rotational push, single stage or mid-point first half: ma -> mb
     GM_ROT2 = 3,    // rotational mid-point second half: field at mb, push from ma -> ma
+    GM_FIELD1 = 4,  // adaptive dt, get_fields: field at ma + thermal draw, store h and hthm,
+                    // reduce max |h + hthm dti|^2 per layer; no push
+    GM_PUSH1 = 5,   // adaptive dt, Heun predictor from the stored h and hthm: ma -> mb, dm
 };
+
+// Diagnostics partials of one workgroup (doubles), per layer k < MRMG_MAX_LAYERS:
+#define D_ERRMX 0                              // max |m x H|^2 / Ms^2
+#define D_ERRMX1 (D_ERRMX + MRMG_MAX_LAYERS)   // max |dm|^2
+#define D_MMAG (D_ERRMX1 + MRMG_MAX_LAYERS)    // sum |m| ms_scale vol
+#define D_MAVE (D_MMAG + MRMG_MAX_LAYERS)      // sum vol m_c, 3 per layer
+#define D_HMAG (D_MAVE + 3 * MRMG_MAX_LAYERS)  // max |h + hthm dti|^2
+#define DIAG_STRIDE (D_HMAG + MRMG_MAX_LAYERS)
 
 struct StepArgs {
     double t;          // time of the field evaluation of this stage
@@ -62,4 +73,5 @@ struct StepArgs {
     unsigned long long seed, step;
     const double* normals;   // injected normals of this step, [G][3]
     int diag;          // write diagnostics partials
+    double dti;        // 1/sqrt(previous dt * max_dmdt), get_fields' |H| estimate
 };

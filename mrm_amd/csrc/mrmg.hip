@@ -476,15 +476,12 @@ __device__ __forceinline__ void rot_push(int type, T scale1, T scale2, const T* 
     }
 }
 
-// Diagnostics partial layout per block: [0] max errmx_l, [1] max errmx1_l,
-// [2 + k] sum |m| ms_scale vol for layer k, [2+NL+3k+c] sum vol m_c for layer k.
-#define DIAG_STRIDE (2 + 4 * MRMG_MAX_LAYERS)
 
 template <typename T, int MODE, int LPG>
 __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ ma, T* __restrict__ mb,
                                                T* __restrict__ dm, T* __restrict__ hthm, T* __restrict__ hout,
                                                const T* __restrict__ Hcell, T* __restrict__ Mcell, size_t CP,
-                                               StepArgs a, double* __restrict__ diag_part)
+                                               StepArgs a, double* __restrict__ diag_part, T* __restrict__ mc)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = tid / LPG, lane = tid % LPG;
@@ -496,7 +493,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         __syncthreads();
     }
     T h[3] = {0, 0, 0};
-    if (g < G) {
+    if (MODE != GM_PUSH1 && g < G) {
         const int k = gd.layer_of[g];
         const LayerParams& L = gd.lp[k];
         // the magnetisation the field is evaluated on
@@ -511,9 +508,10 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         const LayerParams& L = gd.lp[k];
         const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
         T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
+        if (MODE == GM_PUSH1) { h[0] = hout[g]; h[1] = hout[G + g]; h[2] = hout[2 * G + g]; }
         // thermal draw (get_thermal) happens with the first field of a step
         T th[3];
-        if (MODE == GM_HEUN1 || (MODE == GM_ROT1)) {
+        if (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1) {
             th[0] = th[1] = th[2] = 0;
             if (a.noise != 0 && L.thermal) {
                 double n[3];
@@ -530,7 +528,19 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         } else {
             th[0] = hthm[g]; th[1] = hthm[G + g]; th[2] = hthm[2 * G + g];
         }
-        if (MODE != GM_HEUN1) { hout[g] = h[0]; hout[G + g] = h[1]; hout[2 * G + g] = h[2]; }
+        if (MODE != GM_HEUN1 && MODE != GM_PUSH1) { hout[g] = h[0]; hout[G + g] = h[1]; hout[2 * G + g] = h[2]; }
+        if (MODE == GM_FIELD1) {
+            // get_fields, llg.f90:373-381: the step is chosen on the host from these maxima
+            if (a.diag) {
+                const T di = (T)a.dti;
+                T v[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) v[c] = h[c] + th[c] * di;
+                const double q = (double)dot3(v, v);
+                atomicMax(reinterpret_cast<unsigned long long*>(&red[D_HMAG + k]),
+                          (unsigned long long)__double_as_longlong(q));
+            }
+        } else {
         const T sq = (T)a.sqrt_dt;
         T H[3];
 #pragma unroll
@@ -541,7 +551,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
         const T scale2 = gd.alpha_scale[g] * (T)L.alpha;
         if (msc > (T)0) {
-            if (MODE == GM_HEUN1 || MODE == GM_HEUN2) {
+            if (MODE == GM_HEUN1 || MODE == GM_HEUN2 || MODE == GM_PUSH1) {
                 const T scale11 = (T)2 * scale1 / ((T)1 + scale2 * scale2);
                 const T* m = mfield;   // stage 1: m_in, stage 2: predicted m
                 T mt[3];
@@ -582,22 +592,23 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 for (int c = 0; c < 3; c++) mo[c] = mi[c];
             }
         }
-        T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1) ? mb : ma;
+        T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
         dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
         if (a.diag) {
             const T msv = gd.ms[g];
             const double e = msc > (T)0 ? (double)dot3(tq, tq) / ((double)msv * (double)msv) : 0.0;
             const double e1 = msc > (T)0 ? (double)dot3(d, d) : 0.0;
             const double vol = (double)gd.vol[g];
-            atomicMax(reinterpret_cast<unsigned long long*>(&red[0]), (unsigned long long)__double_as_longlong(e));
-            atomicMax(reinterpret_cast<unsigned long long*>(&red[1]), (unsigned long long)__double_as_longlong(e1));
-            atomicAdd(&red[2 + k], sqrt((double)dot3(mo, mo)) * (double)msc * vol);
+            atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
+            atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
+            atomicAdd(&red[D_MMAG + k], sqrt((double)dot3(mo, mo)) * (double)msc * vol);
 #pragma unroll
-            for (int c = 0; c < 3; c++) atomicAdd(&red[2 + MRMG_MAX_LAYERS + 3 * k + c], vol * (double)mo[c]);
+            for (int c = 0; c < 3; c++) atomicAdd(&red[D_MAVE + 3 * k + c], vol * (double)mo[c]);
+        }
         }
     }
     // the new magnetisation of the grain onto its cells, for the next field evaluation
-    if (Mcell && g < G) {
+    if (MODE != GM_FIELD1 && Mcell && g < G) {
         const int src = (threadIdx.x & 63) & ~(LPG - 1);
         T mb3[3];
 #pragma unroll
@@ -918,8 +929,8 @@ struct Engine : EngineBase {
     DBuf<T> xi_w, xu_w, xl_w;
     DBuf<T> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, vol;
     DBuf<LayerParams> lp;
-    // state
-    DBuf<T> ma, mb, dm, hthm, h;
+    // state (mc: corrector output of the adaptive path until the step is accepted)
+    DBuf<T> ma, mb, mc, dm, hthm, h;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
     DBuf<T> Hcell, Mcell;
@@ -991,7 +1002,7 @@ struct Engine : EngineBase {
         b += xi_w.bytes() + xu_w.bytes() + xl_w.bytes();
         b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
              hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + vol.bytes() + lp.bytes();
-        b += ma.bytes() + mb.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
+        b += ma.bytes() + mb.bytes() + mc.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
         b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
              slot2c.bytes();
         b += noise_buf.bytes() + diag_part.bytes();
@@ -1215,7 +1226,7 @@ struct Engine : EngineBase {
         RCHK(upT(cub, h_cub)); RCHK(upT(sigma, h_sig)); RCHK(upT(vol, h_vol));
         if (has_happ) RCHK(upT(happ, h_happ)); else happ.release();
         RCHK(upT(ma, h_m));
-        RCHK(mb.alloc(3 * (size_t)G)); RCHK(dm.alloc(3 * (size_t)G)); RCHK(hthm.alloc(3 * (size_t)G)); RCHK(h.alloc(3 * (size_t)G));
+        RCHK(mb.alloc(3 * (size_t)G)); RCHK(mc.alloc(3 * (size_t)G)); RCHK(dm.alloc(3 * (size_t)G)); RCHK(hthm.alloc(3 * (size_t)G)); RCHK(h.alloc(3 * (size_t)G));
         HIPCHK(hipMemsetAsync(mb.p, 0, mb.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(dm.p, 0, dm.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
@@ -1406,24 +1417,160 @@ struct Engine : EngineBase {
                            c->px, twx.p);
     }
 
-    template <int MODE, int LPG> void launch_grain(const StepArgs& a)
+    template <int MODE, int LPG> void launch_grain(const StepArgs& a, T* mout)
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const T* hc = demag_on() ? Hcell.p : nullptr;
         const size_t threads = (size_t)G * LPG;
         hipLaunchKernelGGL((k_grain<T, MODE, LPG>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
                            gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
-                           diag_part.p);
+                           diag_part.p, mout);
     }
-    template <int MODE> int grain(const StepArgs& a)
+    // mout: destination of the corrector (null: ma)
+    template <int MODE> int grain(const StepArgs& a, T* mout = nullptr)
     {
-        RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1) ? K_GRAIN1 : K_GRAIN2));
-        if (lpg == 4) launch_grain<MODE, 4>(a);
-        else if (lpg == 16) launch_grain<MODE, 16>(a);
-        else launch_grain<MODE, 8>(a);
+        RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1 || MODE == GM_PUSH1) ? K_GRAIN1
+                                                                                                 : K_GRAIN2));
+        if (lpg == 4) launch_grain<MODE, 4>(a, mout);
+        else if (lpg == 16) launch_grain<MODE, 16>(a, mout);
+        else launch_grain<MODE, 8>(a, mout);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         return MRMG_OK;
+    }
+
+    // per-layer reductions of the last diag-enabled grain launch
+    struct LayerRed {
+        std::vector<double> errmx, errmx1, mmag, mave, hmag;
+    };
+    int read_partials(LayerRed& r)
+    {
+        const int nb = (int)(((size_t)G * lpg + 255) / 256), nl = c->nl;
+        std::vector<double> part((size_t)nb * DIAG_STRIDE);
+        HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+        r.errmx.assign(nl, 0.0); r.errmx1.assign(nl, 0.0); r.mmag.assign(nl, 0.0); r.hmag.assign(nl, 0.0);
+        r.mave.assign(3 * nl, 0.0);
+        for (int b = 0; b < nb; b++) {
+            const double* q = &part[(size_t)b * DIAG_STRIDE];
+            for (int k = 0; k < nl; k++) {
+                r.errmx[k] = std::max(r.errmx[k], q[D_ERRMX + k]);
+                r.errmx1[k] = std::max(r.errmx1[k], q[D_ERRMX1 + k]);
+                r.hmag[k] = std::max(r.hmag[k], q[D_HMAG + k]);
+                r.mmag[k] += q[D_MMAG + k];
+                for (int cc = 0; cc < 3; cc++) r.mave[3 * k + cc] += q[D_MAVE + 3 * k + cc];
+            }
+        }
+        return MRMG_OK;
+    }
+    int zero_partials()
+    {
+        const int nb = (int)(((size_t)G * lpg + 255) / 256);
+        HIPCHK(hipMemsetAsync(diag_part.p, 0, (size_t)nb * DIAG_STRIDE * sizeof(double), c->stream));
+        return MRMG_OK;
+    }
+
+    // The llg() loop with get_fields' adaptive dt (llg.f90:320-388) and push_m's rejection
+    // of steps that rotate a layer by more than max_dm (llg.f90:1195-1271), Heun only.
+    int step_adaptive(const mrmg_step_params* p, const double* noise, mrmg_diag* diag)
+    {
+        const int nl = c->nl;
+        const size_t per = 3 * (size_t)G;
+        const double max_dm = p->max_dm * acos(-1.0) / 180.0;
+        if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per));
+        HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
+        if (demag_on()) RCHK(scatter(ma.p));
+        double t = p->t0, dt = p->dt, max_dmdt = 1.0, max_dmdtnext = 1.0;
+        int rejected = 0;
+        LayerRed r;
+        StepArgs a{};
+        a.type = p->type;
+        a.noise = p->noise;
+        a.seed = p->seed;
+        for (int it = 0; it < p->n_steps; it++) {
+            a.step = p->step0 + it;
+            a.normals = nullptr;
+            if (p->noise == MRMG_NOISE_INJECT) {
+                HIPCHK(hipMemcpyAsync(noise_buf.p, noise + per * it, sizeof(double) * per, hipMemcpyHostToDevice,
+                                      c->stream));
+                a.normals = noise_buf.p;
+            }
+            // get_fields(t)
+            if (demag_on()) RCHK(demag());
+            a.t = t;
+            a.dti = 1.0 / sqrt(dt * max_dmdt);
+            a.diag = 1;
+            RCHK(zero_partials());
+            RCHK(grain<GM_FIELD1>(a));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            RCHK(read_partials(r));
+            double nd = 1.e22;
+            for (int k = 0; k < nl; k++) {
+                const HostLayer& H = c->layers[k];
+                const double hmag = sqrt(r.hmag[k]);
+                const double fact1 = (1.0 + H.alpha * H.alpha) / ((1.0 + H.alpha) * H.gamma) * p->dt_frac;
+                const double fact0 = fact1 != 0.0 ? 0.0 : 1.0;
+                const double cand = fact1 / std::max(1.0, hmag) + fact0 * nd;
+                nd = std::min(std::min(nd, p->dt_max), cand);
+            }
+            dt = nd;
+            // push_m(t)
+            for (;;) {
+                const double dtt = dt * max_dmdt;
+                a.dt = dtt; a.dts = dtt; a.sqrt_dt = sqrt(dtt); a.diag = 0;
+                RCHK(grain<GM_PUSH1>(a));
+                if (demag_on()) RCHK(demag());
+                a.t = t + dtt; a.diag = 1;
+                RCHK(zero_partials());
+                RCHK(grain<GM_HEUN2>(a, mc.p));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                RCHK(read_partials(r));
+                bool too_far = false;
+                max_dmdtnext = 1.e22;
+                for (int k = 0; k < nl; k++) {
+                    const double rot = acos(1.0 - 0.5 * r.errmx1[k]) / max_dm;
+                    if (rot > 1.0) {
+                        max_dmdt = 0.9 * max_dmdt / rot;
+                        too_far = true;
+                        break;
+                    }
+                    if (rot < 2.0 / 3.0) max_dmdtnext = std::min(max_dmdtnext, std::min(2.0, max_dmdt / (rot + 1.e-200)));
+                    else max_dmdtnext = std::min(max_dmdtnext, max_dmdt);
+                }
+                if (!too_far) break;
+                rejected++;
+                if (rejected > 1000000) return set_err(MRMG_ERR_STATE, "push_m: no acceptable step");
+            }
+            ma.swap_with(mc);
+            if (max_dmdtnext > 1.e20) max_dmdtnext = max_dmdt;
+            dt = dt * max_dmdt;
+            max_dmdt = max_dmdtnext;
+            t = t + dt;
+        }
+        if (diag) {
+            memset(diag, 0, sizeof(*diag));
+            fill_diag(diag, r, t, dt, p->n_steps);
+            diag->rejected = rejected;
+            diag->max_dmdt = max_dmdt;
+        }
+        return MRMG_OK;
+    }
+
+    void fill_diag(mrmg_diag* d, const LayerRed& r, double t, double dt, int steps)
+    {
+        d->t = t; d->dt = dt; d->steps = steps; d->n_lay = c->nl;
+        if (steps <= 0) return;
+        double e = 0, e1 = 0;
+        for (int k = 0; k < c->nl; k++) {
+            e = std::max(e, sqrt(r.errmx[k]));
+            e1 = std::max(e1, r.errmx1[k]);
+            double sv = 0;
+            for (int n = 0; n < c->layers[k].ng; n++) sv += c->layers[k].vol[n];
+            d->mmag += r.mmag[k] / sv;
+            if (k < 8)
+                for (int cc = 0; cc < 3; cc++) d->mave[k][cc] = r.mave[3 * k + cc] / sv;
+        }
+        d->errmx = e;
+        d->errmx1 = e1;
     }
 
     int step(const mrmg_step_params* p, const double* noise, mrmg_diag* diag) override
@@ -1433,6 +1580,11 @@ struct Engine : EngineBase {
             return set_err(MRMG_ERR_UNSUPPORTED, "llg_type %d not implemented", type);
         if (p->dt <= 0.0) return set_err(MRMG_ERR_ARG, "dt must be > 0");
         if (p->noise == MRMG_NOISE_INJECT && !noise) return set_err(MRMG_ERR_ARG, "noise array is NULL");
+        if (p->dt_frac != 0.0) {
+            if (type != -3) return set_err(MRMG_ERR_UNSUPPORTED, "adaptive dt is implemented for Heun (llg_type -3) only");
+            if (p->max_dm <= 0.0 || p->dt_max <= 0.0) return set_err(MRMG_ERR_ARG, "adaptive dt needs max_dm > 0, dt_max > 0");
+            return step_adaptive(p, noise, diag);
+        }
         // llg() start: init_applied zeroes hthm (field.f90:703)
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         // the cells must hold the current state before the first field evaluation
@@ -1461,6 +1613,7 @@ struct Engine : EngineBase {
                 }
                 a.normals = noise_buf.p + per * ci;
             }
+            if (last) RCHK(zero_partials());
             if (type == -3) {
                 // stage 1 (get_fields at t, push iter=2)
                 if (demag_on()) RCHK(demag());
@@ -1491,7 +1644,14 @@ struct Engine : EngineBase {
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         if (prof) RCHK(ev_collect());
-        if (diag) RCHK(read_diag(diag, t, dt, p->n_steps));
+        if (diag) {
+            memset(diag, 0, sizeof(*diag));
+            LayerRed r;
+            if (p->n_steps > 0) RCHK(read_partials(r));
+            else { r.errmx.assign(c->nl, 0); r.errmx1.assign(c->nl, 0); r.mmag.assign(c->nl, 0); r.mave.assign(3 * c->nl, 0); }
+            fill_diag(diag, r, t, dt, p->n_steps);
+            diag->max_dmdt = 1.0;
+        }
         return MRMG_OK;
     }
 
@@ -1511,37 +1671,6 @@ struct Engine : EngineBase {
             strncpy(out[i].name, names[i], sizeof(out[i].name) - 1);
             out[i].launches = kt_n[i];
             out[i].total_ms = kt_ms[i];
-        }
-        return MRMG_OK;
-    }
-
-    int read_diag(mrmg_diag* d, double t, double dt, int steps)
-    {
-        memset(d, 0, sizeof(*d));
-        d->t = t; d->dt = dt; d->steps = steps; d->n_lay = c->nl;
-        if (steps <= 0) return MRMG_OK;
-        const int nb = (int)(((size_t)G * lpg + 255) / 256);
-        std::vector<double> part((size_t)nb * DIAG_STRIDE);
-        HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-        double e = 0, e1 = 0;
-        std::vector<double> mm(c->nl, 0.0), mv(3 * c->nl, 0.0);
-        for (int b = 0; b < nb; b++) {
-            const double* q = &part[(size_t)b * DIAG_STRIDE];
-            e = std::max(e, q[0]);
-            e1 = std::max(e1, q[1]);
-            for (int k = 0; k < c->nl; k++) {
-                mm[k] += q[2 + k];
-                for (int cc = 0; cc < 3; cc++) mv[3 * k + cc] += q[2 + MRMG_MAX_LAYERS + 3 * k + cc];
-            }
-        }
-        d->errmx = sqrt(e);
-        d->errmx1 = e1;
-        for (int k = 0; k < c->nl; k++) {
-            double sv = 0;
-            for (int n = 0; n < c->layers[k].ng; n++) sv += c->layers[k].vol[n];
-            d->mmag += mm[k] / sv;
-            if (k < 8)
-                for (int cc = 0; cc < 3; cc++) d->mave[k][cc] = mv[3 * k + cc] / sv;
         }
         return MRMG_OK;
     }

@@ -48,6 +48,7 @@ module mrmg_m
     real(c_double) :: t0, dt
     integer(c_int) :: noise, pad_
     integer(c_int64_t) :: seed, step0
+    real(c_double) :: dt_frac, dt_max, max_dm
   end type
 
   type, bind(C) :: mrmg_diag
@@ -55,6 +56,8 @@ module mrmg_m
     integer(c_int) :: steps, n_lay
     real(c_double) :: errmx, errmx1, mmag
     real(c_double) :: mave(3, 8)
+    integer(c_int) :: rejected, pad_
+    real(c_double) :: max_dmdt
   end type
 
   interface

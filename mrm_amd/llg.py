@@ -34,6 +34,9 @@ class Llg:
         self.t_min = -1e22
         self.min_it = 10
         self.step_index = 0
+        self.dt_frac = 0.0      # 0: fixed dt
+        self.dt_max = 1e22
+        self.max_dm = 180.0     # degrees
 
     # -- setup ---------------------------------------------------------------
     def close(self):
@@ -106,8 +109,18 @@ class Llg:
         return b.value
 
     # -- integrator ----------------------------------------------------------
-    def set_llg(self, dt=None, llg_type=None, del_t=None, t_max=None, t_min=None, min_it=None):
-        """set_llg (llg.f90:1452-1487) restricted to the fixed-dt form."""
+    def set_llg(self, dt=None, llg_type=None, del_t=None, t_max=None, t_min=None, min_it=None, dt_frac=None,
+                dt_max=None, max_dm=None, fixed_dt_p=None):
+        """set_llg (llg.f90:1452-1487): fixed dt unless dt_frac != 0 (then get_fields picks
+        dt and push_m rejects steps rotating more than max_dm degrees)."""
+        if dt_frac is not None:
+            self.dt_frac = float(dt_frac)
+        if fixed_dt_p:
+            self.dt_frac = 0.0
+        if dt_max is not None:
+            self.dt_max = float(dt_max)
+        if max_dm is not None:
+            self.max_dm = float(max_dm)
         if dt is not None:
             self.dt = float(dt)
         if llg_type is not None:
@@ -135,7 +148,8 @@ class Llg:
         """Run n_steps fixed-dt steps from time t0.  noise: (n_steps, G, 3) normals to inject,
         or philox_seed to draw them on the device."""
         p = abi.StepParams(type=self.type, n_steps=int(n_steps), t0=float(t0), dt=self.dt,
-                           noise=abi.MRMG_NOISE_NONE, seed=0, step0=self.step_index)
+                           noise=abi.MRMG_NOISE_NONE, seed=0, step0=self.step_index,
+                           dt_frac=self.dt_frac, dt_max=self.dt_max, max_dm=self.max_dm)
         nz = None
         if noise is not None:
             nz = np.ascontiguousarray(noise, dtype=np.float64)
@@ -147,6 +161,8 @@ class Llg:
         d = abi.Diag() if diag else None
         abi.check(self.lib, self.lib.mrmg_step(self.ctx, C.byref(p), abi.dptr(nz), C.byref(d) if d else None))
         self.step_index += int(n_steps)
+        if d is not None and self.dt_frac != 0.0:
+            self.dt = d.dt          # llgp%dt carries over to the next call
         return d
 
     def llg(self, t, noise=None, philox_seed=None):

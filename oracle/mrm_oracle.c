@@ -422,6 +422,7 @@ typedef struct ora_sim {
     int in;        /* double-buffer toggle shared by all layers (0 or 3) */
     /* llgp_s */
     double dt, max_dmdt, max_dmdtnext, errmx, errmx1, mmag;
+    int rejected;
     double *hcell; /* (nx,ny,nl) x3 */
     double tsum_dummy;
 } ora_sim;
@@ -874,7 +875,7 @@ static void push_m(ora_sim *s, double t0, int type, int fixed)
                 if (iter == 1) {
                     if (fixed) { rot_ratio = 1.0; too_far = 0; }
                     else { rot_ratio = acos(1.0 - 0.5 * errmx1_l) / L->max_dm; too_far = rot_ratio > 1.0; }
-                    if (too_far) { s->max_dmdt = 0.9 * s->max_dmdt / rot_ratio; break; }
+                    if (too_far) { s->max_dmdt = 0.9 * s->max_dmdt / rot_ratio; s->rejected++; break; }
                     double sv = 0.0; for (int n = 0; n < L->ng; n++) sv += L->vol[n];
                     s->mmag = s->mmag + mmag1 / sv;
                     if (rot_ratio < 2.0 / 3.0) {
@@ -920,11 +921,14 @@ static void mave_layers(ora_sim *s, mrmg_diag *d)
 /* The llg() loop body (llg.f90:212-285) for p->n_steps steps.  Fixed dt when
  * dt_frac == 0 (set_llg, llg.f90:1484-1486); otherwise get_fields' dt control and
  * push_m's rejection/next-dt logic.  noise: (3,G,n_steps) standard normals or NULL. */
-int ora_step_ex(ora_sim *s, const mrmg_step_params *p, double dt_frac, double dt_max,
-                const double *noise, mrmg_diag *diag)
+int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_diag *diag)
 {
     int G = 0; for (int k = 0; k < s->nl; k++) G += s->L[k].ng;
+    const double dt_frac = p->dt_frac, dt_max = p->dt_frac != 0.0 ? p->dt_max : 1.e22;
     int fixed = (dt_frac == 0.0);
+    if (!fixed && p->max_dm > 0.0)
+        for (int k = 0; k < s->nl; k++) s->L[k].max_dm = p->max_dm * acos(-1.0) / 180.0;
+    s->rejected = 0;
     double t = p->t0;
     /* llg() start: init_applied zeroes hthm (field.f90:703); max_dmdt = 1 (llg.f90:206) */
     for (int k = 0; k < s->nl; k++) memset(s->L[k].hthm, 0, sizeof(double) * 3 * s->L[k].ng);
@@ -959,14 +963,10 @@ int ora_step_ex(ora_sim *s, const mrmg_step_params *p, double dt_frac, double dt
         memset(diag, 0, sizeof(*diag));
         diag->t = t; diag->dt = s->dt; diag->steps = p->n_steps; diag->n_lay = s->nl;
         diag->errmx = s->errmx; diag->errmx1 = s->errmx1; diag->mmag = s->mmag;
+        diag->rejected = s->rejected; diag->max_dmdt = s->max_dmdt;
         mave_layers(s, diag);
     }
     return MRMG_OK;
-}
-
-int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_diag *diag)
-{
-    return ora_step_ex(s, p, 0.0, 1.e22, noise, diag);
 }
 
 /* Demag field of the current state (cells and grain averages) */

@@ -56,8 +56,6 @@ def lib():
             "ora_set_state": ([_cp, C.c_int, _dp], C.c_int),
             "ora_get_state": ([_cp, C.c_int, _dp, _dp], C.c_int),
             "ora_step": ([_cp, C.POINTER(abi.StepParams), _dp, C.POINTER(abi.Diag)], C.c_int),
-            "ora_step_ex": ([_cp, C.POINTER(abi.StepParams), C.c_double, C.c_double, _dp,
-                             C.POINTER(abi.Diag)], C.c_int),
             "ora_demag_field": ([_cp, _dp, _dp], C.c_int),
             "ora_num_threads": ([], C.c_int),
         }
@@ -212,12 +210,14 @@ class Sim:
         lib().ora_get_state(self.h, k, abi.dptr(m), abi.dptr(h))
         return m, h
 
-    def step(self, n_steps, dt, t0=0.0, type=abi.MRMG_TYPE_HEUN, noise=None, dt_frac=0.0, dt_max=1e22):
+    def step(self, n_steps, dt, t0=0.0, type=abi.MRMG_TYPE_HEUN, noise=None, dt_frac=0.0, dt_max=1e22,
+             max_dm=180.0):
         p = abi.StepParams(type=type, n_steps=n_steps, t0=t0, dt=dt,
-                           noise=abi.MRMG_NOISE_INJECT if noise is not None else abi.MRMG_NOISE_NONE)
+                           noise=abi.MRMG_NOISE_INJECT if noise is not None else abi.MRMG_NOISE_NONE,
+                           dt_frac=dt_frac, dt_max=dt_max, max_dm=max_dm)
         d = abi.Diag()
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
-        lib().ora_step_ex(self.h, C.byref(p), dt_frac, dt_max, abi.dptr(nz), C.byref(d))
+        lib().ora_step(self.h, C.byref(p), abi.dptr(nz), C.byref(d))
         return d
 
     def demag_field(self):

@@ -57,7 +57,8 @@ int main(void) {
   F(mrmg_layer, floor_ms) F(mrmg_layer, thermal)
   F(mrmg_ufield, t_start) F(mrmg_ufield, scale)
   F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
-  F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave)
+  F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm)
+  F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave) F(mrmg_diag, rejected) F(mrmg_diag, max_dmdt)
   return 0;
 }
 """

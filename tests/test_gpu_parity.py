@@ -281,6 +281,35 @@ def test_state_survives_layer_rebuild(oracle_mod):
     assert np.allclose(after[2], [0, 0, 1])
 
 
+@pytest.mark.parametrize("max_dm", [180.0, 1.5])
+def test_adaptive_dt_parity(oracle_mod, max_dm):
+    """Adaptive dt (get_fields) and push_m's rejection/retry on the GPU against the oracle:
+    same accepted steps, rejections, dt and trajectory (thermal, injected MT noise)."""
+    say(f"adaptive dt parity max_dm={max_dm}")
+    med = media.config_media("cfg1", nx=64, ny=64, thm_alpha=1.0)
+    s, gpu = pair(oracle_mod, med, ufields=media.ramp_field(4))
+    for k, m in enumerate(random_state(med, 4, tilt=0.2)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    n = 12
+    noise = oracle_mod.mt_noise([34, 35, 36, 37], med.ngs, [True] * 4, 2 * n)
+    gpu.set_llg(dt=1e-14, dt_frac=0.2, dt_max=1e-12, max_dm=max_dm)
+    t, dt, rejected = 0.0, 1e-14, 0
+    for c in range(2):
+        do = s.step(n, dt, t0=t, noise=noise[c * n:(c + 1) * n], dt_frac=0.2, dt_max=1e-12, max_dm=max_dm)
+        dg = gpu.step(n, t0=t, noise=noise[c * n:(c + 1) * n])
+        assert do.rejected == dg.rejected
+        assert abs(do.dt - dg.dt) <= 1e-12 * do.dt and abs(do.t - dg.t) <= 1e-12 * do.t
+        t, dt, rejected = do.t, do.dt, rejected + do.rejected
+    if max_dm < 10:
+        assert rejected > 0
+    assert max_dm_diff(s, gpu) <= 1e-10
+
+
+def max_dm_diff(s, gpu):
+    return max_dm(s, gpu, gpu.n_lay)
+
+
 def test_launch_shapes_do_not_change_results():
     """Row/column launch shapes give bit-identical trajectories; lanes per grain only
     reorder the gather sums (last bits).  The autotuner restores the state it steps."""

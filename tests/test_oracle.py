@@ -384,3 +384,42 @@ def test_fixed_dt_time_accumulates_like_reference(oracle_mod):
     for _ in range(37):
         t = t + 5e-14
     assert d.t == t
+
+
+def test_adaptive_dt_closed_form(oracle_mod):
+    """get_fields (llg.f90:384-386): with dt_frac set, a grain in a uniform field H takes
+    dt = (1+a^2)/((1+a) gamma) dt_frac / max(1, |H|) (capped by dt_max)."""
+    O = oracle_mod
+    alpha, H, dt_frac = 0.1, 8000.0, 0.2
+    g, L = _single_grain(alpha=alpha)
+    s = O.Sim(g)
+    s.set_layer(0, L)
+    s.set_ufield(0, abi.make_ufield((0, 0, 1), scale=H))
+    s.set_state(0, np.array([[0.6, 0.0, 0.8]]))
+    d = s.step(1, 1e-14, t0=0.0, dt_frac=dt_frac, dt_max=1.0)
+    expect = (1 + alpha * alpha) / ((1 + alpha) * 1.76e7) * dt_frac / H
+    assert abs(d.dt - expect) <= 1e-15 * expect and d.t == d.dt
+    d = s.step(1, d.dt, t0=d.t, dt_frac=dt_frac, dt_max=2e-13)
+    assert d.dt == 2e-13          # dt_max caps it
+
+
+def test_adaptive_rejection_bounds_rotation(oracle_mod):
+    """push_m (llg.f90:1229-1236) retries with a smaller step until no layer rotates more
+    than max_dm; every accepted step stays within it."""
+    O = oracle_mod
+    max_dm = 2.0
+    g, L = _single_grain(alpha=0.05)
+    s = O.Sim(g)
+    s.set_layer(0, L)
+    s.set_ufield(0, abi.make_ufield((0, 0, 1), scale=20000.0))
+    m0 = np.array([[1.0, 0.0, 0.0]])
+    s.set_state(0, m0)
+    t, dt, rej = 0.0, 1e-14, 0
+    for _ in range(30):
+        before = s.get_state(0)[0][0]
+        d = s.step(1, dt, t0=t, dt_frac=0.5, dt_max=1.0, max_dm=max_dm)
+        t, dt, rej = d.t, d.dt, rej + d.rejected
+        after = s.get_state(0)[0][0]
+        ang = np.degrees(np.arccos(np.clip(before @ after, -1, 1)))
+        assert ang <= max_dm * 1.01
+    assert rej > 0
