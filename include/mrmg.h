@@ -100,6 +100,13 @@ typedef struct {
     double thm_alpha;                /* alpha used by get_thermal (data_s%thm_alpha)       */
     double floor_ms;                 /* lower bound of |m| in the thermal variance         */
     int thermal;                     /* thermal_p                                          */
+    /* LLB and temperature scaling (push_llb llg.f90:916-1065, temp_scaling.f90:365-431) */
+    int use_mag;                     /* Ms scale = clamp(|m|, 0, 1) every field (use_mag_p) */
+    const double *tc;                /* (ng) Curie temperature (K); NULL means 1e22         */
+    const double *lambda_ms;         /* (ng) lambdaMs; NULL: 3 Tc / (g muB/kB (S+1))        */
+    double spin;                     /* spin S of the Brillouin/Langevin mean field         */
+    double tc_nom;                   /* temperature scaling on when tc_nom <= 1e10          */
+    double msexp, hkexp;             /* Ms, Hk ~ (1 - T/Tc)^exp when >= 0                   */
 } mrmg_layer;
 
 /* Uniform, time-ramped applied field of one layer: the single uniform field branch of
@@ -126,6 +133,8 @@ typedef struct {
     double dt_max;            /* upper bound of the adaptive dt (s)                     */
     double max_dm;            /* max rotation per step (degrees) before push_m rejects
                                  and retries the step (llg.f90:1229-1236)               */
+    int llb;                  /* llgp%llb: Landau-Lifshitz-Bloch push (Heun)              */
+    int pad2_;
 } mrmg_step_params;
 
 /* Diagnostics of the last step (llg.f90:1209-1257, 217-237). */
@@ -166,6 +175,9 @@ int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);
 int mrmg_set_state(mrmg_ctx *ctx, int k, const double *m);
 /* m (3,ng) and the last field h (3,ng) (either may be NULL). */
 int mrmg_get_state(mrmg_ctx *ctx, int k, double *m, double *h);
+/* |m| of layer k (ng): data_s%mag(in/3+1,:), the LLB magnitude (1 for LLG). */
+int mrmg_set_mag(mrmg_ctx *ctx, int k, const double *mag);
+int mrmg_get_mag(mrmg_ctx *ctx, int k, double *mag);
 
 /* Run p->n_steps steps of the llg loop (fixed dt when p->dt_frac == 0; otherwise the
  * adaptive dt of get_fields and the rejection loop of push_m, Heun only; each adaptive

@@ -44,7 +44,9 @@ class Layer(C.Structure):
                 ("temp", _dp), ("h_app", _dp),
                 ("alpha", C.c_double), ("gamma", C.c_double),
                 ("thm_alpha", C.c_double), ("floor_ms", C.c_double),
-                ("thermal", C.c_int)]
+                ("thermal", C.c_int), ("use_mag", C.c_int),
+                ("tc", _dp), ("lambda_ms", _dp), ("spin", C.c_double), ("tc_nom", C.c_double),
+                ("msexp", C.c_double), ("hkexp", C.c_double)]
 
 
 class UField(C.Structure):
@@ -55,7 +57,8 @@ class UField(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("type", C.c_int), ("n_steps", C.c_int), ("t0", C.c_double), ("dt", C.c_double),
                 ("noise", C.c_int), ("pad_", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint64),
-                ("dt_frac", C.c_double), ("dt_max", C.c_double), ("max_dm", C.c_double)]
+                ("dt_frac", C.c_double), ("dt_max", C.c_double), ("max_dm", C.c_double),
+                ("llb", C.c_int), ("pad2_", C.c_int)]
 
 
 class Diag(C.Structure):
@@ -93,7 +96,7 @@ class LayerArrays:
     """
 
     FIELDS_D = ("grain_vol", "ms", "hk", "uni_axis", "cubic_axis", "hx", "uhx", "lhx",
-                "ms_scale", "alpha_scale", "hk_scale", "temp", "h_app")
+                "ms_scale", "alpha_scale", "hk_scale", "temp", "h_app", "tc", "lambda_ms")
     FIELDS_I = ("grid2grain", "grain2gridindex", "grain2grid", "hxgrainindex", "hxgrain",
                 "uhxgrainindex", "uhxgrain", "lhxgrainindex", "lhxgrain")
 
@@ -110,6 +113,11 @@ class LayerArrays:
         self.thm_alpha = float(kw.get("thm_alpha", 1.0))
         self.floor_ms = float(kw.get("floor_ms", 0.01))
         self.thermal = int(bool(kw.get("thermal", False)))
+        self.use_mag = int(bool(kw.get("use_mag", False)))
+        self.spin = float(kw.get("spin", 0.5))
+        self.tc_nom = float(kw.get("tc_nom", 1e22))
+        self.msexp = float(kw.get("msexp", -1.0))
+        self.hkexp = float(kw.get("hkexp", -1.0))
 
     def struct(self):
         s = Layer()
@@ -120,6 +128,7 @@ class LayerArrays:
             setattr(s, f, iptr(getattr(self, f)))
         s.alpha, s.gamma, s.thm_alpha, s.floor_ms = self.alpha, self.gamma, self.thm_alpha, self.floor_ms
         s.thermal = self.thermal
+        s.use_mag, s.spin, s.tc_nom, s.msexp, s.hkexp = self.use_mag, self.spin, self.tc_nom, self.msexp, self.hkexp
         return s
 
 
@@ -167,6 +176,8 @@ def load(path=None):
         ("mrmg_set_ufield", [C.c_void_p, C.c_int, C.POINTER(UField)]),
         ("mrmg_set_state", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_get_state", [C.c_void_p, C.c_int, _dp, _dp]),
+        ("mrmg_set_mag", [C.c_void_p, C.c_int, _dp]),
+        ("mrmg_get_mag", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_step", [C.c_void_p, C.POINTER(StepParams), _dp, C.POINTER(Diag)]),
         ("mrmg_philox_normals", [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
         ("mrmg_demag_field", [C.c_void_p, _dp, _dp]),

@@ -2,6 +2,9 @@
 //
 // Device data layout (G = grains over all layers in layer order, P = nx*ny cells per layer):
 //   grain vectors      component-major, v[c*G + g]                 (coalesced per component)
+//   grain state        5 rows of G: m_x, m_y, m_z, |m| (data_s%mag), and the Ms scale the
+//                      field sees (ms_scale, or clamp(|m|) with use_mag_p); the double
+//                      buffer swaps move |m| with m
 //   cell2grain         [k][y][x] -> global grain or -1               (get_mag scatter, data.f90:344-353)
 //   spectra S1, S2     [k][c][y][kx], kx = 0..nx/2                   (x = halved axis, fftw.f90:66-84)
 //   Hcell              [c][k][y][x]                                   (rx2d, ry2d, rz2d of demag.f90)
@@ -16,9 +19,16 @@
 struct LayerParams {
     double alpha, gamma;
     double uf_dir[3], uf_t0, uf_t1, uf_s0, uf_s1, uf_scale;
+    double spin;          // S of the LLB mean field
     int has_uf, has_uni, has_cub, thermal;
     int off, ng;
+    int use_mag, pad_;    // Ms scale = clamp(|m|) (use_mag_p with tc_nom <= 1e10)
 };
+
+// rows of the grain state buffers
+#define ST_MAG 3
+#define ST_MSC 4
+#define ST_ROWS 5
 
 // Per-grain static data of the field assembly and push.
 template <typename T> struct GrainData {
@@ -42,6 +52,10 @@ template <typename T> struct GrainData {
     const int* xu_off; const int* xu_nb; const T* xu_w;   // to the layer above
     const int* xl_off; const int* xl_nb; const T* xl_w;   // to the layer below
     const LayerParams* lp;
+    // LLB (FP64 in both forms): Tc, lambda = lambdaMs (S+1)/S, grain temperature
+    const double* tc;
+    const double* lam;
+    const double* temp;
 };
 
 // The grain kernel modes
@@ -74,4 +88,5 @@ struct StepArgs {
     const double* normals;   // injected normals of this step, [G][3]
     int diag;          // write diagnostics partials
     double dti;        // 1/sqrt(previous dt * max_dmdt), get_fields' |H| estimate
+    int llb;           // push_llb instead of push_llg
 };

@@ -296,15 +296,17 @@ __device__ __forceinline__ T lane_sum(T v)
     return v;
 }
 
+// neighbour terms hx(j) ms_scale(nb) m(nb); the Ms scale is the state's row ST_MSC
 template <typename T, int LPG>
 __device__ __forceinline__ void csr_sum(const int* __restrict__ off, const int* __restrict__ nb_,
-                                        const T* __restrict__ w_, const T* __restrict__ ms_scale,
-                                        const T* __restrict__ mf, int G, int g, int lane, T* h)
+                                        const T* __restrict__ w_, const T* __restrict__ mf, int G, int g,
+                                        int lane, T* h)
 {
     T s0 = 0, s1 = 0, s2 = 0;
+    const T* __restrict__ msc = mf + (size_t)ST_MSC * G;
     for (int j = off[g] + lane; j < off[g + 1]; j += LPG) {
         const int nb = nb_[j];
-        const T w = w_[j] * ms_scale[nb];
+        const T w = w_[j] * msc[nb];
         s0 += w * mf[nb];
         s1 += w * mf[G + nb];
         s2 += w * mf[2 * G + nb];
@@ -378,9 +380,9 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         for (int i = 0; i < 3; i++) h[i] = h[i] + s * u[i];
     }
     // exchange CSRs (field.f90:326-363): intralayer, layer above, layer below
-    if (gd.xi_off) csr_sum<T, LPG>(gd.xi_off, gd.xi_nb, gd.xi_w, gd.ms_scale, mf, gd.G, g, lane, h);
-    if (gd.xu_off) csr_sum<T, LPG>(gd.xu_off, gd.xu_nb, gd.xu_w, gd.ms_scale, mf, gd.G, g, lane, h);
-    if (gd.xl_off) csr_sum<T, LPG>(gd.xl_off, gd.xl_nb, gd.xl_w, gd.ms_scale, mf, gd.G, g, lane, h);
+    if (gd.xi_off) csr_sum<T, LPG>(gd.xi_off, gd.xi_nb, gd.xi_w, mf, gd.G, g, lane, h);
+    if (gd.xu_off) csr_sum<T, LPG>(gd.xu_off, gd.xu_nb, gd.xu_w, mf, gd.G, g, lane, h);
+    if (gd.xl_off) csr_sum<T, LPG>(gd.xl_off, gd.xl_nb, gd.xl_w, mf, gd.G, g, lane, h);
 }
 
 // get_mag (data.f90:344-353): M = Ms*ms_scale*m on every cell of grain g, lanes split the
@@ -406,7 +408,178 @@ __global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, const T* __res
     const int g = tid / LPG, lane = tid % LPG;
     if (g >= gd.G) return;
     const T mm[3] = {m[g], m[gd.G + g], m[2 * gd.G + g]};
-    scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, gd.msf[g], mm);
+    const T f = gd.lp[gd.layer_of[g]].use_mag ? gd.ms[g] * m[(size_t)ST_MSC * gd.G + g] : gd.msf[g];
+    scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, f, mm);
+}
+
+// ------------------------------------------------------------------------- LLB push
+// Langevin function and its derivative (llg.f90:704-710, 726-733)
+__device__ __forceinline__ double llb_lv(double x) { return fabs(x) > 1.e-2 ? 1.0 / tanh(x) - 1.0 / x : x / 3.0; }
+__device__ __forceinline__ double llb_dlv(double x)
+{
+    if (fabs(x) > 4.e-3) {
+        const double b = sinh(x);
+        return 1.0 / (x * x) - 1.0 / (b * b);
+    }
+    return 1.0 / 3.0;
+}
+__device__ __forceinline__ double dot3d(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void cross3d(const double* a, const double* b, double* t)
+{
+    t[0] = a[1] * b[2] - a[2] * b[1];
+    t[1] = a[2] * b[0] - a[0] * b[2];
+    t[2] = a[0] * b[1] - a[1] * b[0];
+}
+#define GMUB_KB 1.343427768e-4   // g muB / kB (llg.f90:925)
+
+struct LlbGrain {
+    double T, Tc, lam, spin, alpha, as, ms;
+};
+
+// push_llb (llg.f90:916-1065) for one grain, in FP64.  stage2: the corrector (Heun) or
+// second half (mid-point) that starts from the predicted state mp/magp.  mi/magi: state
+// at the start of the step; ph: field without the thermal part; th: the thermal draw.
+// Heun's dm is read (stage 2) or written (stage 1) through dmv.  Returns the new
+// direction and |m|, and the errmx/errmx1 contributions.
+__device__ void llb_push(const LlbGrain& q, int type, bool stage2, double scale1, double dt0,
+                         const double* mi, double magi, const double* mp, double magp, const double* ph,
+                         const double* th, double* dmv, double* mo, double* mago, double* e, double* e1)
+{
+    const double T = q.T, Tc = q.Tc, lam = q.lam, alpha = q.alpha, spin = q.spin;
+    double m[3], h[3], t[3], d[3], o[3], htot[3], mag, mhtot, meq, aperp, apar, y, scale11, scale2;
+    *e = 0.0;
+    if (magi > 0.0) {
+        if (type > -3 && type < 3) {
+            scale11 = scale1;
+            if (!stage2) {   // first or only stage: relax |m| (llg.f90:937-947)
+                for (int c = 0; c < 3; c++) m[c] = mi[c];
+                mag = magi;
+                for (int c = 0; c < 3; c++) htot[c] = ph[c] + lam * mag * m[c];
+                mhtot = sqrt(dot3d(htot, htot));
+                if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+                meq = GMUB_KB * spin * mhtot / T;
+                mag = llb_lv(meq) * dot3d(m, htot);
+                mag = mag + (magi - mag) * exp(scale1 * 2.0 * alpha * lam * lam * T * T * llb_lv(meq) /
+                                               (3.0 * Tc * Tc * llb_dlv(meq) * (mhtot + 1.e-20)));
+            } else {
+                for (int c = 0; c < 3; c++) m[c] = mp[c];
+                mag = magp;
+            }
+            for (int c = 0; c < 3; c++) htot[c] = ph[c] + lam * mag * m[c];
+            mhtot = sqrt(dot3d(htot, htot));
+            if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+            meq = GMUB_KB * spin * mhtot / T;
+            aperp = alpha * (1.0 - lam * llb_lv(meq) * T / (3.0 * mhtot * Tc));
+            apar = alpha * 2.0 * T / (3.0 * Tc);
+            scale2 = -aperp / (fabs(mag) + 1.e-20);
+            const double sth = sqrt(fabs(scale2 / (alpha * q.as * dt0 * (fabs(mag) + 1.e-20))));
+            for (int c = 0; c < 3; c++) h[c] = ph[c] + th[c] * sth;
+            for (int c = 0; c < 3; c++) m[c] = mi[c];
+            cross3d(m, h, t);
+            *e = dot3d(t, t) / (q.ms * q.ms);
+            if (type != 2 && type != -2)
+                for (int c = 0; c < 3; c++) o[c] = (1.0 + scale2 * scale2) * scale11 * h[c] + scale2 * m[c];
+            else
+                for (int c = 0; c < 3; c++) o[c] = scale11 * h[c] + scale2 * m[c];
+            double mo3[3], to[3], mh[3];
+            cross3d(m, o, mo3);
+            for (int c = 0; c < 3; c++) t[c] = m[c] + mo3[c];
+            y = 2.0 / (1.0 + dot3d(o, o));
+            cross3d(t, o, to);
+            for (int c = 0; c < 3; c++) d[c] = y * to[c];
+            for (int c = 0; c < 3; c++) mo[c] = m[c] + d[c];
+            for (int c = 0; c < 3; c++) mh[c] = m[c] + 0.5 * d[c];
+            for (int c = 0; c < 3; c++) htot[c] = ph[c] + lam * mag * mh[c];
+            mhtot = sqrt(dot3d(htot, htot));
+            if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+            meq = GMUB_KB * spin * mhtot / T;
+            const double sg = signbit(mag) ? -1.0 : 1.0;
+            double smh[3];
+            for (int c = 0; c < 3; c++) smh[c] = sg * mh[c];
+            mag = llb_lv(meq) * dot3d(smh, htot);
+            meq = mag + (magi - mag) * exp(scale1 * 4.0 * alpha * lam * lam * T * T * llb_lv(meq) /
+                                           (3.0 * Tc * Tc * llb_dlv(meq) * (mhtot + 1.e-20)));
+            const double spar = sqrt(fabs(apar / (alpha * q.as * dt0)));
+            double tp[3];
+            for (int c = 0; c < 3; c++) tp[c] = th[c] * spar;
+            meq = meq - 2.0 * scale11 * dot3d(tp, mh);
+            *mago = fabs(meq);
+            if (meq < 0.0) {
+                for (int c = 0; c < 3; c++) o[c] = -mh[c];
+                const double r = sqrt(dot3d(o, o));
+                for (int c = 0; c < 3; c++) mo[c] = o[c] / r;
+                for (int c = 0; c < 3; c++) d[c] = mo[c] - m[c];
+            }
+            y = 0.5 * (*mago + magi);
+            const double dmg = *mago - magi;
+            double ev[3];
+            for (int c = 0; c < 3; c++) ev[c] = d[c] * y + m[c] * dmg;
+            *e1 = dot3d(ev, ev);
+        } else {   // Heun (llg.f90:992-1041)
+            if (stage2) {
+                for (int c = 0; c < 3; c++) m[c] = mp[c];
+                mag = magp;
+                if (mag <= 0.0) {
+                    mag = 0.0;
+                    y = sqrt(dot3d(ph, ph));
+                    if (y != 0.0) for (int c = 0; c < 3; c++) m[c] = ph[c] / y;
+                }
+            } else {
+                for (int c = 0; c < 3; c++) m[c] = mi[c];
+                mag = magi;
+            }
+            for (int c = 0; c < 3; c++) htot[c] = ph[c] + lam * mag * m[c];
+            mhtot = sqrt(dot3d(htot, htot));
+            if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+            meq = GMUB_KB * spin * mhtot / T;
+            aperp = alpha * (1.0 - lam * llb_lv(meq) * T / (3.0 * mhtot * Tc));
+            apar = alpha * 2.0 * T / (3.0 * Tc);
+            const double sq = sqrt(alpha * q.as * dt0);
+            double hpar[3], hperp[3];
+            for (int c = 0; c < 3; c++) h[c] = th[c] / sq;
+            const double hm = dot3d(h, m);
+            for (int c = 0; c < 3; c++) { hpar[c] = hm * m[c]; hperp[c] = h[c] - hpar[c]; }
+            const double sp = sqrt(apar), sr = sqrt((aperp - apar > 0.0 ? aperp - apar : 0.0) / (aperp * aperp));
+            for (int c = 0; c < 3; c++) { hpar[c] = hpar[c] * sp; hperp[c] = hperp[c] * sr; }
+            scale11 = 2.0 * scale1;
+            meq = apar * (lam / mhtot) * lam * 2.0 * scale1 * T * llb_lv(meq) / (3.0 * llb_dlv(meq) * Tc) *
+                  (mag - llb_lv(meq) * dot3d(m, htot));
+            double hp[3], mxh[3], mxt[3];
+            for (int c = 0; c < 3; c++) hp[c] = ph[c] + hperp[c];
+            cross3d(m, hp, t);
+            cross3d(m, ph, mxh);
+            cross3d(m, t, mxt);
+            const double am = fabs(meq) < 0.5 ? fabs(meq) : 0.5, rl = signbit(meq) ? -am : am;
+            for (int c = 0; c < 3; c++) d[c] = scale11 * (mag * mxh[c] + aperp * mxt[c] + hpar[c]) + rl * m[c];
+            if (stage2) for (int c = 0; c < 3; c++) d[c] = 0.5 * (d[c] + dmv[c]);
+            else for (int c = 0; c < 3; c++) dmv[c] = d[c];
+            *e = dot3d(t, t) / (q.ms * q.ms);
+            *e1 = dot3d(d, d);
+            for (int c = 0; c < 3; c++) o[c] = mi[c] * magi + d[c];
+            y = sqrt(dot3d(o, o));
+            if (y > 0.0) {
+                for (int c = 0; c < 3; c++) mo[c] = o[c] / y;
+            } else {
+                y = 0.0;
+                for (int c = 0; c < 3; c++) mo[c] = -mi[c];
+            }
+            *mago = y;
+        }
+    } else {   // |m| = 0 (llg.f90:1042-1056)
+        y = sqrt(dot3d(ph, ph));
+        if (y > 0.0) {
+            for (int c = 0; c < 3; c++) mo[c] = ph[c] / y;
+            scale11 = 2.0 * scale1;
+            meq = GMUB_KB * spin * y / T;
+            scale2 = lam * T / (3.0 * llb_dlv(meq) * Tc);
+            *mago = (y / scale2 + llb_lv(meq)) * (1.0 - exp(scale11 * scale2 * 2.0 * T * alpha / (3.0 * Tc)));
+        } else {
+            for (int c = 0; c < 3; c++) mo[c] = mi[c];
+            *mago = magi;
+        }
+        const double dmg = *mago - magi;
+        *e1 = dmg * dmg;
+    }
 }
 
 // Rotational pushes of push_llg (llg.f90:1086-1121): Boris (|type|==2) or analytic
@@ -503,6 +676,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     }
     // after the lane reductions only lane 0 of each grain reads the state and writes it
     T mo[3] = {0, 0, 0};
+    T fsc = 0;   // Ms * ms_scale of the new state: the cell magnetisation factor
     if (g < G && w0) {
         const int k = gd.layer_of[g];
         const LayerParams& L = gd.lp[k];
@@ -540,12 +714,54 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 atomicMax(reinterpret_cast<unsigned long long*>(&red[D_HMAG + k]),
                           (unsigned long long)__double_as_longlong(q));
             }
+        } else if (a.llb) {
+            // push_llb in FP64 (llg.f90:916-1065)
+            const bool stage2 = MODE == GM_HEUN2 || MODE == GM_ROT2;
+            const bool heun = MODE == GM_HEUN1 || MODE == GM_HEUN2 || MODE == GM_PUSH1;
+            LlbGrain q;
+            q.T = gd.temp[g]; q.Tc = gd.tc[g]; q.lam = gd.lam[g]; q.spin = L.spin; q.alpha = L.alpha;
+            q.as = (double)gd.alpha_scale[g]; q.ms = (double)gd.ms[g];
+            double mid[3], mpd[3], phd[3], thd[3], dmd[3] = {0, 0, 0}, mod[3], mago, e, e1;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                mid[c] = (double)ma[(size_t)c * G + g];
+                mpd[c] = (double)mb[(size_t)c * G + g];
+                phd[c] = (double)h[c];
+                thd[c] = (double)th[c];
+            }
+            const double magi = (double)ma[(size_t)ST_MAG * G + g];
+            const double magp = stage2 ? (double)mb[(size_t)ST_MAG * G + g] : 0.0;
+            if (heun)
+#pragma unroll
+                for (int c = 0; c < 3; c++) dmd[c] = (double)dm[(size_t)c * G + g];
+            llb_push(q, a.type, stage2, -0.5 * L.gamma * a.dts, a.dt, mid, magi, mpd, magp, phd, thd, dmd, mod, &mago,
+                     &e, &e1);
+            if (heun && !stage2)
+#pragma unroll
+                for (int c = 0; c < 3; c++) dm[(size_t)c * G + g] = (T)dmd[c];
+#pragma unroll
+            for (int c = 0; c < 3; c++) mo[c] = (T)mod[c];
+            const T mscn = L.use_mag ? (T)(mago < 1.0 ? mago : 1.0) : gd.ms_scale[g];
+            fsc = L.use_mag ? gd.ms[g] * mscn : gd.msf[g];
+            T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
+            dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
+            dst[(size_t)ST_MAG * G + g] = (T)mago;
+            dst[(size_t)ST_MSC * G + g] = mscn;
+            if (a.diag) {
+                const double vol = (double)gd.vol[g];
+                atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
+                atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
+                atomicAdd(&red[D_MMAG + k], mago * vol);
+#pragma unroll
+                for (int c = 0; c < 3; c++) atomicAdd(&red[D_MAVE + 3 * k + c], vol * (double)mo[c]);
+            }
         } else {
         const T sq = (T)a.sqrt_dt;
         T H[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
-        const T msc = gd.ms_scale[g];
+        const T msc = ma[(size_t)ST_MSC * G + g];
+        fsc = L.use_mag ? gd.ms[g] * msc : gd.msf[g];
         T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
         T d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
@@ -613,7 +829,8 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         T mb3[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) mb3[c] = __shfl(mo[c], src);
-        scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, gd.msf[g], mb3);
+        const T f = __shfl(fsc, src);
+        scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, f, mb3);
     }
     if (a.diag) {
         __syncthreads();
@@ -856,8 +1073,37 @@ struct HostLayer {
     bool has_uni = false, has_cub = false, has_happ = false, has_hx = false, has_uhx = false, has_lhx = false;
     mrmg_ufield uf{};
     bool has_uf = false;
-    std::vector<double> m;   // (3,ng) host copy used before the device state exists
+    // LLB and temperature scaling: Tc, lambdaMs, S; use_mag = use_mag_p with tc_nom <= 1e10
+    std::vector<double> tc, lambda_ms;
+    double spin = 0.5;
+    bool use_mag = false;
+    std::vector<double> m;     // (3,ng) host copy used before the device state exists
+    std::vector<double> mag;   // (ng) |m|, likewise
 };
+
+// get_thermal's standard deviation per grain (field.f90:407-423): the LLB form (no_ms_p),
+// else the |m| form when |m| != 1, else the ms_scale form
+static double thermal_sigma(const HostLayer& H, int n, double mag, bool no_ms)
+{
+    if (!H.thermal) return 0.0;
+    const double tsigma = sqrt(2.0 * 1.380662e-16 * H.thm_alpha / H.gamma);
+    double x;
+    if (no_ms) {
+        x = H.alpha_scale[n] * H.temp[n] / (H.ms[n] * H.vol[n]);
+    } else if (mag != 1.0) {
+        x = H.alpha_scale[n] * H.temp[n] / (H.ms[n] * H.vol[n] * std::max(H.floor_ms, fabs(mag)));
+    } else {
+        const double msc = H.use_mag ? 1.0 : H.ms_scale[n];
+        x = msc > 0.0 ? H.alpha_scale[n] * H.temp[n] / (H.ms[n] * H.vol[n] * std::max(H.floor_ms, msc)) : 0.0;
+    }
+    return sqrt(x) * tsigma;
+}
+
+// the Ms scale the field sees for |m| = mag (find_temp_scaling's use_mag_p form)
+static double field_ms_scale(const HostLayer& H, int n, double mag)
+{
+    return H.use_mag ? std::min(1.0, std::max(0.0, fabs(mag))) : H.ms_scale[n];
+}
 
 struct EngineBase {
     virtual ~EngineBase() {}
@@ -865,6 +1111,8 @@ struct EngineBase {
     virtual int set_tensor_planes(const double* d_real) = 0;
     virtual int set_state(int k, const double* m) = 0;
     virtual int get_state(int k, double* m, double* h) = 0;
+    virtual int set_mag(int k, const double* mag) = 0;
+    virtual int get_mag(int k, double* mag) = 0;
     virtual int update_ufield(std::vector<HostLayer>& L) = 0;
     virtual int step(const mrmg_step_params* p, const double* noise, mrmg_diag* diag) = 0;
     virtual int demag_field(double* h_cell, double* h_grain) = 0;
@@ -927,10 +1175,14 @@ struct Engine : EngineBase {
     DBuf<int> layer_of, gc_off, gc_idx, cell2grain;
     DBuf<int> xi_off, xi_nb, xu_off, xu_nb, xl_off, xl_nb;
     DBuf<T> xi_w, xu_w, xl_w;
-    DBuf<T> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, vol;
+    DBuf<T> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
+    DBuf<double> tc, lam, temp;
     DBuf<LayerParams> lp;
-    // state (mc: corrector output of the adaptive path until the step is accepted)
+    // state, ST_ROWS rows of G (mc: corrector output of the adaptive path until the step
+    // is accepted)
     DBuf<T> ma, mb, mc, dm, hthm, h;
+    // |m| on the device differs from the host copy the LLG thermal variance was built from
+    bool mag_dirty = false;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
     DBuf<T> Hcell, Mcell;
@@ -947,6 +1199,7 @@ struct Engine : EngineBase {
     double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool tensor_ok = false, has_happ = false, has_xi = false, has_xu = false, has_xl = false;
+    bool cur_llb = false;   // the step in flight uses push_llb (selects the thermal variance)
     int maxb_row = 1, maxb_col = 1;
     size_t lds_row = 0, lds_col = 0;
     std::vector<LayerParams> hlp;
@@ -1001,7 +1254,8 @@ struct Engine : EngineBase {
         b += xi_off.bytes() + xi_nb.bytes() + xu_off.bytes() + xu_nb.bytes() + xl_off.bytes() + xl_nb.bytes();
         b += xi_w.bytes() + xu_w.bytes() + xl_w.bytes();
         b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
-             hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + vol.bytes() + lp.bytes();
+             hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + sigma_llb.bytes() + vol.bytes() +
+             lp.bytes() + tc.bytes() + lam.bytes() + temp.bytes();
         b += ma.bytes() + mb.bytes() + mc.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
         b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
              slot2c.bytes();
@@ -1117,8 +1371,9 @@ struct Engine : EngineBase {
         for (int k = 0; k < nl; k++) Ctot += L[k].ncell;
         std::vector<int> h_layer(G), h_gco(G + 1), h_gci(Ctot), h_c2g(nl * P, -1);
         std::vector<double> h_ms(G), h_msf(G), h_mss(G), h_as(G), h_hks(G), h_hk1(G), h_hk2(G), h_uni(3 * (size_t)G, 0.0),
-            h_cub(9 * (size_t)G, 0.0), h_happ(3 * (size_t)G, 0.0), h_sig(G, 0.0), h_vol(G);
-        std::vector<double> h_m(3 * (size_t)G);
+            h_cub(9 * (size_t)G, 0.0), h_happ(3 * (size_t)G, 0.0), h_sig(G, 0.0), h_sigl(G, 0.0), h_vol(G);
+        std::vector<double> h_tc(G), h_lam(G), h_temp(G);
+        std::vector<double> h_m((size_t)ST_ROWS * G);
         has_happ = has_xi = has_xu = has_xl = false;
         hlp.assign(nl, LayerParams{});
         int q = 0;
@@ -1147,17 +1402,14 @@ struct Engine : EngineBase {
                 if (H.has_uni) for (int cc = 0; cc < 3; cc++) h_uni[(size_t)cc * G + g] = H.uni[3 * n + cc];
                 if (H.has_cub) for (int cc = 0; cc < 9; cc++) h_cub[(size_t)cc * G + g] = H.cub[9 * n + cc];
                 if (H.has_happ) for (int cc = 0; cc < 3; cc++) h_happ[(size_t)cc * G + g] = H.happ[3 * n + cc];
-                // get_thermal variance, LLG branch with |m| = 1 (field.f90:407-423)
-                if (H.thermal) {
-                    const double tsigma = sqrt(2.0 * 1.380662e-16 * H.thm_alpha / H.gamma);
-                    double x;
-                    if (H.ms_scale[n] > 0.0)
-                        x = H.alpha_scale[n] * H.temp[n] / (H.ms[n] * H.vol[n] * std::max(H.floor_ms, H.ms_scale[n]));
-                    else
-                        x = 0.0;
-                    h_sig[g] = sqrt(x) * tsigma;
-                }
+                h_sig[g] = thermal_sigma(H, n, H.mag[n], false);
+                h_sigl[g] = thermal_sigma(H, n, H.mag[n], true);
+                h_tc[g] = H.tc[n];
+                h_lam[g] = H.lambda_ms[n] * (H.spin + 1) / H.spin;
+                h_temp[g] = H.temp[n];
                 for (int cc = 0; cc < 3; cc++) h_m[(size_t)cc * G + g] = H.m[3 * n + cc];
+                h_m[(size_t)ST_MAG * G + g] = H.mag[n];
+                h_m[(size_t)ST_MSC * G + g] = field_ms_scale(H, n, H.mag[n]);
             }
             for (size_t p = 0; p < P; p++) {
                 const int gr = H.grid2grain[p];
@@ -1171,6 +1423,8 @@ struct Engine : EngineBase {
             LayerParams& lpk = hlp[k];
             lpk.alpha = H.alpha;
             lpk.gamma = H.gamma;
+            lpk.spin = H.spin;
+            lpk.use_mag = H.use_mag;
             lpk.has_uni = H.has_uni;
             lpk.has_cub = H.has_cub;
             lpk.thermal = H.thermal;
@@ -1223,10 +1477,14 @@ struct Engine : EngineBase {
         RCHK(cell2grain.upload(h_c2g.data(), h_c2g.size(), c->stream));
         RCHK(upT(ms, h_ms)); RCHK(upT(msf, h_msf)); RCHK(upT(ms_scale, h_mss)); RCHK(upT(alpha_scale, h_as));
         RCHK(upT(hk_scale, h_hks)); RCHK(upT(hk1, h_hk1)); RCHK(upT(hk2, h_hk2)); RCHK(upT(uni, h_uni));
-        RCHK(upT(cub, h_cub)); RCHK(upT(sigma, h_sig)); RCHK(upT(vol, h_vol));
+        RCHK(upT(cub, h_cub)); RCHK(upT(sigma, h_sig)); RCHK(upT(sigma_llb, h_sigl)); RCHK(upT(vol, h_vol));
+        RCHK(tc.upload(h_tc.data(), G, c->stream));
+        RCHK(lam.upload(h_lam.data(), G, c->stream));
+        RCHK(temp.upload(h_temp.data(), G, c->stream));
         if (has_happ) RCHK(upT(happ, h_happ)); else happ.release();
         RCHK(upT(ma, h_m));
-        RCHK(mb.alloc(3 * (size_t)G)); RCHK(mc.alloc(3 * (size_t)G)); RCHK(dm.alloc(3 * (size_t)G)); RCHK(hthm.alloc(3 * (size_t)G)); RCHK(h.alloc(3 * (size_t)G));
+        mag_dirty = false;
+        RCHK(mb.alloc((size_t)ST_ROWS * G)); RCHK(mc.alloc((size_t)ST_ROWS * G)); RCHK(dm.alloc(3 * (size_t)G)); RCHK(hthm.alloc(3 * (size_t)G)); RCHK(h.alloc(3 * (size_t)G));
         HIPCHK(hipMemsetAsync(mb.p, 0, mb.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(dm.p, 0, dm.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
@@ -1358,7 +1616,63 @@ struct Engine : EngineBase {
         d.xu_off = has_xu ? xu_off.p : nullptr; d.xu_nb = xu_nb.p; d.xu_w = xu_w.p;
         d.xl_off = has_xl ? xl_off.p : nullptr; d.xl_nb = xl_nb.p; d.xl_w = xl_w.p;
         d.lp = lp.p;
+        d.tc = tc.p; d.lam = lam.p; d.temp = temp.p;
+        if (cur_llb) d.sigma = sigma_llb.p;
         return d;
+    }
+
+    // |m| of the state: rows ST_MAG and ST_MSC of ma are the truth; the other buffers take
+    // a copy before a step so that the buffer swaps of the LLG pushes (which leave |m|
+    // alone) keep it
+    int sync_aux()
+    {
+        const size_t off3 = (size_t)ST_MAG * G, n = 2 * (size_t)G;
+        HIPCHK(hipMemcpyAsync(mb.p + off3, ma.p + off3, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(mc.p + off3, ma.p + off3, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        return MRMG_OK;
+    }
+    // the LLG thermal variance depends on |m| when it is not 1: rebuild it from the device |m|
+    int refresh_sigma()
+    {
+        std::vector<T> t(G);
+        HIPCHK(hipMemcpyAsync(t.data(), ma.p + (size_t)ST_MAG * G, sizeof(T) * G, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<T> sg(G);
+        for (int k = 0; k < c->nl; k++) {
+            HostLayer& H = c->layers[k];
+            H.mag.resize(H.ng);
+            for (int n = 0; n < H.ng; n++) {
+                H.mag[n] = (double)t[off[k] + n];
+                sg[off[k] + n] = (T)thermal_sigma(H, n, H.mag[n], false);
+            }
+        }
+        RCHK(sigma.upload(sg.data(), G, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        mag_dirty = false;
+        return MRMG_OK;
+    }
+    int set_mag(int k, const double* mg) override
+    {
+        const HostLayer& H = c->layers[k];
+        std::vector<T> a(H.ng), b(H.ng);
+        for (int n = 0; n < H.ng; n++) {
+            a[n] = (T)mg[n];
+            b[n] = (T)field_ms_scale(H, n, mg[n]);
+        }
+        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MAG * G + off[k], a.data(), sizeof(T) * H.ng, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MSC * G + off[k], b.data(), sizeof(T) * H.ng, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        mag_dirty = true;
+        return MRMG_OK;
+    }
+    int get_mag(int k, double* mg) override
+    {
+        const HostLayer& H = c->layers[k];
+        std::vector<T> a(H.ng);
+        HIPCHK(hipMemcpyAsync(a.data(), ma.p + (size_t)ST_MAG * G + off[k], sizeof(T) * H.ng, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int n = 0; n < H.ng; n++) mg[n] = (double)a[n];
+        return MRMG_OK;
     }
 
     bool demag_on() const { return c->demag && tensor_ok; }
@@ -1483,6 +1797,7 @@ struct Engine : EngineBase {
         int rejected = 0;
         LayerRed r;
         StepArgs a{};
+        a.llb = p->llb;
         a.type = p->type;
         a.noise = p->noise;
         a.seed = p->seed;
@@ -1583,8 +1898,24 @@ struct Engine : EngineBase {
         if (p->dt_frac != 0.0) {
             if (type != -3) return set_err(MRMG_ERR_UNSUPPORTED, "adaptive dt is implemented for Heun (llg_type -3) only");
             if (p->max_dm <= 0.0 || p->dt_max <= 0.0) return set_err(MRMG_ERR_ARG, "adaptive dt needs max_dm > 0, dt_max > 0");
-            return step_adaptive(p, noise, diag);
         }
+        if (p->llb) {
+            for (int k = 0; k < c->nl; k++)
+                if (!(c->layers[k].spin > 0.0))
+                    return set_err(MRMG_ERR_ARG, "layer %d: LLB needs spin > 0", k);
+        }
+        cur_llb = p->llb != 0;
+        RCHK(sync_aux());
+        if (!cur_llb && mag_dirty) RCHK(refresh_sigma());
+        const int rc = p->dt_frac != 0.0 ? step_adaptive(p, noise, diag) : step_fixed(p, noise, diag);
+        if (cur_llb) mag_dirty = true;
+        cur_llb = false;
+        return rc;
+    }
+
+    int step_fixed(const mrmg_step_params* p, const double* noise, mrmg_diag* diag)
+    {
+        const int type = p->type;
         // llg() start: init_applied zeroes hthm (field.f90:703)
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         // the cells must hold the current state before the first field evaluation
@@ -1595,6 +1926,7 @@ struct Engine : EngineBase {
         double t = p->t0;
         const double dt = p->dt;
         StepArgs a{};
+        a.llb = p->llb;
         a.dt = dt;
         a.sqrt_dt = sqrt(dt);
         a.type = type;
@@ -1862,7 +2194,9 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
         for (int q = 0; q < c->nl; q++) {
             HostLayer& H = c->layers[q];
             H.m.resize(3 * (size_t)H.ng);
+            H.mag.resize(H.ng);
             RCHK(c->eng->get_state(q, H.m.data(), nullptr));
+            RCHK(c->eng->get_mag(q, H.mag.data()));
         }
     }
     HostLayer H;
@@ -1909,9 +2243,29 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
     if (H.has_happ) copy_vec(H.happ, in->h_app, 3 * (size_t)ng);
     H.alpha = in->alpha; H.gamma = in->gamma; H.thm_alpha = in->thm_alpha; H.floor_ms = in->floor_ms;
     H.thermal = in->thermal;
-    // init_data_one: m = +z (data.f90:201-202)
+    // LLB parameters: Tc defaults to 1e22 (data.f90:201), lambdaMs to
+    // 3 Tc / (g muB/kB (S+1)) (grain.f90:61)
+    H.spin = in->spin;
+    H.tc.assign(ng, 1.e22); if (in->tc) copy_vec(H.tc, in->tc, ng);
+    H.lambda_ms.resize(ng);
+    for (int n = 0; n < ng; n++)
+        H.lambda_ms[n] = in->lambda_ms ? in->lambda_ms[n] : 3.0 * H.tc[n] / (1.343427768e-4 * (H.spin + 1.0));
+    // find_temp_scaling (temp_scaling.f90:365-431): nothing when tc_nom > 1e10; the
+    // power laws of the (static) grain temperature are applied here, the |m| form at
+    // every field evaluation on the device
+    H.use_mag = false;
+    if (in->tc_nom <= 1.e10) {
+        H.use_mag = in->use_mag != 0;
+        auto cl = [](double x) { return x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0; };
+        if (!H.use_mag && in->msexp >= 0.0)
+            for (int n = 0; n < ng; n++) H.ms_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->msexp);
+        if (in->hkexp >= 0.0)
+            for (int n = 0; n < ng; n++) H.hk_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->hkexp);
+    }
+    // init_data_one: m = +z (data.f90:201-202), |m| = 1 (grain.f90:61)
     H.m.assign(3 * (size_t)ng, 0.0);
     for (int n = 0; n < ng; n++) H.m[3 * n + 2] = 1.0;
+    H.mag.assign(ng, 1.0);
     H.uf = c->layers[k].uf;
     H.has_uf = c->layers[k].has_uf;
     c->layers[k] = std::move(H);
@@ -1940,6 +2294,30 @@ int mrmg_set_state(mrmg_ctx* c, int k, const double* m)
         return MRMG_OK;
     }
     return c->eng->set_state(k, m);
+}
+
+int mrmg_set_mag(mrmg_ctx* c, int k, const double* mag)
+{
+    if (!c || !mag) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (!c->layers[k].set) return set_err(MRMG_ERR_STATE, "layer %d not set", k);
+    if (c->dirty) {
+        c->layers[k].mag.assign(mag, mag + c->layers[k].ng);
+        return MRMG_OK;
+    }
+    return c->eng->set_mag(k, mag);
+}
+
+int mrmg_get_mag(mrmg_ctx* c, int k, double* mag)
+{
+    if (!c || !mag) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (!c->layers[k].set) return set_err(MRMG_ERR_STATE, "layer %d not set", k);
+    if (c->dirty) {
+        memcpy(mag, c->layers[k].mag.data(), sizeof(double) * c->layers[k].ng);
+        return MRMG_OK;
+    }
+    return c->eng->get_mag(k, mag);
 }
 
 int mrmg_get_state(mrmg_ctx* c, int k, double* m, double* h)

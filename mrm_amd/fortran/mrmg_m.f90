@@ -13,7 +13,7 @@ module mrmg_m
   private
   public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag
   public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_set_layer, &
-            mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_step, mrmg_error
+            mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error
   public :: MRMG_FP64, MRMG_FP32, MRMG_NOISE_NONE, MRMG_NOISE_INJECT, MRMG_NOISE_PHILOX, MRMG_TYPE_HEUN
 
   integer(c_int), parameter :: MRMG_FP64 = 0, MRMG_FP32 = 1
@@ -35,7 +35,9 @@ module mrmg_m
     type(c_ptr) :: lhxgrainindex, lhxgrain, lhx
     type(c_ptr) :: ms_scale, alpha_scale, hk_scale, temp, h_app
     real(c_double) :: alpha, gamma, thm_alpha, floor_ms
-    integer(c_int) :: thermal
+    integer(c_int) :: thermal, use_mag
+    type(c_ptr) :: tc, lambda_ms
+    real(c_double) :: spin, tc_nom, msexp, hkexp
   end type
 
   type, bind(C) :: mrmg_ufield
@@ -49,6 +51,7 @@ module mrmg_m
     integer(c_int) :: noise, pad_
     integer(c_int64_t) :: seed, step0
     real(c_double) :: dt_frac, dt_max, max_dm
+    integer(c_int) :: llb, pad2_
   end type
 
   type, bind(C) :: mrmg_diag
@@ -106,6 +109,19 @@ module mrmg_m
       type(c_ptr), value :: ctx
       integer(c_int), value :: k
       real(c_double), intent(out) :: m(*), h(*)
+    end function
+    ! mag: data_s%mag(in/3+1,:), the LLB |m| (ng)
+    integer(c_int) function mrmg_set_mag(ctx, k, mag) bind(C, name='mrmg_set_mag')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double), intent(in) :: mag(*)
+    end function
+    integer(c_int) function mrmg_get_mag(ctx, k, mag) bind(C, name='mrmg_get_mag')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double), intent(out) :: mag(*)
     end function
     integer(c_int) function mrmg_step(ctx, p, noise, diag) bind(C, name='mrmg_step')
       import :: c_int, c_ptr, mrmg_step_params, mrmg_diag

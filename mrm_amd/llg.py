@@ -37,6 +37,7 @@ class Llg:
         self.dt_frac = 0.0      # 0: fixed dt
         self.dt_max = 1e22
         self.max_dm = 180.0     # degrees
+        self.llb = False        # llgp%llb
 
     # -- setup ---------------------------------------------------------------
     def close(self):
@@ -81,6 +82,17 @@ class Llg:
         abi.check(self.lib, self.lib.mrmg_get_state(self.ctx, k, abi.dptr(m), abi.dptr(h)))
         return m, h
 
+    def set_mag(self, k, mag):
+        """|m| of layer k (data_s%mag, the LLB magnitude; 1 for LLG)."""
+        mag = np.ascontiguousarray(mag, dtype=np.float64)
+        assert mag.size == self.ngs[k]
+        abi.check(self.lib, self.lib.mrmg_set_mag(self.ctx, k, abi.dptr(mag)))
+
+    def get_mag(self, k):
+        mag = np.zeros(self.ngs[k])
+        abi.check(self.lib, self.lib.mrmg_get_mag(self.ctx, k, abi.dptr(mag)))
+        return mag
+
     def profile(self, enable=True):
         abi.check(self.lib, self.lib.mrmg_profile(self.ctx, int(bool(enable))))
 
@@ -110,9 +122,12 @@ class Llg:
 
     # -- integrator ----------------------------------------------------------
     def set_llg(self, dt=None, llg_type=None, del_t=None, t_max=None, t_min=None, min_it=None, dt_frac=None,
-                dt_max=None, max_dm=None, fixed_dt_p=None):
+                dt_max=None, max_dm=None, fixed_dt_p=None, llb=None):
         """set_llg (llg.f90:1452-1487): fixed dt unless dt_frac != 0 (then get_fields picks
-        dt and push_m rejects steps rotating more than max_dm degrees)."""
+        dt and push_m rejects steps rotating more than max_dm degrees); llb selects the
+        Landau-Lifshitz-Bloch push (push_llb, llg.f90:916-1065)."""
+        if llb is not None:
+            self.llb = bool(llb)
         if dt_frac is not None:
             self.dt_frac = float(dt_frac)
         if fixed_dt_p:
@@ -149,7 +164,7 @@ class Llg:
         or philox_seed to draw them on the device."""
         p = abi.StepParams(type=self.type, n_steps=int(n_steps), t0=float(t0), dt=self.dt,
                            noise=abi.MRMG_NOISE_NONE, seed=0, step0=self.step_index,
-                           dt_frac=self.dt_frac, dt_max=self.dt_max, max_dm=self.max_dm)
+                           dt_frac=self.dt_frac, dt_max=self.dt_max, max_dm=self.max_dm, llb=int(self.llb))
         nz = None
         if noise is not None:
             nz = np.ascontiguousarray(noise, dtype=np.float64)

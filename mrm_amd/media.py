@@ -147,12 +147,15 @@ class Media:
 def make_media(nx, ny, dx=2e-7, dy=2e-7, dz=8e-7, n_lay=1, pitch=8e-7, ms=700.0, ku=6e6, ku_sigma=0.1,
                cone_deg=3.0, a_ex=1e-6, intergrain_frac=0.05, boundary_width=2e-7, layer_ku=None,
                physical=None, alpha=0.02, gamma=1.76e7, thm_alpha=None, temperature=0.0, thermal=False,
-               seed=1, boundary=True, exchange=True):
+               seed=1, boundary=True, exchange=True, tc_nom=None, tc_sigma=0.0, spin=0.5, msexp=-1.0,
+               hkexp=-1.0, use_mag=False):
     """Build media following the recipe of the benchmark configurations.
 
     layer_ku: per layer Ku (erg/cc); physical: physical-layer id of each layer (sub-layers
     of one physical layer are coupled with 2*A/dz, neighbouring physical layers with
-    intergrain_frac of that).
+    intergrain_frac of that).  tc_nom (K): per-grain Curie temperatures tc_nom (1 +
+    tc_sigma z) for LLB and temperature scaling (Ms, Hk ~ (1-T/Tc)^exp, or Ms from |m|
+    with use_mag); None leaves Tc at 1e22 and scaling off.
     """
     g2g = voronoi_grid(nx, ny, dx, dy, pitch, seed, boundary=boundary)
     g2gi, cells, counts = grain_cells(g2g)
@@ -169,6 +172,11 @@ def make_media(nx, ny, dx=2e-7, dy=2e-7, dz=8e-7, n_lay=1, pitch=8e-7, ms=700.0,
     layers = []
     msv = np.full(ng, float(ms))
     lat = lateral_exchange(g2g, msv, area, ex, dx) if exchange else (None, None, None)
+    llb = {}
+    if tc_nom is not None:
+        zt = np.random.default_rng(seed + 3).standard_normal(ng)
+        llb = dict(tc=tc_nom * (1.0 + tc_sigma * zt), spin=spin, tc_nom=tc_nom, msexp=msexp, hkexp=hkexp,
+                   use_mag=use_mag)
     one = np.arange(0, ng + 1, dtype=np.int32)
     self_nb = np.arange(1, ng + 1, dtype=np.int32)
     for k in range(n_lay):
@@ -177,7 +185,7 @@ def make_media(nx, ny, dx=2e-7, dy=2e-7, dz=8e-7, n_lay=1, pitch=8e-7, ms=700.0,
         kw = dict(ng=ng, grid2grain=g2g, grain2gridindex=g2gi, grain2grid=cells,
                   grain_vol=area * dzs[k], ms=msv, hk=hk, uni_axis=axes,
                   alpha=alpha, gamma=gamma, thm_alpha=alpha if thm_alpha is None else thm_alpha,
-                  floor_ms=0.01, thermal=thermal, temp=np.full(ng, float(temperature)))
+                  floor_ms=0.01, thermal=thermal, temp=np.full(ng, float(temperature)), **llb)
         if exchange:
             kw.update(hxgrainindex=lat[0], hxgrain=lat[1], hx=lat[2])
             for nbk, name in ((k - 1, "u"), (k + 1, "l")):

@@ -20,7 +20,9 @@
  *   applied.f90:863-883, 210-320 uniform field with a linear ramp               (ufield_at)
  *   llg.f90:320-388             get_fields: dt control                          (ora_step)
  *   llg.f90:1066-1155           push_llg: Heun / Boris / quaternion             (push_llg)
+ *   llg.f90:916-1065, 704-733   push_llb: LLB Heun / rotational, lv, dlv        (push_llb)
  *   llg.f90:1195-1271           push_m: stage control, rejection, next dt       (push_m)
+ *   temp_scaling.f90:365-431    find_temp_scaling: |m|, (1-T/Tc)^exp forms       (temp_scaling)
  *   transform.f90:16-20,65-115  cross, rotate_q, rotate_2                       (cross3 / rotate_2)
  *
  * Parity pins (see tests/): MT19937 against numpy's MT19937 and the flang-built
@@ -29,7 +31,11 @@
  * sum; the pushes against the analytic macrospin solution (2nd-order convergence for
  * Heun); exchange against reciprocity; thermal sigma against the fluctuation-dissipation
  * value.  The adaptive-dt rejection path (push_m with rot_ratio > 1) is pinned only by
- * invariants: parity unpinned for its exact retry sequence.
+ * invariants: parity unpinned for its exact retry sequence.  push_llb is pinned by its
+ * limits (Langevin series, the mean-field fixed point |m| = L(beta(H + lambda |m|))
+ * reached at T > 0 without noise, T -> 0 reducing to the LLG precession/damping, the
+ * longitudinal relaxation rate), not by a reference trajectory: parity unpinned for the
+ * exact thermal LLB trajectory.
  */
 #include <math.h>
 #include <stdint.h>
@@ -405,6 +411,9 @@ typedef struct {
     double *ms_scale, *alpha_scale, *hk_scale, *temp, *happ_s;
     double alpha, gamma, thm_alpha, floor_ms;
     int thermal;
+    /* LLB and temperature scaling */
+    double *tc, *lambda_ms, spin, tc_nom, msexp, hkexp;
+    int use_mag;
     double *m;      /* (6,ng) double buffer, slot `in` */
     double *mag;    /* (2,ng) */
     double *h, *hthm, *dm, *h_app, *h_demag;
@@ -452,7 +461,8 @@ static void free_layer(olayer *L)
 {
     void *ps[] = {L->g2gi, L->g2g, L->grid2grain, L->vol, L->ms, L->hk, L->uni, L->cub, L->hxi, L->hxg,
                   L->uhxi, L->uhxg, L->lhxi, L->lhxg, L->hx, L->uhx, L->lhx, L->ms_scale, L->alpha_scale,
-                  L->hk_scale, L->temp, L->happ_s, L->m, L->mag, L->h, L->hthm, L->dm, L->h_app, L->h_demag};
+                  L->hk_scale, L->temp, L->happ_s, L->m, L->mag, L->h, L->hthm, L->dm, L->h_app, L->h_demag,
+                  L->tc, L->lambda_ms};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     memset(L, 0, sizeof(*L));
 }
@@ -469,6 +479,29 @@ static double *fill1(const double *p, int n)
     double *q = (double *)malloc(sizeof(double) * (n ? n : 1));
     for (int i = 0; i < n; i++) q[i] = p ? p[i] : 1.0;
     return q;
+}
+
+#define GMUB_KB 1.343427768e-4   /* g muB / kB (llg.f90:925) */
+
+/* find_temp_scaling (temp_scaling.f90:365-431) for the scale factors this path uses:
+ * Ms from |m| (use_mag_p) or (1-T/Tc)^msexp, Hk from (1-T/Tc)^hkexp.  Tables are
+ * per-grain arrays supplied by the caller (ms_scale, alpha_scale, hk_scale).  The
+ * power laws depend on the static grain temperature only and are evaluated when the
+ * layer is set (t_changed); the |m| form runs at every field evaluation. */
+static double clamp01(double x) { return x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0; }
+static void temp_scaling(olayer *L, int t_changed)
+{
+    if (L->tc_nom > 1.e10) return;
+    if (!L->use_mag && t_changed && L->msexp >= 0.0) {
+        for (int n = 0; n < L->ng; n++) L->ms_scale[n] = pow(clamp01(1.0 - L->temp[n] / L->tc[n]), L->msexp);
+    }
+    if (t_changed && L->hkexp >= 0.0)
+        for (int n = 0; n < L->ng; n++) L->hk_scale[n] = pow(clamp01(1.0 - L->temp[n] / L->tc[n]), L->hkexp);
+}
+static void temp_scaling_mag(olayer *L, int in)
+{
+    if (L->tc_nom > 1.e10 || !L->use_mag) return;
+    for (int n = 0; n < L->ng; n++) L->ms_scale[n] = clamp01(fabs(L->mag[2 * n + in / 3]));
 }
 
 int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
@@ -515,6 +548,17 @@ int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
     for (int n = 0; n < ng; n++) { L->m[6 * n + 2] = 1.0; L->m[6 * n + 5] = 1.0; }
     L->mag = (double *)malloc(sizeof(double) * (2 * ng + 1));
     for (int n = 0; n < 2 * ng; n++) L->mag[n] = 1.0;
+    /* tc defaults to 1e22 (data.f90:201); lambdaMs = 3 Tc / (g muB/kB (S+1)) (grain.f90:61) */
+    L->spin = in->spin; L->tc_nom = in->tc_nom; L->msexp = in->msexp; L->hkexp = in->hkexp;
+    L->use_mag = in->use_mag;
+    L->tc = (double *)malloc(sizeof(double) * (ng ? ng : 1));
+    L->lambda_ms = (double *)malloc(sizeof(double) * (ng ? ng : 1));
+    for (int n = 0; n < ng; n++) {
+        L->tc[n] = in->tc ? in->tc[n] : 1.e22;
+        L->lambda_ms[n] = in->lambda_ms ? in->lambda_ms[n] : 3.0 * L->tc[n] / (GMUB_KB * (L->spin + 1.0));
+    }
+    temp_scaling(L, 1);
+    temp_scaling_mag(L, s->in);
     L->h = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
     L->hthm = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
     L->dm = (double *)calloc((size_t)3 * ng + 1, sizeof(double));
@@ -545,6 +589,23 @@ int ora_set_state(ora_sim *s, int k, const double *m)
     olayer *L = &s->L[k];
     for (int n = 0; n < L->ng; n++)
         for (int c = 0; c < 3; c++) L->m[6 * n + s->in + c] = m[3 * n + c];
+    return MRMG_OK;
+}
+
+/* |m| of layer k: both slots of data_s%mag are set, so the state is the same whichever
+ * slot the next step reads (the reference initialises both to 1, grain.f90:61) */
+int ora_set_mag(ora_sim *s, int k, const double *mag)
+{
+    olayer *L = &s->L[k];
+    for (int n = 0; n < L->ng; n++) { L->mag[2 * n] = mag[n]; L->mag[2 * n + 1] = mag[n]; }
+    temp_scaling_mag(L, s->in);
+    return MRMG_OK;
+}
+
+int ora_get_mag(ora_sim *s, int k, double *mag)
+{
+    olayer *L = &s->L[k];
+    for (int n = 0; n < L->ng; n++) mag[n] = L->mag[2 * n + s->in / 3];
     return MRMG_OK;
 }
 
@@ -667,10 +728,12 @@ static void get_field(ora_sim *s, double t)
 {
     int nx = s->nx, ny = s->ny, nl = s->nl, in = s->in;
     size_t plane = (size_t)nx * ny;
-    /* get_applied: per-grain average of the (static) cell field + uniform field */
+    /* get_applied: per-grain average of the (static) cell field + uniform field, then
+     * find_temp_scaling (field.f90:590-592: t_changed false, so only the |m| form runs) */
     for (int k = 0; k < nl; k++) {
         olayer *L = &s->L[k];
         double u[3]; ufield_at(L, t, u);
+        temp_scaling_mag(L, in);
         for (int n = 0; n < L->ng; n++)
             for (int c = 0; c < 3; c++) L->h_app[3 * n + c] = (L->happ_s ? L->happ_s[3 * n + c] : 0.0) + u[c];
     }
@@ -734,8 +797,8 @@ static void get_field(ora_sim *s, double t)
     }
 }
 
-/* get_thermal (field.f90:397-436), LLG branch; normals = (3,G) for this step */
-static void thermal(ora_sim *s, const double *normals)
+/* get_thermal (field.f90:397-436); no_ms = llgp%llb (llg.f90:363); normals = (3,G) */
+static void thermal(ora_sim *s, const double *normals, int no_ms)
 {
     int goff = 0;
     for (int k = 0; k < s->nl; k++) {
@@ -744,7 +807,9 @@ static void thermal(ora_sim *s, const double *normals)
             double tsigma = sqrt(2.0 * 1.380662e-16 * L->thm_alpha / L->gamma);
             for (int n = 0; n < L->ng; n++) {
                 double x, magv = L->mag[2 * n + s->in / 3];
-                if (magv != 1.0) {
+                if (no_ms) {
+                    x = L->alpha_scale[n] * L->temp[n] / (L->ms[n] * L->vol[n]);
+                } else if (magv != 1.0) {
                     double am = fabs(magv);
                     x = L->alpha_scale[n] * L->temp[n] / (L->ms[n] * L->vol[n] * (L->floor_ms > am ? L->floor_ms : am));
                 } else if (L->ms_scale[n] > 0.0) {
@@ -856,8 +921,163 @@ static void push_llg(ora_sim *s, int k, int type, int iter, double dt, double dt
     }
 }
 
+/* Langevin function and its derivative (llg.f90:704-710, 726-733) */
+static double lv(double x) { return fabs(x) > 1.e-2 ? 1.0 / tanh(x) - 1.0 / x : x / 3.0; }
+static double dlv(double x)
+{
+    if (fabs(x) > 4.e-3) { double b = sinh(x); return 1.0 / (x * x) - 1.0 / (b * b); }
+    return 1.0 / 3.0;
+}
+double ora_lv(double x) { return lv(x); }
+double ora_dlv(double x) { return dlv(x); }
+
+/* push_llb (llg.f90:916-1065) for one layer: the Landau-Lifshitz-Bloch push with the
+ * Langevin mean field.  |m| lives in mag(2,ng), double-buffered like m.  Heun
+ * (type -3) and the rotational branch (|type| < 3: Boris-style update of the direction,
+ * exponential relaxation of |m|); grains with |m| <= 0 relax along the field. */
+static void push_llb(ora_sim *s, int k, int type, int iter, double dt, double dt0,
+                     double *errmx_l, double *errmx1_l, double *mmag1)
+{
+    olayer *L = &s->L[k];
+    int id = s->in, od = 3 - s->in, mi_ = id / 3, mo_ = 1 - id / 3;
+    double scale1 = -0.5 * L->gamma * dt;
+    const double alpha = L->alpha, spin = L->spin;
+    for (int n = 0; n < L->ng; n++) {
+        double *min_ = L->m + 6 * n + id, *mout = L->m + 6 * n + od;
+        const double *ph = L->h + 3 * n, *th = L->hthm + 3 * n;
+        const double T = L->temp[n], Tc = L->tc[n];
+        const double lambdam = L->lambda_ms[n] * (spin + 1) / spin;
+        double m[3], h[3], t[3], d[3], o[3], htot[3], mag, mhtot, meq, aperp, apar, y, scale11, scale2;
+        if (L->mag[2 * n + mi_] > 0.0) {
+            if (abs(type) < 3) {
+                scale11 = scale1;
+                if (iter == 2 || type > 0) {   /* first or only stage: relax |m| */
+                    for (int c = 0; c < 3; c++) m[c] = min_[c];
+                    mag = L->mag[2 * n + mi_];
+                    for (int c = 0; c < 3; c++) htot[c] = ph[c] + lambdam * mag * m[c];
+                    mhtot = sqrt(dot3(htot, htot));
+                    if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+                    meq = GMUB_KB * spin * mhtot / T;
+                    mag = lv(meq) * dot3(m, htot);
+                    mag = mag + (L->mag[2 * n + mi_] - mag) *
+                          exp(scale1 * 2.0 * alpha * lambdam * lambdam * T * T * lv(meq) /
+                              (3.0 * Tc * Tc * dlv(meq) * (mhtot + 1.e-20)));
+                } else {
+                    for (int c = 0; c < 3; c++) m[c] = mout[c];
+                    mag = L->mag[2 * n + mo_];
+                }
+                for (int c = 0; c < 3; c++) htot[c] = ph[c] + lambdam * mag * m[c];
+                mhtot = sqrt(dot3(htot, htot));
+                if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+                meq = GMUB_KB * spin * mhtot / T;
+                aperp = alpha * (1.0 - lambdam * lv(meq) * T / (3.0 * mhtot * Tc));
+                apar = alpha * 2.0 * T / (3.0 * Tc);
+                scale2 = -aperp / (fabs(mag) + 1.e-20);
+                double sth = sqrt(fabs(scale2 / (alpha * L->alpha_scale[n] * dt0 * (fabs(mag) + 1.e-20))));
+                for (int c = 0; c < 3; c++) h[c] = ph[c] + th[c] * sth;
+                for (int c = 0; c < 3; c++) m[c] = min_[c];
+                cross3(m, h, t); y = dot3(t, t);
+                if (y > *errmx_l * L->ms[n] * L->ms[n]) *errmx_l = y / (L->ms[n] * L->ms[n]);
+                if (abs(type) != 2) for (int c = 0; c < 3; c++) o[c] = (1.0 + scale2 * scale2) * scale11 * h[c] + scale2 * m[c];
+                else for (int c = 0; c < 3; c++) o[c] = scale11 * h[c] + scale2 * m[c];
+                double mo3[3]; cross3(m, o, mo3);
+                for (int c = 0; c < 3; c++) t[c] = m[c] + mo3[c];
+                y = 2.0 / (1.0 + dot3(o, o));
+                double to[3]; cross3(t, o, to);
+                for (int c = 0; c < 3; c++) d[c] = y * to[c];
+                for (int c = 0; c < 3; c++) mout[c] = m[c] + d[c];
+                double mh[3];
+                for (int c = 0; c < 3; c++) mh[c] = m[c] + 0.5 * d[c];
+                for (int c = 0; c < 3; c++) htot[c] = ph[c] + lambdam * mag * mh[c];
+                mhtot = sqrt(dot3(htot, htot));
+                if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+                meq = GMUB_KB * spin * mhtot / T;
+                double sg = fsign(1.0, mag), smh[3];
+                for (int c = 0; c < 3; c++) smh[c] = sg * mh[c];
+                mag = lv(meq) * dot3(smh, htot);
+                meq = mag + (L->mag[2 * n + mi_] - mag) *
+                      exp(scale1 * 4.0 * alpha * lambdam * lambdam * T * T * lv(meq) /
+                          (3.0 * Tc * Tc * dlv(meq) * (mhtot + 1.e-20)));
+                double spar = sqrt(fabs(apar / (alpha * L->alpha_scale[n] * dt0))), tp[3];
+                for (int c = 0; c < 3; c++) tp[c] = th[c] * spar;
+                meq = meq - 2.0 * scale11 * dot3(tp, mh);
+                L->mag[2 * n + mo_] = fabs(meq);
+                if (meq < 0.0) {
+                    for (int c = 0; c < 3; c++) o[c] = -mh[c];
+                    double r = sqrt(dot3(o, o));
+                    for (int c = 0; c < 3; c++) mout[c] = o[c] / r;
+                    for (int c = 0; c < 3; c++) d[c] = mout[c] - m[c];
+                }
+                y = 0.5 * (L->mag[2 * n + mo_] + L->mag[2 * n + mi_]);
+                double dmg = L->mag[2 * n + mo_] - L->mag[2 * n + mi_], e[3];
+                for (int c = 0; c < 3; c++) e[c] = d[c] * y + m[c] * dmg;
+                double ee = dot3(e, e); if (ee > *errmx1_l) *errmx1_l = ee;
+            } else {   /* Heun */
+                if (iter == 1) {
+                    for (int c = 0; c < 3; c++) m[c] = mout[c];
+                    mag = L->mag[2 * n + mo_];
+                    if (mag <= 0.0) {
+                        mag = 0.0; y = sqrt(dot3(ph, ph));
+                        if (y != 0.0) for (int c = 0; c < 3; c++) m[c] = ph[c] / y;
+                    }
+                } else {
+                    for (int c = 0; c < 3; c++) m[c] = min_[c];
+                    mag = L->mag[2 * n + mi_];
+                }
+                for (int c = 0; c < 3; c++) htot[c] = ph[c] + lambdam * mag * m[c];
+                mhtot = sqrt(dot3(htot, htot));
+                if (mhtot != 0.0) for (int c = 0; c < 3; c++) htot[c] = htot[c] / mhtot;
+                meq = GMUB_KB * spin * mhtot / T;
+                aperp = alpha * (1.0 - lambdam * lv(meq) * T / (3.0 * mhtot * Tc));
+                apar = alpha * 2.0 * T / (3.0 * Tc);
+                double sq = sqrt(alpha * L->alpha_scale[n] * dt0), hpar[3], hperp[3];
+                for (int c = 0; c < 3; c++) h[c] = th[c] / sq;
+                double hm = dot3(h, m);
+                for (int c = 0; c < 3; c++) { hpar[c] = hm * m[c]; hperp[c] = h[c] - hpar[c]; }
+                double sp = sqrt(apar), sr = sqrt((aperp - apar > 0.0 ? aperp - apar : 0.0) / (aperp * aperp));
+                for (int c = 0; c < 3; c++) { hpar[c] = hpar[c] * sp; hperp[c] = hperp[c] * sr; }
+                scale11 = 2.0 * scale1;
+                meq = apar * (lambdam / mhtot) * lambdam * 2.0 * scale1 * T * lv(meq) / (3.0 * dlv(meq) * Tc) *
+                      (mag - lv(meq) * dot3(m, htot));
+                double hp[3], mxh[3], mxt[3];
+                for (int c = 0; c < 3; c++) hp[c] = ph[c] + hperp[c];
+                cross3(m, hp, t);
+                cross3(m, ph, mxh);
+                cross3(m, t, mxt);
+                double am = fabs(meq) < 0.5 ? fabs(meq) : 0.5, rl = fsign(am, meq);
+                for (int c = 0; c < 3; c++) d[c] = scale11 * (mag * mxh[c] + aperp * mxt[c] + hpar[c]) + rl * m[c];
+                if (iter == 1) for (int c = 0; c < 3; c++) d[c] = 0.5 * (d[c] + L->dm[3 * n + c]);
+                else for (int c = 0; c < 3; c++) L->dm[3 * n + c] = d[c];
+                y = dot3(t, t);
+                if (y > *errmx_l * L->ms[n] * L->ms[n]) *errmx_l = y / (L->ms[n] * L->ms[n]);
+                double dd = dot3(d, d); if (dd > *errmx1_l) *errmx1_l = dd;
+                for (int c = 0; c < 3; c++) o[c] = min_[c] * L->mag[2 * n + mi_] + d[c];
+                y = sqrt(dot3(o, o));
+                if (y > 0.0) { for (int c = 0; c < 3; c++) mout[c] = o[c] / y; }
+                else { y = 0.0; for (int c = 0; c < 3; c++) mout[c] = -min_[c]; }
+                L->mag[2 * n + mo_] = y;
+            }
+        } else {   /* |m| = 0: along the field with the longitudinal relaxation */
+            y = sqrt(dot3(ph, ph));
+            if (y > 0.0) {
+                for (int c = 0; c < 3; c++) mout[c] = ph[c] / y;
+                scale11 = 2.0 * scale1;
+                meq = GMUB_KB * spin * y / T;
+                scale2 = lambdam * T / (3.0 * dlv(meq) * Tc);
+                L->mag[2 * n + mo_] = (y / scale2 + lv(meq)) * (1.0 - exp(scale11 * scale2 * 2.0 * T * alpha / (3.0 * Tc)));
+            } else {
+                for (int c = 0; c < 3; c++) mout[c] = min_[c];
+                L->mag[2 * n + mo_] = L->mag[2 * n + mi_];
+            }
+            double dmg = L->mag[2 * n + mo_] - L->mag[2 * n + mi_];
+            if (dmg * dmg > *errmx1_l) *errmx1_l = dmg * dmg;
+        }
+        *mmag1 = *mmag1 + L->mag[2 * n + mo_] * L->vol[n];
+    }
+}
+
 /* push_m, llg.f90:1195-1271 */
-static void push_m(ora_sim *s, double t0, int type, int fixed)
+static void push_m(ora_sim *s, double t0, int type, int fixed, int llb)
 {
     int too_far = 1;
     while (too_far) {
@@ -871,7 +1091,8 @@ static void push_m(ora_sim *s, double t0, int type, int fixed)
             for (int k = 0; k < s->nl; k++) {
                 olayer *L = &s->L[k];
                 double errmx_l = 0.0, errmx1_l = 0.0, mmag1 = 0.0, rot_ratio;
-                push_llg(s, k, type, iter, dt * dt_s, dt, &errmx_l, &errmx1_l, &mmag1);
+                if (llb) push_llb(s, k, type, iter, dt * dt_s, dt, &errmx_l, &errmx1_l, &mmag1);
+                else push_llg(s, k, type, iter, dt * dt_s, dt, &errmx_l, &errmx1_l, &mmag1);
                 if (iter == 1) {
                     if (fixed) { rot_ratio = 1.0; too_far = 0; }
                     else { rot_ratio = acos(1.0 - 0.5 * errmx1_l) / L->max_dm; too_far = rot_ratio > 1.0; }
@@ -937,7 +1158,7 @@ int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_di
     for (int it = 0; it < p->n_steps; it++) {
         /* get_fields(t), llg.f90:320-388 */
         get_field(s, t);
-        if (p->noise != MRMG_NOISE_NONE && noise) thermal(s, noise + (size_t)3 * G * it);
+        if (p->noise != MRMG_NOISE_NONE && noise) thermal(s, noise + (size_t)3 * G * it, p->llb);
         double dti = 1.0 / sqrt(s->dt * s->max_dmdt);
         if (dt_frac != 0.0) s->dt = 1.e22;
         for (int k = 0; k < s->nl; k++) {
@@ -955,7 +1176,7 @@ int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_di
             double nd = s->dt < dt_max ? s->dt : dt_max;
             s->dt = nd < cand ? nd : cand;
         }
-        push_m(s, t, p->type, fixed);
+        push_m(s, t, p->type, fixed, p->llb);
         s->dt = s->dt * s->max_dmdt; s->max_dmdt = s->max_dmdtnext;
         t = t + s->dt;
     }

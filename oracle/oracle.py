@@ -55,9 +55,13 @@ def lib():
             "ora_set_max_dm": ([_cp, C.c_int, C.c_double], None),
             "ora_set_state": ([_cp, C.c_int, _dp], C.c_int),
             "ora_get_state": ([_cp, C.c_int, _dp, _dp], C.c_int),
+            "ora_set_mag": ([_cp, C.c_int, _dp], C.c_int),
+            "ora_get_mag": ([_cp, C.c_int, _dp], C.c_int),
             "ora_step": ([_cp, C.POINTER(abi.StepParams), _dp, C.POINTER(abi.Diag)], C.c_int),
             "ora_demag_field": ([_cp, _dp, _dp], C.c_int),
             "ora_num_threads": ([], C.c_int),
+            "ora_lv": ([C.c_double], C.c_double),
+            "ora_dlv": ([C.c_double], C.c_double),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -210,11 +214,20 @@ class Sim:
         lib().ora_get_state(self.h, k, abi.dptr(m), abi.dptr(h))
         return m, h
 
+    def set_mag(self, k, mag):
+        mag = np.ascontiguousarray(mag, dtype=np.float64)
+        lib().ora_set_mag(self.h, k, abi.dptr(mag))
+
+    def get_mag(self, k):
+        mag = np.zeros(self.ngs[k])
+        lib().ora_get_mag(self.h, k, abi.dptr(mag))
+        return mag
+
     def step(self, n_steps, dt, t0=0.0, type=abi.MRMG_TYPE_HEUN, noise=None, dt_frac=0.0, dt_max=1e22,
-             max_dm=180.0):
+             max_dm=180.0, llb=False):
         p = abi.StepParams(type=type, n_steps=n_steps, t0=t0, dt=dt,
                            noise=abi.MRMG_NOISE_INJECT if noise is not None else abi.MRMG_NOISE_NONE,
-                           dt_frac=dt_frac, dt_max=dt_max, max_dm=max_dm)
+                           dt_frac=dt_frac, dt_max=dt_max, max_dm=max_dm, llb=int(bool(llb)))
         d = abi.Diag()
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
         lib().ora_step(self.h, C.byref(p), abi.dptr(nz), C.byref(d))

@@ -54,10 +54,11 @@ int main(void) {
          sizeof(mrmg_geom), sizeof(mrmg_layer), sizeof(mrmg_ufield), sizeof(mrmg_step_params), sizeof(mrmg_diag));
   F(mrmg_geom, dx) F(mrmg_geom, dz) F(mrmg_geom, il_space)
   F(mrmg_layer, grid2grain) F(mrmg_layer, hx) F(mrmg_layer, lhx) F(mrmg_layer, h_app) F(mrmg_layer, alpha)
-  F(mrmg_layer, floor_ms) F(mrmg_layer, thermal)
+  F(mrmg_layer, floor_ms) F(mrmg_layer, thermal) F(mrmg_layer, use_mag) F(mrmg_layer, tc) F(mrmg_layer, lambda_ms)
+  F(mrmg_layer, spin) F(mrmg_layer, hkexp)
   F(mrmg_ufield, t_start) F(mrmg_ufield, scale)
   F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
-  F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm)
+  F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm) F(mrmg_step_params, llb)
   F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave) F(mrmg_diag, rejected) F(mrmg_diag, max_dmdt)
   return 0;
 }
