@@ -63,16 +63,46 @@ def col_bytes(med, P):
 
 
 def kernel_bytes(med, P):
+    """Algorithmic bytes per launch of each kernel class: the row r2c reads the cell
+    magnetisation and writes the spectra; the column kernel see col_bytes; the row c2r
+    reads the spectra and writes the cell field; a grain stage reads ~25 P per grain
+    (state, constants, dm, thermal), the exchange CSRs (4 + P per entry), the cell index
+    map (4 per magnetic cell) and gathers/scatters the cell field/magnetisation (6 P per
+    magnetic cell)."""
     nl, C = med.n_lay, med.cells
     kc = (med.nx // 2 + 1) * med.ny
     G = med.G
+    cm = sum(int(L.grain2gridindex[-1]) for L in med.layers)
+    E = 0
+    for L in med.layers:
+        for f in ("hxgrainindex", "uhxgrainindex", "lhxgrainindex"):
+            v = getattr(L, f)
+            if v is not None:
+                E += int(v[-1])
+    grain = 25 * P * G + E * (4 + P) + 4 * cm + 6 * cm * P
     return {
-        "row_fwd": 4 * C + 6 * nl * kc * P + 4 * P * G,
+        "row_fwd": 3 * C * P + 6 * nl * kc * P,
         "col": col_bytes(med, P),
         "row_inv": 6 * nl * kc * P + 3 * C * P,
-        "grain1": 3 * C * P + 4 * C + 25 * P * G,
-        "grain2": 3 * C * P + 4 * C + 25 * P * G,
+        "grain1": grain,
+        "grain2": grain,
     }
+
+
+def pmc_traffic(kernel, tag):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == kernel:
+            try:
+                return int(float(f[3]))
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(med, tensor, target_s=15.0):
@@ -142,7 +172,7 @@ def main():
     log("kernels: " + ", ".join(f"{k} {v['avg_us']:.2f}us {v['share'] * 100:.0f}% {v['gbs']:.0f}GB/s" for k, v in kern.items()))
     ach = kern[dom]["gbs"]
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, "cfg1_fp64"),
             "bytes_per_launch": kb[dom], "avg_launch_us": round(kern[dom]["avg_us"], 3),
             "copy_ceiling_gbs": None}
     try:

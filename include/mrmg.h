@@ -8,6 +8,8 @@
  *   llg.f90:320-388   get_fields() field + thermal draw + dt       -> stage-1 kernel
  *   llg.f90:1195-1271 push_m()     two-stage Heun control          -> mrmg_step
  *   llg.f90:1066-1155 push_llg()   Heun / Boris / quaternion push  -> grain stage kernels
+ *   llg.f90:916-1065  push_llb()   LLB Heun / rotational push      -> grain stage kernels
+ *   temp_scaling.f90:365-431 find_temp_scaling()                   -> mrmg_set_layer + grain kernels
  *   field.f90:160-273 get_field()  H = Happ + Hdemag + Hk + Hex    -> grain stage kernels
  *   field.f90:397-436 get_thermal()                                -> injected normals or Philox
  *   field.f90:438-517 get_magnetostatic() cell -> grain average    -> grain stage kernels
@@ -175,7 +177,9 @@ int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);
 int mrmg_set_state(mrmg_ctx *ctx, int k, const double *m);
 /* m (3,ng) and the last field h (3,ng) (either may be NULL). */
 int mrmg_get_state(mrmg_ctx *ctx, int k, double *m, double *h);
-/* |m| of layer k (ng): data_s%mag(in/3+1,:), the LLB magnitude (1 for LLG). */
+/* |m| of layer k (ng): data_s%mag(in/3+1,:) (data.f90:198, set to 1 by grain.f90:61), the
+ * LLB magnitude; push_llg leaves it alone.  set_mag writes both slots of the reference's
+ * (2,ng) array, so the value holds whichever slot the next step reads. */
 int mrmg_set_mag(mrmg_ctx *ctx, int k, const double *mag);
 int mrmg_get_mag(mrmg_ctx *ctx, int k, double *mag);
 
