@@ -80,12 +80,17 @@ def kernel_bytes(med, P):
             if v is not None:
                 E += int(v[-1])
     grain = 25 * P * G + E * (4 + P) + 4 * cm + 6 * cm * P
+    tu = 6 * nl * (nl + 1) // 2
     return {
         "row_fwd": 3 * C * P + 6 * nl * kc * P,
         "col": col_bytes(med, P),
         "row_inv": 6 * nl * kc * P + 3 * C * P,
         "grain1": grain,
         "grain2": grain,
+        # split column path: column FFTs in place, product streaming spectra and tensor
+        "col_fwd": 12 * nl * kc * P,
+        "prod": 12 * nl * kc * P + 2 * tu * kc * P,
+        "col_inv": 12 * nl * kc * P,
     }
 
 
@@ -163,8 +168,8 @@ def main():
     st = x64.kernel_stats()
     x64.profile(False)
     kb = kernel_bytes(med, 8)
-    kern = {k: {"launches": n, "avg_us": 1e3 * ms / max(n, 1), "share": 0.0} for k, (n, ms) in st.items()}
-    tot = sum(ms for _, ms in st.values())
+    kern = {k: {"launches": n, "avg_us": 1e3 * ms / max(n, 1), "share": 0.0} for k, (n, ms) in st.items() if n > 0}
+    tot = sum(ms for n, ms in st.values() if n > 0)
     for k in kern:
         kern[k]["share"] = st[k][1] / tot if tot else 0.0
         kern[k]["gbs"] = kb[k] / (kern[k]["avg_us"] * 1e-6) / 1e9 if kern[k]["avg_us"] > 0 else 0.0

@@ -284,6 +284,175 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
     }
 }
 
+// ------------------------------------------------------------------------- split column path
+// For grids whose 3*n_lay columns of one kx do not fit one workgroup's LDS, the demag
+// runs as five kernels over column-contiguous spectra ST[k][c][kx][y]:
+//   k_rowT_fwd  r2c of TY rows of one (component, layer), written as TY-long y runs
+//   k_colT      y-FFT of whole columns (CB columns per workgroup), in place
+//   k_prod      the k-space product (demag.f90:117-121), one thread per (kx, ky)
+//   k_colT      inverse y-FFT, in place
+//   k_rowT_inv  c2r of TY rows, read back from TY-long y runs
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+                                                  const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    const int B = TY / 2;
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(B * nx);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
+    const int y0 = blockIdx.x * TY, kc = blockIdx.y;        // kc = k*3 + c
+    const int k = kc / 3, c = kc - 3 * k;
+    const size_t CP = (size_t)nl * ny * nx;
+    const T* src = Mcell + c * CP + (size_t)k * ny * nx;
+    for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
+        const int p = idx / nx, x = idx - p * nx;
+        const int ya = y0 + 2 * p, yb = ya + 1;
+        re[LP(idx)] = ya < ny ? src[(size_t)ya * nx + x] : (T)0;
+        im[LP(idx)] = yb < ny ? src[(size_t)yb * nx + x] : (T)0;
+    }
+    __syncthreads();
+    lds_fft<T, E, false>(re, im, B, px, tw);
+    C2<T>* dst = S + (size_t)kc * nxh * ny;
+    for (int idx = threadIdx.x; idx < nxh * TY; idx += blockDim.x) {
+        const int kx = idx / TY, yy = idx - kx * TY;
+        const int y = y0 + yy;
+        if (y >= ny) continue;
+        const int p = yy >> 1;
+        const int kn = kx == 0 ? 0 : nx - kx;
+        const T zr = re[LP(p * nx + kx)], zi = im[LP(p * nx + kx)], nr = re[LP(p * nx + kn)], ni = im[LP(p * nx + kn)];
+        C2<T> v;
+        if ((yy & 1) == 0) v = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
+        else v = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
+        dst[(size_t)kx * ny + y] = v;
+    }
+}
+
+// y-FFT of ncols contiguous columns of length ny, CB columns per workgroup, in place.
+// Twiddles are read from the global table (cached), not staged.
+template <typename T, int E, bool INV>
+__global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int ncols, int CB, FFTPlan py,
+                                              const C2<T>* __restrict__ twy)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* re = reinterpret_cast<T*>(smem_raw);
+    const int q0 = blockIdx.x * CB;
+    const int nb = min(CB, ncols - q0);
+    T* im = re + LPN(CB * ny);
+    C2<T>* base = S + (size_t)q0 * ny;
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
+        const C2<T> v = base[idx];
+        re[LP(idx)] = v.x;
+        im[LP(idx)] = v.y;
+    }
+    __syncthreads();
+    lds_fft<T, E, INV>(re, im, nb, py, twy);
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
+}
+
+// The k-space product over column-contiguous spectra: thread per (kx, ky), observation
+// layers in chunks of OC, each output summed over (ks, b) in the reference's order.
+template <typename T, bool SYM, int OC>
+__global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+                                              const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t Kc = (size_t)nxh * ny;
+    if (idx >= Kc) return;
+    const int kx = (int)(idx / ny), ky = (int)(idx - (size_t)kx * ny);
+    for (int kc0 = 0; kc0 < nl; kc0 += OC) {
+        C2<T> h[OC][3];
+#pragma unroll
+        for (int j = 0; j < OC; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) h[j][a] = {0, 0};
+        for (int ks = 0; ks < nl; ks++) {
+            const C2<T> mx = S1[((size_t)(3 * ks + 0)) * Kc + idx];
+            const C2<T> my = S1[((size_t)(3 * ks + 1)) * Kc + idx];
+            const C2<T> mz = S1[((size_t)(3 * ks + 2)) * Kc + idx];
+#pragma unroll
+            for (int j = 0; j < OC; j++) {
+                const int ko = kc0 + j;
+                if (ko >= nl) break;
+                const C2<T>* t = Tspec + ((((size_t)kx * nl + ko) * nl + ks) * nslot) * ny + ky;
+                C2<T> a;
+                if (SYM) {
+                    const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
+                    a = cmul(txx, mx); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txy, my); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txz, mz); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txy, mx); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(tyy, my); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(tyz, mz); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(txz, mx); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(tyz, my); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(tzz, mz); h[j][2].x += a.x; h[j][2].y += a.y;
+                } else {
+                    a = cmul(t[0 * ny], mx); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[1 * ny], my); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[2 * ny], mz); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[3 * ny], mx); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[4 * ny], my); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[5 * ny], mz); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[6 * ny], mx); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(t[7 * ny], my); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(t[8 * ny], mz); h[j][2].x += a.x; h[j][2].y += a.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < OC; j++) {
+            const int ko = kc0 + j;
+            if (ko >= nl) break;
+#pragma unroll
+            for (int a = 0; a < 3; a++) S2[((size_t)(3 * ko + a)) * Kc + idx] = h[j][a];
+        }
+    }
+}
+
+// c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
+// imaginary parts of the DC and Nyquist bins ignored).
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T* __restrict__ Hcell,
+                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+                                                  const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    const int B = TY / 2;
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(B * nx);
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
+    const int y0 = blockIdx.x * TY, kc = blockIdx.y;
+    const int k = kc / 3, c = kc - 3 * k;
+    const C2<T>* src = S + (size_t)kc * nxh * ny;
+    // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
+    for (int idx = threadIdx.x; idx < nxh * B; idx += blockDim.x) {
+        const int kx = idx / B, p = idx - kx * B;
+        const int ya = y0 + 2 * p, yb = ya + 1;
+        C2<T> A = ya < ny ? src[(size_t)kx * ny + ya] : C2<T>{0, 0};
+        C2<T> Bv = yb < ny ? src[(size_t)kx * ny + yb] : C2<T>{0, 0};
+        if (kx == 0 || 2 * kx == nx) { A.y = 0; Bv.y = 0; }
+        re[LP(p * nx + kx)] = A.x - Bv.y;
+        im[LP(p * nx + kx)] = A.y + Bv.x;
+        if (kx > 0 && nx - kx != kx) {
+            re[LP(p * nx + nx - kx)] = A.x + Bv.y;
+            im[LP(p * nx + nx - kx)] = Bv.x - A.y;
+        }
+    }
+    __syncthreads();
+    lds_fft<T, E, true>(re, im, B, px, tw);
+    T* hc = Hcell + ((size_t)c * nl + k) * (size_t)nx * ny;
+    for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
+        const int p = idx / nx, x = idx - p * nx;
+        const int ya = y0 + 2 * p, yb = ya + 1;
+        if (ya < ny) hc[(size_t)ya * nx + x] = re[LP(idx)];
+        if (yb < ny) hc[(size_t)yb * nx + x] = im[LP(idx)];
+    }
+}
+
 // ------------------------------------------------------------------------- grain kernel
 // LPG lanes cooperate on one grain: the cell gather and the exchange sums are split over
 // the lanes and reduced with a fixed xor-butterfly (deterministic); the rest of the grain
@@ -1190,6 +1359,11 @@ struct Engine : EngineBase {
     // launch shapes (mrmg_tune): threads of the row and column workgroups, observation-layer
     // groups per column workgroup, lanes per grain in the grain kernels
     int row_threads = 128, col_threads = 512, kg = 1, lpg = 8;
+    // split column path (column-contiguous spectra): rows per row workgroup, columns per
+    // column workgroup, and whether it is in use (forced by col_path or by the grid size)
+    bool split = false, fused_ok = false, split_ok = false;
+    int TYr = 8, CBc = 1, maxb_rowT = 4, maxb_colT = 4;
+    size_t lds_rowT = 0, lds_colT = 0;
     DBuf<double> noise_buf, diag_part;
     // instrumentation: event pairs of the current mrmg_step call, per-class totals
     bool prof = false;
@@ -1211,7 +1385,8 @@ struct Engine : EngineBase {
     }
 
     // kernel classes of the profile
-    enum { K_ROW_FWD = 0, K_COL = 1, K_ROW_INV = 2, K_GRAIN1 = 3, K_GRAIN2 = 4, K_NCLASS = 5 };
+    enum { K_ROW_FWD = 0, K_COL = 1, K_ROW_INV = 2, K_GRAIN1 = 3, K_GRAIN2 = 4, K_COLF = 5, K_PROD = 6, K_COLI = 7,
+           K_NCLASS = 8 };
     int ev_begin(int cls)
     {
         if (!prof) return MRMG_OK;
@@ -1273,14 +1448,31 @@ struct Engine : EngineBase {
         for (int i = 0; i < ny; i++) ty[i] = {(T)wy[i].x, (T)wy[i].y};
         RCHK(twx.upload(tx.data(), nx, c->stream));
         RCHK(twy.upload(ty.data(), ny, c->stream));
-        RCHK(shapes());
-        if (maxb_row < 0 || maxb_col < 0)
-            return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs", nx, ny, nl);
+        // fused column path: all 3*n_lay columns of one kx in one workgroup's LDS
         lds_row = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN(3 * nx);
         lds_col = sizeof(C2<T>) * ny + sizeof(T) * 2 * LPN(3 * nl * ny);
-        if (lds_row > 160 * 1024 || lds_col > 160 * 1024)
-            return set_err(MRMG_ERR_UNSUPPORTED, "LDS need %zu/%zu bytes exceeds 160 KiB (ny=%d, n_lay=%d)", lds_row,
-                           lds_col, ny, nl);
+        fused_ok = lds_row <= 160 * 1024 && lds_col <= 160 * 1024 && pick_e(3, c->px, 256) > 0 &&
+                   pick_e(3 * nl, c->py, col_threads) > 0 && nl * ny <= 4 * col_threads;
+        // split path: TY rows per row workgroup, CB columns per column workgroup
+        const size_t lim = 160 * 1024;
+        split_ok = false;
+        for (int ty = 8; ty >= 2; ty /= 2) {
+            lds_rowT = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN((ty / 2) * nx);
+            if (lds_rowT <= lim && pick_e(ty / 2, c->px, 256) > 0 && (lds_rowT <= 64 * 1024 || ty == 2)) {
+                TYr = ty;
+                split_ok = true;
+                break;
+            }
+        }
+        CBc = std::max(1, std::min(16, 2048 / ny));
+        CBc = std::min(CBc, 3 * nl * nxh);
+        lds_colT = sizeof(T) * 2 * LPN(CBc * ny);
+        split_ok = split_ok && lds_colT <= lim && pick_e(CBc, c->py, 256) > 0;
+        if (!fused_ok && !split_ok)
+            return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: a row or column FFT does not fit one workgroup", nx, ny,
+                           nl);
+        split = !fused_ok;
+        RCHK(shapes());
         RCHK(lds_attr());
         RCHK(S1.alloc((size_t)nl * 3 * ny * nxh));
         RCHK(S2.alloc((size_t)nl * 3 * ny * nxh));
@@ -1296,6 +1488,14 @@ struct Engine : EngineBase {
     int shapes()
     {
         const int nl = c->nl, ny = c->ny;
+        if (split) {
+            maxb_rowT = pick_e(TYr / 2, c->px, row_threads);
+            maxb_colT = pick_e(CBc, c->py, 256);
+            if (maxb_rowT < 0 || maxb_colT < 0)
+                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: split-path FFT too large for %d threads", c->nx, ny,
+                               nl, row_threads);
+            return MRMG_OK;
+        }
         maxb_row = pick_e(3, c->px, row_threads);
         maxb_col = pick_e(3 * nl, c->py, col_threads);
         if (maxb_row < 0 || maxb_col < 0)
@@ -1326,6 +1526,13 @@ struct Engine : EngineBase {
         } else if (k == "lanes_per_grain") {
             if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 4, 8 or 16");
             lpg = v;
+        } else if (k == "col_path") {
+            if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused) or 1 (split)");
+            if (v == 0 && !fused_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the fused column path does not fit this grid");
+            if (v == 1 && !split_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the split column path does not fit this grid");
+            const bool old_split = split;
+            split = v == 1;
+            if (shapes() != MRMG_OK) { split = old_split; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else {
             return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         }
@@ -1338,6 +1545,7 @@ struct Engine : EngineBase {
         else if (k == "col_threads") *v = col_threads;
         else if (k == "col_groups") *v = kg;
         else if (k == "lanes_per_grain") *v = lpg;
+        else if (k == "col_path") *v = split ? 1 : 0;
         else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         return MRMG_OK;
     }
@@ -1351,12 +1559,26 @@ struct Engine : EngineBase {
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
     }
+    template <int MB> int lds_attr_split()
+    {
+        const int r = (int)lds_rowT, cl = (int)lds_colT;
+        HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        return MRMG_OK;
+    }
     // dynamic LDS above 64 KiB must be requested per kernel
     int lds_attr()
     {
-        if (lds_row <= 64 * 1024 && lds_col <= 64 * 1024) return MRMG_OK;
-        RCHK(lds_attr_mb<4>()); RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
-        RCHK(lds_attr_mb<32>()); RCHK(lds_attr_mb<64>());
+        if (fused_ok && (lds_row > 64 * 1024 || lds_col > 64 * 1024)) {
+            RCHK(lds_attr_mb<4>()); RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
+            RCHK(lds_attr_mb<32>()); RCHK(lds_attr_mb<64>());
+        }
+        if (split_ok && (lds_rowT > 64 * 1024 || lds_colT > 64 * 1024)) {
+            RCHK(lds_attr_split<4>()); RCHK(lds_attr_split<8>()); RCHK(lds_attr_split<16>());
+            RCHK(lds_attr_split<32>()); RCHK(lds_attr_split<64>());
+        }
         return MRMG_OK;
     }
 
@@ -1528,6 +1750,12 @@ struct Engine : EngineBase {
         {
             dim3 grid((ny + 1) / 2, P), blk(256);
             size_t lds = sizeof(double) * (2 * nx + 2 * LPN(nx));
+            if (lds > 160 * 1024) return set_err(MRMG_ERR_UNSUPPORTED, "tensor row FFT of %d does not fit LDS", nx);
+            if (lds > 64 * 1024) {
+                const void* fk[5] = {(const void*)k_tensor_rows<4>, (const void*)k_tensor_rows<8>, (const void*)k_tensor_rows<16>,
+                                     (const void*)k_tensor_rows<32>, (const void*)k_tensor_rows<64>};
+                for (const void* f : fk) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
 #define KTR(MB, ...) hipLaunchKernelGGL(k_tensor_rows<MB>, grid, blk, lds, c->stream, __VA_ARGS__)
             switch (mbr) {
             case 4: KTR(4, d_real, TC.p, nx, ny, nxh, c->px, tdx.p); break;
@@ -1542,6 +1770,12 @@ struct Engine : EngineBase {
         {
             dim3 grid(nxh, P), blk(256);
             size_t lds = sizeof(double) * (2 * ny + 2 * LPN(ny));
+            if (lds > 160 * 1024) return set_err(MRMG_ERR_UNSUPPORTED, "tensor column FFT of %d does not fit LDS", ny);
+            if (lds > 64 * 1024) {
+                const void* fk[5] = {(const void*)k_tensor_cols<4>, (const void*)k_tensor_cols<8>, (const void*)k_tensor_cols<16>,
+                                     (const void*)k_tensor_cols<32>, (const void*)k_tensor_cols<64>};
+                for (const void* f : fk) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
 #define KTC(MB) hipLaunchKernelGGL(k_tensor_cols<MB>, grid, blk, lds, c->stream, TC.p, ny, nxh, c->py, tdy.p)
             switch (mbc) {
             case 4: KTC(4); break;
@@ -1691,6 +1925,7 @@ struct Engine : EngineBase {
     // get_demag: row r2c of the cell magnetisation, column FFT * tensor, row c2r
     int demag()
     {
+        if (split) return demag_split();
         const int nx = c->nx, ny = c->ny, nl = c->nl, nxh = c->nxh;
         dim3 grow((ny + 1) / 2, nl), blk(row_threads);
         RCHK(ev_begin(K_ROW_FWD));
@@ -1709,6 +1944,58 @@ struct Engine : EngineBase {
         RCHK(ev_end());
         (void)nx;
         return MRMG_OK;
+    }
+    // the split path over column-contiguous spectra
+    int demag_split()
+    {
+        const int ny = c->ny, nl = c->nl, nxh = c->nxh;
+        const int ncols = 3 * nl * nxh;
+        dim3 grow((ny + TYr - 1) / TYr, 3 * nl), blk(row_threads);
+        RCHK(ev_begin(K_ROW_FWD));
+        LAUNCH_E(maxb_rowT, launch_rowT_fwd, grow, blk);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        const dim3 gcol((ncols + CBc - 1) / CBc), b256(256);
+        RCHK(ev_begin(K_COLF));
+        LAUNCH_E(maxb_colT, launch_colT_fwd, gcol, b256);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        const size_t Kc = (size_t)nxh * ny;
+        RCHK(ev_begin(K_PROD));
+        const dim3 gp((unsigned)((Kc + 255) / 256));
+        if (nslot == 6) hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
+        else hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        RCHK(ev_begin(K_COLI));
+        LAUNCH_E(maxb_colT, launch_colT_inv, gcol, b256);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        RCHK(ev_begin(K_ROW_INV));
+        LAUNCH_E(maxb_rowT, launch_rowT_inv, grow, blk);
+        HIPCHK(hipGetLastError());
+        RCHK(ev_end());
+        return MRMG_OK;
+    }
+    template <int MB> void launch_rowT_fwd(dim3 g, dim3 b)
+    {
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
+                           TYr, c->px, twx.p);
+    }
+    template <int MB> void launch_colT_fwd(dim3 g, dim3 b)
+    {
+        hipLaunchKernelGGL((k_colT<T, MB, false>), g, b, lds_colT, c->stream, S1.p, c->ny, 3 * c->nl * c->nxh, CBc,
+                           c->py, twy.p);
+    }
+    template <int MB> void launch_colT_inv(dim3 g, dim3 b)
+    {
+        hipLaunchKernelGGL((k_colT<T, MB, true>), g, b, lds_colT, c->stream, S2.p, c->ny, 3 * c->nl * c->nxh, CBc,
+                           c->py, twy.p);
+    }
+    template <int MB> void launch_rowT_inv(dim3 g, dim3 b)
+    {
+        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
+                           TYr, c->px, twx.p);
     }
     template <int MB> void launch_row_fwd(dim3 g, dim3 b)
     {
@@ -1996,7 +2283,8 @@ struct Engine : EngineBase {
     }
     int kernel_stats(mrmg_kstat* out, int max, int* n) override
     {
-        static const char* names[K_NCLASS] = {"row_fwd", "col", "row_inv", "grain1", "grain2"};
+        static const char* names[K_NCLASS] = {"row_fwd", "col", "row_inv", "grain1", "grain2",
+                                              "col_fwd", "prod", "col_inv"};
         *n = K_NCLASS;
         for (int i = 0; i < K_NCLASS && i < max; i++) {
             memset(out[i].name, 0, sizeof(out[i].name));

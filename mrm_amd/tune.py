@@ -8,7 +8,9 @@ are restored afterwards, so tuning does not change the trajectory.
 """
 import numpy as np
 
+DEMAG = ("row_fwd", "col", "row_inv", "col_fwd", "prod", "col_inv")
 SPACE = {
+    "col_path": ([0, 1], DEMAG),              # fused column kernel or the split path
     "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
     "col_threads": ([256, 512], ("col",)),
     "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)
@@ -21,12 +23,13 @@ def _cost(x, steps, classes, philox_seed):
     x.step(steps, philox_seed=philox_seed, diag=False)
     st = x.kernel_stats()
     x.profile(False)
-    return sum(st[k][1] / max(st[k][0], 1) for k in classes)
+    return sum(st[k][1] / max(st[k][0], 1) for k in classes if k in st)
 
 
 def autotune(x, steps=10, philox_seed=12345, verbose=False):
     """Choose every tunable of context x (an mrm_amd.llg.Llg).  Returns {key: value}."""
     saved = [x.get_state(k)[0] for k in range(x.n_lay)]
+    saved_mag = [x.get_mag(k) for k in range(x.n_lay)]
     step_index = x.step_index
     chosen = {}
     for key, (values, classes) in SPACE.items():
@@ -48,6 +51,7 @@ def autotune(x, steps=10, philox_seed=12345, verbose=False):
         chosen[key] = best_v
     for k, m in enumerate(saved):
         x.set_state(k, m)
+        x.set_mag(k, saved_mag[k])
     x.step_index = step_index
     return chosen
 
