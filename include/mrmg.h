@@ -120,6 +120,38 @@ typedef struct {
     double t_start, t_end, s_start, s_end, scale;
 } mrmg_ufield;
 
+/* A moving applied source of one layer (get_appl, applied.f90:827-1034), sampled at the
+ * cell centres every field evaluation and averaged over each grain's cells like
+ * get_applied (field.f90:545-556):
+ *   MRMG_SRC_FIELD_TABLE  vector field table vals (3,nxp,nyp) on increasing node
+ *                         positions xpos (nxp), ypos (nyp) (cm), bilinear (set +
+ *                         interpolate, applied.f90:1757-1776, 1471-1522)
+ *   MRMG_SRC_TEMP_TABLE   scalar temperature table vals (nxp,nyp), likewise
+ *   MRMG_SRC_TEMP_SG      super-Gaussian temperature dtemp * gx(x) * gy(y),
+ *                         g(u) = exp(-((u * isig)^expn)) (set_sg/get_sg, applied.f90:1778-1811)
+ * Position along each axis at time t: offset + vel*(t - t_origin) - ref (tables) and
+ * ref - vel*(t - t_origin) - offset (super-Gaussian), as get_appl forms them.  The value
+ * is scaled by the linear ramp of mrmg_ufield (t_start, t_end, s_start, s_end, scale).
+ * A grain's field is its static h_app + the source average + the uniform field; its
+ * temperature is the source average + its static temp.  A temperature source makes the
+ * temperature scalings (msexp, hkexp) and the thermal variance follow the temperature. */
+#define MRMG_SRC_NONE 0
+#define MRMG_SRC_FIELD_TABLE 1
+#define MRMG_SRC_TEMP_TABLE 2
+#define MRMG_SRC_TEMP_SG 3
+typedef struct {
+    int kind;
+    int nxp, nyp;
+    int expn;                        /* super-Gaussian power (the reference's 2*expn)        */
+    const double *xpos, *ypos;       /* table nodes (cm)                                      */
+    const double *vals;              /* table values, x fastest                               */
+    double vel_x, vel_y;             /* cm/s                                                  */
+    double offset_x, offset_y, ref_x, ref_y, t_origin;
+    double isigx, isigy;             /* 1/sigma (cm^-1), 0 = flat along that axis           */
+    double dtemp;                    /* super-Gaussian peak (K)                               */
+    double t_start, t_end, s_start, s_end, scale;
+} mrmg_source;
+
 typedef struct {
     int type;                 /* MRMG_TYPE_*                                            */
     int n_steps;              /* accepted steps to run in this call                     */
@@ -172,6 +204,9 @@ int mrmg_build_tensor(mrmg_ctx *ctx, int num_images, double *n_real_out);
 
 int mrmg_set_layer(mrmg_ctx *ctx, int k, const mrmg_layer *layer);
 int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);
+/* Set (or with kind MRMG_SRC_NONE clear) the field source (slot 0) or the temperature
+ * source (slot 1) of layer k. */
+int mrmg_set_source(mrmg_ctx *ctx, int k, int slot, const mrmg_source *src);
 
 /* Magnetisation state of layer k: m (3,ng) unit vectors (the `in` slot of data_s%m). */
 int mrmg_set_state(mrmg_ctx *ctx, int k, const double *m);

@@ -54,6 +54,45 @@ class UField(C.Structure):
                 ("s_start", C.c_double), ("s_end", C.c_double), ("scale", C.c_double)]
 
 
+MRMG_SRC_NONE, MRMG_SRC_FIELD_TABLE, MRMG_SRC_TEMP_TABLE, MRMG_SRC_TEMP_SG = 0, 1, 2, 3
+
+
+class Source(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nxp", C.c_int), ("nyp", C.c_int), ("expn", C.c_int),
+                ("xpos", _dp), ("ypos", _dp), ("vals", _dp),
+                ("vel_x", C.c_double), ("vel_y", C.c_double),
+                ("offset_x", C.c_double), ("offset_y", C.c_double), ("ref_x", C.c_double), ("ref_y", C.c_double),
+                ("t_origin", C.c_double), ("isigx", C.c_double), ("isigy", C.c_double), ("dtemp", C.c_double),
+                ("t_start", C.c_double), ("t_end", C.c_double), ("s_start", C.c_double), ("s_end", C.c_double),
+                ("scale", C.c_double)]
+
+
+class SourceArrays:
+    """A moving source with its tables kept alive (mrmg_source).  Tables: xpos (nxp),
+    ypos (nyp), vals (nyp, nxp, 3) for a field or (nyp, nxp) for a temperature (x fastest,
+    the Fortran (3,nxp,nyp) / (nxp,nyp) layouts)."""
+
+    def __init__(self, kind, xpos=None, ypos=None, vals=None, vel=(0.0, 0.0), offset=(0.0, 0.0), ref=(0.0, 0.0),
+                 t_origin=0.0, sigma=(0.0, 0.0), expn=2, dtemp=0.0, t_start=0.0, t_end=1.0, s_start=1.0, s_end=1.0,
+                 scale=1.0):
+        self.kind = int(kind)
+        self.xpos = None if xpos is None else np.ascontiguousarray(xpos, dtype=np.float64)
+        self.ypos = None if ypos is None else np.ascontiguousarray(ypos, dtype=np.float64)
+        self.vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+        self.kw = dict(vel_x=vel[0], vel_y=vel[1], offset_x=offset[0], offset_y=offset[1], ref_x=ref[0], ref_y=ref[1],
+                       t_origin=t_origin, isigx=(1.0 / sigma[0]) if sigma[0] else 0.0,
+                       isigy=(1.0 / sigma[1]) if sigma[1] else 0.0, dtemp=dtemp, t_start=t_start, t_end=t_end,
+                       s_start=s_start, s_end=s_end, scale=scale)
+        self.expn = int(expn)
+
+    def struct(self):
+        s = Source(kind=self.kind, expn=self.expn, **self.kw)
+        if self.xpos is not None:
+            s.nxp, s.nyp = len(self.xpos), len(self.ypos)
+            s.xpos, s.ypos, s.vals = dptr(self.xpos), dptr(self.ypos), dptr(self.vals)
+        return s
+
+
 class StepParams(C.Structure):
     _fields_ = [("type", C.c_int), ("n_steps", C.c_int), ("t0", C.c_double), ("dt", C.c_double),
                 ("noise", C.c_int), ("pad_", C.c_int), ("seed", C.c_uint64), ("step0", C.c_uint64),
@@ -174,6 +213,7 @@ def load(path=None):
         ("mrmg_build_tensor", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_set_layer", [C.c_void_p, C.c_int, C.POINTER(Layer)]),
         ("mrmg_set_ufield", [C.c_void_p, C.c_int, C.POINTER(UField)]),
+        ("mrmg_set_source", [C.c_void_p, C.c_int, C.c_int, C.POINTER(Source)]),
         ("mrmg_set_state", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_get_state", [C.c_void_p, C.c_int, _dp, _dp]),
         ("mrmg_set_mag", [C.c_void_p, C.c_int, _dp]),

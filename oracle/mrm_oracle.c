@@ -44,6 +44,8 @@
 #include <stdio.h>
 #include "../include/mrmg.h"
 
+#define MRMG_MAX_LAYERS_ORA 16
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -409,6 +411,7 @@ typedef struct {
     int *hxi, *hxg, *uhxi, *uhxg, *lhxi, *lhxg;
     double *hx, *uhx, *lhx;
     double *ms_scale, *alpha_scale, *hk_scale, *temp, *happ_s;
+    double *temp_s;   /* the static temperature the sources add to */
     double alpha, gamma, thm_alpha, floor_ms;
     int thermal;
     /* LLB and temperature scaling */
@@ -422,8 +425,16 @@ typedef struct {
     double max_dm;
 } olayer;
 
+/* a moving applied source (applied.f90:827-1034) with its own copies of the tables */
+typedef struct {
+    mrmg_source s;
+    double *xpos, *ypos, *vals;
+    int on;
+} osource;
+
 typedef struct ora_sim {
     int nx, ny, nl, demag;
+    osource src[MRMG_MAX_LAYERS_ORA][2];
     double dx, dy, *dz, *il;
     olayer *L;
     cplx *nh;      /* tensor spectra (nxh,ny,9,nl,nl) */
@@ -462,14 +473,21 @@ static void free_layer(olayer *L)
     void *ps[] = {L->g2gi, L->g2g, L->grid2grain, L->vol, L->ms, L->hk, L->uni, L->cub, L->hxi, L->hxg,
                   L->uhxi, L->uhxg, L->lhxi, L->lhxg, L->hx, L->uhx, L->lhx, L->ms_scale, L->alpha_scale,
                   L->hk_scale, L->temp, L->happ_s, L->m, L->mag, L->h, L->hthm, L->dm, L->h_app, L->h_demag,
-                  L->tc, L->lambda_ms};
+                  L->tc, L->lambda_ms, L->temp_s};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     memset(L, 0, sizeof(*L));
+}
+
+static void free_source(osource *o)
+{
+    free(o->xpos); free(o->ypos); free(o->vals);
+    memset(o, 0, sizeof(*o));
 }
 
 void ora_destroy(ora_sim *s)
 {
     if (!s) return;
+    for (int k = 0; k < s->nl && k < MRMG_MAX_LAYERS_ORA; k++) { free_source(&s->src[k][0]); free_source(&s->src[k][1]); }
     for (int k = 0; k < s->nl; k++) free_layer(&s->L[k]);
     free(s->L); free(s->dz); free(s->il); free(s->nh); free(s->hcell); free(s);
 }
@@ -540,6 +558,8 @@ int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
     L->hk_scale = fill1(in->hk_scale, ng);
     L->temp = (double *)calloc(ng ? ng : 1, sizeof(double));
     if (in->temp) memcpy(L->temp, in->temp, sizeof(double) * ng);
+    L->temp_s = (double *)malloc(sizeof(double) * (ng ? ng : 1));
+    memcpy(L->temp_s, L->temp, sizeof(double) * ng);
     L->happ_s = (double *)dupa(in->h_app, sizeof(double) * 3 * ng);
     L->alpha = in->alpha; L->gamma = in->gamma; L->thm_alpha = in->thm_alpha; L->floor_ms = in->floor_ms;
     L->thermal = in->thermal;
@@ -723,6 +743,139 @@ static void hk_cubic(const olayer *L, int n, const double *m, double *h)
         h[r] = -(A[r + 0] * d3[0] + A[r + 3] * d3[1] + A[r + 6] * d3[2]) * L->hk_scale[n];
 }
 
+/* ---- moving applied sources (applied.f90) ---- */
+int ora_set_source(ora_sim *s, int k, int slot, const mrmg_source *src)
+{
+    if (k < 0 || k >= s->nl || slot < 0 || slot > 1) return MRMG_ERR_ARG;
+    osource *o = &s->src[k][slot];
+    free_source(o);
+    if (!src || src->kind == MRMG_SRC_NONE) return MRMG_OK;
+    o->s = *src;
+    /* set_temp/set_fld normalisation of a constant scale (applied.f90:307-310) */
+    if (o->s.s_start == o->s.s_end) { o->s.scale = o->s.scale * o->s.s_start; o->s.s_start = 1.0; o->s.s_end = 1.0; }
+    if (src->kind != MRMG_SRC_TEMP_SG) {
+        int nv = src->kind == MRMG_SRC_FIELD_TABLE ? 3 : 1;
+        o->xpos = (double *)dupa(src->xpos, sizeof(double) * src->nxp);
+        o->ypos = (double *)dupa(src->ypos, sizeof(double) * src->nyp);
+        o->vals = (double *)dupa(src->vals, sizeof(double) * nv * src->nxp * src->nyp);
+    }
+    o->on = 1;
+    return MRMG_OK;
+}
+
+/* set (applied.f90:1757-1776): 1-based node index ind(i) and weight f(i) of cell i */
+static void src_set(double offset, int n, double dd, const double *pos, int npos, int *ind, double *f)
+{
+    int i0 = 2;
+    for (int i = 1; i <= n; i++) {
+        double x = (i - .5) * dd - offset;
+        while (pos[i0 - 1] <= x && i0 < npos) i0 = i0 + 1;
+        ind[i - 1] = npos - 1 < i0 - 1 ? npos - 1 : i0 - 1;
+        double fr = (pos[i0 - 1] - x) / (pos[i0 - 1] - pos[i0 - 2]);
+        fr = 1.0 < fr ? 1.0 : fr;
+        f[i - 1] = 0.0 > fr ? 0.0 : fr;
+    }
+}
+
+/* x**n for integer n >= 0 by binary powering */
+static double ipow(double x, int n)
+{
+    double r = 1.0;
+    while (n > 0) { if (n & 1) r = r * x; x = x * x; n >>= 1; }
+    return r;
+}
+
+/* set_sg (applied.f90:1778-1790) */
+static void src_set_sg(int expn, double sig, double del, int n, double dd, double *f)
+{
+    for (int i = 1; i <= n; i++) f[i - 1] = exp(-ipow((dd * (i - 0.5) + del) * sig, expn));
+}
+
+static double src_scale(const mrmg_source *q, double t)
+{
+    double c = (t - q->t_start) / (q->t_end - q->t_start);
+    c = c < 1.0 ? c : 1.0; c = c > 0.0 ? c : 0.0;
+    return ((q->s_end - q->s_start) * c + q->s_start) * q->scale;
+}
+
+/* Values of a source at every cell of the layer, (nv, nx, ny) x fastest per component:
+ * interpolate (applied.f90:1471-1522) or get_sg (1792-1811) at time t */
+static void src_cells(const ora_sim *s, const osource *o, double t, double *out)
+{
+    const mrmg_source *q = &o->s;
+    int nx = s->nx, ny = s->ny;
+    double sc = src_scale(q, t);
+    if (q->kind == MRMG_SRC_TEMP_SG) {
+        double *fx = (double *)malloc(sizeof(double) * nx), *fy = (double *)malloc(sizeof(double) * ny);
+        src_set_sg(q->expn, q->isigx, q->ref_x - q->vel_x * (t - q->t_origin) - q->offset_x, nx, s->dx, fx);
+        src_set_sg(q->expn, q->isigy, q->ref_y - q->vel_y * (t - q->t_origin) - q->offset_y, ny, s->dy, fy);
+        double a = q->dtemp * sc;
+        for (int j = 0; j < ny; j++) {
+            double x = a * fy[j];
+            for (int i = 0; i < nx; i++) out[i + (size_t)nx * j] = x * fx[i];
+        }
+        free(fx); free(fy);
+        return;
+    }
+    int nv = q->kind == MRMG_SRC_FIELD_TABLE ? 3 : 1, np = q->nxp;
+    int *ix = (int *)malloc(sizeof(int) * nx), *iy = (int *)malloc(sizeof(int) * ny);
+    double *fx = (double *)malloc(sizeof(double) * nx), *fy = (double *)malloc(sizeof(double) * ny);
+    src_set(q->offset_x + q->vel_x * (t - q->t_origin) - q->ref_x, nx, s->dx, o->xpos, q->nxp, ix, fx);
+    src_set(q->offset_y + q->vel_y * (t - q->t_origin) - q->ref_y, ny, s->dy, o->ypos, q->nyp, iy, fy);
+    for (int j = 0; j < ny; j++) {
+        double yf = sc * fy[j], yfc = sc * (1.0 - fy[j]);
+        for (int i = 0; i < nx; i++) {
+            size_t a00 = (size_t)(ix[i] - 1) + (size_t)np * (iy[j] - 1), a10 = a00 + 1, a01 = a00 + np, a11 = a01 + 1;
+            for (int c = 0; c < nv; c++)
+                out[(size_t)c * nx * ny + i + (size_t)nx * j] =
+                    fx[i] * yf * o->vals[nv * a00 + c] + (1.0 - fx[i]) * yf * o->vals[nv * a10 + c] +
+                    fx[i] * yfc * o->vals[nv * a01 + c] + (1.0 - fx[i]) * yfc * o->vals[nv * a11 + c];
+        }
+    }
+    free(ix); free(iy); free(fx); free(fy);
+}
+
+int ora_source_cells(ora_sim *s, int k, int slot, double t, double *out)
+{
+    if (k < 0 || k >= s->nl || slot < 0 || slot > 1 || !s->src[k][slot].on) return MRMG_ERR_ARG;
+    src_cells(s, &s->src[k][slot], t, out);
+    return MRMG_OK;
+}
+
+/* get_applied's per-grain averages (field.f90:545-556) of the layer's sources */
+static void apply_sources(ora_sim *s, int k, double t, double *hsrc)
+{
+    olayer *L = &s->L[k];
+    size_t P = (size_t)s->nx * s->ny;
+    if (s->src[k][0].on) {
+        double *v = (double *)malloc(sizeof(double) * 3 * P);
+        src_cells(s, &s->src[k][0], t, v);
+        for (int n = 0; n < L->ng; n++) {
+            double y[3] = {0, 0, 0};
+            for (int q = L->g2gi[n]; q < L->g2gi[n + 1]; q++) {
+                size_t ci = (size_t)(L->g2g[2 * q] - 1) + (size_t)s->nx * (L->g2g[2 * q + 1] - 1);
+                for (int c = 0; c < 3; c++) y[c] = y[c] + v[c * P + ci];
+            }
+            int cnt = L->g2gi[n + 1] - L->g2gi[n];
+            for (int c = 0; c < 3; c++) hsrc[3 * n + c] = y[c] / cnt;
+        }
+        free(v);
+    }
+    if (s->src[k][1].on) {
+        double *v = (double *)malloc(sizeof(double) * P);
+        src_cells(s, &s->src[k][1], t, v);
+        for (int n = 0; n < L->ng; n++) {
+            double x = 0.0;
+            for (int q = L->g2gi[n]; q < L->g2gi[n + 1]; q++)
+                x = x + v[(size_t)(L->g2g[2 * q] - 1) + (size_t)s->nx * (L->g2g[2 * q + 1] - 1)];
+            int cnt = L->g2gi[n + 1] - L->g2gi[n];
+            L->temp[n] = x / cnt + L->temp_s[n];
+        }
+        free(v);
+        temp_scaling(L, 1);   /* find_temp_scaling with t_changed */
+    }
+}
+
 /* get_field (field.f90:160-273) without the thermal part; t = time of the applied field */
 static void get_field(ora_sim *s, double t)
 {
@@ -733,9 +886,13 @@ static void get_field(ora_sim *s, double t)
     for (int k = 0; k < nl; k++) {
         olayer *L = &s->L[k];
         double u[3]; ufield_at(L, t, u);
+        double *hs = s->src[k][0].on ? (double *)calloc((size_t)3 * L->ng + 1, sizeof(double)) : NULL;
+        apply_sources(s, k, t, hs);
         temp_scaling_mag(L, in);
         for (int n = 0; n < L->ng; n++)
-            for (int c = 0; c < 3; c++) L->h_app[3 * n + c] = (L->happ_s ? L->happ_s[3 * n + c] : 0.0) + u[c];
+            for (int c = 0; c < 3; c++)
+                L->h_app[3 * n + c] = ((L->happ_s ? L->happ_s[3 * n + c] : 0.0) + (hs ? hs[3 * n + c] : 0.0)) + u[c];
+        free(hs);
     }
     /* get_magnetostatic */
     if (s->demag && s->have_tensor) {

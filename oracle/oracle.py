@@ -52,6 +52,8 @@ def lib():
             "ora_set_tensor": ([_cp, _dp], C.c_int),
             "ora_set_layer": ([_cp, C.c_int, C.POINTER(abi.Layer)], C.c_int),
             "ora_set_ufield": ([_cp, C.c_int, C.POINTER(abi.UField)], C.c_int),
+            "ora_set_source": ([_cp, C.c_int, C.c_int, C.POINTER(abi.Source)], C.c_int),
+            "ora_source_cells": ([_cp, C.c_int, C.c_int, C.c_double, _dp], C.c_int),
             "ora_set_max_dm": ([_cp, C.c_int, C.c_double], None),
             "ora_set_state": ([_cp, C.c_int, _dp], C.c_int),
             "ora_get_state": ([_cp, C.c_int, _dp, _dp], C.c_int),
@@ -200,6 +202,17 @@ class Sim:
 
     def set_ufield(self, k, uf):
         assert lib().ora_set_ufield(self.h, k, C.byref(uf)) == 0
+
+    def set_source(self, k, slot, src):
+        self._keep.append(src)
+        st = src.struct() if src is not None else abi.Source(kind=abi.MRMG_SRC_NONE)
+        assert lib().ora_set_source(self.h, k, slot, C.byref(st)) == 0
+
+    def source_cells(self, k, slot, t):
+        nv = 3 if slot == 0 else 1
+        out = np.zeros((nv, self.ny, self.nx))
+        assert lib().ora_source_cells(self.h, k, slot, t, abi.dptr(out)) == 0
+        return out
 
     def set_max_dm(self, k, deg):
         lib().ora_set_max_dm(self.h, k, deg)
