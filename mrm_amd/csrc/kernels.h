@@ -16,13 +16,40 @@
 
 #define MRMG_MAX_LAYERS 16
 
+// A moving applied source of one layer (slot 0 field, slot 1 temperature).
+struct SrcParams {
+    int kind, nxp, nyp, expn;
+    int poff_x, poff_y, voff;   // offsets of the node positions and values in the tables
+    int pad_;
+    double vel_x, vel_y, offset_x, offset_y, ref_x, ref_y, t_origin, isigx, isigy, dtemp;
+    double t0, t1, s0, s1, scale;
+};
+
 struct LayerParams {
     double alpha, gamma;
     double uf_dir[3], uf_t0, uf_t1, uf_s0, uf_s1, uf_scale;
     double spin;          // S of the LLB mean field
+    double tsigma;        // sqrt(2 kB thm_alpha / gamma) of get_thermal
+    double floor_ms;
+    double msexp, hkexp;  // temperature power laws (active when >= 0 and tscale)
     int has_uf, has_uni, has_cub, thermal;
     int off, ng;
-    int use_mag, pad_;    // Ms scale = clamp(|m|) (use_mag_p with tc_nom <= 1e10)
+    int use_mag;          // Ms scale = clamp(|m|) (use_mag_p with tc_nom <= 1e10)
+    int tscale;           // tc_nom <= 1e10
+    SrcParams src[2];
+};
+
+// Per-stage axis tables of the moving sources, [layer][slot][time][n], time 0 = the
+// evaluation, 1 = the next evaluation:
+//   tables:          xi, xa = set()'s node index (1-based) and weight along x;
+//                    yi, ya, yb = index, s*w, s*(1-w) along y (s the time scale)
+//   super-Gaussian:  xa = gx, ya = dtemp*s*gy
+struct SrcAxes {
+    const int* xi; const double* xa;
+    const int* yi; const double* ya; const double* yb;
+    const double* pos;    // node positions
+    const double* vals;   // table values
+    int nx, ny;
 };
 
 // rows of the grain state buffers
@@ -56,6 +83,8 @@ template <typename T> struct GrainData {
     const double* tc;
     const double* lam;
     const double* temp;
+    int any_src;          // some layer has a moving source
+    SrcAxes ax;
 };
 
 // The grain kernel modes
@@ -89,4 +118,5 @@ struct StepArgs {
     int diag;          // write diagnostics partials
     double dti;        // 1/sqrt(previous dt * max_dmdt), get_fields' |H| estimate
     int llb;           // push_llb instead of push_llg
+    double* tlast;     // adaptive path: grain temperature of the last field evaluation
 };

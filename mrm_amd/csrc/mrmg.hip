@@ -516,16 +516,79 @@ __device__ __forceinline__ void csr_sum(const int* __restrict__ off, const int* 
     h[2] += lane_sum<LPG>(s2);
 }
 
+// Value of a moving source at cell (i, j) of layer k from the stage's axis tables:
+// interpolate (applied.f90:1471-1522) for tables, get_sg (1792-1811) for the super-Gaussian.
+__device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& sp, int k, int slot, int ts, int i,
+                                           int j, int comp, int nv)
+{
+    const size_t bx = ((size_t)(k * 2 + slot) * 2 + ts) * ax.nx, by = ((size_t)(k * 2 + slot) * 2 + ts) * ax.ny;
+    if (sp.kind == MRMG_SRC_TEMP_SG) return ax.ya[by + j] * ax.xa[bx + i];
+    const int ix = ax.xi[bx + i], iy = ax.yi[by + j];
+    const double fx = ax.xa[bx + i], yf = ax.ya[by + j], yfc = ax.yb[by + j];
+    const double* v = ax.vals + sp.voff;
+    const size_t a00 = (size_t)(ix - 1) + (size_t)sp.nxp * (iy - 1), a10 = a00 + 1, a01 = a00 + sp.nxp, a11 = a01 + 1;
+    return fx * yf * v[nv * a00 + comp] + (1.0 - fx) * yf * v[nv * a10 + comp] + fx * yfc * v[nv * a01 + comp] +
+           (1.0 - fx) * yfc * v[nv * a11 + comp];
+}
+
+// tout (may be null): the grain temperature at the evaluation and at the next evaluation
+// from the layer's temperature source (field.f90:545-556); unchanged without one.
 template <typename T, int LPG>
 __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int lane,
                                          const T* __restrict__ mf, const T* __restrict__ Hcell, size_t CP,
-                                         double t, const T* m, T* h)
+                                         double t, const T* m, T* h, double* tout = nullptr)
 {
-    // get_applied: per-grain static average + uniform ramped field (applied.f90:871-883)
+    const int k = gd.layer_of[g];
+    const bool fsrc = L.src[0].kind == MRMG_SRC_FIELD_TABLE, tsrc = L.src[1].kind != MRMG_SRC_NONE;
+    // get_magnetostatic gather (field.f90:463-468), with the moving sources of get_applied
+    T hd[3] = {0, 0, 0};
+    double hs[3] = {0, 0, 0}, ts0 = 0, ts1 = 0;
+    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
+    if (Hcell || fsrc || tsrc) {
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int q = b0 + lane; q < b1; q += LPG) {
+            const int ci = gd.gc_idx[q];
+            if (Hcell) {
+                s0 += Hcell[ci];
+                s1 += Hcell[CP + ci];
+                s2 += Hcell[2 * CP + ci];
+            }
+            if (fsrc || tsrc) {
+                const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
+                if (fsrc)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
+                if (tsrc) {
+                    ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+                    ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+                }
+            }
+        }
+        const T cnt = (T)(b1 - b0);
+        if (Hcell) {
+            hd[0] = lane_sum<LPG>(s0) / cnt;
+            hd[1] = lane_sum<LPG>(s1) / cnt;
+            hd[2] = lane_sum<LPG>(s2) / cnt;
+        }
+        const double dc = (double)(b1 - b0);
+        if (fsrc)
+#pragma unroll
+            for (int c = 0; c < 3; c++) hs[c] = lane_sum<LPG>(hs[c]) / dc;
+        if (tsrc && tout) {
+            const double tst = gd.temp[g];
+            tout[0] = lane_sum<LPG>(ts0) / dc + tst;
+            tout[1] = lane_sum<LPG>(ts1) / dc + tst;
+        }
+    }
+    // get_applied: static per-grain field + source average + uniform ramped field
+    // (applied.f90:871-883)
     T happ[3] = {0, 0, 0};
     if (gd.happ) {
         happ[0] = gd.happ[g]; happ[1] = gd.happ[gd.G + g]; happ[2] = gd.happ[2 * gd.G + g];
     }
+    if (fsrc)
+#pragma unroll
+        for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)hs[i];
     if (L.has_uf) {
         double c = (t - L.uf_t0) / (L.uf_t1 - L.uf_t0);
         c = c < 1.0 ? c : 1.0;
@@ -534,25 +597,15 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
 #pragma unroll
         for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)(sc * L.uf_dir[i]);
     }
-    // get_magnetostatic gather (field.f90:463-468)
-    T hd[3] = {0, 0, 0};
-    if (Hcell) {
-        const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
-        T s0 = 0, s1 = 0, s2 = 0;
-        for (int q = b0 + lane; q < b1; q += LPG) {
-            const int ci = gd.gc_idx[q];
-            s0 += Hcell[ci];
-            s1 += Hcell[CP + ci];
-            s2 += Hcell[2 * CP + ci];
-        }
-        const T cnt = (T)(b1 - b0);
-        hd[0] = lane_sum<LPG>(s0) / cnt;
-        hd[1] = lane_sum<LPG>(s1) / cnt;
-        hd[2] = lane_sum<LPG>(s2) / cnt;
-    }
 #pragma unroll
     for (int i = 0; i < 3; i++) h[i] = happ[i] + hd[i];
-    const T hks = gd.hk_scale[g];
+    // find_temp_scaling of Hk at the evaluation's temperature (temp_scaling.f90:392-395)
+    T hks = gd.hk_scale[g];
+    if (tsrc && L.tscale && L.hkexp >= 0.0 && tout) {
+        double x = 1.0 - tout[0] / gd.tc[g];
+        x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
+        hks = (T)pow(x, L.hkexp);
+    }
     // get_cubic_hk (field.f90:377-395)
     if (L.has_cub && hks > (T)0) {
         const T* A = gd.cub;
@@ -585,6 +638,118 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
     if (gd.xl_off) csr_sum<T, LPG>(gd.xl_off, gd.xl_nb, gd.xl_w, mf, gd.G, g, lane, h);
 }
 
+// Grain temperature of a layer's temperature source at the two times of the stage's axes
+// (lanes split the grain's cells; every lane gets the sums): tout = mean + static temp.
+template <typename T, int LPG>
+__device__ __forceinline__ void src_temp(const GrainData<T>& gd, const LayerParams& L, int g, int lane, double* tout)
+{
+    const int k = gd.layer_of[g];
+    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
+    double ts0 = 0, ts1 = 0;
+    for (int q = b0 + lane; q < b1; q += LPG) {
+        const int ci = gd.gc_idx[q];
+        const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
+        ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+        ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+    }
+    const double dc = (double)(b1 - b0), tst = gd.temp[g];
+    tout[0] = lane_sum<LPG>(ts0) / dc + tst;
+    tout[1] = lane_sum<LPG>(ts1) / dc + tst;
+}
+
+// (1 - T/Tc)^e clamped to [0, 1] (temp_scaling.f90:390-395)
+__device__ __forceinline__ double temp_power(double T, double Tc, double e)
+{
+    double x = 1.0 - T / Tc;
+    x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
+    return pow(x, e);
+}
+
+// get_thermal's standard deviation (field.f90:407-423) for a grain whose temperature moves
+__device__ __forceinline__ double thermal_sigma_dev(double tsig, double floor_ms, double as, double T, double ms,
+                                                    double vol, bool no_ms, double mag, double msc)
+{
+    double x;
+    if (no_ms) x = as * T / (ms * vol);
+    else if (mag != 1.0) x = as * T / (ms * vol * fmax(floor_ms, fabs(mag)));
+    else if (msc > 0.0) x = as * T / (ms * vol * fmax(floor_ms, msc));
+    else x = 0.0;
+    return sqrt(x) * tsig;
+}
+
+__device__ __forceinline__ double ipow_dev(double x, int n)
+{
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r = r * x;
+        x = x * x;
+        n >>= 1;
+    }
+    return r;
+}
+
+// Axis tables of every moving source at the evaluation time t0 and the next evaluation
+// time t1: set() (applied.f90:1757-1776, node search by bisection, which finds the index
+// its monotone scan finds for increasing nodes) or set_sg() (1778-1790).
+__global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, int ny, double dx, double dy,
+                           double t0, double t1, int* __restrict__ xi, double* __restrict__ xa, int* __restrict__ yi,
+                           double* __restrict__ ya, double* __restrict__ yb, const double* __restrict__ pos)
+{
+    const int n = nx + ny;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nl * 4 * n) return;
+    int r = tid;
+    const int e = r % n; r /= n;
+    const int ts = r % 2; r /= 2;
+    const int slot = r % 2; r /= 2;
+    const int k = r;
+    const SrcParams& sp = lp[k].src[slot];
+    if (sp.kind == MRMG_SRC_NONE) return;
+    const double t = ts ? t1 : t0;
+    double c = (t - sp.t0) / (sp.t1 - sp.t0);
+    c = c < 1.0 ? c : 1.0;
+    c = c > 0.0 ? c : 0.0;
+    const double sc = ((sp.s1 - sp.s0) * c + sp.s0) * sp.scale;
+    const bool isx = e < nx;
+    const int i = isx ? e : e - nx;   // 0-based cell index; the reference's i is i+1
+    const size_t base = (size_t)(k * 2 + slot) * 2 + ts;
+    if (sp.kind == MRMG_SRC_TEMP_SG) {
+        if (isx) {
+            const double del = sp.ref_x - sp.vel_x * (t - sp.t_origin) - sp.offset_x;
+            xa[base * nx + i] = exp(-ipow_dev((dx * ((i + 1) - 0.5) + del) * sp.isigx, sp.expn));
+        } else {
+            const double del = sp.ref_y - sp.vel_y * (t - sp.t_origin) - sp.offset_y;
+            ya[base * ny + i] = sp.dtemp * sc * exp(-ipow_dev((dy * ((i + 1) - 0.5) + del) * sp.isigy, sp.expn));
+        }
+        return;
+    }
+    const double off = isx ? sp.offset_x + sp.vel_x * (t - sp.t_origin) - sp.ref_x
+                           : sp.offset_y + sp.vel_y * (t - sp.t_origin) - sp.ref_y;
+    const double dd = isx ? dx : dy;
+    const int npos = isx ? sp.nxp : sp.nyp;
+    const double* P = pos + (isx ? sp.poff_x : sp.poff_y);
+    const double x = ((i + 1) - .5) * dd - off;
+    int lo = 2, hi = npos;   // first i0 in [2, npos] with P(i0) > x, else npos (1-based)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (P[mid - 1] > x) hi = mid;
+        else lo = mid + 1;
+    }
+    const int i0 = lo;
+    const int ind = npos - 1 < i0 - 1 ? npos - 1 : i0 - 1;
+    double fr = (P[i0 - 1] - x) / (P[i0 - 1] - P[i0 - 2]);
+    fr = 1.0 < fr ? 1.0 : fr;
+    fr = 0.0 > fr ? 0.0 : fr;
+    if (isx) {
+        xi[base * nx + i] = ind;
+        xa[base * nx + i] = fr;
+    } else {
+        yi[base * ny + i] = ind;
+        ya[base * ny + i] = sc * fr;
+        yb[base * ny + i] = sc * (1.0 - fr);
+    }
+}
+
 // get_mag (data.f90:344-353): M = Ms*ms_scale*m on every cell of grain g, lanes split the
 // grain's cells.  Non-magnetic cells are never written and stay 0.
 template <typename T, int LPG>
@@ -600,15 +765,24 @@ __device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, T* __restr
     }
 }
 
+// recompute: layers whose Ms follows a moving temperature take it at the axes' first time
 template <typename T, int LPG>
-__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, const T* __restrict__ m, T* __restrict__ Mcell,
-                                                 size_t CP)
+__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict__ m, T* __restrict__ Mcell,
+                                                 size_t CP, int recompute)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = tid / LPG, lane = tid % LPG;
     if (g >= gd.G) return;
+    const LayerParams& L = gd.lp[gd.layer_of[g]];
     const T mm[3] = {m[g], m[gd.G + g], m[2 * gd.G + g]};
-    const T f = gd.lp[gd.layer_of[g]].use_mag ? gd.ms[g] * m[(size_t)ST_MSC * gd.G + g] : gd.msf[g];
+    T f = L.use_mag ? gd.ms[g] * m[(size_t)ST_MSC * gd.G + g] : gd.msf[g];
+    if (recompute && L.src[1].kind != MRMG_SRC_NONE && L.tscale && !L.use_mag && L.msexp >= 0.0) {
+        double tt[2];
+        src_temp<T, LPG>(gd, L, g, lane, tt);
+        const T msc = (T)temp_power(tt[0], gd.tc[g], L.msexp);
+        if (lane == 0) m[(size_t)ST_MSC * gd.G + g] = msc;
+        f = gd.ms[g] * msc;
+    }
     scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, f, mm);
 }
 
@@ -866,14 +1040,17 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         __syncthreads();
     }
     T h[3] = {0, 0, 0};
+    double tev[2] = {0, 0};   // grain temperature now and at the next evaluation (moving sources)
     if (MODE != GM_PUSH1 && g < G) {
         const int k = gd.layer_of[g];
         const LayerParams& L = gd.lp[k];
         // the magnetisation the field is evaluated on
         const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
         T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
-        field_at<T, LPG>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h);
+        field_at<T, LPG>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
     }
+    if (MODE == GM_PUSH1 && g < G && gd.lp[gd.layer_of[g]].src[1].kind != MRMG_SRC_NONE)
+        src_temp<T, LPG>(gd, gd.lp[gd.layer_of[g]], g, lane, tev);
     // after the lane reductions only lane 0 of each grain reads the state and writes it
     T mo[3] = {0, 0, 0};
     T fsc = 0;   // Ms * ms_scale of the new state: the cell magnetisation factor
@@ -883,6 +1060,20 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
         T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
         if (MODE == GM_PUSH1) { h[0] = hout[g]; h[1] = hout[G + g]; h[2] = hout[2 * G + g]; }
+        // the grain temperature of this evaluation and the next one
+        const bool tsrc = L.src[1].kind != MRMG_SRC_NONE;
+        double Tev = gd.temp[g], Tnx = Tev;
+        if (tsrc) {
+            Tnx = tev[1];
+            if (MODE == GM_PUSH1) {
+                Tev = a.tlast[g];
+            } else {
+                Tev = tev[0];
+                if (a.tlast && (MODE == GM_FIELD1 || MODE == GM_HEUN2)) a.tlast[g] = Tev;
+            }
+        }
+        // the Ms scale of the new state at the next evaluation follows the temperature
+        const bool dyn_ms = tsrc && L.tscale && !L.use_mag && L.msexp >= 0.0;
         // thermal draw (get_thermal) happens with the first field of a step
         T th[3];
         if (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1) {
@@ -894,7 +1085,11 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 } else {
                     philox_normals3(a.seed, a.step, g, n);
                 }
-                const T sg = gd.sigma[g];
+                T sg = gd.sigma[g];
+                if (tsrc)
+                    sg = (T)thermal_sigma_dev(L.tsigma, L.floor_ms, (double)gd.alpha_scale[g], Tev, (double)gd.ms[g],
+                                              (double)gd.vol[g], a.llb != 0, (double)ma[(size_t)ST_MAG * G + g],
+                                              (double)ma[(size_t)ST_MSC * G + g]);
 #pragma unroll
                 for (int c = 0; c < 3; c++) th[c] = sg * (T)n[c];
             }
@@ -919,7 +1114,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
             const bool stage2 = MODE == GM_HEUN2 || MODE == GM_ROT2;
             const bool heun = MODE == GM_HEUN1 || MODE == GM_HEUN2 || MODE == GM_PUSH1;
             LlbGrain q;
-            q.T = gd.temp[g]; q.Tc = gd.tc[g]; q.lam = gd.lam[g]; q.spin = L.spin; q.alpha = L.alpha;
+            q.T = Tev; q.Tc = gd.tc[g]; q.lam = gd.lam[g]; q.spin = L.spin; q.alpha = L.alpha;
             q.as = (double)gd.alpha_scale[g]; q.ms = (double)gd.ms[g];
             double mid[3], mpd[3], phd[3], thd[3], dmd[3] = {0, 0, 0}, mod[3], mago, e, e1;
 #pragma unroll
@@ -941,8 +1136,9 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 for (int c = 0; c < 3; c++) dm[(size_t)c * G + g] = (T)dmd[c];
 #pragma unroll
             for (int c = 0; c < 3; c++) mo[c] = (T)mod[c];
-            const T mscn = L.use_mag ? (T)(mago < 1.0 ? mago : 1.0) : gd.ms_scale[g];
-            fsc = L.use_mag ? gd.ms[g] * mscn : gd.msf[g];
+            const T mscn = L.use_mag ? (T)(mago < 1.0 ? mago : 1.0)
+                                     : dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : gd.ms_scale[g];
+            fsc = (L.use_mag || dyn_ms) ? gd.ms[g] * mscn : gd.msf[g];
             T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
             dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
             dst[(size_t)ST_MAG * G + g] = (T)mago;
@@ -960,8 +1156,9 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         T H[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
-        const T msc = ma[(size_t)ST_MSC * G + g];
-        fsc = L.use_mag ? gd.ms[g] * msc : gd.msf[g];
+        const T msc = mf[(size_t)ST_MSC * G + g];
+        const T mscn = dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : msc;
+        fsc = (L.use_mag || dyn_ms) ? gd.ms[g] * mscn : gd.msf[g];
         T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
         T d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
@@ -1010,6 +1207,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         }
         T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
         dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
+        if (dyn_ms) dst[(size_t)ST_MSC * G + g] = mscn;
         if (a.diag) {
             const T msv = gd.ms[g];
             const double e = msc > (T)0 ? (double)dot3(tq, tq) / ((double)msv * (double)msv) : 0.0;
@@ -1309,8 +1507,11 @@ struct HostLayer {
     bool has_uf = false;
     // LLB and temperature scaling: Tc, lambdaMs, S; use_mag = use_mag_p with tc_nom <= 1e10
     std::vector<double> tc, lambda_ms;
-    double spin = 0.5;
-    bool use_mag = false;
+    double spin = 0.5, msexp = -1.0, hkexp = -1.0;
+    bool use_mag = false, tscale = false;
+    // moving applied sources: slot 0 field table, slot 1 temperature (table or super-Gaussian)
+    mrmg_source src[2] = {};
+    std::vector<double> src_x[2], src_y[2], src_v[2];
     std::vector<double> m;     // (3,ng) host copy used before the device state exists
     std::vector<double> mag;   // (ng) |m|, likewise
 };
@@ -1417,6 +1618,11 @@ struct Engine : EngineBase {
     DBuf<T> ma, mb, mc, dm, hthm, h;
     // |m| on the device differs from the host copy the LLG thermal variance was built from
     bool mag_dirty = false;
+    // moving sources: node positions and values of every table, per-stage axis tables,
+    // and the grain temperature of the last field evaluation (adaptive path)
+    DBuf<double> src_pos, src_vals, ax_xa, ax_ya, ax_yb, tlast;
+    DBuf<int> ax_xi, ax_yi;
+    bool any_src = false;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
     // FP32 form: the compressed tensor (real/imaginary parts by parity, half of ky)
@@ -1724,12 +1930,18 @@ struct Engine : EngineBase {
             lpk.gamma = H.gamma;
             lpk.spin = H.spin;
             lpk.use_mag = H.use_mag;
+            lpk.tsigma = sqrt(2.0 * 1.380662e-16 * H.thm_alpha / H.gamma);
+            lpk.floor_ms = H.floor_ms;
+            lpk.msexp = H.msexp;
+            lpk.hkexp = H.hkexp;
+            lpk.tscale = H.tscale;
             lpk.has_uni = H.has_uni;
             lpk.has_cub = H.has_cub;
             lpk.thermal = H.thermal;
             lpk.off = off[k];
             lpk.ng = H.ng;
         }
+        RCHK(fill_sources(L));
         RCHK(fill_ufield(L));
         // exchange CSRs in global numbering
         auto csr = [&](int which, DBuf<int>& doff, DBuf<int>& dnb, DBuf<T>& dw) -> int {
@@ -1791,6 +2003,66 @@ struct Engine : EngineBase {
         RCHK(diag_part.alloc((size_t)(((size_t)G * 16 + 255) / 256) * DIAG_STRIDE));
         HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        return MRMG_OK;
+    }
+
+    // the moving sources of every layer into LayerParams and the device tables
+    int fill_sources(std::vector<HostLayer>& L)
+    {
+        const int nl = c->nl, nx = c->nx, ny = c->ny;
+        std::vector<double> pos, vals;
+        any_src = false;
+        for (int k = 0; k < nl; k++) {
+            for (int sl = 0; sl < 2; sl++) {
+                SrcParams& sp = hlp[k].src[sl];
+                sp = SrcParams{};
+                const mrmg_source& q = L[k].src[sl];
+                sp.kind = q.kind;
+                if (q.kind == MRMG_SRC_NONE) continue;
+                any_src = true;
+                sp.nxp = q.nxp; sp.nyp = q.nyp; sp.expn = q.expn;
+                sp.vel_x = q.vel_x; sp.vel_y = q.vel_y; sp.offset_x = q.offset_x; sp.offset_y = q.offset_y;
+                sp.ref_x = q.ref_x; sp.ref_y = q.ref_y; sp.t_origin = q.t_origin; sp.isigx = q.isigx; sp.isigy = q.isigy;
+                sp.dtemp = q.dtemp;
+                // set_temp/set_fld normalisation of a constant scale (applied.f90:307-310)
+                sp.t0 = q.t_start; sp.t1 = q.t_end; sp.s0 = q.s_start; sp.s1 = q.s_end; sp.scale = q.scale;
+                if (sp.s0 == sp.s1) { sp.scale = sp.scale * sp.s0; sp.s0 = 1.0; sp.s1 = 1.0; }
+                if (q.kind != MRMG_SRC_TEMP_SG) {
+                    sp.poff_x = (int)pos.size();
+                    pos.insert(pos.end(), L[k].src_x[sl].begin(), L[k].src_x[sl].end());
+                    sp.poff_y = (int)pos.size();
+                    pos.insert(pos.end(), L[k].src_y[sl].begin(), L[k].src_y[sl].end());
+                    sp.voff = (int)vals.size();
+                    vals.insert(vals.end(), L[k].src_v[sl].begin(), L[k].src_v[sl].end());
+                }
+            }
+        }
+        if (!any_src) {
+            src_pos.release(); src_vals.release(); ax_xa.release(); ax_ya.release(); ax_yb.release();
+            ax_xi.release(); ax_yi.release(); tlast.release();
+            return MRMG_OK;
+        }
+        if (pos.empty()) pos.push_back(0.0);
+        if (vals.empty()) vals.push_back(0.0);
+        RCHK(src_pos.upload(pos.data(), pos.size(), c->stream));
+        RCHK(src_vals.upload(vals.data(), vals.size(), c->stream));
+        const size_t nax = (size_t)nl * 4 * nx, nay = (size_t)nl * 4 * ny;
+        RCHK(ax_xa.alloc(nax)); RCHK(ax_xi.alloc(nax));
+        RCHK(ax_ya.alloc(nay)); RCHK(ax_yb.alloc(nay)); RCHK(ax_yi.alloc(nay));
+        HIPCHK(hipMemsetAsync(ax_xi.p, 0, ax_xi.bytes(), c->stream));
+        HIPCHK(hipMemsetAsync(ax_yi.p, 0, ax_yi.bytes(), c->stream));
+        RCHK(tlast.alloc(G));
+        return MRMG_OK;
+    }
+
+    // the axis tables of the moving sources at the evaluation time t0 and the next t1
+    int src_axes(double t0, double t1)
+    {
+        if (!any_src) return MRMG_OK;
+        const int n = c->nl * 4 * (c->nx + c->ny);
+        hipLaunchKernelGGL(k_src_axes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, lp.p, c->nl, c->nx,
+                           c->ny, c->dx, c->dy, t0, t1, ax_xi.p, ax_xa.p, ax_yi.p, ax_ya.p, ax_yb.p, src_pos.p);
+        HIPCHK(hipGetLastError());
         return MRMG_OK;
     }
 
@@ -1952,6 +2224,9 @@ struct Engine : EngineBase {
         d.lp = lp.p;
         d.tc = tc.p; d.lam = lam.p; d.temp = temp.p;
         if (cur_llb) d.sigma = sigma_llb.p;
+        d.any_src = any_src;
+        d.ax.xi = ax_xi.p; d.ax.xa = ax_xa.p; d.ax.yi = ax_yi.p; d.ax.ya = ax_ya.p; d.ax.yb = ax_yb.p;
+        d.ax.pos = src_pos.p; d.ax.vals = src_vals.p; d.ax.nx = c->nx; d.ax.ny = c->ny;
         return d;
     }
 
@@ -2011,13 +2286,15 @@ struct Engine : EngineBase {
 
     bool demag_on() const { return c->demag && tensor_ok; }
 
-    // get_mag for the state m (used where no grain kernel has written the cells yet)
-    int scatter(const T* m)
+    // get_mag for the state m (used where no grain kernel has written the cells yet); with
+    // t0 given, Ms scales that follow a moving temperature are taken at t0
+    int scatter(T* m, const double* t0 = nullptr)
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const size_t threads = (size_t)G * 8;
+        if (t0) RCHK(src_axes(*t0, *t0));
         hipLaunchKernelGGL((k_scatter<T, 8>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
-                           gdata(), m, Mcell.p, CP);
+                           gdata(), m, Mcell.p, CP, t0 && any_src ? 1 : 0);
         HIPCHK(hipGetLastError());
         return MRMG_OK;
     }
@@ -2148,9 +2425,11 @@ struct Engine : EngineBase {
                            gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
                            diag_part.p, mout);
     }
-    // mout: destination of the corrector (null: ma)
-    template <int MODE> int grain(const StepArgs& a, T* mout = nullptr)
+    // mout: destination of the corrector (null: ma); t_next: time of the next field
+    // evaluation, for the moving sources
+    template <int MODE> int grain(const StepArgs& a, T* mout = nullptr, double t_next = 0.0)
     {
+        RCHK(src_axes(a.t, t_next));
         RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1 || MODE == GM_PUSH1) ? K_GRAIN1
                                                                                                  : K_GRAIN2));
         if (lpg == 4) launch_grain<MODE, 4>(a, mout);
@@ -2200,12 +2479,13 @@ struct Engine : EngineBase {
         const double max_dm = p->max_dm * acos(-1.0) / 180.0;
         if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per));
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
-        if (demag_on()) RCHK(scatter(ma.p));
+        if (demag_on()) RCHK(scatter(ma.p, &p->t0));
         double t = p->t0, dt = p->dt, max_dmdt = 1.0, max_dmdtnext = 1.0;
         int rejected = 0;
         LayerRed r;
         StepArgs a{};
         a.llb = p->llb;
+        a.tlast = any_src ? tlast.p : nullptr;
         a.type = p->type;
         a.noise = p->noise;
         a.seed = p->seed;
@@ -2223,7 +2503,7 @@ struct Engine : EngineBase {
             a.dti = 1.0 / sqrt(dt * max_dmdt);
             a.diag = 1;
             RCHK(zero_partials());
-            RCHK(grain<GM_FIELD1>(a));
+            RCHK(grain<GM_FIELD1>(a, nullptr, t));
             HIPCHK(hipStreamSynchronize(c->stream));
             RCHK(read_partials(r));
             double nd = 1.e22;
@@ -2240,11 +2520,11 @@ struct Engine : EngineBase {
             for (;;) {
                 const double dtt = dt * max_dmdt;
                 a.dt = dtt; a.dts = dtt; a.sqrt_dt = sqrt(dtt); a.diag = 0;
-                RCHK(grain<GM_PUSH1>(a));
+                RCHK(grain<GM_PUSH1>(a, nullptr, t + dtt));
                 if (demag_on()) RCHK(demag());
                 a.t = t + dtt; a.diag = 1;
                 RCHK(zero_partials());
-                RCHK(grain<GM_HEUN2>(a, mc.p));
+                RCHK(grain<GM_HEUN2>(a, mc.p, t + dtt));
                 HIPCHK(hipStreamSynchronize(c->stream));
                 RCHK(read_partials(r));
                 bool too_far = false;
@@ -2327,7 +2607,7 @@ struct Engine : EngineBase {
         // llg() start: init_applied zeroes hthm (field.f90:703)
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         // the cells must hold the current state before the first field evaluation
-        if (demag_on()) RCHK(scatter(ma.p));
+        if (demag_on()) RCHK(scatter(ma.p, &p->t0));
         const int chunk = 64;
         const size_t per = 3 * (size_t)G;
         if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per * std::min(chunk, std::max(p->n_steps, 1))));
@@ -2358,22 +2638,22 @@ struct Engine : EngineBase {
                 // stage 1 (get_fields at t, push iter=2)
                 if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt; a.diag = 0;
-                RCHK(grain<GM_HEUN1>(a));
+                RCHK(grain<GM_HEUN1>(a, nullptr, t + dt));
                 // stage 2 (get_field at t+dt on the predicted m, push iter=1)
                 if (demag_on()) RCHK(demag());
                 a.t = t + dt; a.diag = last;
-                RCHK(grain<GM_HEUN2>(a));
+                RCHK(grain<GM_HEUN2>(a, nullptr, t + dt));
             } else if (type < 0) {
                 if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt * 0.5; a.diag = 0;
-                RCHK(grain<GM_ROT1>(a));
+                RCHK(grain<GM_ROT1>(a, nullptr, t + dt * 0.5));
                 if (demag_on()) RCHK(demag());
                 a.t = t + dt * 0.5; a.dts = dt; a.diag = last;
-                RCHK(grain<GM_ROT2>(a));
+                RCHK(grain<GM_ROT2>(a, nullptr, t + dt));
             } else {
                 if (demag_on()) RCHK(demag());
                 a.t = t; a.dts = dt; a.diag = last;
-                RCHK(grain<GM_ROT1>(a));
+                RCHK(grain<GM_ROT1>(a, nullptr, t + dt));
                 ma.swap_with(mb);
             }
             t = t + dt;
@@ -2591,6 +2871,21 @@ template <typename V> static void copy_vec(std::vector<V>& dst, const V* src, si
     else dst.clear();
 }
 
+// before a rebuild: the device state of every layer back into the host copies
+static int save_device_state(mrmg_ctx* c)
+{
+    if (c->dirty) return MRMG_OK;
+    for (int q = 0; q < c->nl; q++) {
+        HostLayer& H = c->layers[q];
+        if (!H.set) continue;
+        H.m.resize(3 * (size_t)H.ng);
+        H.mag.resize(H.ng);
+        RCHK(c->eng->get_state(q, H.m.data(), nullptr));
+        RCHK(c->eng->get_mag(q, H.mag.data()));
+    }
+    return MRMG_OK;
+}
+
 int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
 {
     if (!c || !in) return set_err(MRMG_ERR_ARG, "null argument");
@@ -2598,16 +2893,7 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
     if (in->ng < 1) return set_err(MRMG_ERR_ARG, "ng must be >= 1");
     if (!in->grid2grain || !in->grain2gridindex || !in->grain2grid || !in->grain_vol || !in->ms || !in->hk)
         return set_err(MRMG_ERR_ARG, "layer %d: required array is NULL", k);
-    // keep the device state of the other layers across the rebuild
-    if (!c->dirty) {
-        for (int q = 0; q < c->nl; q++) {
-            HostLayer& H = c->layers[q];
-            H.m.resize(3 * (size_t)H.ng);
-            H.mag.resize(H.ng);
-            RCHK(c->eng->get_state(q, H.m.data(), nullptr));
-            RCHK(c->eng->get_mag(q, H.mag.data()));
-        }
-    }
+    RCHK(save_device_state(c));
     HostLayer H;
     H.set = true;
     const int ng = in->ng;
@@ -2663,6 +2949,9 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
     // power laws of the (static) grain temperature are applied here, the |m| form at
     // every field evaluation on the device
     H.use_mag = false;
+    H.tscale = in->tc_nom <= 1.e10;
+    H.msexp = in->msexp;
+    H.hkexp = in->hkexp;
     if (in->tc_nom <= 1.e10) {
         H.use_mag = in->use_mag != 0;
         auto cl = [](double x) { return x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0; };
@@ -2677,6 +2966,12 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
     H.mag.assign(ng, 1.0);
     H.uf = c->layers[k].uf;
     H.has_uf = c->layers[k].has_uf;
+    for (int sl = 0; sl < 2; sl++) {
+        H.src[sl] = c->layers[k].src[sl];
+        H.src_x[sl] = c->layers[k].src_x[sl];
+        H.src_y[sl] = c->layers[k].src_y[sl];
+        H.src_v[sl] = c->layers[k].src_v[sl];
+    }
     c->layers[k] = std::move(H);
     c->dirty = true;
     return MRMG_OK;
@@ -2690,6 +2985,44 @@ int mrmg_set_ufield(mrmg_ctx* c, int k, const mrmg_ufield* uf)
     c->layers[k].uf = *uf;
     c->layers[k].has_uf = true;
     if (!c->dirty) return c->eng->update_ufield(c->layers);
+    return MRMG_OK;
+}
+
+int mrmg_set_source(mrmg_ctx* c, int k, int slot, const mrmg_source* src)
+{
+    if (!c || !src) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    if (slot != 0 && slot != 1) return set_err(MRMG_ERR_ARG, "source slot must be 0 (field) or 1 (temperature)");
+    const int kind = src->kind;
+    if (kind != MRMG_SRC_NONE) {
+        if (slot == 0 && kind != MRMG_SRC_FIELD_TABLE)
+            return set_err(MRMG_ERR_ARG, "slot 0 takes a field table (MRMG_SRC_FIELD_TABLE)");
+        if (slot == 1 && kind != MRMG_SRC_TEMP_TABLE && kind != MRMG_SRC_TEMP_SG)
+            return set_err(MRMG_ERR_ARG, "slot 1 takes a temperature table or super-Gaussian");
+        if (src->t_end == src->t_start) return set_err(MRMG_ERR_ARG, "t_end == t_start");
+        if (kind == MRMG_SRC_TEMP_SG) {
+            if (src->expn < 1) return set_err(MRMG_ERR_ARG, "super-Gaussian power must be >= 1");
+        } else {
+            if (src->nxp < 2 || src->nyp < 2 || !src->xpos || !src->ypos || !src->vals)
+                return set_err(MRMG_ERR_ARG, "a table needs at least 2 x 2 nodes and its arrays");
+            for (int i = 1; i < src->nxp; i++)
+                if (!(src->xpos[i] > src->xpos[i - 1])) return set_err(MRMG_ERR_ARG, "table x nodes must increase");
+            for (int i = 1; i < src->nyp; i++)
+                if (!(src->ypos[i] > src->ypos[i - 1])) return set_err(MRMG_ERR_ARG, "table y nodes must increase");
+        }
+    }
+    RCHK(save_device_state(c));
+    HostLayer& H = c->layers[k];
+    H.src[slot] = *src;
+    H.src_x[slot].clear(); H.src_y[slot].clear(); H.src_v[slot].clear();
+    if (kind == MRMG_SRC_FIELD_TABLE || kind == MRMG_SRC_TEMP_TABLE) {
+        const int nv = kind == MRMG_SRC_FIELD_TABLE ? 3 : 1;
+        H.src_x[slot].assign(src->xpos, src->xpos + src->nxp);
+        H.src_y[slot].assign(src->ypos, src->ypos + src->nyp);
+        H.src_v[slot].assign(src->vals, src->vals + (size_t)nv * src->nxp * src->nyp);
+    }
+    H.src[slot].xpos = H.src[slot].ypos = H.src[slot].vals = nullptr;
+    c->dirty = true;
     return MRMG_OK;
 }
 

@@ -72,6 +72,12 @@ class Llg:
     def set_ufield(self, k, uf):
         abi.check(self.lib, self.lib.mrmg_set_ufield(self.ctx, k, C.byref(uf)))
 
+    def set_source(self, k, slot, src):
+        """A moving applied source of layer k (slot 0: field table, slot 1: temperature
+        table or super-Gaussian; None clears): get_appl, applied.f90:827-1034."""
+        st = src.struct() if src is not None else abi.Source(kind=abi.MRMG_SRC_NONE, t_end=1.0)
+        abi.check(self.lib, self.lib.mrmg_set_source(self.ctx, k, slot, C.byref(st)))
+
     def set_state(self, k, m):
         m = np.ascontiguousarray(m, dtype=np.float64)
         abi.check(self.lib, self.lib.mrmg_set_state(self.ctx, k, abi.dptr(m)))
