@@ -150,35 +150,17 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
     }
 }
 
-// Compressed tensor of the FP32 form.  The tensor of a periodic lattice of prisms has
-// parities that make its spectrum real (xx yy zz xy) or imaginary (xz yz), and even or
-// odd in ky (xy and yz odd): one float per element for ky in [0, ny/2] holds it.
-// ct_acc adds sum_b N_ab m_b of one (ko, ks) pair at ky to h0, h1, h2.
-__device__ __forceinline__ void ct_acc(const float* __restrict__ tc, int nyh1, int ky, int ny, C2<float> mx,
-                                       C2<float> my, C2<float> mz, C2<float>& h0, C2<float>& h1, C2<float>& h2)
-{
-    const bool up = ky > ny / 2;
-    const int kk = up ? ny - ky : ky;
-    const float sg = up ? -1.f : 1.f;
-    const float xx = tc[kk], yy = tc[nyh1 + kk], zz = tc[2 * nyh1 + kk];
-    const float xy = sg * tc[3 * nyh1 + kk], xz = tc[4 * nyh1 + kk], yz = sg * tc[5 * nyh1 + kk];
-    // real v: v m; imaginary i v: (-v m.y, v m.x)
-    h0.x += xx * mx.x; h0.y += xx * mx.y; h0.x += xy * my.x; h0.y += xy * my.y; h0.x -= xz * mz.y; h0.y += xz * mz.x;
-    h1.x += xy * mx.x; h1.y += xy * mx.y; h1.x += yy * my.x; h1.y += yy * my.y; h1.x -= yz * mz.y; h1.y += yz * mz.x;
-    h2.x -= xz * mx.y; h2.y += xz * mx.x; h2.x -= yz * my.y; h2.y += yz * my.x; h2.x += zz * mz.x; h2.y += zz * mz.y;
-}
-
 // Column pass with the k-space product fused (demag.f90:117-121): one workgroup per
 // (kx, group of observation layers).  The 3*n_lay source columns of this kx are
 // transformed along y in LDS, the field columns of the group's layers are formed in place
 // (all products read into registers, barrier, then written over the group's own source
 // columns), and those columns are transformed back.  kg groups per kx keep all CUs busy.
 #define COL_MAX_THREADS 512
-template <typename T, int E, bool SYM, bool CT = false>
+template <typename T, int E, bool SYM>
 __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
                                                      const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
                                                      int nslot, int kg, int nb8, FFTPlan py,
-                                                     const C2<T>* __restrict__ twy, const float* __restrict__ Tct)
+                                                     const C2<T>* __restrict__ twy)
 {
     const int L = xcd_remap(blockIdx.x, nb8);
     const int kx = L / kg, grp = L - kx * kg;
@@ -215,11 +197,6 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
                 const C2<T> mx = {re[LP((3 * ks + 0) * ny + ky)], im[LP((3 * ks + 0) * ny + ky)]};
                 const C2<T> my = {re[LP((3 * ks + 1) * ny + ky)], im[LP((3 * ks + 1) * ny + ky)]};
                 const C2<T> mz = {re[LP((3 * ks + 2) * ny + ky)], im[LP((3 * ks + 2) * ny + ky)]};
-                if constexpr (CT) {
-                    const int nyh1 = ny / 2 + 1;
-                    ct_acc(Tct + ((((size_t)kx * nl + ko) * nl + ks) * 6) * nyh1, nyh1, ky, ny, mx, my, mz, h0, h1, h2);
-                    continue;
-                }
                 const C2<T>* t = Tk + (size_t)ks * nslot * ny;
                 C2<T> a;
                 if (SYM) {  // slots xx yy zz xy xz yz
@@ -378,10 +355,9 @@ __global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int
 
 // The k-space product over column-contiguous spectra: thread per (kx, ky), observation
 // layers in chunks of OC, each output summed over (ks, b) in the reference's order.
-template <typename T, bool SYM, int OC, bool CT = false>
+template <typename T, bool SYM, int OC>
 __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
-                                              const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot,
-                                              const float* __restrict__ Tct)
+                                              const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t Kc = (size_t)nxh * ny;
@@ -401,12 +377,6 @@ __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T
             for (int j = 0; j < OC; j++) {
                 const int ko = kc0 + j;
                 if (ko >= nl) break;
-                if constexpr (CT) {
-                    const int nyh1 = ny / 2 + 1;
-                    ct_acc(Tct + ((((size_t)kx * nl + ko) * nl + ks) * 6) * nyh1, nyh1, ky, ny, mx, my, mz, h[j][0],
-                           h[j][1], h[j][2]);
-                    continue;
-                }
                 const C2<T>* t = Tspec + ((((size_t)kx * nl + ko) * nl + ks) * nslot) * ny + ky;
                 C2<T> a;
                 if (SYM) {
@@ -1406,40 +1376,6 @@ __global__ void k_tensor_pack(const C2<double>* __restrict__ TC, C2<T>* __restri
     Tspec[i] = C2<T>{(T)v.x, (T)v.y};
 }
 
-// The compressed FP32 tensor (see ct_acc) from the FP64 spectra, with the largest
-// departure from its symmetries (dev[0]) and the largest |N| (dev[1]) as ordered bit
-// patterns of non-negative doubles.
-__global__ void k_tensor_pack_ct(const C2<double>* __restrict__ TC, float* __restrict__ Tct, int ny, int nxh, int nl,
-                                 const int* __restrict__ slot2c, unsigned long long* __restrict__ dev)
-{
-    const int nyh1 = ny / 2 + 1;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)nxh * nl * nl * 6 * nyh1;
-    if (i >= total) return;
-    size_t r = i;
-    const int ky = r % nyh1; r /= nyh1;
-    const int s = r % 6; r /= 6;
-    const int ks = r % nl; r /= nl;
-    const int ko = r % nl; r /= nl;
-    const int kx = (int)r;
-    const size_t plane = (size_t)slot2c[s] + 9 * ((size_t)ko + (size_t)nl * ks);
-    const C2<double>* P = TC + plane * nxh * ny;
-    const C2<double> v = P[(size_t)ky * nxh + kx];
-    const C2<double> w = P[(size_t)((ny - ky) % ny) * nxh + kx];
-    const double sg = (s == 3 || s == 5) ? -1.0 : 1.0;
-    double val, d;
-    if (s < 4) {
-        val = v.x;
-        d = fmax(fabs(v.y), fmax(fabs(w.x - sg * val), fabs(w.y)));
-    } else {
-        val = v.y;
-        d = fmax(fabs(v.x), fmax(fabs(w.y - sg * val), fabs(w.x)));
-    }
-    Tct[i] = (float)val;
-    atomicMax(&dev[0], (unsigned long long)__double_as_longlong(d));
-    atomicMax(&dev[1], (unsigned long long)__double_as_longlong(fmax(fabs(v.x), fabs(v.y))));
-}
-
 // ------------------------------------------------------------------------- host side
 static bool factorize(int n, FFTPlan& p)
 {
@@ -1625,10 +1561,6 @@ struct Engine : EngineBase {
     bool any_src = false;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
-    // FP32 form: the compressed tensor (real/imaginary parts by parity, half of ky)
-    DBuf<float> Tct;
-    bool ct = false, ct_ok = false;
-    double ct_dev = 0.0;
     DBuf<T> Hcell, Mcell;
     DBuf<int> slot2c;
     // launch shapes (mrmg_tune): threads of the row and column workgroups, observation-layer
@@ -1707,7 +1639,7 @@ struct Engine : EngineBase {
              hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + sigma_llb.bytes() + vol.bytes() +
              lp.bytes() + tc.bytes() + lam.bytes() + temp.bytes();
         b += ma.bytes() + mb.bytes() + mc.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
-        b += S1.bytes() + S2.bytes() + Tspec.bytes() + Tct.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
+        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
              slot2c.bytes();
         b += noise_buf.bytes() + diag_part.bytes();
         return b;
@@ -1801,11 +1733,6 @@ struct Engine : EngineBase {
         } else if (k == "lanes_per_grain") {
             if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 4, 8 or 16");
             lpg = v;
-        } else if (k == "tensor_compress") {
-            if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "tensor_compress must be 0 or 1");
-            if (v == 1 && !ct_ok)
-                return set_err(MRMG_ERR_UNSUPPORTED, "no compressed tensor (FP64 form, or the tensor lacks the parities)");
-            ct = v == 1;
         } else if (k == "col_path") {
             if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused) or 1 (split)");
             if (v == 0 && !fused_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the fused column path does not fit this grid");
@@ -1826,7 +1753,6 @@ struct Engine : EngineBase {
         else if (k == "col_groups") *v = kg;
         else if (k == "lanes_per_grain") *v = lpg;
         else if (k == "col_path") *v = split ? 1 : 0;
-        else if (k == "tensor_compress") *v = ct ? 1 : 0;
         else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         return MRMG_OK;
     }
@@ -1838,8 +1764,6 @@ struct Engine : EngineBase {
         HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
-        if constexpr (std::is_same<T, float>::value)
-            HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
     }
     template <int MB> int lds_attr_split()
@@ -2142,29 +2066,6 @@ struct Engine : EngineBase {
         hipLaunchKernelGGL(k_tensor_pack<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, TC.p,
                            Tspec.p, ny, nxh, nl, nslot, slot2c.p);
         HIPCHK(hipGetLastError());
-        // FP32 form: the compressed tensor when the spectra keep the lattice's parities to
-        // 1e-6 of the largest element (the FP64 form always uses the full spectra)
-        ct = ct_ok = false;
-        Tct.release();
-        if (std::is_same<T, float>::value && nslot == 6) {
-            const size_t tct = (size_t)nxh * nl * nl * 6 * (ny / 2 + 1);
-            DBuf<unsigned long long> dev;
-            RCHK(dev.alloc(2));
-            HIPCHK(hipMemsetAsync(dev.p, 0, dev.bytes(), c->stream));
-            RCHK(Tct.alloc(tct));
-            hipLaunchKernelGGL(k_tensor_pack_ct, dim3((unsigned)((tct + 255) / 256)), dim3(256), 0, c->stream, TC.p,
-                               Tct.p, ny, nxh, nl, slot2c.p, dev.p);
-            HIPCHK(hipGetLastError());
-            unsigned long long hd[2];
-            HIPCHK(hipMemcpyAsync(hd, dev.p, sizeof(hd), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            double d0, d1;
-            memcpy(&d0, &hd[0], 8);
-            memcpy(&d1, &hd[1], 8);
-            ct_dev = d1 > 0.0 ? d0 / d1 : 0.0;
-            ct_ok = ct = ct_dev <= 1e-6;
-            if (!ct_ok) Tct.release();
-        }
         HIPCHK(hipStreamSynchronize(c->stream));
         tensor_ok = true;
         return MRMG_OK;
@@ -2340,22 +2241,10 @@ struct Engine : EngineBase {
         const size_t Kc = (size_t)nxh * ny;
         RCHK(ev_begin(K_PROD));
         const dim3 gp((unsigned)((Kc + 255) / 256));
-        bool done = false;
-        if constexpr (std::is_same<T, float>::value) {
-            if (ct) {
-                hipLaunchKernelGGL((k_prod<T, true, 4, true>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh,
-                                   nslot, Tct.p);
-                done = true;
-            }
-        }
-        if (!done) {
-            if (nslot == 6)
-                hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot,
-                                   (const float*)nullptr);
-            else
-                hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot,
-                                   (const float*)nullptr);
-        }
+        if (nslot == 6)
+            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
+        else
+            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         RCHK(ev_begin(K_COLI));
@@ -2395,20 +2284,13 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_col_sym(int nb8)
     {
-        if constexpr (std::is_same<T, float>::value) {
-            if (ct) {
-                hipLaunchKernelGGL((k_col<T, MB, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
-                                   S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p, Tct.p);
-                return;
-            }
-        }
         hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
-                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p, (const float*)nullptr);
+                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
     template <int MB> void launch_col_gen(int nb8)
     {
         hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
-                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p, (const float*)nullptr);
+                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
     template <int MB> void launch_row_inv(dim3 g, dim3 b)
     {

@@ -113,27 +113,3 @@ def test_track_grid_precision_form():
     say(f"cfg2 grid fp32 vs fp64 after 200 steps: max {dmax:.2e} rms {(ss / n) ** 0.5:.2e}")
     assert dmax <= 1e-4
 
-
-@pytest.mark.parametrize("col_path", [0, 1])
-def test_fp32_compressed_tensor(oracle_mod, col_path):
-    """The FP32 form's compressed tensor (real or imaginary by parity, half of ky) gives
-    the demag field of the full FP32 tensor to FP32 rounding, on both column paths."""
-    med = media.make_media(64, 48, dz=[2e-7, 3e-7, 2e-7], n_lay=3, pitch=6e-7, seed=12)
-    il = [0.0, 5e-8, 5e-8]
-    tensor = oracle_mod.calculate_tensor(64, 48, 3, med.dx, med.dy, med.dz, il_space=il)
-    s, g64 = pair(oracle_mod, med, tensor=tensor, il=il)
-    _, g32 = pair(oracle_mod, med, tensor=tensor, il=il, precision=abi.MRMG_FP32)
-    assert g64.get_tune("tensor_compress") == 0
-    assert g32.get_tune("tensor_compress") == 1
-    g32.tune("col_path", col_path)
-    for k, m in enumerate(random_state(med, 5)):
-        g64.set_state(k, m)
-        g32.set_state(k, m)
-    h64, _ = g64.demag_field()
-    hc, _ = g32.demag_field()
-    g32.tune("tensor_compress", 0)
-    hf, _ = g32.demag_field()
-    sc = np.abs(h64).max()
-    e_c, e_f = np.abs(hc - h64).max() / sc, np.abs(hf - h64).max() / sc
-    say(f"fp32 demag vs fp64: compressed {e_c:.2e}, full {e_f:.2e}")
-    assert e_c <= 2e-6 and e_f <= 2e-6
