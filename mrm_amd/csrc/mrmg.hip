@@ -156,7 +156,9 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
 // (all products read into registers, barrier, then written over the group's own source
 // columns), and those columns are transformed back.  kg groups per kx keep all CUs busy.
 #define COL_MAX_THREADS 512
-template <typename T, int E, bool SYM>
+// TL: spectra in the column-contiguous layout [k][c][kx][y] of the split path (whole
+// columns are contiguous and the loads coalesce), else [k][c][y][kx].
+template <typename T, int E, bool SYM, bool TL = false>
 __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
                                                      const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
                                                      int nslot, int kg, int nb8, FFTPlan py,
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     T* im = re + LPN(ncol * ny);
     for (int idx = threadIdx.x; idx < ncol * ny; idx += blockDim.x) {
         const int col = idx / ny, y = idx - col * ny;
-        const C2<T> v = S1[((size_t)col * ny + y) * nxh + kx];
+        const C2<T> v = TL ? S1[((size_t)col * nxh + kx) * ny + y] : S1[((size_t)col * ny + y) * nxh + kx];
         re[LP(idx)] = v.x;
         im[LP(idx)] = v.y;
     }
@@ -244,7 +246,9 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const int idx = e0 + i;
         const int col = idx / ny, y = idx - col * ny;
-        S2[((size_t)col * ny + y) * nxh + kx] = C2<T>{re[LP(idx)], im[LP(idx)]};
+        const C2<T> v = {re[LP(idx)], im[LP(idx)]};
+        if (TL) S2[((size_t)col * nxh + kx) * ny + y] = v;
+        else S2[((size_t)col * ny + y) * nxh + kx] = v;
     }
 }
 
@@ -1000,7 +1004,12 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                                                const T* __restrict__ Hcell, T* __restrict__ Mcell, size_t CP,
                                                StepArgs a, double* __restrict__ diag_part, T* __restrict__ mc)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // grid padded to a multiple of 8: each XCD takes a contiguous range of grains, so the
+    // cell lines neighbouring grains share stay in one L2
+    const int nblk = (gd.G * LPG + 255) / 256;
+    const int bl = xcd_remap(blockIdx.x, gridDim.x);
+    if (bl >= nblk) return;
+    const int tid = bl * blockDim.x + threadIdx.x;
     const int g = tid / LPG, lane = tid % LPG;
     const bool w0 = lane == 0;
     const int G = gd.G;
@@ -1202,7 +1211,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     }
     if (a.diag) {
         __syncthreads();
-        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)blockIdx.x * DIAG_STRIDE + i] = red[i];
+        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)bl * DIAG_STRIDE + i] = red[i];
     }
 }
 
@@ -1569,6 +1578,7 @@ struct Engine : EngineBase {
     // split column path (column-contiguous spectra): rows per row workgroup, columns per
     // column workgroup, and whether it is in use (forced by col_path or by the grid size)
     bool split = false, fused_ok = false, split_ok = false;
+    bool tl = false;   // fused column kernel over the split path's column-contiguous spectra
     int TYr = 8, CBc = 1, maxb_rowT = 4, maxb_colT = 4;
     size_t lds_rowT = 0, lds_colT = 0;
     DBuf<double> noise_buf, diag_part;
@@ -1695,6 +1705,12 @@ struct Engine : EngineBase {
     int shapes()
     {
         const int nl = c->nl, ny = c->ny;
+        if (split || tl) {
+            maxb_rowT = pick_e(TYr / 2, c->px, row_threads);
+            if (maxb_rowT < 0)
+                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: row FFT too large for %d threads", c->nx, ny, nl,
+                               row_threads);
+        }
         if (split) {
             maxb_rowT = pick_e(TYr / 2, c->px, row_threads);
             maxb_colT = pick_e(CBc, c->py, 256);
@@ -1734,12 +1750,24 @@ struct Engine : EngineBase {
             if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 4, 8 or 16");
             lpg = v;
         } else if (k == "col_path") {
-            if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused) or 1 (split)");
-            if (v == 0 && !fused_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the fused column path does not fit this grid");
-            if (v == 1 && !split_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the split column path does not fit this grid");
-            const bool old_split = split;
+            if (v < 0 || v > 2)
+                return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused), 1 (split) or 2 (fused, column-contiguous)");
+            if (v != 1 && !fused_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the fused column kernel does not fit this grid");
+            if (v != 0 && !split_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the transposing row kernels do not fit this grid");
+            const bool os = split, ot = tl;
             split = v == 1;
-            if (shapes() != MRMG_OK) { split = old_split; return shapes(), MRMG_ERR_UNSUPPORTED; }
+            tl = v == 2;
+            if (shapes() != MRMG_OK) { split = os; tl = ot; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "row_block") {
+            if (v != 2 && v != 4 && v != 8) return set_err(MRMG_ERR_ARG, "row_block must be 2, 4 or 8");
+            const size_t l = sizeof(C2<T>) * c->nx + sizeof(T) * 2 * LPN((v / 2) * c->nx);
+            if (l > 160 * 1024 || pick_e(v / 2, c->px, 256) < 0)
+                return set_err(MRMG_ERR_UNSUPPORTED, "row_block %d does not fit LDS", v);
+            const int ot = TYr;
+            const size_t ol = lds_rowT;
+            TYr = v;
+            lds_rowT = l;
+            if (shapes() != MRMG_OK || lds_attr() != MRMG_OK) { TYr = ot; lds_rowT = ol; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else {
             return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         }
@@ -1752,7 +1780,8 @@ struct Engine : EngineBase {
         else if (k == "col_threads") *v = col_threads;
         else if (k == "col_groups") *v = kg;
         else if (k == "lanes_per_grain") *v = lpg;
-        else if (k == "col_path") *v = split ? 1 : 0;
+        else if (k == "col_path") *v = split ? 1 : tl ? 2 : 0;
+        else if (k == "row_block") *v = TYr;
         else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         return MRMG_OK;
     }
@@ -1764,6 +1793,8 @@ struct Engine : EngineBase {
         HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
     }
     template <int MB> int lds_attr_split()
@@ -2204,10 +2235,11 @@ struct Engine : EngineBase {
     int demag()
     {
         if (split) return demag_split();
-        const int nx = c->nx, ny = c->ny, nl = c->nl, nxh = c->nxh;
-        dim3 grow((ny + 1) / 2, nl), blk(row_threads);
+        const int ny = c->ny, nl = c->nl, nxh = c->nxh;
+        dim3 grow((ny + 1) / 2, nl), growT((ny + TYr - 1) / TYr, 3 * nl), blk(row_threads);
         RCHK(ev_begin(K_ROW_FWD));
-        LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
+        if (tl) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
+        else LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         const int nb8 = (nxh * kg + 7) / 8 * 8;
@@ -2217,10 +2249,10 @@ struct Engine : EngineBase {
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
         RCHK(ev_begin(K_ROW_INV));
-        LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
+        if (tl) LAUNCH_E(maxb_rowT, launch_rowT_inv, growT, blk);
+        else LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
-        (void)nx;
         return MRMG_OK;
     }
     // the split path over column-contiguous spectra
@@ -2284,13 +2316,21 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_col_sym(int nb8)
     {
-        hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
-                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+        if (tl)
+            hipLaunchKernelGGL((k_col<T, MB, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+        else
+            hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
     template <int MB> void launch_col_gen(int nb8)
     {
-        hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p, Tspec.p,
-                           c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+        if (tl)
+            hipLaunchKernelGGL((k_col<T, MB, false, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
+                               S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+        else
+            hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
     }
     template <int MB> void launch_row_inv(dim3 g, dim3 b)
     {
@@ -2303,7 +2343,8 @@ struct Engine : EngineBase {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const T* hc = demag_on() ? Hcell.p : nullptr;
         const size_t threads = (size_t)G * LPG;
-        hipLaunchKernelGGL((k_grain<T, MODE, LPG>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+        const unsigned nb8 = (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);
+        hipLaunchKernelGGL((k_grain<T, MODE, LPG>), dim3(nb8), dim3(256), 0, c->stream,
                            gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
                            diag_part.p, mout);
     }

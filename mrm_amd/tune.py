@@ -10,7 +10,8 @@ import numpy as np
 
 DEMAG = ("row_fwd", "col", "row_inv", "col_fwd", "prod", "col_inv")
 SPACE = {
-    "col_path": ([0, 1], DEMAG),              # fused column kernel or the split path
+    "col_path": ([0, 1, 2], DEMAG),           # fused column kernel, split path, fused on column-contiguous spectra
+    "row_block": ([2, 4, 8], DEMAG),          # rows per transposing row workgroup (paths 1, 2)
     "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
     "col_threads": ([256, 512], ("col",)),
     "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)

@@ -113,3 +113,23 @@ def test_track_grid_precision_form():
     say(f"cfg2 grid fp32 vs fp64 after 200 steps: max {dmax:.2e} rms {(ss / n) ** 0.5:.2e}")
     assert dmax <= 1e-4
 
+
+
+@pytest.mark.parametrize("row_block", [2, 4, 8])
+def test_column_contiguous_fused_path(oracle_mod, row_block):
+    """col_path 2 (transposing row kernels + the fused column kernel on column-contiguous
+    spectra) gives the oracle's demag field, for every rows-per-workgroup block."""
+    med = media.make_media(48, 45, dz=[2e-7, 3e-7, 2e-7], n_lay=3, pitch=6e-7, seed=93)
+    il = [0.0, 5e-8, 5e-8]
+    s, gpu = pair(oracle_mod, med, il=il)
+    gpu.tune("col_path", 2)
+    gpu.tune("row_block", row_block)
+    assert gpu.get_tune("col_path") == 2 and gpu.get_tune("row_block") == row_block
+    for k, m in enumerate(random_state(med, 4)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    hc0, hg0 = s.demag_field()
+    hc1, hg1 = gpu.demag_field()
+    err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
+    say(f"col_path 2, row_block {row_block}: rel err {err:.2e}")
+    assert err <= 1e-12
