@@ -184,9 +184,11 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twy[i];
     __syncthreads();
     lds_fft<T, E, false>(re, im, ncol, py, tw);
-    // product: work item (ko, ky); up to 4 items per thread held in registers
+    // product: work item (ko, ky); up to 4 items per thread held in registers.  The tensor
+    // values of KSB source layers are loaded before any of them is used, so a thread waits
+    // for memory once per KSB layers instead of once per layer.
     const int items = (ko1 - ko0) * ny;
-    constexpr int MAXI = 4;
+    constexpr int MAXI = 4, KSB = 4, NS = SYM ? 6 : 9;
     C2<T> hv[MAXI][3];
 #pragma unroll
     for (int q = 0; q < MAXI; q++) {
@@ -195,33 +197,47 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
             const int ko = ko0 + it / ny, ky = it % ny;
             const C2<T>* Tk = Tspec + (((size_t)kx * nl + ko) * nl) * nslot * ny + ky;
             C2<T> h0 = {0, 0}, h1 = {0, 0}, h2 = {0, 0};
-            for (int ks = 0; ks < nl; ks++) {
-                const C2<T> mx = {re[LP((3 * ks + 0) * ny + ky)], im[LP((3 * ks + 0) * ny + ky)]};
-                const C2<T> my = {re[LP((3 * ks + 1) * ny + ky)], im[LP((3 * ks + 1) * ny + ky)]};
-                const C2<T> mz = {re[LP((3 * ks + 2) * ny + ky)], im[LP((3 * ks + 2) * ny + ky)]};
-                const C2<T>* t = Tk + (size_t)ks * nslot * ny;
-                C2<T> a;
-                if (SYM) {  // slots xx yy zz xy xz yz
-                    const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
-                    a = cmul(txx, mx); h0.x += a.x; h0.y += a.y;
-                    a = cmul(txy, my); h0.x += a.x; h0.y += a.y;
-                    a = cmul(txz, mz); h0.x += a.x; h0.y += a.y;
-                    a = cmul(txy, mx); h1.x += a.x; h1.y += a.y;
-                    a = cmul(tyy, my); h1.x += a.x; h1.y += a.y;
-                    a = cmul(tyz, mz); h1.x += a.x; h1.y += a.y;
-                    a = cmul(txz, mx); h2.x += a.x; h2.y += a.y;
-                    a = cmul(tyz, my); h2.x += a.x; h2.y += a.y;
-                    a = cmul(tzz, mz); h2.x += a.x; h2.y += a.y;
-                } else {    // slot 3a+b: field component a from source component b
-                    a = cmul(t[0 * ny], mx); h0.x += a.x; h0.y += a.y;
-                    a = cmul(t[1 * ny], my); h0.x += a.x; h0.y += a.y;
-                    a = cmul(t[2 * ny], mz); h0.x += a.x; h0.y += a.y;
-                    a = cmul(t[3 * ny], mx); h1.x += a.x; h1.y += a.y;
-                    a = cmul(t[4 * ny], my); h1.x += a.x; h1.y += a.y;
-                    a = cmul(t[5 * ny], mz); h1.x += a.x; h1.y += a.y;
-                    a = cmul(t[6 * ny], mx); h2.x += a.x; h2.y += a.y;
-                    a = cmul(t[7 * ny], my); h2.x += a.x; h2.y += a.y;
-                    a = cmul(t[8 * ny], mz); h2.x += a.x; h2.y += a.y;
+            for (int kb = 0; kb < nl; kb += KSB) {
+                C2<T> tv[KSB][NS];
+#pragma unroll
+                for (int j = 0; j < KSB; j++) {
+                    if (kb + j < nl) {
+                        const C2<T>* t = Tk + (size_t)(kb + j) * nslot * ny;
+#pragma unroll
+                        for (int sl = 0; sl < NS; sl++) tv[j][sl] = t[sl * ny];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < KSB; j++) {
+                    const int ks = kb + j;
+                    if (ks >= nl) break;
+                    const C2<T> mx = {re[LP((3 * ks + 0) * ny + ky)], im[LP((3 * ks + 0) * ny + ky)]};
+                    const C2<T> my = {re[LP((3 * ks + 1) * ny + ky)], im[LP((3 * ks + 1) * ny + ky)]};
+                    const C2<T> mz = {re[LP((3 * ks + 2) * ny + ky)], im[LP((3 * ks + 2) * ny + ky)]};
+                    C2<T> a;
+                    if (SYM) {  // slots xx yy zz xy xz yz
+                        const C2<T> txx = tv[j][0], tyy = tv[j][1], tzz = tv[j][2], txy = tv[j][3], txz = tv[j][4],
+                                    tyz = tv[j][5];
+                        a = cmul(txx, mx); h0.x += a.x; h0.y += a.y;
+                        a = cmul(txy, my); h0.x += a.x; h0.y += a.y;
+                        a = cmul(txz, mz); h0.x += a.x; h0.y += a.y;
+                        a = cmul(txy, mx); h1.x += a.x; h1.y += a.y;
+                        a = cmul(tyy, my); h1.x += a.x; h1.y += a.y;
+                        a = cmul(tyz, mz); h1.x += a.x; h1.y += a.y;
+                        a = cmul(txz, mx); h2.x += a.x; h2.y += a.y;
+                        a = cmul(tyz, my); h2.x += a.x; h2.y += a.y;
+                        a = cmul(tzz, mz); h2.x += a.x; h2.y += a.y;
+                    } else {    // slot 3a+b: field component a from source component b
+                        a = cmul(tv[j][0], mx); h0.x += a.x; h0.y += a.y;
+                        a = cmul(tv[j][1], my); h0.x += a.x; h0.y += a.y;
+                        a = cmul(tv[j][2], mz); h0.x += a.x; h0.y += a.y;
+                        a = cmul(tv[j][3], mx); h1.x += a.x; h1.y += a.y;
+                        a = cmul(tv[j][4], my); h1.x += a.x; h1.y += a.y;
+                        a = cmul(tv[j][5], mz); h1.x += a.x; h1.y += a.y;
+                        a = cmul(tv[j][6], mx); h2.x += a.x; h2.y += a.y;
+                        a = cmul(tv[j][7], my); h2.x += a.x; h2.y += a.y;
+                        a = cmul(tv[j][8], mz); h2.x += a.x; h2.y += a.y;
+                    }
                 }
             }
             hv[q][0] = h0; hv[q][1] = h1; hv[q][2] = h2;

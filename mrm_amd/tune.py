@@ -27,13 +27,44 @@ def _cost(x, steps, classes, philox_seed):
     return sum(st[k][1] / max(st[k][0], 1) for k in classes if k in st)
 
 
+ROW = ("row_fwd", "row_inv")
+
+
+def _try(x, settings, classes, steps, philox_seed):
+    try:
+        for k, v in settings.items():
+            x.tune(k, v)
+    except Exception:
+        return None
+    x.step(2, philox_seed=philox_seed, diag=False)   # warm the shape
+    return _cost(x, steps, classes, philox_seed)
+
+
 def autotune(x, steps=10, philox_seed=12345, verbose=False):
-    """Choose every tunable of context x (an mrm_amd.llg.Llg).  Returns {key: value}."""
+    """Choose every tunable of context x (an mrm_amd.llg.Llg).  Returns {key: value}.
+    The column path, the rows per transposing row workgroup and the row threads are
+    searched together (the paths use different row kernels); the others one at a time."""
     saved = [x.get_state(k)[0] for k in range(x.n_lay)]
     saved_mag = [x.get_mag(k) for k in range(x.n_lay)]
     step_index = x.step_index
     chosen = {}
+    best, best_s = None, None
+    for path in SPACE["col_path"][0]:
+        for rb in (SPACE["row_block"][0] if path != 0 else [x.get_tune("row_block")]):
+            for rt in SPACE["row_threads"][0]:
+                st = {"col_path": path, "row_block": rb, "row_threads": rt}
+                c = _try(x, st, DEMAG, steps, philox_seed)
+                if verbose:
+                    print(f"  tune {st}: {c if c is None else round(c * 1e3, 2)} us", flush=True)
+                if c is not None and (best is None or c < best):
+                    best, best_s = c, st
+    if best_s is not None:
+        for k, v in best_s.items():
+            x.tune(k, v)
+        chosen.update(best_s)
     for key, (values, classes) in SPACE.items():
+        if key in ("col_path", "row_block", "row_threads"):
+            continue
         if values is None:
             values = list(range(1, min(x.n_lay, 4) + 1))
         best, best_v = None, x.get_tune(key)
