@@ -1402,13 +1402,15 @@ __global__ void k_tensor_pack(const C2<double>* __restrict__ TC, C2<T>* __restri
 }
 
 // ------------------------------------------------------------------------- host side
-static bool factorize(int n, FFTPlan& p)
+// Stockham passes of the largest radices up to maxr: fewer, heavier passes for large
+// batches, more and lighter ones when a workgroup's batch has few butterflies per pass
+static bool factorize(int n, FFTPlan& p, int maxr = 16)
 {
     p.n = n;
     p.nfac = 0;
     int m = n;
-    while (m % 16 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 16; m /= 16; }
-    while (m % 8 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 8; m /= 8; }
+    while (maxr >= 16 && m % 16 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 16; m /= 16; }
+    while (maxr >= 8 && m % 8 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 8; m /= 8; }
     while (m % 4 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 4; m /= 4; }
     while (m % 2 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 3; m /= 3; }
@@ -1596,6 +1598,9 @@ struct Engine : EngineBase {
     bool split = false, fused_ok = false, split_ok = false;
     bool tl = false;   // fused column kernel over the split path's column-contiguous spectra
     int TYr = 8, CBc = 1, maxb_rowT = 4, maxb_colT = 4;
+    // FFT plans of the step's row and column transforms and their largest radix
+    FFTPlan prow{}, pcol{};
+    int row_radix = 16, col_radix = 16;
     size_t lds_rowT = 0, lds_colT = 0;
     DBuf<double> noise_buf, diag_part;
     // instrumentation: event pairs of the current mrmg_step call, per-class totals
@@ -1681,6 +1686,8 @@ struct Engine : EngineBase {
         for (int i = 0; i < ny; i++) ty[i] = {(T)wy[i].x, (T)wy[i].y};
         RCHK(twx.upload(tx.data(), nx, c->stream));
         RCHK(twy.upload(ty.data(), ny, c->stream));
+        prow = c->px;
+        pcol = c->py;
         // fused column path: all 3*n_lay columns of one kx in one workgroup's LDS
         lds_row = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN(3 * nx);
         lds_col = sizeof(C2<T>) * ny + sizeof(T) * 2 * LPN(3 * nl * ny);
@@ -1774,6 +1781,13 @@ struct Engine : EngineBase {
             split = v == 1;
             tl = v == 2;
             if (shapes() != MRMG_OK) { split = os; tl = ot; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "row_radix" || k == "col_radix") {
+            if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "%s must be 4, 8 or 16", k.c_str());
+            FFTPlan q;
+            if (!factorize(k == "row_radix" ? c->nx : c->ny, q, v) || q.nfac > MRMG_MAX_FAC)
+                return set_err(MRMG_ERR_UNSUPPORTED, "%s %d: no plan", k.c_str(), v);
+            if (k == "row_radix") { prow = q; row_radix = v; }
+            else { pcol = q; col_radix = v; }
         } else if (k == "row_block") {
             if (v != 2 && v != 4 && v != 8) return set_err(MRMG_ERR_ARG, "row_block must be 2, 4 or 8");
             const size_t l = sizeof(C2<T>) * c->nx + sizeof(T) * 2 * LPN((v / 2) * c->nx);
@@ -1798,6 +1812,8 @@ struct Engine : EngineBase {
         else if (k == "lanes_per_grain") *v = lpg;
         else if (k == "col_path") *v = split ? 1 : tl ? 2 : 0;
         else if (k == "row_block") *v = TYr;
+        else if (k == "row_radix") *v = row_radix;
+        else if (k == "col_radix") *v = col_radix;
         else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
         return MRMG_OK;
     }
@@ -2308,50 +2324,50 @@ struct Engine : EngineBase {
     template <int MB> void launch_rowT_fwd(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
-                           TYr, c->px, twx.p);
+                           TYr, prow, twx.p);
     }
     template <int MB> void launch_colT_fwd(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_colT<T, MB, false>), g, b, lds_colT, c->stream, S1.p, c->ny, 3 * c->nl * c->nxh, CBc,
-                           c->py, twy.p);
+                           pcol, twy.p);
     }
     template <int MB> void launch_colT_inv(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_colT<T, MB, true>), g, b, lds_colT, c->stream, S2.p, c->ny, 3 * c->nl * c->nxh, CBc,
-                           c->py, twy.p);
+                           pcol, twy.p);
     }
     template <int MB> void launch_rowT_inv(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
-                           TYr, c->px, twx.p);
+                           TYr, prow, twx.p);
     }
     template <int MB> void launch_row_fwd(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
-                           c->px, twx.p);
+                           prow, twx.p);
     }
     template <int MB> void launch_col_sym(int nb8)
     {
         if (tl)
             hipLaunchKernelGGL((k_col<T, MB, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
         else
             hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
     }
     template <int MB> void launch_col_gen(int nb8)
     {
         if (tl)
             hipLaunchKernelGGL((k_col<T, MB, false, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
-                               S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+                               S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
         else
             hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, c->py, twy.p);
+                               Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
     }
     template <int MB> void launch_row_inv(dim3 g, dim3 b)
     {
         hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
-                           c->px, twx.p);
+                           prow, twx.p);
     }
 
     template <int MODE, int LPG> void launch_grain(const StepArgs& a, T* mout)

@@ -13,6 +13,8 @@ SPACE = {
     "col_path": ([0, 1, 2], DEMAG),           # fused column kernel, split path, fused on column-contiguous spectra
     "row_block": ([2, 4, 8], DEMAG),          # rows per transposing row workgroup (paths 1, 2)
     "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
+    "row_radix": ([4, 8, 16], ("row_fwd", "row_inv")),   # largest Stockham radix of the row FFTs
+    "col_radix": ([4, 8, 16], ("col", "col_fwd", "col_inv")),
     "col_threads": ([256, 512], ("col",)),
     "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)
     "lanes_per_grain": ([4, 8, 16], ("grain1", "grain2")),
@@ -52,18 +54,19 @@ def autotune(x, steps=10, philox_seed=12345, verbose=False):
     for path in SPACE["col_path"][0]:
         for rb in (SPACE["row_block"][0] if path != 0 else [x.get_tune("row_block")]):
             for rt in SPACE["row_threads"][0]:
-                st = {"col_path": path, "row_block": rb, "row_threads": rt}
-                c = _try(x, st, DEMAG, steps, philox_seed)
-                if verbose:
-                    print(f"  tune {st}: {c if c is None else round(c * 1e3, 2)} us", flush=True)
-                if c is not None and (best is None or c < best):
-                    best, best_s = c, st
+                for rr in SPACE["row_radix"][0]:
+                    st = {"col_path": path, "row_block": rb, "row_threads": rt, "row_radix": rr}
+                    c = _try(x, st, DEMAG, steps, philox_seed)
+                    if verbose:
+                        print(f"  tune {st}: {c if c is None else round(c * 1e3, 2)} us", flush=True)
+                    if c is not None and (best is None or c < best):
+                        best, best_s = c, st
     if best_s is not None:
         for k, v in best_s.items():
             x.tune(k, v)
         chosen.update(best_s)
     for key, (values, classes) in SPACE.items():
-        if key in ("col_path", "row_block", "row_threads"):
+        if key in ("col_path", "row_block", "row_threads", "row_radix"):
             continue
         if values is None:
             values = list(range(1, min(x.n_lay, 4) + 1))

@@ -133,3 +133,22 @@ def test_column_contiguous_fused_path(oracle_mod, row_block):
     err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
     say(f"col_path 2, row_block {row_block}: rel err {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("path,rr,cr", [(0, 4, 8), (0, 8, 4), (1, 4, 4), (2, 8, 8), (2, 4, 16)])
+def test_fft_radix_choices(oracle_mod, path, rr, cr):
+    """Row and column FFTs with a smaller largest radix (more, lighter Stockham passes)
+    give the oracle's demag field on every column path."""
+    med = media.make_media(64, 48, dz=[2e-7, 3e-7], n_lay=2, pitch=6e-7, seed=97)
+    s, gpu = pair(oracle_mod, med)
+    gpu.tune("col_path", path)
+    gpu.tune("row_radix", rr)
+    gpu.tune("col_radix", cr)
+    for k, m in enumerate(random_state(med, 6)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    hc0, hg0 = s.demag_field()
+    hc1, hg1 = gpu.demag_field()
+    err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
+    say(f"path {path} row radix {rr} col radix {cr}: rel err {err:.2e}")
+    assert err <= 1e-12
