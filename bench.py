@@ -110,6 +110,64 @@ def pmc_traffic(kernel, tag):
     return None
 
 
+def max_rms(a, b, n_lay):
+    dmax, ss, n = 0.0, 0.0, 0
+    for k in range(n_lay):
+        d = np.abs(a.get_state(k)[0] - b.get_state(k)[0])
+        dmax = max(dmax, float(d.max()))
+        ss += float((d ** 2).sum())
+        n += d.size
+    return dmax, (ss / max(n, 1)) ** 0.5
+
+
+def accuracy(med, N, steps):
+    """FP32 form against the FP64 form from the same start with the same Philox noise."""
+    xs = [build_ctx(med, p, tensor=N)[0] for p in (abi.MRMG_FP64, abi.MRMG_FP32)]
+    for x in xs:
+        x.step(steps, philox_seed=SEED, diag=False)
+    return max_rms(xs[0], xs[1], med.n_lay)
+
+
+def track_form(log, steps=100, warmup=10):
+    """The 512 x 4096 x 4 recording-track grid (1.5 nm cells, the two-layer media, T=300 K):
+    both forms timed, and compared after the same steps."""
+    med = media.config_media("cfg2")
+    x64, N = build_ctx(med, abi.MRMG_FP64)
+    x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
+    del N
+    out = {"workload": f"{med.nx}x{med.ny}x{med.n_lay} cells of 1.5 nm, {med.G} grains, T=300 K, Heun dt=50 fs",
+           "steps": steps}
+    for name, x in (("fp64", x64), ("fp32", x32)):
+        shapes = autotune(x, steps=3)
+        x.step(warmup, philox_seed=SEED, diag=False)
+        el, _ = timed(x, steps, warmup * DT)
+        out[name] = {"value": med.cells * steps / el, "ms_per_step": el * 1e3 / steps, "launch_shapes": shapes}
+        log(f"track grid {name}: {el * 1e3 / steps:.3f} ms/step, {med.cells * steps / el:.4e} cell-updates/s")
+    out["speedup_fp32"] = out["fp32"]["value"] / out["fp64"]["value"]
+    dmax, drms = max_rms(x64, x32, med.n_lay)
+    out["max_dm_fp32_vs_fp64"], out["rms_dm_fp32_vs_fp64"] = dmax, drms
+    out["steps_compared"] = warmup + steps
+    log(f"track grid fp32 x{out['speedup_fp32']:.2f}, max|dm| {dmax:.2e} rms {drms:.2e}")
+    x64.close()
+    x32.close()
+    return out
+
+
+def serving(log, n_requests=40, rate=40.0):
+    """The request loop under a Poisson load of config-1-style requests (FP32 form)."""
+    from mrm_amd import serve
+    reqs = serve.build_load(n_requests, rate=rate, sizes=(64, 384), steps=(200, 2000), seed=11)
+    srv = serve.Server(slots=8)
+    wall = srv.run(reqs)
+    st = serve.latency_stats(reqs, wall)
+    st["peak_admitted_bytes"] = srv.peak
+    st["load"] = (f"{n_requests} requests, Poisson {rate}/s, Nx, Ny log-uniform 64-384 (2^a 3^b 5^c), n_lay 1/2/4, "
+                  "T 0/300/400 K, 200-2000 steps, 8 slots, FP32 form")
+    srv.close()
+    log(f"serving: p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, {st['cell_updates_per_s']:.3e} cell-updates/s")
+    return st
+
+
 def cpu_baseline(med, tensor, target_s=15.0):
     """The oracle (C restatement, OpenMP) on the same workload: a bounded sample of steps."""
     from oracle import oracle as O
@@ -144,6 +202,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=10000)
+    ap.add_argument("--no-track", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -207,19 +268,17 @@ def main():
         el32, _ = timed(x32, a.steps, t0)
         v32 = med.cells * a.steps / el32
         # accuracy of the precision form: same start, same noise, same number of steps
-        ref, _ = build_ctx(med, abi.MRMG_FP64, tensor=N)
-        ref.step(a.warmup + a.steps, philox_seed=SEED)
-        dmax, drms = 0.0, 0.0
-        for k in range(med.n_lay):
-            d = np.abs(ref.get_state(k)[0] - x32.get_state(k)[0])
-            dmax = max(dmax, float(d.max()))
-            drms += float((d ** 2).sum())
-        drms = (drms / (3 * med.G)) ** 0.5
+        x32.close()
+        dmax, drms = accuracy(med, N, a.accuracy_steps)
         log(f"fp32: {el32 * 1e3 / a.steps:.4f} ms/step, {v32:.4e} cell-updates/s, x{v32 / v64:.2f}, "
-            f"max|dm| {dmax:.2e} rms {drms:.2e} after {a.warmup + a.steps} steps")
+            f"max|dm| {dmax:.2e} rms {drms:.2e} after {a.accuracy_steps} steps")
         out["fp32_form"] = {"value": v32, "unit": "cell-updates/s", "ms_per_step": el32 * 1e3 / a.steps,
                             "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64": dmax, "rms_dm_vs_fp64": drms,
-                            "steps_compared": a.warmup + a.steps, "launch_shapes": shapes32}
+                            "steps_compared": a.accuracy_steps, "launch_shapes": shapes32}
+    if not a.no_track:
+        out["track_grid"] = track_form(log)
+    if not a.no_serve:
+        out["serving"] = serving(log)
 
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(med, N)
