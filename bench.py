@@ -120,12 +120,46 @@ def max_rms(a, b, n_lay):
     return dmax, (ss / max(n, 1)) ** 0.5
 
 
-def accuracy(med, N, steps):
-    """FP32 form against the FP64 form from the same start with the same Philox noise."""
-    xs = [build_ctx(med, p, tensor=N)[0] for p in (abi.MRMG_FP64, abi.MRMG_FP32)]
-    for x in xs:
+def accuracy(med, N, steps, log):
+    """The FP32 form against the FP64 form after `steps` steps from the same start:
+    deterministic (T = 0, same media) and thermal (T = 300 K, the same Philox noise).  The
+    thermal runs switch grains by activation, which is chaotic: the control compares two
+    FP64 runs that differ only in the summation order of the cell gathers (4 instead of 8
+    lanes per grain), and `switched` counts grains whose final m_z sign differs."""
+    out = {"steps": steps}
+
+    def run(prec, thermal, lanes=None):
+        m = media.config_media("cfg1", thermal=thermal, temperature=300.0 if thermal else 0.0)
+        x = build_ctx(m, prec, tensor=N)[0]
+        if lanes:
+            x.tune("lanes_per_grain", lanes)
         x.step(steps, philox_seed=SEED, diag=False)
-    return max_rms(xs[0], xs[1], med.n_lay)
+        return x, m
+
+    def cmp(a, b, m):
+        dmax, drms = max_rms(a, b, m.n_lay)
+        sw = sum(int((np.sign(a.get_state(k)[0][:, 2]) != np.sign(b.get_state(k)[0][:, 2])).sum())
+                 for k in range(m.n_lay))
+        return {"max_dm": dmax, "rms_dm": drms, "switched": sw, "grains": m.G}
+
+    a, m = run(abi.MRMG_FP64, False)
+    b, _ = run(abi.MRMG_FP32, False)
+    out["T0_fp32_vs_fp64"] = cmp(a, b, m)
+    for x in (a, b):
+        x.close()
+    a, m = run(abi.MRMG_FP64, True)
+    b, _ = run(abi.MRMG_FP32, True)
+    c, _ = run(abi.MRMG_FP64, True, lanes=4)
+    out["T300_fp32_vs_fp64"] = cmp(a, b, m)
+    out["T300_fp64_vs_fp64_reordered"] = cmp(a, c, m)
+    for x in (a, b, c):
+        x.close()
+    log(f"accuracy after {steps} steps: T=0 fp32 vs fp64 max|dm| {out['T0_fp32_vs_fp64']['max_dm']:.2e}; "
+        f"T=300 fp32 vs fp64 {out['T300_fp32_vs_fp64']['max_dm']:.2e} "
+        f"({out['T300_fp32_vs_fp64']['switched']} grains switched differently), "
+        f"fp64 vs fp64 reordered {out['T300_fp64_vs_fp64_reordered']['max_dm']:.2e} "
+        f"({out['T300_fp64_vs_fp64_reordered']['switched']})")
+    return out
 
 
 def track_form(log, steps=100, warmup=10):
@@ -269,12 +303,11 @@ def main():
         v32 = med.cells * a.steps / el32
         # accuracy of the precision form: same start, same noise, same number of steps
         x32.close()
-        dmax, drms = accuracy(med, N, a.accuracy_steps)
-        log(f"fp32: {el32 * 1e3 / a.steps:.4f} ms/step, {v32:.4e} cell-updates/s, x{v32 / v64:.2f}, "
-            f"max|dm| {dmax:.2e} rms {drms:.2e} after {a.accuracy_steps} steps")
+        log(f"fp32: {el32 * 1e3 / a.steps:.4f} ms/step, {v32:.4e} cell-updates/s, x{v32 / v64:.2f}")
+        acc = accuracy(med, N, a.accuracy_steps, log)
         out["fp32_form"] = {"value": v32, "unit": "cell-updates/s", "ms_per_step": el32 * 1e3 / a.steps,
-                            "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64": dmax, "rms_dm_vs_fp64": drms,
-                            "steps_compared": a.accuracy_steps, "launch_shapes": shapes32}
+                            "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64_T0": acc["T0_fp32_vs_fp64"]["max_dm"],
+                            "accuracy": acc, "launch_shapes": shapes32}
     if not a.no_track:
         out["track_grid"] = track_form(log)
     if not a.no_serve:
