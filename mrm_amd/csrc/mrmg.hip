@@ -486,20 +486,33 @@ __device__ __forceinline__ T lane_sum(T v)
     return v;
 }
 
-// neighbour terms hx(j) ms_scale(nb) m(nb); the Ms scale is the state's row ST_MSC
+// neighbour terms hx(j) ms_scale(nb) m(nb) of entries [j0, j1); the Ms scale is the
+// state's row ST_MSC.  Each lane takes entries j0+lane, j0+lane+LPG, ... in order; two
+// per round, with all index and value loads of a round issued before they are used.
 template <typename T, int LPG>
-__device__ __forceinline__ void csr_sum(const int* __restrict__ off, const int* __restrict__ nb_,
-                                        const T* __restrict__ w_, const T* __restrict__ mf, int G, int g,
-                                        int lane, T* h)
+__device__ __forceinline__ void csr_sum(int j0, int j1, const int* __restrict__ nb_, const T* __restrict__ w_,
+                                        const T* __restrict__ mf, int G, int lane, T* h)
 {
     T s0 = 0, s1 = 0, s2 = 0;
     const T* __restrict__ msc = mf + (size_t)ST_MSC * G;
-    for (int j = off[g] + lane; j < off[g + 1]; j += LPG) {
-        const int nb = nb_[j];
-        const T w = w_[j] * msc[nb];
-        s0 += w * mf[nb];
-        s1 += w * mf[G + nb];
-        s2 += w * mf[2 * G + nb];
+    for (int ja = j0 + lane; ja < j1; ja += 2 * LPG) {
+        const int jb = ja + LPG;
+        const bool hb = jb < j1;
+        const int na = nb_[ja], nbb = hb ? nb_[jb] : 0;
+        const T wa = w_[ja], wb = hb ? w_[jb] : (T)0;
+        const T ca = msc[na], cb = msc[nbb];
+        const T ax = mf[na], ay = mf[G + na], az = mf[2 * G + na];
+        const T bx = mf[nbb], by = mf[G + nbb], bz = mf[2 * G + nbb];
+        T w = wa * ca;
+        s0 += w * ax;
+        s1 += w * ay;
+        s2 += w * az;
+        if (hb) {
+            w = wb * cb;
+            s0 += w * bx;
+            s1 += w * by;
+            s2 += w * bz;
+        }
     }
     h[0] += lane_sum<LPG>(s0);
     h[1] += lane_sum<LPG>(s1);
@@ -530,11 +543,33 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
 {
     const int k = gd.layer_of[g];
     const bool fsrc = L.src[0].kind == MRMG_SRC_FIELD_TABLE, tsrc = L.src[1].kind != MRMG_SRC_NONE;
+    // every offset this grain needs, issued together
+    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
+    int xi0 = 0, xi1 = 0, xu0 = 0, xu1 = 0, xl0 = 0, xl1 = 0;
+    if (gd.xi_off) { xi0 = gd.xi_off[g]; xi1 = gd.xi_off[g + 1]; }
+    if (gd.xu_off) { xu0 = gd.xu_off[g]; xu1 = gd.xu_off[g + 1]; }
+    if (gd.xl_off) { xl0 = gd.xl_off[g]; xl1 = gd.xl_off[g + 1]; }
     // get_magnetostatic gather (field.f90:463-468), with the moving sources of get_applied
     T hd[3] = {0, 0, 0};
     double hs[3] = {0, 0, 0}, ts0 = 0, ts1 = 0;
-    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
-    if (Hcell || fsrc || tsrc) {
+    if (Hcell && !fsrc && !tsrc) {
+        // each lane takes cells b0+lane, b0+lane+LPG, ... in order, two per round with the
+        // index and field loads of a round issued before use
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int qa = b0 + lane; qa < b1; qa += 2 * LPG) {
+            const int qb = qa + LPG;
+            const bool hb = qb < b1;
+            const int ca = gd.gc_idx[qa], cb = hb ? gd.gc_idx[qb] : ca;
+            const T ax = Hcell[ca], ay = Hcell[CP + ca], az = Hcell[2 * CP + ca];
+            const T bx = Hcell[cb], by = Hcell[CP + cb], bz = Hcell[2 * CP + cb];
+            s0 += ax; s1 += ay; s2 += az;
+            if (hb) { s0 += bx; s1 += by; s2 += bz; }
+        }
+        const T cnt = (T)(b1 - b0);
+        hd[0] = lane_sum<LPG>(s0) / cnt;
+        hd[1] = lane_sum<LPG>(s1) / cnt;
+        hd[2] = lane_sum<LPG>(s2) / cnt;
+    } else if (Hcell || fsrc || tsrc) {
         T s0 = 0, s1 = 0, s2 = 0;
         for (int q = b0 + lane; q < b1; q += LPG) {
             const int ci = gd.gc_idx[q];
@@ -623,9 +658,9 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         for (int i = 0; i < 3; i++) h[i] = h[i] + s * u[i];
     }
     // exchange CSRs (field.f90:326-363): intralayer, layer above, layer below
-    if (gd.xi_off) csr_sum<T, LPG>(gd.xi_off, gd.xi_nb, gd.xi_w, mf, gd.G, g, lane, h);
-    if (gd.xu_off) csr_sum<T, LPG>(gd.xu_off, gd.xu_nb, gd.xu_w, mf, gd.G, g, lane, h);
-    if (gd.xl_off) csr_sum<T, LPG>(gd.xl_off, gd.xl_nb, gd.xl_w, mf, gd.G, g, lane, h);
+    if (gd.xi_off) csr_sum<T, LPG>(xi0, xi1, gd.xi_nb, gd.xi_w, mf, gd.G, lane, h);
+    if (gd.xu_off) csr_sum<T, LPG>(xu0, xu1, gd.xu_nb, gd.xu_w, mf, gd.G, lane, h);
+    if (gd.xl_off) csr_sum<T, LPG>(xl0, xl1, gd.xl_nb, gd.xl_w, mf, gd.G, lane, h);
 }
 
 // Grain temperature of a layer's temperature source at the two times of the stage's axes
