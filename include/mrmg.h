@@ -51,7 +51,8 @@ extern "C" {
 
 /* ---- precision forms -------------------------------------------------------- */
 #define MRMG_FP64 0   /* plain form: every stage in FP64 (reference built with FFTD)      */
-#define MRMG_FP32 1   /* precision form: FFT, convolution and push in FP32, |m|=1 renorm */
+#define MRMG_FP32 1   /* precision form: cell fields, FFT and convolution in FP32; grain state,
+                        field assembly and push in FP64 (the reference built with ffttr = 4)  */
 
 /* ---- thermal noise sources --------------------------------------------------- */
 #define MRMG_NOISE_NONE   0  /* no thermal field (thermal_p = .false.)                    */

@@ -536,9 +536,9 @@ __device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& s
 
 // tout (may be null): the grain temperature at the evaluation and at the next evaluation
 // from the layer's temperature source (field.f90:545-556); unchanged without one.
-template <typename T, int LPG>
+template <typename T, int LPG, typename TC>
 __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int lane,
-                                         const T* __restrict__ mf, const T* __restrict__ Hcell, size_t CP,
+                                         const T* __restrict__ mf, const TC* __restrict__ Hcell, size_t CP,
                                          double t, const T* m, T* h, double* tout = nullptr)
 {
     const int k = gd.layer_of[g];
@@ -560,8 +560,8 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
             const int qb = qa + LPG;
             const bool hb = qb < b1;
             const int ca = gd.gc_idx[qa], cb = hb ? gd.gc_idx[qb] : ca;
-            const T ax = Hcell[ca], ay = Hcell[CP + ca], az = Hcell[2 * CP + ca];
-            const T bx = Hcell[cb], by = Hcell[CP + cb], bz = Hcell[2 * CP + cb];
+            const T ax = (T)Hcell[ca], ay = (T)Hcell[CP + ca], az = (T)Hcell[2 * CP + ca];
+            const T bx = (T)Hcell[cb], by = (T)Hcell[CP + cb], bz = (T)Hcell[2 * CP + cb];
             s0 += ax; s1 += ay; s2 += az;
             if (hb) { s0 += bx; s1 += by; s2 += bz; }
         }
@@ -574,9 +574,9 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         for (int q = b0 + lane; q < b1; q += LPG) {
             const int ci = gd.gc_idx[q];
             if (Hcell) {
-                s0 += Hcell[ci];
-                s1 += Hcell[CP + ci];
-                s2 += Hcell[2 * CP + ci];
+                s0 += (T)Hcell[ci];
+                s1 += (T)Hcell[CP + ci];
+                s2 += (T)Hcell[2 * CP + ci];
             }
             if (fsrc || tsrc) {
                 const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
@@ -777,11 +777,11 @@ __global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, i
 
 // get_mag (data.f90:344-353): M = Ms*ms_scale*m on every cell of grain g, lanes split the
 // grain's cells.  Non-magnetic cells are never written and stay 0.
-template <typename T, int LPG>
-__device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, T* __restrict__ Mcell, size_t CP, int g,
+template <typename T, int LPG, typename TC>
+__device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, TC* __restrict__ Mcell, size_t CP, int g,
                                               int lane, T f, const T* m)
 {
-    const T v0 = f * m[0], v1 = f * m[1], v2 = f * m[2];
+    const TC v0 = (TC)(f * m[0]), v1 = (TC)(f * m[1]), v2 = (TC)(f * m[2]);
     for (int q = gd.gc_off[g] + lane; q < gd.gc_off[g + 1]; q += LPG) {
         const int ci = gd.gc_idx[q];
         Mcell[ci] = v0;
@@ -791,8 +791,8 @@ __device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, T* __restr
 }
 
 // recompute: layers whose Ms follows a moving temperature take it at the axes' first time
-template <typename T, int LPG>
-__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict__ m, T* __restrict__ Mcell,
+template <typename T, int LPG, typename TC>
+__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict__ m, TC* __restrict__ Mcell,
                                                  size_t CP, int recompute)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -808,7 +808,7 @@ __global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict_
         if (lane == 0) m[(size_t)ST_MSC * gd.G + g] = msc;
         f = gd.ms[g] * msc;
     }
-    scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, f, mm);
+    scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, f, mm);
 }
 
 // ------------------------------------------------------------------------- LLB push
@@ -1049,10 +1049,10 @@ __device__ __forceinline__ void rot_push(int type, T scale1, T scale2, const T* 
 }
 
 
-template <typename T, int MODE, int LPG>
+template <typename T, int MODE, int LPG, typename TC>
 __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ ma, T* __restrict__ mb,
                                                T* __restrict__ dm, T* __restrict__ hthm, T* __restrict__ hout,
-                                               const T* __restrict__ Hcell, T* __restrict__ Mcell, size_t CP,
+                                               const TC* __restrict__ Hcell, TC* __restrict__ Mcell, size_t CP,
                                                StepArgs a, double* __restrict__ diag_part, T* __restrict__ mc)
 {
     // grid padded to a multiple of 8: each XCD takes a contiguous range of grains, so the
@@ -1077,7 +1077,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         // the magnetisation the field is evaluated on
         const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
         T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
-        field_at<T, LPG>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
+        field_at<T, LPG, TC>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
     }
     if (MODE == GM_PUSH1 && g < G && gd.lp[gd.layer_of[g]].src[1].kind != MRMG_SRC_NONE)
         src_temp<T, LPG>(gd, gd.lp[gd.layer_of[g]], g, lane, tev);
@@ -1258,7 +1258,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
 #pragma unroll
         for (int c = 0; c < 3; c++) mb3[c] = __shfl(mo[c], src);
         const T f = __shfl(fsc, src);
-        scatter_cells<T, LPG>(gd, Mcell, CP, g, lane, f, mb3);
+        scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, f, mb3);
     }
     if (a.diag) {
         __syncthreads();
@@ -1599,21 +1599,25 @@ static int pick_e(int batch, const FFTPlan& p, int threads)
         }                                                                                             \
     } while (0)
 
+// T: the type of the demag path (cell fields, spectra, tensor, FFT).  The grain state, the
+// grain constants and the field assembly are FP64 in both forms, as in the reference's
+// build with single-precision FFTs (ffttr = 4, data.f90:326).
 template <typename T>
 struct Engine : EngineBase {
+    using S = double;
     mrmg_ctx* c;
     int G = 0, Ctot = 0, nslot = 0;
     std::vector<int> off;
     // grain data
     DBuf<int> layer_of, gc_off, gc_idx, cell2grain;
     DBuf<int> xi_off, xi_nb, xu_off, xu_nb, xl_off, xl_nb;
-    DBuf<T> xi_w, xu_w, xl_w;
-    DBuf<T> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
+    DBuf<S> xi_w, xu_w, xl_w;
+    DBuf<S> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
     DBuf<double> tc, lam, temp;
     DBuf<LayerParams> lp;
     // state, ST_ROWS rows of G (mc: corrector output of the adaptive path until the step
     // is accepted)
-    DBuf<T> ma, mb, mc, dm, hthm, h;
+    DBuf<S> ma, mb, mc, dm, hthm, h;
     // |m| on the device differs from the host copy the LLG thermal variance was built from
     bool mag_dirty = false;
     // moving sources: node positions and values of every table, per-stage axis tables,
@@ -1966,9 +1970,9 @@ struct Engine : EngineBase {
         RCHK(fill_sources(L));
         RCHK(fill_ufield(L));
         // exchange CSRs in global numbering
-        auto csr = [&](int which, DBuf<int>& doff, DBuf<int>& dnb, DBuf<T>& dw) -> int {
+        auto csr = [&](int which, DBuf<int>& doff, DBuf<int>& dnb, DBuf<S>& dw) -> int {
             std::vector<int> o(G + 1, 0), nb;
-            std::vector<T> w;
+            std::vector<S> w;
             for (int k = 0; k < nl; k++) {
                 const HostLayer& H = L[k];
                 const std::vector<int>* ix = which == 0 ? &H.hxi : which == 1 ? &H.uhxi : &H.lhxi;
@@ -1984,7 +1988,7 @@ struct Engine : EngineBase {
                             const int gg = (*gx)[e];
                             if (gg < 1 || gg > L[tk].ng) return set_err(MRMG_ERR_ARG, "layer %d: exchange neighbour %d out of range", k, gg);
                             nb.push_back(off[tk] + gg - 1);
-                            w.push_back((T)(*vx)[e]);
+                            w.push_back((*vx)[e]);
                         }
                     }
                     o[g + 1] = (int)nb.size();
@@ -1999,11 +2003,7 @@ struct Engine : EngineBase {
         if (has_xu) RCHK(csr(1, xu_off, xu_nb, xu_w)); else { xu_off.release(); xu_nb.release(); xu_w.release(); }
         if (has_xl) RCHK(csr(2, xl_off, xl_nb, xl_w)); else { xl_off.release(); xl_nb.release(); xl_w.release(); }
 
-        auto upT = [&](DBuf<T>& d, const std::vector<double>& v) -> int {
-            std::vector<T> t(v.size());
-            for (size_t i = 0; i < v.size(); i++) t[i] = (T)v[i];
-            return d.upload(t.data(), t.size(), c->stream);
-        };
+        auto upT = [&](DBuf<S>& d, const std::vector<double>& v) -> int { return d.upload(v.data(), v.size(), c->stream); };
         RCHK(layer_of.upload(h_layer.data(), G, c->stream));
         RCHK(gc_off.upload(h_gco.data(), G + 1, c->stream));
         RCHK(gc_idx.upload(h_gci.data(), Ctot, c->stream));
@@ -2181,11 +2181,11 @@ struct Engine : EngineBase {
     int set_state(int k, const double* m) override
     {
         const int ng = c->layers[k].ng;
-        std::vector<T> t(3 * (size_t)ng);
+        std::vector<S> t(3 * (size_t)ng);
         for (int n = 0; n < ng; n++)
-            for (int cc = 0; cc < 3; cc++) t[(size_t)cc * ng + n] = (T)m[3 * n + cc];
+            for (int cc = 0; cc < 3; cc++) t[(size_t)cc * ng + n] = (S)m[3 * n + cc];
         for (int cc = 0; cc < 3; cc++)
-            HIPCHK(hipMemcpyAsync(ma.p + (size_t)cc * G + off[k], t.data() + (size_t)cc * ng, sizeof(T) * ng,
+            HIPCHK(hipMemcpyAsync(ma.p + (size_t)cc * G + off[k], t.data() + (size_t)cc * ng, sizeof(S) * ng,
                                   hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return MRMG_OK;
@@ -2194,13 +2194,13 @@ struct Engine : EngineBase {
     int get_state(int k, double* m, double* hh) override
     {
         const int ng = c->layers[k].ng;
-        std::vector<T> t(3 * (size_t)ng);
+        std::vector<S> t(3 * (size_t)ng);
         for (int pass = 0; pass < 2; pass++) {
             double* out = pass == 0 ? m : hh;
-            T* src = pass == 0 ? ma.p : h.p;
+            S* src = pass == 0 ? ma.p : h.p;
             if (!out) continue;
             for (int cc = 0; cc < 3; cc++)
-                HIPCHK(hipMemcpyAsync(t.data() + (size_t)cc * ng, src + (size_t)cc * G + off[k], sizeof(T) * ng,
+                HIPCHK(hipMemcpyAsync(t.data() + (size_t)cc * ng, src + (size_t)cc * G + off[k], sizeof(S) * ng,
                                       hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             for (int n = 0; n < ng; n++)
@@ -2209,9 +2209,9 @@ struct Engine : EngineBase {
         return MRMG_OK;
     }
 
-    GrainData<T> gdata() const
+    GrainData<S> gdata() const
     {
-        GrainData<T> d;
+        GrainData<S> d;
         d.G = G;
         d.layer_of = layer_of.p;
         d.ms = ms.p; d.msf = msf.p; d.ms_scale = ms_scale.p; d.alpha_scale = alpha_scale.p; d.hk_scale = hk_scale.p;
@@ -2235,23 +2235,23 @@ struct Engine : EngineBase {
     int sync_aux()
     {
         const size_t off3 = (size_t)ST_MAG * G, n = 2 * (size_t)G;
-        HIPCHK(hipMemcpyAsync(mb.p + off3, ma.p + off3, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(mc.p + off3, ma.p + off3, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(mb.p + off3, ma.p + off3, n * sizeof(S), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(mc.p + off3, ma.p + off3, n * sizeof(S), hipMemcpyDeviceToDevice, c->stream));
         return MRMG_OK;
     }
     // the LLG thermal variance depends on |m| when it is not 1: rebuild it from the device |m|
     int refresh_sigma()
     {
-        std::vector<T> t(G);
-        HIPCHK(hipMemcpyAsync(t.data(), ma.p + (size_t)ST_MAG * G, sizeof(T) * G, hipMemcpyDeviceToHost, c->stream));
+        std::vector<S> t(G);
+        HIPCHK(hipMemcpyAsync(t.data(), ma.p + (size_t)ST_MAG * G, sizeof(S) * G, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        std::vector<T> sg(G);
+        std::vector<S> sg(G);
         for (int k = 0; k < c->nl; k++) {
             HostLayer& H = c->layers[k];
             H.mag.resize(H.ng);
             for (int n = 0; n < H.ng; n++) {
                 H.mag[n] = (double)t[off[k] + n];
-                sg[off[k] + n] = (T)thermal_sigma(H, n, H.mag[n], false);
+                sg[off[k] + n] = (S)thermal_sigma(H, n, H.mag[n], false);
             }
         }
         RCHK(sigma.upload(sg.data(), G, c->stream));
@@ -2262,13 +2262,13 @@ struct Engine : EngineBase {
     int set_mag(int k, const double* mg) override
     {
         const HostLayer& H = c->layers[k];
-        std::vector<T> a(H.ng), b(H.ng);
+        std::vector<S> a(H.ng), b(H.ng);
         for (int n = 0; n < H.ng; n++) {
-            a[n] = (T)mg[n];
-            b[n] = (T)field_ms_scale(H, n, mg[n]);
+            a[n] = (S)mg[n];
+            b[n] = (S)field_ms_scale(H, n, mg[n]);
         }
-        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MAG * G + off[k], a.data(), sizeof(T) * H.ng, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MSC * G + off[k], b.data(), sizeof(T) * H.ng, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MAG * G + off[k], a.data(), sizeof(S) * H.ng, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(ma.p + (size_t)ST_MSC * G + off[k], b.data(), sizeof(S) * H.ng, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         mag_dirty = true;
         return MRMG_OK;
@@ -2276,8 +2276,8 @@ struct Engine : EngineBase {
     int get_mag(int k, double* mg) override
     {
         const HostLayer& H = c->layers[k];
-        std::vector<T> a(H.ng);
-        HIPCHK(hipMemcpyAsync(a.data(), ma.p + (size_t)ST_MAG * G + off[k], sizeof(T) * H.ng, hipMemcpyDeviceToHost, c->stream));
+        std::vector<S> a(H.ng);
+        HIPCHK(hipMemcpyAsync(a.data(), ma.p + (size_t)ST_MAG * G + off[k], sizeof(S) * H.ng, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int n = 0; n < H.ng; n++) mg[n] = (double)a[n];
         return MRMG_OK;
@@ -2287,12 +2287,12 @@ struct Engine : EngineBase {
 
     // get_mag for the state m (used where no grain kernel has written the cells yet); with
     // t0 given, Ms scales that follow a moving temperature are taken at t0
-    int scatter(T* m, const double* t0 = nullptr)
+    int scatter(S* m, const double* t0 = nullptr)
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const size_t threads = (size_t)G * 8;
         if (t0) RCHK(src_axes(*t0, *t0));
-        hipLaunchKernelGGL((k_scatter<T, 8>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_scatter<S, 8, T>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
                            gdata(), m, Mcell.p, CP, t0 && any_src ? 1 : 0);
         HIPCHK(hipGetLastError());
         return MRMG_OK;
@@ -2405,19 +2405,19 @@ struct Engine : EngineBase {
                            prow, twx.p);
     }
 
-    template <int MODE, int LPG> void launch_grain(const StepArgs& a, T* mout)
+    template <int MODE, int LPG> void launch_grain(const StepArgs& a, S* mout)
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
         const T* hc = demag_on() ? Hcell.p : nullptr;
         const size_t threads = (size_t)G * LPG;
         const unsigned nb8 = (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);
-        hipLaunchKernelGGL((k_grain<T, MODE, LPG>), dim3(nb8), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_grain<S, MODE, LPG, T>), dim3(nb8), dim3(256), 0, c->stream,
                            gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
                            diag_part.p, mout);
     }
     // mout: destination of the corrector (null: ma); t_next: time of the next field
     // evaluation, for the moving sources
-    template <int MODE> int grain(const StepArgs& a, T* mout = nullptr, double t_next = 0.0)
+    template <int MODE> int grain(const StepArgs& a, S* mout = nullptr, double t_next = 0.0)
     {
         RCHK(src_axes(a.t, t_next));
         RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1 || MODE == GM_PUSH1) ? K_GRAIN1

@@ -46,7 +46,7 @@ def footprint_estimate(med, precision):
     tensor = 6 * nl * nl * kc * 2 * P
     spectra = 2 * 3 * nl * kc * 2 * P
     cells = 2 * 3 * med.cells * P
-    grains = 40 * P * med.G
+    grains = 40 * 8 * med.G          # grain state and constants are FP64 in both forms
     return tensor + spectra + cells + grains + (1 << 20)
 
 
