@@ -21,13 +21,39 @@
 // elements of an LDS array holding n padded entries
 #define LPN(n) ((n) + ((n) >> 5) + 1)
 
+// Pass p has radix fac[p] and span ns[p] (the product of the earlier radices).  The
+// divisions of a pass's index arithmetic use multipliers fixed when the plan is made:
+// x / d == umulhi(x, mnr or mns) for the indices of one workgroup (x * d < 2^32), d == 1
+// taken separately.  Reading them per pass also keeps the compiler from hoisting the
+// index arithmetic of every radix branch out of the pass loop.
 struct FFTPlan {
     int n;
     int nfac;
     int fac[MRMG_MAX_FAC];
+    int ns[MRMG_MAX_FAC];
+    int tstep[MRMG_MAX_FAC];        // twiddle stride n / (ns * fac)
+    unsigned mnr[MRMG_MAX_FAC];     // multiplier of n / fac
+    unsigned mns[MRMG_MAX_FAC];     // multiplier of ns
 };
 
+__host__ __device__ inline unsigned fdiv_mul(int d) { return d > 1 ? (unsigned)((0x100000000ull + d - 1) / d) : 0u; }
+__device__ __forceinline__ int fdiv(int x, int d, unsigned m) { return d == 1 ? x : (int)__umulhi((unsigned)x, m); }
+
+// fill the per-pass fields of a plan whose n, nfac and fac are set
+inline void plan_passes(FFTPlan& p)
+{
+    int ns = 1;
+    for (int i = 0; i < p.nfac; i++) {
+        p.ns[i] = ns;
+        p.tstep[i] = p.n / (ns * p.fac[i]);
+        p.mnr[i] = fdiv_mul(p.n / p.fac[i]);
+        p.mns[i] = fdiv_mul(ns);
+        ns *= p.fac[i];
+    }
+}
+
 template <typename T> struct C2 { T x, y; };
+template <typename T> struct C4 { C2<T> a, b; };
 
 template <typename T>
 __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
@@ -170,23 +196,36 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
     }
 };
 
-// One Stockham pass over B transforms of length N with span Ns.
-template <typename T, int R, int MAXB, bool INV>
-__device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, int B, const C2<T>* __restrict__ tw)
+// Ordering point between the read and write halves of a wave-local pass: the wave's own
+// LDS operations are complete (s_waitcnt lgkmcnt(0)) and the compiler moves no memory
+// access across it.  Other waves are not involved.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0), vmcnt and expcnt left alone
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham pass over B transforms of length N with span Ns, by nt threads numbered t
+// (the workgroup, or one wave when WAVE: then only that wave synchronises).
+template <typename T, int R, int MAXB, bool INV, bool WAVE = false>
+__device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, int B, const C2<T>* __restrict__ tw,
+                                         int t, int nt, int tstep, unsigned mnr, unsigned mns)
 {
     const int NR = N / R;
     const int total = B * NR;
-    const int tstep = N / (Ns * R);
     C2<T> v[MAXB][R];
     int dst[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
-        const int idx = threadIdx.x + q * blockDim.x;
+        const int idx = t + q * nt;
         dst[q] = -1;
         if (idx < total) {
-            const int b = idx / NR;
+            const int b = fdiv(idx, NR, mnr);
             const int j = idx - b * NR;
-            const int k = j % Ns;
+            const int jq = fdiv(j, Ns, mns);
+            const int k = j - jq * Ns;
             const int base = off + b * N;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -202,10 +241,11 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
                 }
             }
             Dft<T, R, INV>::run(v[q]);
-            dst[q] = base + (j / Ns) * Ns * R + k;
+            dst[q] = base + jq * Ns * R + k;
         }
     }
-    __syncthreads();
+    if (WAVE) wave_sync();
+    else __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
         if (dst[q] >= 0) {
@@ -216,7 +256,8 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
             }
         }
     }
-    __syncthreads();
+    if (WAVE) wave_sync();
+    else __syncthreads();
 }
 
 // Full transform: every thread of the workgroup must call it (it synchronises).
@@ -226,24 +267,78 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
 template <typename T, int E, bool INV>
 __device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw, int off = 0)
 {
-    int Ns = 1;
     const int N = plan.n;
+    const int t = threadIdx.x, nt = blockDim.x;
     for (int p = 0; p < plan.nfac; p++) {
-        const int R = plan.fac[p];
-        if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV>(re, im, off, N, Ns, B, tw);
-        else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV>(re, im, off, N, Ns, B, tw);
-        else if (R == 4) fft_pass<T, 4, (E + 3) / 4, INV>(re, im, off, N, Ns, B, tw);
-        else if (R == 2) fft_pass<T, 2, (E + 1) / 2, INV>(re, im, off, N, Ns, B, tw);
-        else if (R == 3) fft_pass<T, 3, (E + 2) / 3, INV>(re, im, off, N, Ns, B, tw);
-        else fft_pass<T, 5, (E + 4) / 5, INV>(re, im, off, N, Ns, B, tw);
-        Ns *= R;
+        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.tstep[p];
+        const unsigned a = plan.mnr[p], c = plan.mns[p];
+        if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
+        else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
+        else if (R == 4) fft_pass<T, 4, (E + 3) / 4, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
+        else if (R == 2) fft_pass<T, 2, (E + 1) / 2, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
+        else if (R == 3) fft_pass<T, 3, (E + 2) / 3, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
+        else fft_pass<T, 5, (E + 4) / 5, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
     }
 }
 
-// Copy a twiddle table into LDS (every thread calls; synchronises).
-template <typename T>
-__device__ __forceinline__ void stage_twiddles(C2<T>* dst, const C2<T>* __restrict__ src, int n)
+// Wave-local form: the B transforms are dealt out to the waves in contiguous runs of
+// ceil(B / waves), and each wave transforms its run alone, with no workgroup barrier
+// (the runs are disjoint).  E bounds the elements one lane holds: ceil(B/waves)*N <= 64*E.
+// The caller synchronises the workgroup before any wave reads another wave's transforms.
+template <typename T, int E, bool INV>
+__device__ void lds_fft_wave(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw, int off = 0)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
-    __syncthreads();
+    const int nw = (int)blockDim.x >> 6, w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int bpw = (B + nw - 1) / nw;
+    const int b0 = w * bpw;
+    const int nb = B - b0 < bpw ? B - b0 : bpw;
+    if (nb <= 0) return;   // uniform over the wave
+    const int N = plan.n;
+    const int o = off + b0 * N;
+    for (int p = 0; p < plan.nfac; p++) {
+        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.tstep[p];
+        const unsigned a = plan.mnr[p], c = plan.mns[p];
+        if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+        else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+        else if (R == 4) fft_pass<T, 4, (E + 3) / 4, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+        else if (R == 2) fft_pass<T, 2, (E + 1) / 2, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+        else if (R == 3) fft_pass<T, 3, (E + 2) / 3, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+        else fft_pass<T, 5, (E + 4) / 5, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
+    }
+}
+
+// Global -> LDS staging of a strided loop over [0, total): each thread issues the loads of
+// R of its elements before it stores any of them, so it waits for memory once per R
+// elements instead of once per element (a load followed by its LDS store in one loop
+// iteration serialises on the load's latency).  ld(idx) loads element idx and must be
+// unconditional (an index past the end is clamped to the last element and its value
+// dropped); st(idx, v) applies any masking and stores.  tw_dst/tw_src/n_tw: a twiddle
+// table copied alongside, its loads issued after the first chunk's.  No barrier.
+template <int R, typename T, typename Ld, typename St>
+__device__ __forceinline__ void staged_load(int total, Ld ld, St st, C2<T>* tw_dst = nullptr,
+                                            const C2<T>* __restrict__ tw_src = nullptr, int n_tw = 0)
+{
+    const int nt = (int)blockDim.x, t0 = (int)threadIdx.x;
+    bool tw_done = tw_dst == nullptr;
+    for (int base = 0; base < total || !tw_done; base += R * nt) {
+        decltype(ld(0)) v[R];
+        if (base < total) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = base + r * nt + t0;
+                v[r] = ld(idx < total ? idx : total - 1);
+            }
+        }
+        if (!tw_done) {
+            for (int i = t0; i < n_tw; i += nt) tw_dst[i] = tw_src[i];
+            tw_done = true;
+        }
+        if (base < total) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = base + r * nt + t0;
+                if (idx < total) st(idx, v[r]);
+            }
+        }
+    }
 }

@@ -126,16 +126,17 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(3 * nx);
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
     const size_t CP = (size_t)nl * ny * nx;
     const T* r0 = Mcell + ((size_t)k * ny + y0) * nx;
-    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
-        const int c = idx / nx, x = idx - c * nx;
-        re[LP(idx)] = r0[c * CP + x];
-        im[LP(idx)] = has1 ? r0[c * CP + nx + x] : (T)0;
-    }
+    staged_load<(E < 8 ? E : 8)>(
+        3 * nx,
+        [&](int idx) {
+            const int c = idx / nx, x = idx - c * nx;
+            return C2<T>{r0[c * CP + x], r0[c * CP + (has1 ? nx : 0) + x]};
+        },
+        [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = has1 ? v.y : (T)0; }, tw, twx, nx);
     __syncthreads();
     lds_fft<T, E, false>(re, im, 3, px, tw);
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
@@ -158,7 +159,9 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
 #define COL_MAX_THREADS 512
 // TL: spectra in the column-contiguous layout [k][c][kx][y] of the split path (whole
 // columns are contiguous and the loads coalesce), else [k][c][y][kx].
-template <typename T, int E, bool SYM, bool TL = false>
+// WV: each wave transforms its own run of columns (no workgroup barrier inside the FFTs);
+// E then bounds one lane's elements of its wave's run.
+template <typename T, int E, bool SYM, bool TL = false, bool WV = false>
 __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
                                                      const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
                                                      int nslot, int kg, int nb8, FFTPlan py,
@@ -175,15 +178,20 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + ny);
     T* im = re + LPN(ncol * ny);
-    for (int idx = threadIdx.x; idx < ncol * ny; idx += blockDim.x) {
-        const int col = idx / ny, y = idx - col * ny;
-        const C2<T> v = TL ? S1[((size_t)col * nxh + kx) * ny + y] : S1[((size_t)col * ny + y) * nxh + kx];
-        re[LP(idx)] = v.x;
-        im[LP(idx)] = v.y;
-    }
-    for (int i = threadIdx.x; i < ny; i += blockDim.x) tw[i] = twy[i];
+    staged_load<(E < 8 ? E : 8)>(
+        ncol * ny,
+        [&](int idx) {
+            const int col = idx / ny, y = idx - col * ny;
+            return TL ? S1[((size_t)col * nxh + kx) * ny + y] : S1[((size_t)col * ny + y) * nxh + kx];
+        },
+        [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; }, tw, twy, ny);
     __syncthreads();
-    lds_fft<T, E, false>(re, im, ncol, py, tw);
+    if (WV) {
+        lds_fft_wave<T, E, false>(re, im, ncol, py, tw);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, false>(re, im, ncol, py, tw);
+    }
     // product: work item (ko, ky); up to 4 items per thread held in registers.  The tensor
     // values of KSB source layers are loaded before any of them is used, so a thread waits
     // for memory once per KSB layers instead of once per layer.
@@ -258,7 +266,12 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     }
     __syncthreads();
     const int e0 = 3 * ko0 * ny, ne = 3 * (ko1 - ko0) * ny;
-    lds_fft<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
+    if (WV) {
+        lds_fft_wave<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
+    }
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const int idx = e0 + i;
         const int col = idx / ny, y = idx - col * ny;
@@ -278,22 +291,27 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(3 * nx);
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
-    for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
-        const int c = idx / nxh, kx = idx - c * nxh;
-        const C2<T>* row = S2 + (((size_t)k * 3 + c) * ny + y0) * nxh;
-        C2<T> A = row[kx];
-        C2<T> B = has1 ? row[nxh + kx] : C2<T>{0, 0};
-        if (kx == 0 || 2 * kx == nx) { A.y = 0; B.y = 0; }
-        re[LP(c * nx + kx)] = A.x - B.y;
-        im[LP(c * nx + kx)] = A.y + B.x;
-        if (kx > 0 && nx - kx != kx) {
-            re[LP(c * nx + nx - kx)] = A.x + B.y;
-            im[LP(c * nx + nx - kx)] = B.x - A.y;
-        }
-    }
+    staged_load<(E < 4 ? E : 4)>(
+        3 * nxh,
+        [&](int idx) {
+            const int c = idx / nxh, kx = idx - c * nxh;
+            const C2<T>* row = S2 + (((size_t)k * 3 + c) * ny + y0) * nxh;
+            return C4<T>{row[kx], row[(has1 ? nxh : 0) + kx]};
+        },
+        [&](int idx, C4<T> v) {
+            const int c = idx / nxh, kx = idx - c * nxh;
+            C2<T> A = v.a, B = has1 ? v.b : C2<T>{0, 0};
+            if (kx == 0 || 2 * kx == nx) { A.y = 0; B.y = 0; }
+            re[LP(c * nx + kx)] = A.x - B.y;
+            im[LP(c * nx + kx)] = A.y + B.x;
+            if (kx > 0 && nx - kx != kx) {
+                re[LP(c * nx + nx - kx)] = A.x + B.y;
+                im[LP(c * nx + nx - kx)] = B.x - A.y;
+            }
+        },
+        tw, twx, nx);
     __syncthreads();
     lds_fft<T, E, true>(re, im, 3, px, tw);
     const size_t plane = (size_t)nx * ny;
@@ -313,7 +331,7 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
 //   k_prod      the k-space product (demag.f90:117-121), one thread per (kx, ky)
 //   k_colT      inverse y-FFT, in place
 //   k_rowT_inv  c2r of TY rows, read back from TY-long y runs
-template <typename T, int E>
+template <typename T, int E, bool WV = false>
 __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
                                                   int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
@@ -323,19 +341,30 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(B * nx);
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = blockIdx.x * TY, kc = blockIdx.y;        // kc = k*3 + c
     const int k = kc / 3, c = kc - 3 * k;
     const size_t CP = (size_t)nl * ny * nx;
     const T* src = Mcell + c * CP + (size_t)k * ny * nx;
-    for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
-        const int p = idx / nx, x = idx - p * nx;
-        const int ya = y0 + 2 * p, yb = ya + 1;
-        re[LP(idx)] = ya < ny ? src[(size_t)ya * nx + x] : (T)0;
-        im[LP(idx)] = yb < ny ? src[(size_t)yb * nx + x] : (T)0;
-    }
+    staged_load<(E < 8 ? E : 8)>(
+        B * nx,
+        [&](int idx) {
+            const int p = idx / nx, x = idx - p * nx;
+            const int ya = y0 + 2 * p, yb = ya + 1;
+            return C2<T>{src[(size_t)(ya < ny ? ya : ny - 1) * nx + x], src[(size_t)(yb < ny ? yb : ny - 1) * nx + x]};
+        },
+        [&](int idx, C2<T> v) {
+            const int p = idx / nx, ya = y0 + 2 * p;
+            re[LP(idx)] = ya < ny ? v.x : (T)0;
+            im[LP(idx)] = ya + 1 < ny ? v.y : (T)0;
+        },
+        tw, twx, nx);
     __syncthreads();
-    lds_fft<T, E, false>(re, im, B, px, tw);
+    if (WV) {
+        lds_fft_wave<T, E, false>(re, im, B, px, tw);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, false>(re, im, B, px, tw);
+    }
     C2<T>* dst = S + (size_t)kc * nxh * ny;
     for (int idx = threadIdx.x; idx < nxh * TY; idx += blockDim.x) {
         const int kx = idx / TY, yy = idx - kx * TY;
@@ -363,11 +392,8 @@ __global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int
     const int nb = min(CB, ncols - q0);
     T* im = re + LPN(CB * ny);
     C2<T>* base = S + (size_t)q0 * ny;
-    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
-        const C2<T> v = base[idx];
-        re[LP(idx)] = v.x;
-        im[LP(idx)] = v.y;
-    }
+    staged_load<(E < 8 ? E : 8), T>(
+        nb * ny, [&](int idx) { return base[idx]; }, [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
     __syncthreads();
     lds_fft<T, E, INV>(re, im, nb, py, twy);
     for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
@@ -435,7 +461,7 @@ __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T
 
 // c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
 // imaginary parts of the DC and Nyquist bins ignored).
-template <typename T, int E>
+template <typename T, int E, bool WV = false>
 __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T* __restrict__ Hcell,
                                                   int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
@@ -445,26 +471,37 @@ __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(B * nx);
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twx[i];
     const int y0 = blockIdx.x * TY, kc = blockIdx.y;
     const int k = kc / 3, c = kc - 3 * k;
     const C2<T>* src = S + (size_t)kc * nxh * ny;
     // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
-    for (int idx = threadIdx.x; idx < nxh * B; idx += blockDim.x) {
-        const int kx = idx / B, p = idx - kx * B;
-        const int ya = y0 + 2 * p, yb = ya + 1;
-        C2<T> A = ya < ny ? src[(size_t)kx * ny + ya] : C2<T>{0, 0};
-        C2<T> Bv = yb < ny ? src[(size_t)kx * ny + yb] : C2<T>{0, 0};
-        if (kx == 0 || 2 * kx == nx) { A.y = 0; Bv.y = 0; }
-        re[LP(p * nx + kx)] = A.x - Bv.y;
-        im[LP(p * nx + kx)] = A.y + Bv.x;
-        if (kx > 0 && nx - kx != kx) {
-            re[LP(p * nx + nx - kx)] = A.x + Bv.y;
-            im[LP(p * nx + nx - kx)] = Bv.x - A.y;
-        }
-    }
+    staged_load<(E < 4 ? E : 4)>(
+        nxh * B,
+        [&](int idx) {
+            const int kx = idx / B, p = idx - kx * B;
+            const int ya = y0 + 2 * p, yb = ya + 1;
+            return C4<T>{src[(size_t)kx * ny + (ya < ny ? ya : ny - 1)], src[(size_t)kx * ny + (yb < ny ? yb : ny - 1)]};
+        },
+        [&](int idx, C4<T> v) {
+            const int kx = idx / B, p = idx - kx * B;
+            const int ya = y0 + 2 * p;
+            C2<T> A = ya < ny ? v.a : C2<T>{0, 0}, Bv = ya + 1 < ny ? v.b : C2<T>{0, 0};
+            if (kx == 0 || 2 * kx == nx) { A.y = 0; Bv.y = 0; }
+            re[LP(p * nx + kx)] = A.x - Bv.y;
+            im[LP(p * nx + kx)] = A.y + Bv.x;
+            if (kx > 0 && nx - kx != kx) {
+                re[LP(p * nx + nx - kx)] = A.x + Bv.y;
+                im[LP(p * nx + nx - kx)] = Bv.x - A.y;
+            }
+        },
+        tw, twx, nx);
     __syncthreads();
-    lds_fft<T, E, true>(re, im, B, px, tw);
+    if (WV) {
+        lds_fft_wave<T, E, true>(re, im, B, px, tw);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, true>(re, im, B, px, tw);
+    }
     T* hc = Hcell + ((size_t)c * nl + k) * (size_t)nx * ny;
     for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
         const int p = idx / nx, x = idx - p * nx;
@@ -1450,6 +1487,7 @@ static bool factorize(int n, FFTPlan& p, int maxr = 16)
     while (m % 2 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 3; m /= 3; }
     while (m % 5 == 0 && p.nfac < MRMG_MAX_FAC) { p.fac[p.nfac++] = 5; m /= 5; }
+    plan_passes(p);
     return m == 1;
 }
 
@@ -1636,6 +1674,7 @@ struct Engine : EngineBase {
     // column workgroup, and whether it is in use (forced by col_path or by the grid size)
     bool split = false, fused_ok = false, split_ok = false;
     bool tl = false;   // fused column kernel over the split path's column-contiguous spectra
+    bool wv = false;   // wave-local FFTs in the transposing row kernels and the path-2 column kernel
     int TYr = 8, CBc = 1, maxb_rowT = 4, maxb_colT = 4;
     // FFT plans of the step's row and column transforms and their largest radix
     FFTPlan prow{}, pcol{};
@@ -1767,14 +1806,18 @@ struct Engine : EngineBase {
     int shapes()
     {
         const int nl = c->nl, ny = c->ny;
+        // wave-local transforms: a lane's bound covers its wave's run of ceil(B / waves)
+        auto row_e = [&]() {
+            const int nw = row_threads / 64, b = TYr / 2;
+            return wv ? pick_e((b + nw - 1) / nw, c->px, 64) : pick_e(b, c->px, row_threads);
+        };
         if (split || tl) {
-            maxb_rowT = pick_e(TYr / 2, c->px, row_threads);
+            maxb_rowT = row_e();
             if (maxb_rowT < 0)
                 return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: row FFT too large for %d threads", c->nx, ny, nl,
                                row_threads);
         }
         if (split) {
-            maxb_rowT = pick_e(TYr / 2, c->px, row_threads);
             maxb_colT = pick_e(CBc, c->py, 256);
             if (maxb_rowT < 0 || maxb_colT < 0)
                 return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: split-path FFT too large for %d threads", c->nx, ny,
@@ -1782,7 +1825,8 @@ struct Engine : EngineBase {
             return MRMG_OK;
         }
         maxb_row = pick_e(3, c->px, row_threads);
-        maxb_col = pick_e(3 * nl, c->py, col_threads);
+        const int ncw = col_threads / 64;
+        maxb_col = tl && wv ? pick_e((3 * nl + ncw - 1) / ncw, c->py, 64) : pick_e(3 * nl, c->py, col_threads);
         if (maxb_row < 0 || maxb_col < 0)
             return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs (threads %d/%d)",
                            c->nx, ny, nl, row_threads, col_threads);
@@ -1820,6 +1864,11 @@ struct Engine : EngineBase {
             split = v == 1;
             tl = v == 2;
             if (shapes() != MRMG_OK) { split = os; tl = ot; return shapes(), MRMG_ERR_UNSUPPORTED; }
+        } else if (k == "wave_fft") {
+            if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "wave_fft must be 0 or 1");
+            const bool ow = wv;
+            wv = v == 1;
+            if (shapes() != MRMG_OK) { wv = ow; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else if (k == "row_radix" || k == "col_radix") {
             if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "%s must be 4, 8 or 16", k.c_str());
             FFTPlan q;
@@ -1851,6 +1900,7 @@ struct Engine : EngineBase {
         else if (k == "lanes_per_grain") *v = lpg;
         else if (k == "col_path") *v = split ? 1 : tl ? 2 : 0;
         else if (k == "row_block") *v = TYr;
+        else if (k == "wave_fft") *v = wv ? 1 : 0;
         else if (k == "row_radix") *v = row_radix;
         else if (k == "col_radix") *v = col_radix;
         else return set_err(MRMG_ERR_ARG, "unknown tunable '%s'", k.c_str());
@@ -1866,6 +1916,8 @@ struct Engine : EngineBase {
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
     }
     template <int MB> int lds_attr_split()
@@ -1873,6 +1925,8 @@ struct Engine : EngineBase {
         const int r = (int)lds_rowT, cl = (int)lds_colT;
         HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
         HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+        HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
         HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
@@ -2358,8 +2412,12 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_rowT_fwd(dim3 g, dim3 b)
     {
-        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
-                           TYr, prow, twx.p);
+        if (wv)
+            hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
+                               c->nxh, TYr, prow, twx.p);
+        else
+            hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
+                               c->nxh, TYr, prow, twx.p);
     }
     template <int MB> void launch_colT_fwd(dim3 g, dim3 b)
     {
@@ -2373,8 +2431,12 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_rowT_inv(dim3 g, dim3 b)
     {
-        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
-                           TYr, prow, twx.p);
+        if (wv)
+            hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
+                               c->nxh, TYr, prow, twx.p);
+        else
+            hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
+                               c->nxh, TYr, prow, twx.p);
     }
     template <int MB> void launch_row_fwd(dim3 g, dim3 b)
     {
@@ -2383,7 +2445,10 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_col_sym(int nb8)
     {
-        if (tl)
+        if (tl && wv)
+            hipLaunchKernelGGL((k_col<T, MB, true, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
+                               S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
+        else if (tl)
             hipLaunchKernelGGL((k_col<T, MB, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
                                Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
         else
@@ -2392,7 +2457,10 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_col_gen(int nb8)
     {
-        if (tl)
+        if (tl && wv)
+            hipLaunchKernelGGL((k_col<T, MB, false, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream,
+                               S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
+        else if (tl)
             hipLaunchKernelGGL((k_col<T, MB, false, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
                                S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
         else

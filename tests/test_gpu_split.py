@@ -152,3 +152,32 @@ def test_fft_radix_choices(oracle_mod, path, rr, cr):
     err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
     say(f"path {path} row radix {rr} col radix {cr}: rel err {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("path,row_block,row_threads,col_threads,cr,precision", [
+    (2, 8, 256, 512, 16, abi.MRMG_FP64), (2, 4, 128, 512, 8, abi.MRMG_FP64), (2, 2, 64, 256, 4, abi.MRMG_FP64),
+    (1, 8, 256, 512, 16, abi.MRMG_FP64), (2, 8, 128, 256, 8, abi.MRMG_FP32), (1, 4, 256, 512, 4, abi.MRMG_FP32)])
+def test_wave_local_ffts(oracle_mod, path, row_block, row_threads, col_threads, cr, precision):
+    """wave_fft 1: each wave transforms its own run of rows or columns with no workgroup
+    barrier inside the FFT.  The butterflies are those of the workgroup form, so the demag
+    field is bit-for-bit the same, and (FP64) within 1e-12 of the oracle.  Sizes with
+    radix-3 and radix-5 passes and runs that leave waves idle (9 columns over 4 or 8 waves)."""
+    med = media.make_media(48, 45, dz=[2e-7, 3e-7, 2e-7], n_lay=3, pitch=6e-7, seed=95)
+    s, gpu = pair(oracle_mod, med, precision=precision)
+    fields = []
+    for wv in (0, 1):
+        for key, v in (("col_path", path), ("row_block", row_block), ("row_threads", row_threads),
+                       ("col_threads", col_threads), ("col_radix", cr), ("wave_fft", wv)):
+            gpu.tune(key, v)
+        assert gpu.get_tune("wave_fft") == wv and gpu.get_tune("col_path") == path
+        for k, m in enumerate(random_state(med, 7)):
+            s.set_state(k, m)
+            gpu.set_state(k, m)
+        fields.append(gpu.demag_field())
+    assert np.array_equal(fields[0][0], fields[1][0]) and np.array_equal(fields[0][1], fields[1][1])
+    if precision == abi.MRMG_FP64:
+        hc0, hg0 = s.demag_field()
+        hc1, hg1 = fields[1]
+        err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
+        say(f"wave ffts path {path} rb {row_block} rt {row_threads} ct {col_threads} cr {cr}: rel err {err:.2e}")
+        assert err <= 1e-12
