@@ -75,10 +75,11 @@ template <typename T> struct GrainData {
     const T* vol;
     const int* gc_off;    // grain -> cells CSR (grain2gridindex)
     const int* gc_idx;    //   global cell index k*P + j*nx + i
-    const int* xi_off; const int* xi_nb; const T* xi_w;   // intralayer exchange CSR
-    const int* xu_off; const int* xu_nb; const T* xu_w;   // to the layer above
-    const int* xl_off; const int* xl_nb; const T* xl_w;   // to the layer below
+    // exchange CSR, per grain the intralayer entries, then those to the layer above, then
+    // to the layer below; x_nb/x_w and gc_idx carry one spare entry at the end
+    const int* x_off; const int* x_nb; const T* x_w;
     const LayerParams* lp;
+    int nl;
     // LLB (FP64 in both forms): Tc, lambda = lambdaMs (S+1)/S, grain temperature
     const double* tc;
     const double* lam;

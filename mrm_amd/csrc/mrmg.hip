@@ -573,40 +573,71 @@ __device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& s
 
 // tout (may be null): the grain temperature at the evaluation and at the next evaluation
 // from the layer's temperature source (field.f90:545-556); unchanged without one.
+// Loads are arranged for memory-level parallelism: the offsets and constants first, then
+// per round (each lane two cells and two exchange entries) all index loads, then all value
+// loads, then the sums.  Out-of-range positions load a valid entry (the arrays carry one
+// spare entry at the end) and add nothing.
 template <typename T, int LPG, typename TC>
-__device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int g, int lane,
+__device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int k, int g, int lane,
                                          const T* __restrict__ mf, const TC* __restrict__ Hcell, size_t CP,
                                          double t, const T* m, T* h, double* tout = nullptr)
 {
-    const int k = gd.layer_of[g];
+    const int G = gd.G;
     const bool fsrc = L.src[0].kind == MRMG_SRC_FIELD_TABLE, tsrc = L.src[1].kind != MRMG_SRC_NONE;
-    // every offset this grain needs, issued together
     const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
-    int xi0 = 0, xi1 = 0, xu0 = 0, xu1 = 0, xl0 = 0, xl1 = 0;
-    if (gd.xi_off) { xi0 = gd.xi_off[g]; xi1 = gd.xi_off[g + 1]; }
-    if (gd.xu_off) { xu0 = gd.xu_off[g]; xu1 = gd.xu_off[g + 1]; }
-    if (gd.xl_off) { xl0 = gd.xl_off[g]; xl1 = gd.xl_off[g + 1]; }
-    // get_magnetostatic gather (field.f90:463-468), with the moving sources of get_applied
-    T hd[3] = {0, 0, 0};
+    const int x0 = gd.x_off[g], x1 = gd.x_off[g + 1];
+    const T hk_s = gd.hk_scale[g], hk1 = gd.hk1[g], hk2 = gd.hk2[g];
+    const T u[3] = {gd.uni[g], gd.uni[G + g], gd.uni[2 * G + g]};
+    T happ[3] = {0, 0, 0};
+    if (gd.happ) {
+        happ[0] = gd.happ[g]; happ[1] = gd.happ[G + g]; happ[2] = gd.happ[2 * G + g];
+    }
+    const T* __restrict__ msc = mf + (size_t)ST_MSC * G;
+    // get_magnetostatic gather (field.f90:463-468) and the exchange CSR (field.f90:326-363:
+    // intralayer, then the layers above and below, merged per grain)
+    T hd[3] = {0, 0, 0}, hx[3] = {0, 0, 0};
     double hs[3] = {0, 0, 0}, ts0 = 0, ts1 = 0;
-    if (Hcell && !fsrc && !tsrc) {
-        // each lane takes cells b0+lane, b0+lane+LPG, ... in order, two per round with the
-        // index and field loads of a round issued before use
-        T s0 = 0, s1 = 0, s2 = 0;
-        for (int qa = b0 + lane; qa < b1; qa += 2 * LPG) {
-            const int qb = qa + LPG;
-            const bool hb = qb < b1;
-            const int ca = gd.gc_idx[qa], cb = hb ? gd.gc_idx[qb] : ca;
-            const T ax = (T)Hcell[ca], ay = (T)Hcell[CP + ca], az = (T)Hcell[2 * CP + ca];
-            const T bx = (T)Hcell[cb], by = (T)Hcell[CP + cb], bz = (T)Hcell[2 * CP + cb];
-            s0 += ax; s1 += ay; s2 += az;
-            if (hb) { s0 += bx; s1 += by; s2 += bz; }
+    if (!fsrc && !tsrc) {
+        T s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0, e2 = 0;
+        const int nc = b1 - b0, ne = x1 - x0;
+        const int ncr = (nc + 2 * LPG - 1) / (2 * LPG), ner = (ne + 2 * LPG - 1) / (2 * LPG);
+        const int nr = ncr > ner ? ncr : ner;
+        for (int r = 0; r < nr; r++) {
+            const int qa = b0 + lane + 2 * r * LPG, qb = qa + LPG;
+            const int ja = x0 + lane + 2 * r * LPG, jb = ja + LPG;
+            const bool hqa = qa < b1, hqb = qb < b1, hja = ja < x1, hjb = jb < x1;
+            const int ca = gd.gc_idx[hqa ? qa : b0], cb = gd.gc_idx[hqb ? qb : b0];
+            const int na = gd.x_nb[hja ? ja : x0], nbb = gd.x_nb[hjb ? jb : x0];
+            const T wa = gd.x_w[hja ? ja : x0], wb = gd.x_w[hjb ? jb : x0];
+            T c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
+            if (Hcell) {
+                c0 = (T)Hcell[ca]; c1 = (T)Hcell[CP + ca]; c2 = (T)Hcell[2 * CP + ca];
+                d0 = (T)Hcell[cb]; d1 = (T)Hcell[CP + cb]; d2 = (T)Hcell[2 * CP + cb];
+            }
+            const T pa = msc[na], pb = msc[nbb];
+            const T ax = mf[na], ay = mf[G + na], az = mf[2 * G + na];
+            const T bx = mf[nbb], by = mf[G + nbb], bz = mf[2 * G + nbb];
+            if (hqa) { s0 += c0; s1 += c1; s2 += c2; }
+            if (hqb) { s0 += d0; s1 += d1; s2 += d2; }
+            if (hja) {
+                const T w = wa * pa;
+                e0 += w * ax; e1 += w * ay; e2 += w * az;
+            }
+            if (hjb) {
+                const T w = wb * pb;
+                e0 += w * bx; e1 += w * by; e2 += w * bz;
+            }
         }
-        const T cnt = (T)(b1 - b0);
-        hd[0] = lane_sum<LPG>(s0) / cnt;
-        hd[1] = lane_sum<LPG>(s1) / cnt;
-        hd[2] = lane_sum<LPG>(s2) / cnt;
-    } else if (Hcell || fsrc || tsrc) {
+        if (Hcell) {
+            const T cnt = (T)nc;
+            hd[0] = lane_sum<LPG>(s0) / cnt;
+            hd[1] = lane_sum<LPG>(s1) / cnt;
+            hd[2] = lane_sum<LPG>(s2) / cnt;
+        }
+        hx[0] = lane_sum<LPG>(e0);
+        hx[1] = lane_sum<LPG>(e1);
+        hx[2] = lane_sum<LPG>(e2);
+    } else {
         T s0 = 0, s1 = 0, s2 = 0;
         for (int q = b0 + lane; q < b1; q += LPG) {
             const int ci = gd.gc_idx[q];
@@ -615,15 +646,13 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
                 s1 += (T)Hcell[CP + ci];
                 s2 += (T)Hcell[2 * CP + ci];
             }
-            if (fsrc || tsrc) {
-                const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
-                if (fsrc)
+            const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
+            if (fsrc)
 #pragma unroll
-                    for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
-                if (tsrc) {
-                    ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
-                    ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
-                }
+                for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
+            if (tsrc) {
+                ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+                ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
             }
         }
         const T cnt = (T)(b1 - b0);
@@ -641,13 +670,10 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
             tout[0] = lane_sum<LPG>(ts0) / dc + tst;
             tout[1] = lane_sum<LPG>(ts1) / dc + tst;
         }
+        csr_sum<T, LPG>(x0, x1, gd.x_nb, gd.x_w, mf, G, lane, hx);
     }
     // get_applied: static per-grain field + source average + uniform ramped field
     // (applied.f90:871-883)
-    T happ[3] = {0, 0, 0};
-    if (gd.happ) {
-        happ[0] = gd.happ[g]; happ[1] = gd.happ[gd.G + g]; happ[2] = gd.happ[2 * gd.G + g];
-    }
     if (fsrc)
 #pragma unroll
         for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)hs[i];
@@ -662,7 +688,7 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
 #pragma unroll
     for (int i = 0; i < 3; i++) h[i] = happ[i] + hd[i];
     // find_temp_scaling of Hk at the evaluation's temperature (temp_scaling.f90:392-395)
-    T hks = gd.hk_scale[g];
+    T hks = hk_s;
     if (tsrc && L.tscale && L.hkexp >= 0.0 && tout) {
         double x = 1.0 - tout[0] / gd.tc[g];
         x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
@@ -671,33 +697,29 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
     // get_cubic_hk (field.f90:377-395)
     if (L.has_cub && hks > (T)0) {
         const T* A = gd.cub;
-        const int G = gd.G;
         T d1[3], d2[3], d3[3];
 #pragma unroll
         for (int c = 0; c < 3; c++)
             d1[c] = A[(3 * c + 0) * G + g] * m[0] + A[(3 * c + 1) * G + g] * m[1] + A[(3 * c + 2) * G + g] * m[2];
 #pragma unroll
         for (int c = 0; c < 3; c++) d2[c] = d1[c] * d1[c];
-        const T k1 = gd.hk1[g], k2 = gd.hk2[g];
-        d3[0] = d1[0] * (k1 * (d2[1] + d2[2]) + k2 * d2[1] * d2[2]);
-        d3[1] = d1[1] * (k1 * (d2[0] + d2[2]) + k2 * d2[0] * d2[2]);
-        d3[2] = d1[2] * (k1 * (d2[0] + d2[1]) + k2 * d2[0] * d2[1]);
+        d3[0] = d1[0] * (hk1 * (d2[1] + d2[2]) + hk2 * d2[1] * d2[2]);
+        d3[1] = d1[1] * (hk1 * (d2[0] + d2[2]) + hk2 * d2[0] * d2[2]);
+        d3[2] = d1[2] * (hk1 * (d2[0] + d2[1]) + hk2 * d2[0] * d2[1]);
 #pragma unroll
         for (int r = 0; r < 3; r++)
             h[r] = h[r] - (A[(r + 0) * G + g] * d3[0] + A[(r + 3) * G + g] * d3[1] + A[(r + 6) * G + g] * d3[2]) * hks;
     }
     // get_uni_hk (field.f90:366-375)
     if (L.has_uni && hks > (T)0) {
-        const T u[3] = {gd.uni[g], gd.uni[gd.G + g], gd.uni[2 * gd.G + g]};
         const T p = dot3(m, u), yy = (T)1 - p * p;
-        const T s = p * (gd.hk1[g] + gd.hk2[g] * yy) * hks;
+        const T s = p * (hk1 + hk2 * yy) * hks;
 #pragma unroll
         for (int i = 0; i < 3; i++) h[i] = h[i] + s * u[i];
     }
-    // exchange CSRs (field.f90:326-363): intralayer, layer above, layer below
-    if (gd.xi_off) csr_sum<T, LPG>(xi0, xi1, gd.xi_nb, gd.xi_w, mf, gd.G, lane, h);
-    if (gd.xu_off) csr_sum<T, LPG>(xu0, xu1, gd.xu_nb, gd.xu_w, mf, gd.G, lane, h);
-    if (gd.xl_off) csr_sum<T, LPG>(xl0, xl1, gd.xl_nb, gd.xl_w, mf, gd.G, lane, h);
+    // exchange, summed over the merged CSR above
+#pragma unroll
+    for (int i = 0; i < 3; i++) h[i] = h[i] + hx[i];
 }
 
 // Grain temperature of a layer's temperature source at the two times of the stage's axes
@@ -1098,45 +1120,61 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     const int bl = xcd_remap(blockIdx.x, gridDim.x);
     if (bl >= nblk) return;
     const int tid = bl * blockDim.x + threadIdx.x;
-    const int g = tid / LPG, lane = tid % LPG;
-    const bool w0 = lane == 0;
     const int G = gd.G;
+    // threads past the last grain repeat the last grain's work and store nothing, so the
+    // loads below need no guard; every lane of a grain computes its results, lane 0 stores
+    const bool act = tid / LPG < G;
+    const int g = act ? tid / LPG : G - 1, lane = tid % LPG;
+    const bool w0 = act && lane == 0;
     __shared__ double red[DIAG_STRIDE];
-    if (a.diag) {
-        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
-        __syncthreads();
+    __shared__ LayerParams slp[MRMG_MAX_LAYERS];
+    {
+        const int nw = gd.nl * (int)(sizeof(LayerParams) / sizeof(int));
+        const int* src = reinterpret_cast<const int*>(gd.lp);
+        int* dst = reinterpret_cast<int*>(slp);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
     }
+    if (a.diag)
+        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
+    __syncthreads();
+    const int k = gd.layer_of[g];
+    const LayerParams& L = slp[k];
+    // the grain's state and constants, loaded together ahead of the field's gathers
+    const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
+    T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
+    const T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
+    const T msc = mf[(size_t)ST_MSC * G + g];
+    const T g_ms = gd.ms[g], g_msf = gd.msf[g], g_as = gd.alpha_scale[g], g_vol = gd.vol[g], g_sig = gd.sigma[g];
+    const double g_temp = gd.temp[g];
+    T dmv[3] = {0, 0, 0}, thv[3] = {0, 0, 0};
+    if (MODE == GM_HEUN2)
+#pragma unroll
+        for (int c = 0; c < 3; c++) dmv[c] = dm[(size_t)c * G + g];
+    if (!(MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1))
+#pragma unroll
+        for (int c = 0; c < 3; c++) thv[c] = hthm[(size_t)c * G + g];
     T h[3] = {0, 0, 0};
     double tev[2] = {0, 0};   // grain temperature now and at the next evaluation (moving sources)
-    if (MODE != GM_PUSH1 && g < G) {
-        const int k = gd.layer_of[g];
-        const LayerParams& L = gd.lp[k];
-        // the magnetisation the field is evaluated on
-        const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
-        T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
-        field_at<T, LPG, TC>(gd, L, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
+    if (MODE != GM_PUSH1) {
+        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) h[c] = hout[(size_t)c * G + g];
+        if (L.src[1].kind != MRMG_SRC_NONE) src_temp<T, LPG>(gd, L, g, lane, tev);
     }
-    if (MODE == GM_PUSH1 && g < G && gd.lp[gd.layer_of[g]].src[1].kind != MRMG_SRC_NONE)
-        src_temp<T, LPG>(gd, gd.lp[gd.layer_of[g]], g, lane, tev);
-    // after the lane reductions only lane 0 of each grain reads the state and writes it
     T mo[3] = {0, 0, 0};
     T fsc = 0;   // Ms * ms_scale of the new state: the cell magnetisation factor
-    if (g < G && w0) {
-        const int k = gd.layer_of[g];
-        const LayerParams& L = gd.lp[k];
-        const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
-        T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
-        if (MODE == GM_PUSH1) { h[0] = hout[g]; h[1] = hout[G + g]; h[2] = hout[2 * G + g]; }
+    {
         // the grain temperature of this evaluation and the next one
         const bool tsrc = L.src[1].kind != MRMG_SRC_NONE;
-        double Tev = gd.temp[g], Tnx = Tev;
+        double Tev = g_temp, Tnx = Tev;
         if (tsrc) {
             Tnx = tev[1];
             if (MODE == GM_PUSH1) {
                 Tev = a.tlast[g];
             } else {
                 Tev = tev[0];
-                if (a.tlast && (MODE == GM_FIELD1 || MODE == GM_HEUN2)) a.tlast[g] = Tev;
+                if (w0 && a.tlast && (MODE == GM_FIELD1 || MODE == GM_HEUN2)) a.tlast[g] = Tev;
             }
         }
         // the Ms scale of the new state at the next evaluation follows the temperature
@@ -1152,22 +1190,22 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 } else {
                     philox_normals3(a.seed, a.step, g, n);
                 }
-                T sg = gd.sigma[g];
+                T sg = g_sig;
                 if (tsrc)
-                    sg = (T)thermal_sigma_dev(L.tsigma, L.floor_ms, (double)gd.alpha_scale[g], Tev, (double)gd.ms[g],
-                                              (double)gd.vol[g], a.llb != 0, (double)ma[(size_t)ST_MAG * G + g],
+                    sg = (T)thermal_sigma_dev(L.tsigma, L.floor_ms, (double)g_as, Tev, (double)g_ms, (double)g_vol,
+                                              a.llb != 0, (double)ma[(size_t)ST_MAG * G + g],
                                               (double)ma[(size_t)ST_MSC * G + g]);
 #pragma unroll
                 for (int c = 0; c < 3; c++) th[c] = sg * (T)n[c];
             }
-            hthm[g] = th[0]; hthm[G + g] = th[1]; hthm[2 * G + g] = th[2];
+            if (w0) { hthm[g] = th[0]; hthm[G + g] = th[1]; hthm[2 * G + g] = th[2]; }
         } else {
-            th[0] = hthm[g]; th[1] = hthm[G + g]; th[2] = hthm[2 * G + g];
+            th[0] = thv[0]; th[1] = thv[1]; th[2] = thv[2];
         }
-        if (MODE != GM_HEUN1 && MODE != GM_PUSH1) { hout[g] = h[0]; hout[G + g] = h[1]; hout[2 * G + g] = h[2]; }
+        if (w0 && MODE != GM_HEUN1 && MODE != GM_PUSH1) { hout[g] = h[0]; hout[G + g] = h[1]; hout[2 * G + g] = h[2]; }
         if (MODE == GM_FIELD1) {
             // get_fields, llg.f90:373-381: the step is chosen on the host from these maxima
-            if (a.diag) {
+            if (a.diag && w0) {
                 const T di = (T)a.dti;
                 T v[3];
 #pragma unroll
@@ -1182,7 +1220,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
             const bool heun = MODE == GM_HEUN1 || MODE == GM_HEUN2 || MODE == GM_PUSH1;
             LlbGrain q;
             q.T = Tev; q.Tc = gd.tc[g]; q.lam = gd.lam[g]; q.spin = L.spin; q.alpha = L.alpha;
-            q.as = (double)gd.alpha_scale[g]; q.ms = (double)gd.ms[g];
+            q.as = (double)g_as; q.ms = (double)g_ms;
             double mid[3], mpd[3], phd[3], thd[3], dmd[3] = {0, 0, 0}, mod[3], mago, e, e1;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -1198,20 +1236,22 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 for (int c = 0; c < 3; c++) dmd[c] = (double)dm[(size_t)c * G + g];
             llb_push(q, a.type, stage2, -0.5 * L.gamma * a.dts, a.dt, mid, magi, mpd, magp, phd, thd, dmd, mod, &mago,
                      &e, &e1);
-            if (heun && !stage2)
+            if (w0 && heun && !stage2)
 #pragma unroll
                 for (int c = 0; c < 3; c++) dm[(size_t)c * G + g] = (T)dmd[c];
 #pragma unroll
             for (int c = 0; c < 3; c++) mo[c] = (T)mod[c];
             const T mscn = L.use_mag ? (T)(mago < 1.0 ? mago : 1.0)
                                      : dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : gd.ms_scale[g];
-            fsc = (L.use_mag || dyn_ms) ? gd.ms[g] * mscn : gd.msf[g];
+            fsc = (L.use_mag || dyn_ms) ? g_ms * mscn : g_msf;
             T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
-            dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
-            dst[(size_t)ST_MAG * G + g] = (T)mago;
-            dst[(size_t)ST_MSC * G + g] = mscn;
-            if (a.diag) {
-                const double vol = (double)gd.vol[g];
+            if (w0) {
+                dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
+                dst[(size_t)ST_MAG * G + g] = (T)mago;
+                dst[(size_t)ST_MSC * G + g] = mscn;
+            }
+            if (a.diag && w0) {
+                const double vol = (double)g_vol;
                 atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
                 atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
                 atomicAdd(&red[D_MMAG + k], mago * vol);
@@ -1223,13 +1263,11 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         T H[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
-        const T msc = mf[(size_t)ST_MSC * G + g];
         const T mscn = dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : msc;
-        fsc = (L.use_mag || dyn_ms) ? gd.ms[g] * mscn : gd.msf[g];
-        T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
+        fsc = (L.use_mag || dyn_ms) ? g_ms * mscn : g_msf;
         T d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
-        const T scale2 = gd.alpha_scale[g] * (T)L.alpha;
+        const T scale2 = g_as * (T)L.alpha;
         if (msc > (T)0) {
             if (MODE == GM_HEUN1 || MODE == GM_HEUN2 || MODE == GM_PUSH1) {
                 const T scale11 = (T)2 * scale1 / ((T)1 + scale2 * scale2);
@@ -1241,8 +1279,8 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 for (int c = 0; c < 3; c++) d[c] = scale11 * (tq[c] + scale2 * mt[c]);
                 if (MODE == GM_HEUN2) {
 #pragma unroll
-                    for (int c = 0; c < 3; c++) d[c] = (T)0.5 * (d[c] + dm[(size_t)c * G + g]);
-                } else {
+                    for (int c = 0; c < 3; c++) d[c] = (T)0.5 * (d[c] + dmv[c]);
+                } else if (w0) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) dm[(size_t)c * G + g] = d[c];
                 }
@@ -1273,13 +1311,15 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
             }
         }
         T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
-        dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
-        if (dyn_ms) dst[(size_t)ST_MSC * G + g] = mscn;
-        if (a.diag) {
-            const T msv = gd.ms[g];
+        if (w0) {
+            dst[g] = mo[0]; dst[G + g] = mo[1]; dst[2 * G + g] = mo[2];
+            if (dyn_ms) dst[(size_t)ST_MSC * G + g] = mscn;
+        }
+        if (a.diag && w0) {
+            const T msv = g_ms;
             const double e = msc > (T)0 ? (double)dot3(tq, tq) / ((double)msv * (double)msv) : 0.0;
             const double e1 = msc > (T)0 ? (double)dot3(d, d) : 0.0;
-            const double vol = (double)gd.vol[g];
+            const double vol = (double)g_vol;
             atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
             atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
             atomicAdd(&red[D_MMAG + k], sqrt((double)dot3(mo, mo)) * (double)msc * vol);
@@ -1289,14 +1329,8 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         }
     }
     // the new magnetisation of the grain onto its cells, for the next field evaluation
-    if (MODE != GM_FIELD1 && Mcell && g < G) {
-        const int src = (threadIdx.x & 63) & ~(LPG - 1);
-        T mb3[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) mb3[c] = __shfl(mo[c], src);
-        const T f = __shfl(fsc, src);
-        scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, f, mb3);
-    }
+    // (every lane of the grain holds mo and fsc)
+    if (MODE != GM_FIELD1 && Mcell && act) scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, fsc, mo);
     if (a.diag) {
         __syncthreads();
         for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)bl * DIAG_STRIDE + i] = red[i];
@@ -1648,8 +1682,8 @@ struct Engine : EngineBase {
     std::vector<int> off;
     // grain data
     DBuf<int> layer_of, gc_off, gc_idx, cell2grain;
-    DBuf<int> xi_off, xi_nb, xu_off, xu_nb, xl_off, xl_nb;
-    DBuf<S> xi_w, xu_w, xl_w;
+    DBuf<int> x_off, x_nb;
+    DBuf<S> x_w;
     DBuf<S> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
     DBuf<double> tc, lam, temp;
     DBuf<LayerParams> lp;
@@ -1688,7 +1722,7 @@ struct Engine : EngineBase {
     size_t ev_used = 0;
     double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    bool tensor_ok = false, has_happ = false, has_xi = false, has_xu = false, has_xl = false;
+    bool tensor_ok = false, has_happ = false;
     bool cur_llb = false;   // the step in flight uses push_llb (selects the thermal variance)
     int maxb_row = 1, maxb_col = 1;
     size_t lds_row = 0, lds_col = 0;
@@ -1742,8 +1776,7 @@ struct Engine : EngineBase {
     {
         size_t b = 0;
         b += layer_of.bytes() + gc_off.bytes() + gc_idx.bytes() + cell2grain.bytes();
-        b += xi_off.bytes() + xi_nb.bytes() + xu_off.bytes() + xu_nb.bytes() + xl_off.bytes() + xl_nb.bytes();
-        b += xi_w.bytes() + xu_w.bytes() + xl_w.bytes();
+        b += x_off.bytes() + x_nb.bytes() + x_w.bytes();
         b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
              hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + sigma_llb.bytes() + vol.bytes() +
              lp.bytes() + tc.bytes() + lam.bytes() + temp.bytes();
@@ -1959,7 +1992,7 @@ struct Engine : EngineBase {
             h_cub(9 * (size_t)G, 0.0), h_happ(3 * (size_t)G, 0.0), h_sig(G, 0.0), h_sigl(G, 0.0), h_vol(G);
         std::vector<double> h_tc(G), h_lam(G), h_temp(G);
         std::vector<double> h_m((size_t)ST_ROWS * G);
-        has_happ = has_xi = has_xu = has_xl = false;
+        has_happ = false;
         hlp.assign(nl, LayerParams{});
         int q = 0;
         h_gco[0] = 0;
@@ -2002,9 +2035,6 @@ struct Engine : EngineBase {
                 h_c2g[k * P + p] = gr > 0 ? off[k] + gr - 1 : -1;
             }
             has_happ |= H.has_happ;
-            has_xi |= H.has_hx;
-            has_xu |= H.has_uhx;
-            has_xl |= H.has_lhx;
             LayerParams& lpk = hlp[k];
             lpk.alpha = H.alpha;
             lpk.gamma = H.gamma;
@@ -2023,44 +2053,45 @@ struct Engine : EngineBase {
         }
         RCHK(fill_sources(L));
         RCHK(fill_ufield(L));
-        // exchange CSRs in global numbering
-        auto csr = [&](int which, DBuf<int>& doff, DBuf<int>& dnb, DBuf<S>& dw) -> int {
+        // exchange CSR in global numbering: per grain the intralayer entries, then those to
+        // the layer above, then to the layer below, and one spare entry at the end
+        {
             std::vector<int> o(G + 1, 0), nb;
             std::vector<S> w;
             for (int k = 0; k < nl; k++) {
                 const HostLayer& H = L[k];
-                const std::vector<int>* ix = which == 0 ? &H.hxi : which == 1 ? &H.uhxi : &H.lhxi;
-                const std::vector<int>* gx = which == 0 ? &H.hxg : which == 1 ? &H.uhxg : &H.lhxg;
-                const std::vector<double>* vx = which == 0 ? &H.hx : which == 1 ? &H.uhx : &H.lhx;
-                const bool has = which == 0 ? H.has_hx : which == 1 ? H.has_uhx : H.has_lhx;
-                const int tk = which == 0 ? k : which == 1 ? k - 1 : k + 1;
                 for (int n = 0; n < H.ng; n++) {
-                    const int g = off[k] + n;
-                    if (has) {
+                    for (int which = 0; which < 3; which++) {
+                        const std::vector<int>& ix = which == 0 ? H.hxi : which == 1 ? H.uhxi : H.lhxi;
+                        const std::vector<int>& gx = which == 0 ? H.hxg : which == 1 ? H.uhxg : H.lhxg;
+                        const std::vector<double>& vx = which == 0 ? H.hx : which == 1 ? H.uhx : H.lhx;
+                        const bool has = which == 0 ? H.has_hx : which == 1 ? H.has_uhx : H.has_lhx;
+                        const int tk = which == 0 ? k : which == 1 ? k - 1 : k + 1;
+                        if (!has) continue;
                         if (tk < 0 || tk >= nl) return set_err(MRMG_ERR_ARG, "layer %d: interlayer CSR to missing layer", k);
-                        for (int e = (*ix)[n]; e < (*ix)[n + 1]; e++) {
-                            const int gg = (*gx)[e];
-                            if (gg < 1 || gg > L[tk].ng) return set_err(MRMG_ERR_ARG, "layer %d: exchange neighbour %d out of range", k, gg);
+                        for (int e = ix[n]; e < ix[n + 1]; e++) {
+                            const int gg = gx[e];
+                            if (gg < 1 || gg > L[tk].ng)
+                                return set_err(MRMG_ERR_ARG, "layer %d: exchange neighbour %d out of range", k, gg);
                             nb.push_back(off[tk] + gg - 1);
-                            w.push_back((*vx)[e]);
+                            w.push_back(vx[e]);
                         }
                     }
-                    o[g + 1] = (int)nb.size();
+                    o[off[k] + n + 1] = (int)nb.size();
                 }
             }
-            RCHK(doff.upload(o.data(), o.size(), c->stream));
-            RCHK(dnb.upload(nb.data(), nb.size(), c->stream));
-            RCHK(dw.upload(w.data(), w.size(), c->stream));
-            return MRMG_OK;
-        };
-        if (has_xi) RCHK(csr(0, xi_off, xi_nb, xi_w)); else { xi_off.release(); xi_nb.release(); xi_w.release(); }
-        if (has_xu) RCHK(csr(1, xu_off, xu_nb, xu_w)); else { xu_off.release(); xu_nb.release(); xu_w.release(); }
-        if (has_xl) RCHK(csr(2, xl_off, xl_nb, xl_w)); else { xl_off.release(); xl_nb.release(); xl_w.release(); }
+            nb.push_back(0);
+            w.push_back(0.0);
+            RCHK(x_off.upload(o.data(), o.size(), c->stream));
+            RCHK(x_nb.upload(nb.data(), nb.size(), c->stream));
+            RCHK(x_w.upload(w.data(), w.size(), c->stream));
+        }
 
         auto upT = [&](DBuf<S>& d, const std::vector<double>& v) -> int { return d.upload(v.data(), v.size(), c->stream); };
         RCHK(layer_of.upload(h_layer.data(), G, c->stream));
         RCHK(gc_off.upload(h_gco.data(), G + 1, c->stream));
-        RCHK(gc_idx.upload(h_gci.data(), Ctot, c->stream));
+        h_gci.push_back(0);   // spare entry
+        RCHK(gc_idx.upload(h_gci.data(), h_gci.size(), c->stream));
         RCHK(cell2grain.upload(h_c2g.data(), h_c2g.size(), c->stream));
         RCHK(upT(ms, h_ms)); RCHK(upT(msf, h_msf)); RCHK(upT(ms_scale, h_mss)); RCHK(upT(alpha_scale, h_as));
         RCHK(upT(hk_scale, h_hks)); RCHK(upT(hk1, h_hk1)); RCHK(upT(hk2, h_hk2)); RCHK(upT(uni, h_uni));
@@ -2271,9 +2302,8 @@ struct Engine : EngineBase {
         d.ms = ms.p; d.msf = msf.p; d.ms_scale = ms_scale.p; d.alpha_scale = alpha_scale.p; d.hk_scale = hk_scale.p;
         d.hk1 = hk1.p; d.hk2 = hk2.p; d.uni = uni.p; d.cub = cub.p; d.happ = has_happ ? happ.p : nullptr;
         d.sigma = sigma.p; d.vol = vol.p; d.gc_off = gc_off.p; d.gc_idx = gc_idx.p;
-        d.xi_off = has_xi ? xi_off.p : nullptr; d.xi_nb = xi_nb.p; d.xi_w = xi_w.p;
-        d.xu_off = has_xu ? xu_off.p : nullptr; d.xu_nb = xu_nb.p; d.xu_w = xu_w.p;
-        d.xl_off = has_xl ? xl_off.p : nullptr; d.xl_nb = xl_nb.p; d.xl_w = xl_w.p;
+        d.x_off = x_off.p; d.x_nb = x_nb.p; d.x_w = x_w.p;
+        d.nl = c->nl;
         d.lp = lp.p;
         d.tc = tc.p; d.lam = lam.p; d.temp = temp.p;
         if (cur_llb) d.sigma = sigma_llb.p;
