@@ -38,6 +38,7 @@ struct LayerParams {
     int tscale;           // tc_nom <= 1e10
     SrcParams src[2];
 };
+static_assert(sizeof(LayerParams) % 8 == 0, "layer parameters are copied to LDS in 8-byte words");
 
 // Per-stage axis tables of the moving sources, [layer][slot][time][n], time 0 = the
 // evaluation, 1 = the next evaluation:

@@ -577,21 +577,43 @@ __device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& s
 // per round (each lane two cells and two exchange entries) all index loads, then all value
 // loads, then the sums.  Out-of-range positions load a valid entry (the arrays carry one
 // spare entry at the end) and add nothing.
+// Per-grain offsets and field constants, loaded by the caller with the grain's state.
+template <typename T> struct GrainPre {
+    int b0, b1, x0, x1;   // cell range (grain2gridindex) and exchange CSR range
+    T hk_s, hk1, hk2, u[3], happ[3];
+    // the lane's cells of the first gather round, for the scatter (valid when the grain
+    // needed a single round)
+    int ca, cb;
+    bool one_round;
+};
+
+template <typename T>
+__device__ __forceinline__ void grain_pre(const GrainData<T>& gd, int g, GrainPre<T>& P)
+{
+    const int G = gd.G;
+    P.b0 = gd.gc_off[g]; P.b1 = gd.gc_off[g + 1];
+    P.x0 = gd.x_off[g]; P.x1 = gd.x_off[g + 1];
+    P.hk_s = gd.hk_scale[g]; P.hk1 = gd.hk1[g]; P.hk2 = gd.hk2[g];
+    P.u[0] = gd.uni[g]; P.u[1] = gd.uni[G + g]; P.u[2] = gd.uni[2 * G + g];
+    P.happ[0] = P.happ[1] = P.happ[2] = 0;
+    if (gd.happ) {
+        P.happ[0] = gd.happ[g]; P.happ[1] = gd.happ[G + g]; P.happ[2] = gd.happ[2 * G + g];
+    }
+    P.ca = P.cb = 0;
+    P.one_round = false;
+}
+
 template <typename T, int LPG, typename TC>
 __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int k, int g, int lane,
                                          const T* __restrict__ mf, const TC* __restrict__ Hcell, size_t CP,
-                                         double t, const T* m, T* h, double* tout = nullptr)
+                                         double t, const T* m, T* h, double* tout, GrainPre<T>& P)
 {
     const int G = gd.G;
     const bool fsrc = L.src[0].kind == MRMG_SRC_FIELD_TABLE, tsrc = L.src[1].kind != MRMG_SRC_NONE;
-    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
-    const int x0 = gd.x_off[g], x1 = gd.x_off[g + 1];
-    const T hk_s = gd.hk_scale[g], hk1 = gd.hk1[g], hk2 = gd.hk2[g];
-    const T u[3] = {gd.uni[g], gd.uni[G + g], gd.uni[2 * G + g]};
-    T happ[3] = {0, 0, 0};
-    if (gd.happ) {
-        happ[0] = gd.happ[g]; happ[1] = gd.happ[G + g]; happ[2] = gd.happ[2 * G + g];
-    }
+    const int b0 = P.b0, b1 = P.b1, x0 = P.x0, x1 = P.x1;
+    const T hk_s = P.hk_s, hk1 = P.hk1, hk2 = P.hk2;
+    const T u[3] = {P.u[0], P.u[1], P.u[2]};
+    T happ[3] = {P.happ[0], P.happ[1], P.happ[2]};
     const T* __restrict__ msc = mf + (size_t)ST_MSC * G;
     // get_magnetostatic gather (field.f90:463-468) and the exchange CSR (field.f90:326-363:
     // intralayer, then the layers above and below, merged per grain)
@@ -602,11 +624,13 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         const int nc = b1 - b0, ne = x1 - x0;
         const int ncr = (nc + 2 * LPG - 1) / (2 * LPG), ner = (ne + 2 * LPG - 1) / (2 * LPG);
         const int nr = ncr > ner ? ncr : ner;
+        P.one_round = ncr <= 1;
         for (int r = 0; r < nr; r++) {
             const int qa = b0 + lane + 2 * r * LPG, qb = qa + LPG;
             const int ja = x0 + lane + 2 * r * LPG, jb = ja + LPG;
             const bool hqa = qa < b1, hqb = qb < b1, hja = ja < x1, hjb = jb < x1;
             const int ca = gd.gc_idx[hqa ? qa : b0], cb = gd.gc_idx[hqb ? qb : b0];
+            if (r == 0) { P.ca = ca; P.cb = cb; }
             const int na = gd.x_nb[hja ? ja : x0], nbb = gd.x_nb[hjb ? jb : x0];
             const T wa = gd.x_w[hja ? ja : x0], wb = gd.x_w[hjb ? jb : x0];
             T c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
@@ -838,9 +862,16 @@ __global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, i
 // grain's cells.  Non-magnetic cells are never written and stay 0.
 template <typename T, int LPG, typename TC>
 __device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, TC* __restrict__ Mcell, size_t CP, int g,
-                                              int lane, T f, const T* m)
+                                              int lane, T f, const T* m, const GrainPre<T>* P = nullptr)
 {
     const TC v0 = (TC)(f * m[0]), v1 = (TC)(f * m[1]), v2 = (TC)(f * m[2]);
+    if (P && P->one_round) {
+        // the cells this lane gathered: b0+lane and b0+lane+LPG
+        const int qa = P->b0 + lane, qb = qa + LPG;
+        if (qa < P->b1) { Mcell[P->ca] = v0; Mcell[CP + P->ca] = v1; Mcell[2 * CP + P->ca] = v2; }
+        if (qb < P->b1) { Mcell[P->cb] = v0; Mcell[CP + P->cb] = v1; Mcell[2 * CP + P->cb] = v2; }
+        return;
+    }
     for (int q = gd.gc_off[g] + lane; q < gd.gc_off[g + 1]; q += LPG) {
         const int ci = gd.gc_idx[q];
         Mcell[ci] = v0;
@@ -1128,18 +1159,11 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     const bool w0 = act && lane == 0;
     __shared__ double red[DIAG_STRIDE];
     __shared__ LayerParams slp[MRMG_MAX_LAYERS];
-    {
-        const int nw = gd.nl * (int)(sizeof(LayerParams) / sizeof(int));
-        const int* src = reinterpret_cast<const int*>(gd.lp);
-        int* dst = reinterpret_cast<int*>(slp);
-        for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
-    }
-    if (a.diag)
-        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
-    __syncthreads();
+    // the grain's state and constants, issued first; the layer parameters' copy into LDS
+    // then waits for all of them at once
     const int k = gd.layer_of[g];
-    const LayerParams& L = slp[k];
-    // the grain's state and constants, loaded together ahead of the field's gathers
+    GrainPre<T> P;
+    grain_pre(gd, g, P);
     const T* mf = (MODE == GM_HEUN2 || MODE == GM_ROT2) ? mb : ma;
     T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
     const T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
@@ -1153,10 +1177,21 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     if (!(MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1))
 #pragma unroll
         for (int c = 0; c < 3; c++) thv[c] = hthm[(size_t)c * G + g];
+    {
+        typedef unsigned long long u64;
+        const int nw = gd.nl * (int)(sizeof(LayerParams) / sizeof(u64));
+        const u64* src = reinterpret_cast<const u64*>(gd.lp);
+        u64* dst = reinterpret_cast<u64*>(slp);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+    }
+    if (a.diag)
+        for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) red[i] = 0.0;
+    __syncthreads();
+    const LayerParams& L = slp[k];
     T h[3] = {0, 0, 0};
     double tev[2] = {0, 0};   // grain temperature now and at the next evaluation (moving sources)
     if (MODE != GM_PUSH1) {
-        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hcell, CP, a.t, mfield, h, tev);
+        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hcell, CP, a.t, mfield, h, tev, P);
     } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) h[c] = hout[(size_t)c * G + g];
@@ -1330,7 +1365,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     }
     // the new magnetisation of the grain onto its cells, for the next field evaluation
     // (every lane of the grain holds mo and fsc)
-    if (MODE != GM_FIELD1 && Mcell && act) scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, fsc, mo);
+    if (MODE != GM_FIELD1 && Mcell && act) scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, fsc, mo, &P);
     if (a.diag) {
         __syncthreads();
         for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)bl * DIAG_STRIDE + i] = red[i];
