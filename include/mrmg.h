@@ -172,7 +172,9 @@ typedef struct {
     int pad2_;
 } mrmg_step_params;
 
-/* Diagnostics of the last step (llg.f90:1209-1257, 217-237). */
+/* Diagnostics of the call (llg.f90:1209-1257, 217-237).  mave, mvar and have are the
+ * llg() loop-top statistics of the last step, as llgl(:)%mave etc. hold after llg()
+ * returns: the state before that step's push and h from the field evaluation before it. */
 typedef struct {
     double t;                 /* time after the last step                               */
     double dt;                /* llgp%dt after the last step (pass it to the next call)  */
@@ -181,10 +183,15 @@ typedef struct {
     double errmx;             /* max |m x H|/Ms over all grains (sqrt of errmx_l)        */
     double errmx1;            /* max |dm|^2                                              */
     double mmag;              /* sum over layers of volume-weighted <|m| ms_scale>       */
-    double mave[8][3];        /* per-layer volume-weighted <m> (up to 8 layers)          */
+    double mave[8][3];        /* per-layer volume-weighted <m> (llgl%mave, up to 8)      */
     int rejected;             /* steps push_m rejected and retried in this call          */
     int pad_;
     double max_dmdt;          /* llgp%max_dmdt after the call                             */
+    double mvar[8][3];        /* per-layer volume-weighted variance of m (llgl%mvar)      */
+    double have[8][3];        /* per-layer volume-weighted <h> (llgl%have)                */
+    double mave_all[3];       /* llgp%mave: layers combined with their total volumes     */
+    double mvar_all[3];       /* llgp%mvar                                                */
+    double have_all[3];       /* llgp%have                                                */
 } mrmg_diag;
 
 /* ---- lifecycle ----------------------------------------------------------------- */

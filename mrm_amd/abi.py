@@ -104,7 +104,8 @@ class Diag(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("steps", C.c_int), ("n_lay", C.c_int),
                 ("errmx", C.c_double), ("errmx1", C.c_double), ("mmag", C.c_double),
                 ("mave", (C.c_double * 3) * 8), ("rejected", C.c_int), ("pad_", C.c_int),
-                ("max_dmdt", C.c_double)]
+                ("max_dmdt", C.c_double), ("mvar", (C.c_double * 3) * 8), ("have", (C.c_double * 3) * 8),
+                ("mave_all", C.c_double * 3), ("mvar_all", C.c_double * 3), ("have_all", C.c_double * 3)]
 
 
 class KStat(C.Structure):

@@ -15,11 +15,12 @@
 
 #define MRMG_MAX_FAC 24
 
-// LDS index padding: one spare slot per 32 elements, so the stride-R writes of a
-// Stockham pass spread over the LDS banks instead of piling onto two of them.
-#define LP(i) ((i) + ((i) >> 5))
+// LDS index padding: one spare slot per 16 elements, so the stride-R writes of a
+// Stockham pass (R = 8, 16) from 16 consecutive lanes land in distinct banks for 8-byte
+// elements (and 32 lanes for 4-byte elements) instead of piling onto a few.
+#define LP(i) ((i) + ((i) >> 4))
 // elements of an LDS array holding n padded entries
-#define LPN(n) ((n) + ((n) >> 5) + 1)
+#define LPN(n) ((n) + ((n) >> 4) + 1)
 
 // Pass p has radix fac[p] and span ns[p] (the product of the earlier radices).  The
 // divisions of a pass's index arithmetic use multipliers fixed when the plan is made:

@@ -104,8 +104,7 @@ enum GrainMode {
 #define D_ERRMX 0                              // max |m x H|^2 / Ms^2
 #define D_ERRMX1 (D_ERRMX + MRMG_MAX_LAYERS)   // max |dm|^2
 #define D_MMAG (D_ERRMX1 + MRMG_MAX_LAYERS)    // sum |m| ms_scale vol
-#define D_MAVE (D_MMAG + MRMG_MAX_LAYERS)      // sum vol m_c, 3 per layer
-#define D_HMAG (D_MAVE + 3 * MRMG_MAX_LAYERS)  // max |h + hthm dti|^2
+#define D_HMAG (D_MMAG + MRMG_MAX_LAYERS)      // max |h + hthm dti|^2
 #define DIAG_STRIDE (D_HMAG + MRMG_MAX_LAYERS)
 
 struct StepArgs {

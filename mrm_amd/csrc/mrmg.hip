@@ -1290,8 +1290,6 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
                 atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
                 atomicAdd(&red[D_MMAG + k], mago * vol);
-#pragma unroll
-                for (int c = 0; c < 3; c++) atomicAdd(&red[D_MAVE + 3 * k + c], vol * (double)mo[c]);
             }
         } else {
         const T sq = (T)a.sqrt_dt;
@@ -1358,8 +1356,6 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
             atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX + k]), (unsigned long long)__double_as_longlong(e));
             atomicMax(reinterpret_cast<unsigned long long*>(&red[D_ERRMX1 + k]), (unsigned long long)__double_as_longlong(e1));
             atomicAdd(&red[D_MMAG + k], sqrt((double)dot3(mo, mo)) * (double)msc * vol);
-#pragma unroll
-            for (int c = 0; c < 3; c++) atomicAdd(&red[D_MAVE + 3 * k + c], vol * (double)mo[c]);
         }
         }
     }
@@ -1369,6 +1365,69 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     if (a.diag) {
         __syncthreads();
         for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)bl * DIAG_STRIDE + i] = red[i];
+    }
+}
+
+// llg()'s loop-top statistics (llg.f90:217-237) of layer blockIdx.x: the volume-weighted
+// mean and variance of m and mean of h over the layer's grains, in two passes (mean, then
+// squared deviations) with a fixed reduction order.  out[k*10 + ...] = W, <m>, var, <h>.
+#define STATS_THREADS 1024
+__device__ __forceinline__ double block_sum_fixed(double v, double* sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += sh[w];
+    return r;
+}
+__global__ void __launch_bounds__(STATS_THREADS) k_layer_stats(const double* __restrict__ m,
+                                                               const double* __restrict__ h,
+                                                               const double* __restrict__ vol,
+                                                               const LayerParams* __restrict__ lp, int G,
+                                                               double* __restrict__ out)
+{
+    __shared__ double sh[STATS_THREADS / 64];
+    const int k = blockIdx.x;
+    const int g0 = lp[k].off, g1 = g0 + lp[k].ng;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int g = g0 + (int)threadIdx.x; g < g1; g += blockDim.x) {
+        const double w = vol[g];
+        s[0] += w;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            s[1 + c] += w * m[(size_t)c * G + g];
+            s[4 + c] += w * h[(size_t)c * G + g];
+        }
+    }
+    double r[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) r[i] = block_sum_fixed(s[i], sh);
+    const double W = r[0];
+    const double mean[3] = {r[1] / W, r[2] / W, r[3] / W};
+    double v[3] = {0, 0, 0};
+    for (int g = g0 + (int)threadIdx.x; g < g1; g += blockDim.x) {
+        const double w = vol[g];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double d = m[(size_t)c * G + g] - mean[c];
+            v[c] += w * d * d;
+        }
+    }
+    double vr[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) vr[c] = block_sum_fixed(v[c], sh);
+    if (threadIdx.x == 0) {
+        double* o = out + (size_t)k * 10;
+        o[0] = W;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o[1 + c] = mean[c];
+            o[4 + c] = vr[c] / W;
+            o[7 + c] = r[4 + c] / W;
+        }
     }
 }
 
@@ -1749,7 +1808,7 @@ struct Engine : EngineBase {
     FFTPlan prow{}, pcol{};
     int row_radix = 16, col_radix = 16;
     size_t lds_rowT = 0, lds_colT = 0;
-    DBuf<double> noise_buf, diag_part;
+    DBuf<double> noise_buf, diag_part, stats;
     // instrumentation: event pairs of the current mrmg_step call, per-class totals
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -2143,6 +2202,7 @@ struct Engine : EngineBase {
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(h.p, 0, h.bytes(), c->stream));
         RCHK(diag_part.alloc((size_t)(((size_t)G * 16 + 255) / 256) * DIAG_STRIDE));
+        RCHK(stats.alloc((size_t)10 * nl));
         HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return MRMG_OK;
@@ -2565,7 +2625,7 @@ struct Engine : EngineBase {
 
     // per-layer reductions of the last diag-enabled grain launch
     struct LayerRed {
-        std::vector<double> errmx, errmx1, mmag, mave, hmag;
+        std::vector<double> errmx, errmx1, mmag, hmag;
     };
     int read_partials(LayerRed& r)
     {
@@ -2573,7 +2633,6 @@ struct Engine : EngineBase {
         std::vector<double> part((size_t)nb * DIAG_STRIDE);
         HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
         r.errmx.assign(nl, 0.0); r.errmx1.assign(nl, 0.0); r.mmag.assign(nl, 0.0); r.hmag.assign(nl, 0.0);
-        r.mave.assign(3 * nl, 0.0);
         for (int b = 0; b < nb; b++) {
             const double* q = &part[(size_t)b * DIAG_STRIDE];
             for (int k = 0; k < nl; k++) {
@@ -2581,7 +2640,38 @@ struct Engine : EngineBase {
                 r.errmx1[k] = std::max(r.errmx1[k], q[D_ERRMX1 + k]);
                 r.hmag[k] = std::max(r.hmag[k], q[D_HMAG + k]);
                 r.mmag[k] += q[D_MMAG + k];
-                for (int cc = 0; cc < 3; cc++) r.mave[3 * k + cc] += q[D_MAVE + 3 * k + cc];
+            }
+        }
+        return MRMG_OK;
+    }
+    // llg()'s loop-top statistics of the current state and h, into stats
+    int layer_stats()
+    {
+        hipLaunchKernelGGL(k_layer_stats, dim3(c->nl), dim3(STATS_THREADS), 0, c->stream, ma.p, h.p, vol.p, lp.p, G,
+                           stats.p);
+        HIPCHK(hipGetLastError());
+        return MRMG_OK;
+    }
+    // the statistics into the diagnostics record, the layers combined as llg.f90:231-233 does
+    int read_stats(mrmg_diag* d)
+    {
+        const int nl = c->nl;
+        std::vector<double> o((size_t)10 * nl);
+        HIPCHK(hipMemcpy(o.data(), stats.p, o.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double x = 0.0;
+        for (int k = 0; k < nl; k++) {
+            const double* q = &o[(size_t)10 * k];
+            const double x1 = q[0];
+            x = x + x1;
+            for (int cc = 0; cc < 3; cc++) {
+                d->mave_all[cc] = d->mave_all[cc] + x1 / x * (q[1 + cc] - d->mave_all[cc]);
+                d->have_all[cc] = d->have_all[cc] + x1 / x * (q[7 + cc] - d->have_all[cc]);
+                d->mvar_all[cc] = d->mvar_all[cc] + x1 / x * (q[4 + cc] - d->mvar_all[cc]);
+                if (k < 8) {
+                    d->mave[k][cc] = q[1 + cc];
+                    d->mvar[k][cc] = q[4 + cc];
+                    d->have[k][cc] = q[7 + cc];
+                }
             }
         }
         return MRMG_OK;
@@ -2620,6 +2710,7 @@ struct Engine : EngineBase {
                                       c->stream));
                 a.normals = noise_buf.p;
             }
+            if (diag && it == p->n_steps - 1) RCHK(layer_stats());
             // get_fields(t)
             if (demag_on()) RCHK(demag());
             a.t = t;
@@ -2675,6 +2766,7 @@ struct Engine : EngineBase {
         if (diag) {
             memset(diag, 0, sizeof(*diag));
             fill_diag(diag, r, t, dt, p->n_steps);
+            if (p->n_steps > 0) RCHK(read_stats(diag));
             diag->rejected = rejected;
             diag->max_dmdt = max_dmdt;
         }
@@ -2692,8 +2784,6 @@ struct Engine : EngineBase {
             double sv = 0;
             for (int n = 0; n < c->layers[k].ng; n++) sv += c->layers[k].vol[n];
             d->mmag += r.mmag[k] / sv;
-            if (k < 8)
-                for (int cc = 0; cc < 3; cc++) d->mave[k][cc] = r.mave[3 * k + cc] / sv;
         }
         d->errmx = e;
         d->errmx1 = e1;
@@ -2757,6 +2847,7 @@ struct Engine : EngineBase {
                 a.normals = noise_buf.p + per * ci;
             }
             if (last) RCHK(zero_partials());
+            if (last && diag) RCHK(layer_stats());
             if (type == -3) {
                 // stage 1 (get_fields at t, push iter=2)
                 if (demag_on()) RCHK(demag());
@@ -2791,8 +2882,9 @@ struct Engine : EngineBase {
             memset(diag, 0, sizeof(*diag));
             LayerRed r;
             if (p->n_steps > 0) RCHK(read_partials(r));
-            else { r.errmx.assign(c->nl, 0); r.errmx1.assign(c->nl, 0); r.mmag.assign(c->nl, 0); r.mave.assign(3 * c->nl, 0); }
+            else { r.errmx.assign(c->nl, 0); r.errmx1.assign(c->nl, 0); r.mmag.assign(c->nl, 0); }
             fill_diag(diag, r, t, dt, p->n_steps);
+            if (p->n_steps > 0) RCHK(read_stats(diag));
             diag->max_dmdt = 1.0;
         }
         return MRMG_OK;

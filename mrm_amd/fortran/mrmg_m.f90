@@ -61,6 +61,8 @@ module mrmg_m
     real(c_double) :: mave(3, 8)
     integer(c_int) :: rejected, pad_
     real(c_double) :: max_dmdt
+    real(c_double) :: mvar(3, 8), have(3, 8)
+    real(c_double) :: mave_all(3), mvar_all(3), have_all(3)
   end type
 
   interface

@@ -1281,19 +1281,38 @@ static void push_m(ora_sim *s, double t0, int type, int fixed, int llb)
     if (s->max_dmdtnext > 1.e20) s->max_dmdtnext = s->max_dmdt;
 }
 
-/* Per-layer volume-weighted mean of m (llg.f90:217-237) */
-static void mave_layers(ora_sim *s, mrmg_diag *d)
+/* The llg() loop-top statistics (llg.f90:217-237): per layer the volume-weighted mean and
+ * variance of m (West's incremental update, in grain order) and the mean of h; then the
+ * layers combined with their total volumes.  Written into the diagnostics record. */
+static void loop_stats(ora_sim *s, mrmg_diag *d)
 {
-    for (int k = 0; k < s->nl && k < 8; k++) {
+    double gm[3] = {0, 0, 0}, gv[3] = {0, 0, 0}, gh[3] = {0, 0, 0}, x = 0.0;
+    for (int k = 0; k < s->nl; k++) {
         olayer *L = &s->L[k];
-        double ma[3] = {0, 0, 0}, x1 = 0.0;
+        double ma[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, ha[3] = {0, 0, 0}, x1 = 0.0;
         for (int n = 0; n < L->ng; n++) {
             double x3 = L->vol[n] / (x1 + L->vol[n]);
+            for (int c = 0; c < 3; c++) {
+                double dmc = L->m[6 * n + s->in + c] - ma[c];
+                mv[c] = mv[c] + x3 * x1 * dmc * dmc;
+            }
             x1 = x1 + L->vol[n];
-            for (int c = 0; c < 3; c++) ma[c] = ma[c] + x3 * (L->m[6 * n + s->in + c] - ma[c]);
+            for (int c = 0; c < 3; c++) {
+                ma[c] = ma[c] + x3 * (L->m[6 * n + s->in + c] - ma[c]);
+                ha[c] = ha[c] + x3 * (L->h[3 * n + c] - ha[c]);
+            }
         }
-        for (int c = 0; c < 3; c++) d->mave[k][c] = ma[c];
+        for (int c = 0; c < 3; c++) mv[c] = mv[c] / x1;
+        x = x + x1;
+        for (int c = 0; c < 3; c++) {
+            gm[c] = gm[c] + x1 / x * (ma[c] - gm[c]);
+            gh[c] = gh[c] + x1 / x * (ha[c] - gh[c]);
+            gv[c] = gv[c] + x1 / x * (mv[c] - gv[c]);
+        }
+        if (k < 8)
+            for (int c = 0; c < 3; c++) { d->mave[k][c] = ma[c]; d->mvar[k][c] = mv[c]; d->have[k][c] = ha[c]; }
     }
+    for (int c = 0; c < 3; c++) { d->mave_all[c] = gm[c]; d->mvar_all[c] = gv[c]; d->have_all[c] = gh[c]; }
 }
 
 /* The llg() loop body (llg.f90:212-285) for p->n_steps steps.  Fixed dt when
@@ -1312,7 +1331,10 @@ int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_di
     for (int k = 0; k < s->nl; k++) memset(s->L[k].hthm, 0, sizeof(double) * 3 * s->L[k].ng);
     s->dt = p->dt; s->max_dmdt = 1.0;
     if (fixed) s->max_dmdtnext = 1.0;
+    mrmg_diag st;
+    memset(&st, 0, sizeof(st));
     for (int it = 0; it < p->n_steps; it++) {
+        loop_stats(s, &st);
         /* get_fields(t), llg.f90:320-388 */
         get_field(s, t);
         if (p->noise != MRMG_NOISE_NONE && noise) thermal(s, noise + (size_t)3 * G * it, p->llb);
@@ -1342,7 +1364,12 @@ int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_di
         diag->t = t; diag->dt = s->dt; diag->steps = p->n_steps; diag->n_lay = s->nl;
         diag->errmx = s->errmx; diag->errmx1 = s->errmx1; diag->mmag = s->mmag;
         diag->rejected = s->rejected; diag->max_dmdt = s->max_dmdt;
-        mave_layers(s, diag);
+        memcpy(diag->mave, st.mave, sizeof(st.mave));
+        memcpy(diag->mvar, st.mvar, sizeof(st.mvar));
+        memcpy(diag->have, st.have, sizeof(st.have));
+        memcpy(diag->mave_all, st.mave_all, sizeof(st.mave_all));
+        memcpy(diag->mvar_all, st.mvar_all, sizeof(st.mvar_all));
+        memcpy(diag->have_all, st.have_all, sizeof(st.have_all));
     }
     return MRMG_OK;
 }

@@ -60,6 +60,7 @@ int main(void) {
   F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
   F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm) F(mrmg_step_params, llb)
   F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave) F(mrmg_diag, rejected) F(mrmg_diag, max_dmdt)
+  F(mrmg_diag, mvar) F(mrmg_diag, have) F(mrmg_diag, mave_all) F(mrmg_diag, mvar_all) F(mrmg_diag, have_all)
   return 0;
 }
 """

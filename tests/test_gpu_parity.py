@@ -45,6 +45,19 @@ def pair(O, med, tensor=None, demag=True, precision=abi.MRMG_FP64, ufields=None,
     return s, gpu
 
 
+def check_loop_stats(do, dg, n_lay, tol=1e-10):
+    """llg()'s loop-top statistics of the last step (llg.f90:217-237): oracle vs device."""
+    def close(a, b, scale):
+        return np.abs(np.array(a) - np.array(b)).max() <= tol * scale
+    hs = max(1.0, float(np.abs(np.array(do.have_all)).max()), float(np.abs(np.array(do.have)[:n_lay]).max()))
+    assert close(np.array(do.mave)[:n_lay], np.array(dg.mave)[:n_lay], 1.0)
+    assert close(np.array(do.mvar)[:n_lay], np.array(dg.mvar)[:n_lay], max(1e-6, float(np.abs(np.array(do.mvar)).max())))
+    assert close(np.array(do.have)[:n_lay], np.array(dg.have)[:n_lay], hs)
+    assert close(do.mave_all, dg.mave_all, 1.0)
+    assert close(do.have_all, dg.have_all, hs)
+    assert close(do.mvar_all, dg.mvar_all, max(1e-6, float(np.abs(np.array(do.mvar_all)).max())))
+
+
 def random_state(med, seed, tilt=None):
     rng = np.random.default_rng(seed)
     out = []
@@ -132,6 +145,7 @@ def test_heun_step_parity_cfg0(oracle_mod):
     assert do.t == dg.t
     assert abs(do.errmx - dg.errmx) <= 1e-8 * do.errmx
     assert abs(do.mave[0][2] - dg.mave[0][2]) < 1e-10
+    check_loop_stats(do, dg, 1)
     # the stored field is the stage-2 field of the last step
     ho = s.get_state(0)[1]
     hg = gpu.get_state(0)[1]
@@ -300,6 +314,7 @@ def test_adaptive_dt_parity(oracle_mod, max_dm):
         dg = gpu.step(n, t0=t, noise=noise[c * n:(c + 1) * n])
         assert do.rejected == dg.rejected
         assert abs(do.dt - dg.dt) <= 1e-12 * do.dt and abs(do.t - dg.t) <= 1e-12 * do.t
+        check_loop_stats(do, dg, med.n_lay, tol=1e-9)
         t, dt, rejected = do.t, do.dt, rejected + do.rejected
     if max_dm < 10:
         assert rejected > 0

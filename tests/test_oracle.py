@@ -423,3 +423,59 @@ def test_adaptive_rejection_bounds_rotation(oracle_mod):
         ang = np.degrees(np.arccos(np.clip(before @ after, -1, 1)))
         assert ang <= max_dm * 1.01
     assert rej > 0
+
+
+def _weighted(m, w):
+    W = w.sum()
+    mean = (w[:, None] * m).sum(0) / W
+    var = (w[:, None] * (m - mean) ** 2).sum(0) / W
+    return W, mean, var
+
+
+def test_loop_statistics_closed_form(oracle_mod):
+    """llg()'s loop-top statistics (llg.f90:217-237) of the last step: the volume-weighted
+    mean and variance of m (West's incremental update) and mean of h over the state and h
+    that step starts from, and the layers combined by total volume.  Pinned against the
+    two-pass weighted moments of the state read back before that step."""
+    O = oracle_mod
+    med = media.make_media(24, 20, dz=[2e-7, 3e-7], n_lay=2, pitch=6e-7, seed=31)
+    g = abi.make_geom(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz)
+    s = O.Sim(g)
+    s.set_tensor(O.calculate_tensor(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, num_images=1))
+    for k, L in enumerate(med.layers):
+        s.set_layer(k, L)
+    for k, u in enumerate(media.ramp_field(med.n_lay, h_max=1e4, t_ramp=1e-11)):
+        s.set_ufield(k, u)
+    rng = np.random.default_rng(3)
+    for k, L in enumerate(med.layers):
+        m = rng.standard_normal((L.ng, 3)) + np.array([0.0, 0.0, 2.0])
+        s.set_state(k, m / np.linalg.norm(m, axis=1, keepdims=True))
+    dt = 5e-14
+    s.step(7, dt)
+    before = [s.get_state(k) for k in range(med.n_lay)]
+    d = s.step(3, dt, t0=7 * dt)
+    assert d.steps == 3
+    # the last step starts from the state after 9 steps: step once more from the saved one
+    s2 = O.Sim(g)
+    s2.set_tensor(O.calculate_tensor(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, num_images=1))
+    for k, L in enumerate(med.layers):
+        s2.set_layer(k, L)
+    for k, u in enumerate(media.ramp_field(med.n_lay, h_max=1e4, t_ramp=1e-11)):
+        s2.set_ufield(k, u)
+    for k in range(med.n_lay):
+        s2.set_state(k, before[k][0])
+    s2.step(2, dt, t0=7 * dt)
+    start = [s2.get_state(k) for k in range(med.n_lay)]
+    Ws, means, hs, vars_ = [], [], [], []
+    for k, L in enumerate(med.layers):
+        W, mean, var = _weighted(start[k][0], L.grain_vol)
+        _, hmean, _ = _weighted(start[k][1], L.grain_vol)
+        np.testing.assert_allclose(d.mave[k], mean, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(d.mvar[k], var, rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(d.have[k], hmean, rtol=1e-12, atol=1e-9)
+        Ws.append(W); means.append(mean); hs.append(hmean); vars_.append(var)
+    Ws = np.array(Ws)
+    np.testing.assert_allclose(d.mave_all, (Ws[:, None] * np.array(means)).sum(0) / Ws.sum(), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(d.mvar_all, (Ws[:, None] * np.array(vars_)).sum(0) / Ws.sum(), rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(d.have_all, (Ws[:, None] * np.array(hs)).sum(0) / Ws.sum(), rtol=1e-12, atol=1e-9)
+    assert np.all(np.array(d.mvar[0]) > 1e-3)     # a spread state: the variance is not trivially zero
