@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     // values of KSB source layers are loaded before any of them is used, so a thread waits
     // for memory once per KSB layers instead of once per layer.
     const int items = (ko1 - ko0) * ny;
-    constexpr int MAXI = 4, KSB = 4, NS = SYM ? 6 : 9;
+    constexpr int MAXI = 4, KSB = 2, NS = SYM ? 6 : 9;
     C2<T> hv[MAXI][3];
 #pragma unroll
     for (int q = 0; q < MAXI; q++) {
@@ -341,7 +341,12 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(B * nx);
-    const int y0 = blockIdx.x * TY, kc = blockIdx.y;        // kc = k*3 + c
+    // 1D grid padded to a multiple of 8, XCD-contiguous: the row blocks that share the
+    // cache lines of one kx run (TY*sizeof(C2<T>) < 128 bytes) are read/written in one L2
+    const int nyb = (ny + TY - 1) / TY;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    if (L >= nyb * 3 * nl) return;
+    const int y0 = (L % nyb) * TY, kc = L / nyb;        // kc = k*3 + c
     const int k = kc / 3, c = kc - 3 * k;
     const size_t CP = (size_t)nl * ny * nx;
     const T* src = Mcell + c * CP + (size_t)k * ny * nx;
@@ -471,7 +476,10 @@ __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(B * nx);
-    const int y0 = blockIdx.x * TY, kc = blockIdx.y;
+    const int nyb = (ny + TY - 1) / TY;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    if (L >= nyb * 3 * nl) return;
+    const int y0 = (L % nyb) * TY, kc = L / nyb;
     const int k = kc / 3, c = kc - 3 * k;
     const C2<T>* src = S + (size_t)kc * nxh * ny;
     // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
@@ -2482,7 +2490,7 @@ struct Engine : EngineBase {
     {
         if (split) return demag_split();
         const int ny = c->ny, nl = c->nl, nxh = c->nxh;
-        dim3 grow((ny + 1) / 2, nl), growT((ny + TYr - 1) / TYr, 3 * nl), blk(row_threads);
+        dim3 grow((ny + 1) / 2, nl), growT(rowT_blocks()), blk(row_threads);
         RCHK(ev_begin(K_ROW_FWD));
         if (tl) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
         else LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
@@ -2506,7 +2514,7 @@ struct Engine : EngineBase {
     {
         const int ny = c->ny, nl = c->nl, nxh = c->nxh;
         const int ncols = 3 * nl * nxh;
-        dim3 grow((ny + TYr - 1) / TYr, 3 * nl), blk(row_threads);
+        dim3 grow(rowT_blocks()), blk(row_threads);
         RCHK(ev_begin(K_ROW_FWD));
         LAUNCH_E(maxb_rowT, launch_rowT_fwd, grow, blk);
         HIPCHK(hipGetLastError());
@@ -2535,6 +2543,8 @@ struct Engine : EngineBase {
         RCHK(ev_end());
         return MRMG_OK;
     }
+    // transposing row kernels: (ny/TY) * 3 n_lay blocks on a 1D grid padded to a multiple of 8
+    unsigned rowT_blocks() const { return (unsigned)(((c->ny + TYr - 1) / TYr * 3 * c->nl + 7) / 8 * 8); }
     template <int MB> void launch_rowT_fwd(dim3 g, dim3 b)
     {
         if (wv)
