@@ -112,13 +112,31 @@ typedef struct {
     double msexp, hkexp;             /* Ms, Hk ~ (1 - T/Tc)^exp when >= 0                   */
 } mrmg_layer;
 
-/* Uniform, time-ramped applied field of one layer: the single uniform field branch of
- * get_appl (applied.f90:863-883) with the linear ramp of set_fld (applied.f90:210-320):
- *   h(t) = ((s_end - s_start) * clamp((t - t_start)/(t_end - t_start), 0, 1) + s_start)
- *          * scale * dir                                                               */
+/* Time function of an applied field or source (funct, applied.f90:328-396):
+ *   MRMG_FN_RAMP (0)  the linear ramp of set_fld (applied.f90:210-320):
+ *                     ((s_end - s_start) * clamp((t - t_start)/(t_end - t_start), 0, 1) + s_start) * scale
+ *   otherwise         (amp * f(t) + dc) * scale with f = sin(gamma t + phase) (SIN),
+ *                     exp(gamma t + phase) (EXP), cos(gamma t + phase) (COS), the
+ *                     exponential pulse train of period gamma, time constant param, duty
+ *                     and modulation depth (NEW), or the trapezoid wave of period gamma
+ *                     and ramp width param (STE).
+ * As set_fld does, a function with amp == 0 or gamma == 0 becomes the constant
+ * amp * f(0) + dc, and duty and moddepth are clamped to [0, 1].  ROT is not carried. */
+#define MRMG_FN_RAMP 0
+#define MRMG_FN_SIN 1
+#define MRMG_FN_EXP 2
+#define MRMG_FN_COS 3
+#define MRMG_FN_NEW 4
+#define MRMG_FN_STE 5
+
+/* Uniform applied field of one layer: the single uniform field branch of get_appl
+ * (applied.f90:863-883), h(t) = scale(t) * dir with the time function above. */
 typedef struct {
     double dir[3];
     double t_start, t_end, s_start, s_end, scale;
+    int func;                        /* MRMG_FN_*                                             */
+    int pad_;
+    double amp, gamma, phase, param, duty, moddepth, dc;
 } mrmg_ufield;
 
 /* A moving applied source of one layer (get_appl, applied.f90:827-1034), sampled at the
@@ -132,7 +150,7 @@ typedef struct {
  *                         g(u) = exp(-((u * isig)^expn)) (set_sg/get_sg, applied.f90:1778-1811)
  * Position along each axis at time t: offset + vel*(t - t_origin) - ref (tables) and
  * ref - vel*(t - t_origin) - offset (super-Gaussian), as get_appl forms them.  The value
- * is scaled by the linear ramp of mrmg_ufield (t_start, t_end, s_start, s_end, scale).
+ * is scaled by the time function of mrmg_ufield (ramp or funct).
  * A grain's field is its static h_app + the source average + the uniform field; its
  * temperature is the source average + its static temp.  A temperature source makes the
  * temperature scalings (msexp, hkexp) and the thermal variance follow the temperature. */
@@ -151,6 +169,9 @@ typedef struct {
     double isigx, isigy;             /* 1/sigma (cm^-1), 0 = flat along that axis           */
     double dtemp;                    /* super-Gaussian peak (K)                               */
     double t_start, t_end, s_start, s_end, scale;
+    int func;                        /* time function, as in mrmg_ufield                      */
+    int pad_;
+    double amp, gamma, phase, param, duty, moddepth, dc;
 } mrmg_source;
 
 typedef struct {

@@ -49,9 +49,15 @@ class Layer(C.Structure):
                 ("msexp", C.c_double), ("hkexp", C.c_double)]
 
 
+MRMG_FN_RAMP, MRMG_FN_SIN, MRMG_FN_EXP, MRMG_FN_COS, MRMG_FN_NEW, MRMG_FN_STE = 0, 1, 2, 3, 4, 5
+_TFUN = [("func", C.c_int), ("pad_", C.c_int), ("amp", C.c_double), ("gamma", C.c_double), ("phase", C.c_double),
+         ("param", C.c_double), ("duty", C.c_double), ("moddepth", C.c_double), ("dc", C.c_double)]
+TFUN_KEYS = ("func", "amp", "gamma", "phase", "param", "duty", "moddepth", "dc")
+
+
 class UField(C.Structure):
     _fields_ = [("dir", C.c_double * 3), ("t_start", C.c_double), ("t_end", C.c_double),
-                ("s_start", C.c_double), ("s_end", C.c_double), ("scale", C.c_double)]
+                ("s_start", C.c_double), ("s_end", C.c_double), ("scale", C.c_double)] + _TFUN
 
 
 MRMG_SRC_NONE, MRMG_SRC_FIELD_TABLE, MRMG_SRC_TEMP_TABLE, MRMG_SRC_TEMP_SG = 0, 1, 2, 3
@@ -64,7 +70,7 @@ class Source(C.Structure):
                 ("offset_x", C.c_double), ("offset_y", C.c_double), ("ref_x", C.c_double), ("ref_y", C.c_double),
                 ("t_origin", C.c_double), ("isigx", C.c_double), ("isigy", C.c_double), ("dtemp", C.c_double),
                 ("t_start", C.c_double), ("t_end", C.c_double), ("s_start", C.c_double), ("s_end", C.c_double),
-                ("scale", C.c_double)]
+                ("scale", C.c_double)] + _TFUN
 
 
 class SourceArrays:
@@ -74,7 +80,7 @@ class SourceArrays:
 
     def __init__(self, kind, xpos=None, ypos=None, vals=None, vel=(0.0, 0.0), offset=(0.0, 0.0), ref=(0.0, 0.0),
                  t_origin=0.0, sigma=(0.0, 0.0), expn=2, dtemp=0.0, t_start=0.0, t_end=1.0, s_start=1.0, s_end=1.0,
-                 scale=1.0):
+                 scale=1.0, **tfun):
         self.kind = int(kind)
         self.xpos = None if xpos is None else np.ascontiguousarray(xpos, dtype=np.float64)
         self.ypos = None if ypos is None else np.ascontiguousarray(ypos, dtype=np.float64)
@@ -83,6 +89,10 @@ class SourceArrays:
                        t_origin=t_origin, isigx=(1.0 / sigma[0]) if sigma[0] else 0.0,
                        isigy=(1.0 / sigma[1]) if sigma[1] else 0.0, dtemp=dtemp, t_start=t_start, t_end=t_end,
                        s_start=s_start, s_end=s_end, scale=scale)
+        for key, v in tfun.items():
+            if key not in TFUN_KEYS:
+                raise TypeError(f"unknown time-function argument {key}")
+            self.kw[key] = int(v) if key == "func" else float(v)
         self.expn = int(expn)
 
     def struct(self):
@@ -181,11 +191,16 @@ def make_geom(nx, ny, n_lay, dx, dy, dz, il_space=None, precision=MRMG_FP64, dem
     return g
 
 
-def make_ufield(direction, t_start=-1e22, t_end=1e22, s_start=1.0, s_end=1.0, scale=1.0):
+def make_ufield(direction, t_start=-1e22, t_end=1e22, s_start=1.0, s_end=1.0, scale=1.0, **tfun):
+    """mrmg_ufield; tfun: func (MRMG_FN_*), amp, gamma, phase, param, duty, moddepth, dc."""
     u = UField()
     for i in range(3):
         u.dir[i] = float(direction[i])
     u.t_start, u.t_end, u.s_start, u.s_end, u.scale = t_start, t_end, s_start, s_end, scale
+    for key, v in tfun.items():
+        if key not in TFUN_KEYS:
+            raise TypeError(f"unknown time-function argument {key}")
+        setattr(u, key, int(v) if key == "func" else float(v))
     return u
 
 

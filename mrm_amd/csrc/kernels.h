@@ -16,6 +16,51 @@
 
 #define MRMG_MAX_LAYERS 16
 
+// Time function of an applied field or source after set_fld's normalisation: func 0 is
+// the linear ramp ((s1 - s0) clamp((t - t0)/(t1 - t0)) + s0) * scale, otherwise
+// (amp * funct(t) + dc) * scale (applied.f90:328-396, 866-874).
+struct TFun {
+    int func, pad_;
+    double amp, gamma, phase, param, duty, moddepth, dc;
+};
+
+// funct (applied.f90:328-396): SIN, EXP, COS, the NEW pulse train and the STE trapezoid
+__host__ __device__ inline double funct_eval(int func, double t, double gamm, double phase, double param, double duty,
+                                             double moddepth)
+{
+    if (func == MRMG_FN_SIN) return sin(t * gamm + phase);
+    if (func == MRMG_FN_EXP) return exp(t * gamm + phase);
+    if (func == MRMG_FN_COS) return cos(t * gamm + phase);
+    if (func == MRMG_FN_NEW || func == MRMG_FN_STE) {
+        // the phase-shifted time folded into one period [0, gamm)
+        double x = t + phase;
+        if (x < 0.0) x = x + gamm * (1 + (int)fabs(x / gamm));
+        x = fmod(x, gamm);
+        if (func == MRMG_FN_NEW) {
+            const double a = gamm / param, y = exp(duty * a), w = exp(a);
+            if (x > duty * gamm) return 1.0 - moddepth + exp((gamm - x) / param) * (y - 1.0) * moddepth / (w - 1.0);
+            return 1.0 + exp(-x / param) * (y - w) * moddepth / (w - 1.0);
+        }
+        if (x < 0.5 * param) return 2.0 * x / param;
+        if (x < 0.5 * (gamm - param)) return 1.0;
+        if (x < 0.5 * (gamm + param)) return (gamm - 2.0 * x) / param;
+        if (x < gamm - 0.5 * param) return -1.0;
+        return 2.0 * (x - gamm) / param;
+    }
+    return 0.0;
+}
+
+__host__ __device__ inline double tscale_eval(const TFun& f, double t0, double t1, double s0, double s1, double scale,
+                                              double t)
+{
+    if (f.func != MRMG_FN_RAMP)
+        return (f.amp * funct_eval(f.func, t, f.gamma, f.phase, f.param, f.duty, f.moddepth) + f.dc) * scale;
+    double c = (t - t0) / (t1 - t0);
+    c = c < 1.0 ? c : 1.0;
+    c = c > 0.0 ? c : 0.0;
+    return ((s1 - s0) * c + s0) * scale;
+}
+
 // A moving applied source of one layer (slot 0 field, slot 1 temperature).
 struct SrcParams {
     int kind, nxp, nyp, expn;
@@ -23,6 +68,7 @@ struct SrcParams {
     int pad_;
     double vel_x, vel_y, offset_x, offset_y, ref_x, ref_y, t_origin, isigx, isigy, dtemp;
     double t0, t1, s0, s1, scale;
+    TFun fn;
 };
 
 struct LayerParams {
@@ -37,6 +83,7 @@ struct LayerParams {
     int use_mag;          // Ms scale = clamp(|m|) (use_mag_p with tc_nom <= 1e10)
     int tscale;           // tc_nom <= 1e10
     SrcParams src[2];
+    TFun uf_fn;           // time function of the uniform field
 };
 static_assert(sizeof(LayerParams) % 8 == 0, "layer parameters are copied to LDS in 8-byte words");
 

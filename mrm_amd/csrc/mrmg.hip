@@ -710,10 +710,7 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
 #pragma unroll
         for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)hs[i];
     if (L.has_uf) {
-        double c = (t - L.uf_t0) / (L.uf_t1 - L.uf_t0);
-        c = c < 1.0 ? c : 1.0;
-        c = c > 0.0 ? c : 0.0;
-        const double sc = ((L.uf_s1 - L.uf_s0) * c + L.uf_s0) * L.uf_scale;
+        const double sc = tscale_eval(L.uf_fn, L.uf_t0, L.uf_t1, L.uf_s0, L.uf_s1, L.uf_scale, t);
 #pragma unroll
         for (int i = 0; i < 3; i++) happ[i] = happ[i] + (T)(sc * L.uf_dir[i]);
     }
@@ -822,10 +819,7 @@ __global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, i
     const SrcParams& sp = lp[k].src[slot];
     if (sp.kind == MRMG_SRC_NONE) return;
     const double t = ts ? t1 : t0;
-    double c = (t - sp.t0) / (sp.t1 - sp.t0);
-    c = c < 1.0 ? c : 1.0;
-    c = c > 0.0 ? c : 0.0;
-    const double sc = ((sp.s1 - sp.s0) * c + sp.s0) * sp.scale;
+    const double sc = tscale_eval(sp.fn, sp.t0, sp.t1, sp.s0, sp.s1, sp.scale, t);
     const bool isx = e < nx;
     const int i = isx ? e : e - nx;   // 0-based cell index; the reference's i is i+1
     const size_t base = (size_t)(k * 2 + slot) * 2 + ts;
@@ -1712,6 +1706,35 @@ static double field_ms_scale(const HostLayer& H, int n, double mag)
     return H.use_mag ? std::min(1.0, std::max(0.0, fabs(mag))) : H.ms_scale[n];
 }
 
+// set_fld's normalisation of a time function (applied.f90:276-310): duty and moddepth
+// clamped to [0, 1]; a function with amp == 0 or gamma == 0 becomes the constant
+// amp * f(0) + dc; equal ramp ends fold into the scale
+static TFun norm_tfun(int func, double amp, double gamma, double phase, double param, double duty, double moddepth,
+                      double dc, double& s0, double& s1, double& scale)
+{
+    TFun f{};
+    f.func = func; f.amp = amp; f.gamma = gamma; f.phase = phase; f.param = param; f.dc = dc;
+    f.duty = std::min(1.0, std::max(0.0, duty));
+    f.moddepth = std::min(1.0, std::max(0.0, moddepth));
+    if (f.func != MRMG_FN_RAMP && (f.amp == 0.0 || f.gamma == 0.0)) {
+        s0 = f.amp * funct_eval(f.func, 0.0, f.gamma, f.phase, f.param, f.duty, f.moddepth) + f.dc;
+        s1 = s0;
+        f.func = MRMG_FN_RAMP;
+    }
+    if (s0 == s1) { scale = scale * s0; s0 = 1.0; s1 = 1.0; }
+    return f;
+}
+
+// checks of a time function's arguments
+static int check_tfun(int func, double t_start, double t_end, double amp, double gamma, double param)
+{
+    if (func < MRMG_FN_RAMP || func > MRMG_FN_STE) return set_err(MRMG_ERR_ARG, "time function %d unknown", func);
+    if (func == MRMG_FN_RAMP && t_end == t_start) return set_err(MRMG_ERR_ARG, "t_end == t_start");
+    if ((func == MRMG_FN_NEW || func == MRMG_FN_STE) && amp != 0.0 && gamma != 0.0 && param == 0.0)
+        return set_err(MRMG_ERR_ARG, "NEW and STE need param != 0");
+    return MRMG_OK;
+}
+
 struct EngineBase {
     virtual ~EngineBase() {}
     virtual int build(std::vector<HostLayer>& L) = 0;
@@ -2236,7 +2259,8 @@ struct Engine : EngineBase {
                 sp.dtemp = q.dtemp;
                 // set_temp/set_fld normalisation of a constant scale (applied.f90:307-310)
                 sp.t0 = q.t_start; sp.t1 = q.t_end; sp.s0 = q.s_start; sp.s1 = q.s_end; sp.scale = q.scale;
-                if (sp.s0 == sp.s1) { sp.scale = sp.scale * sp.s0; sp.s0 = 1.0; sp.s1 = 1.0; }
+                sp.fn = norm_tfun(q.func, q.amp, q.gamma, q.phase, q.param, q.duty, q.moddepth, q.dc, sp.s0, sp.s1,
+                                  sp.scale);
                 if (q.kind != MRMG_SRC_TEMP_SG) {
                     sp.poff_x = (int)pos.size();
                     pos.insert(pos.end(), L[k].src_x[sl].begin(), L[k].src_x[sl].end());
@@ -2282,8 +2306,8 @@ struct Engine : EngineBase {
             LayerParams& p = hlp[k];
             p.has_uf = L[k].has_uf;
             mrmg_ufield u = L[k].uf;
-            // set_fld normalisation (applied.f90:307-310)
-            if (u.s_start == u.s_end) { u.scale = u.scale * u.s_start; u.s_end = 1.0; u.s_start = 1.0; }
+            p.uf_fn = norm_tfun(u.func, u.amp, u.gamma, u.phase, u.param, u.duty, u.moddepth, u.dc, u.s_start, u.s_end,
+                                u.scale);
             for (int i = 0; i < 3; i++) p.uf_dir[i] = u.dir[i];
             p.uf_t0 = u.t_start; p.uf_t1 = u.t_end; p.uf_s0 = u.s_start; p.uf_s1 = u.s_end; p.uf_scale = u.scale;
         }
@@ -3206,7 +3230,7 @@ int mrmg_set_ufield(mrmg_ctx* c, int k, const mrmg_ufield* uf)
 {
     if (!c || !uf) return set_err(MRMG_ERR_ARG, "null argument");
     if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
-    if (uf->t_end == uf->t_start) return set_err(MRMG_ERR_ARG, "t_end == t_start");
+    RCHK(check_tfun(uf->func, uf->t_start, uf->t_end, uf->amp, uf->gamma, uf->param));
     c->layers[k].uf = *uf;
     c->layers[k].has_uf = true;
     if (!c->dirty) return c->eng->update_ufield(c->layers);
@@ -3224,7 +3248,7 @@ int mrmg_set_source(mrmg_ctx* c, int k, int slot, const mrmg_source* src)
             return set_err(MRMG_ERR_ARG, "slot 0 takes a field table (MRMG_SRC_FIELD_TABLE)");
         if (slot == 1 && kind != MRMG_SRC_TEMP_TABLE && kind != MRMG_SRC_TEMP_SG)
             return set_err(MRMG_ERR_ARG, "slot 1 takes a temperature table or super-Gaussian");
-        if (src->t_end == src->t_start) return set_err(MRMG_ERR_ARG, "t_end == t_start");
+        RCHK(check_tfun(src->func, src->t_start, src->t_end, src->amp, src->gamma, src->param));
         if (kind == MRMG_SRC_TEMP_SG) {
             if (src->expn < 1) return set_err(MRMG_ERR_ARG, "super-Gaussian power must be >= 1");
         } else {

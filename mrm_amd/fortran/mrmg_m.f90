@@ -43,6 +43,8 @@ module mrmg_m
   type, bind(C) :: mrmg_ufield
     real(c_double) :: dir(3)
     real(c_double) :: t_start, t_end, s_start, s_end, scale
+    integer(c_int) :: func, pad_                ! MRMG_FN_*: 0 ramp, 1 SIN, 2 EXP, 3 COS, 4 NEW, 5 STE
+    real(c_double) :: amp, gamma, phase, param, duty, moddepth, dc
   end type
 
   type, bind(C) :: mrmg_step_params

@@ -589,12 +589,76 @@ int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
     return MRMG_OK;
 }
 
+/* funct (applied.f90:328-396): SIN, EXP, COS, the NEW pulse train and the STE trapezoid
+ * wave (ROT is not restated).  Fortran's mod of reals is fmod (truncating). */
+static double ora_funct(int func, double t, double gamm, double phase, double param, double duty, double moddepth)
+{
+    double x, y, w;
+    if (func == MRMG_FN_SIN) return sin(t * gamm + phase);
+    if (func == MRMG_FN_EXP) return exp(t * gamm + phase);
+    if (func == MRMG_FN_COS) return cos(t * gamm + phase);
+    if (func == MRMG_FN_NEW) {
+        x = gamm / param;
+        y = exp(duty * x); w = exp(x);
+        x = t + phase; if (x < 0.0) x = x + gamm * (1 + (int)fabs(x / gamm));
+        x = fmod(x, gamm);
+        if (x > duty * gamm) {
+            x = exp((gamm - x) / param);
+            return 1.0 - moddepth + x * (y - 1.0) * moddepth / (w - 1.0);
+        }
+        x = exp(-x / param);
+        return 1.0 + x * (y - w) * moddepth / (w - 1.0);
+    }
+    if (func == MRMG_FN_STE) {
+        x = t + phase;
+        if (x < 0.0) x = x + gamm * (1 + (int)fabs(x / gamm));
+        x = fmod(x, gamm);
+        if (x < 0.5 * param) return 2.0 * x / param;
+        if (x < 0.5 * (gamm - param)) return 1.0;
+        if (x < 0.5 * (gamm + param)) return (gamm - 2.0 * x) / param;
+        if (x < gamm - 0.5 * param) return -1.0;
+        return 2.0 * (x - gamm) / param;
+    }
+    return 0.0;
+}
+
+/* set_fld's normalisation of a time function (applied.f90:276-310): duty and moddepth
+ * clamped; amp == 0 or gamma == 0 makes a constant amp*f(0) + dc; equal ramp ends fold
+ * into the scale */
+static void ora_norm_tfun(int *func, double *amp, double *gamma, double *phase, double *param, double *duty,
+                          double *moddepth, double *dc, double *s_start, double *s_end, double *scale)
+{
+    *duty = *duty < 0.0 ? 0.0 : (*duty > 1.0 ? 1.0 : *duty);
+    *moddepth = *moddepth < 0.0 ? 0.0 : (*moddepth > 1.0 ? 1.0 : *moddepth);
+    if (*func != MRMG_FN_RAMP && (*amp == 0.0 || *gamma == 0.0)) {
+        *s_start = *amp * ora_funct(*func, 0.0, *gamma, *phase, *param, *duty, *moddepth) + *dc;
+        *func = MRMG_FN_RAMP; *s_end = *s_start;
+    }
+    if (*s_start == *s_end) { *scale = *scale * *s_start; *s_end = 1.0; *s_start = 1.0; }
+}
+
+/* the time scale of get_appl (applied.f90:866-874) */
+static double ora_tscale(int func, double amp, double gamma, double phase, double param, double duty,
+                         double moddepth, double dc, double t_start, double t_end, double s_start, double s_end,
+                         double scale, double t)
+{
+    if (func != MRMG_FN_RAMP) return (amp * ora_funct(func, t, gamma, phase, param, duty, moddepth) + dc) * scale;
+    double c = (t - t_start) / (t_end - t_start);
+    c = c < 1.0 ? c : 1.0; c = c > 0.0 ? c : 0.0;
+    return ((s_end - s_start) * c + s_start) * scale;
+}
+
+double ora_funct_value(int func, double t, double gamm, double phase, double param, double duty, double moddepth)
+{
+    return ora_funct(func, t, gamm, phase, param, duty, moddepth);
+}
+
 int ora_set_ufield(ora_sim *s, int k, const mrmg_ufield *uf)
 {
     if (k < 0 || k >= s->nl) return MRMG_ERR_ARG;
     mrmg_ufield u = *uf;
-    /* set_fld normalisation, applied.f90:307-310 */
-    if (u.s_start == u.s_end) { u.scale = u.scale * u.s_start; u.s_end = 1.0; u.s_start = 1.0; }
+    ora_norm_tfun(&u.func, &u.amp, &u.gamma, &u.phase, &u.param, &u.duty, &u.moddepth, &u.dc, &u.s_start, &u.s_end,
+                  &u.scale);
     s->L[k].uf = u; s->L[k].has_uf = 1;
     return MRMG_OK;
 }
@@ -711,9 +775,8 @@ static void ufield_at(const olayer *L, double t, double *u)
 {
     if (!L->has_uf) { u[0] = u[1] = u[2] = 0.0; return; }
     const mrmg_ufield *f = &L->uf;
-    double c = (t - f->t_start) / (f->t_end - f->t_start);
-    c = c < 1.0 ? c : 1.0; c = c > 0.0 ? c : 0.0;
-    double sc = ((f->s_end - f->s_start) * c + f->s_start) * f->scale;
+    double sc = ora_tscale(f->func, f->amp, f->gamma, f->phase, f->param, f->duty, f->moddepth, f->dc, f->t_start,
+                           f->t_end, f->s_start, f->s_end, f->scale, t);
     for (int i = 0; i < 3; i++) u[i] = sc * f->dir[i];
 }
 
@@ -751,8 +814,10 @@ int ora_set_source(ora_sim *s, int k, int slot, const mrmg_source *src)
     free_source(o);
     if (!src || src->kind == MRMG_SRC_NONE) return MRMG_OK;
     o->s = *src;
-    /* set_temp/set_fld normalisation of a constant scale (applied.f90:307-310) */
-    if (o->s.s_start == o->s.s_end) { o->s.scale = o->s.scale * o->s.s_start; o->s.s_start = 1.0; o->s.s_end = 1.0; }
+    /* set_temp/set_fld normalisation (applied.f90:276-310) */
+    mrmg_source *q = &o->s;
+    ora_norm_tfun(&q->func, &q->amp, &q->gamma, &q->phase, &q->param, &q->duty, &q->moddepth, &q->dc, &q->s_start,
+                  &q->s_end, &q->scale);
     if (src->kind != MRMG_SRC_TEMP_SG) {
         int nv = src->kind == MRMG_SRC_FIELD_TABLE ? 3 : 1;
         o->xpos = (double *)dupa(src->xpos, sizeof(double) * src->nxp);
@@ -793,9 +858,8 @@ static void src_set_sg(int expn, double sig, double del, int n, double dd, doubl
 
 static double src_scale(const mrmg_source *q, double t)
 {
-    double c = (t - q->t_start) / (q->t_end - q->t_start);
-    c = c < 1.0 ? c : 1.0; c = c > 0.0 ? c : 0.0;
-    return ((q->s_end - q->s_start) * c + q->s_start) * q->scale;
+    return ora_tscale(q->func, q->amp, q->gamma, q->phase, q->param, q->duty, q->moddepth, q->dc, q->t_start,
+                      q->t_end, q->s_start, q->s_end, q->scale, t);
 }
 
 /* Values of a source at every cell of the layer, (nv, nx, ny) x fastest per component:

@@ -64,6 +64,7 @@ def lib():
             "ora_num_threads": ([], C.c_int),
             "ora_lv": ([C.c_double], C.c_double),
             "ora_dlv": ([C.c_double], C.c_double),
+            "ora_funct_value": ([C.c_int] + [C.c_double] * 6, C.c_double),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -172,6 +173,11 @@ def irfft2(c, nx):
 
 
 # ---------------------------------------------------------------- simulation
+def funct(func, t, gamma, phase=0.0, param=0.0, duty=0.0, moddepth=0.0):
+    """funct (applied.f90:328-396) as restated in the oracle."""
+    return lib().ora_funct_value(int(func), t, gamma, phase, param, duty, moddepth)
+
+
 class Sim:
     """The restated llg/field/demag path on the CPU, driven like the C ABI."""
 

@@ -56,7 +56,9 @@ int main(void) {
   F(mrmg_layer, grid2grain) F(mrmg_layer, hx) F(mrmg_layer, lhx) F(mrmg_layer, h_app) F(mrmg_layer, alpha)
   F(mrmg_layer, floor_ms) F(mrmg_layer, thermal) F(mrmg_layer, use_mag) F(mrmg_layer, tc) F(mrmg_layer, lambda_ms)
   F(mrmg_layer, spin) F(mrmg_layer, hkexp)
-  F(mrmg_ufield, t_start) F(mrmg_ufield, scale)
+  F(mrmg_ufield, t_start) F(mrmg_ufield, scale) F(mrmg_ufield, func) F(mrmg_ufield, amp) F(mrmg_ufield, dc)
+  F(mrmg_source, scale) F(mrmg_source, func) F(mrmg_source, amp) F(mrmg_source, dc)
+  printf("mrmg_source %zu\n", sizeof(mrmg_source));
   F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
   F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm) F(mrmg_step_params, llb)
   F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave) F(mrmg_diag, rejected) F(mrmg_diag, max_dmdt)
@@ -82,8 +84,9 @@ def test_ctypes_layout_matches_header():
     assert C.sizeof(abi.UField) == want["mrmg_ufield"]
     assert C.sizeof(abi.StepParams) == want["mrmg_step_params"]
     assert C.sizeof(abi.Diag) == want["mrmg_diag"]
+    assert C.sizeof(abi.Source) == want["mrmg_source"]
     types = {"mrmg_geom": abi.Geom, "mrmg_layer": abi.Layer, "mrmg_ufield": abi.UField,
-             "mrmg_step_params": abi.StepParams, "mrmg_diag": abi.Diag}
+             "mrmg_step_params": abi.StepParams, "mrmg_diag": abi.Diag, "mrmg_source": abi.Source}
     for k, v in want.items():
         if "." in k:
             t, f = k.split(".")

@@ -135,3 +135,50 @@ def test_source_argument_checks():
         x.set_source(0, 0, bad)                 # nodes not increasing
     with pytest.raises(abi.MrmgError):
         x.set_source(0, 1, head_table(med))     # a field table in the temperature slot
+
+
+@pytest.mark.parametrize("ltype", [-3, 2])
+def test_time_functions_parity(oracle_mod, ltype):
+    """Time functions (funct, applied.f90:328-396) on the device: a uniform field under the
+    NEW pulse train, a moving head table under SIN and a laser under the STE wave."""
+    O = oracle_mod
+    med = llb_media(nx=48, ny=40, seed=51, hkexp=1.5, tc_nom=700.0)
+    for L in med.layers:
+        L.temp = np.full(L.ng, 300.0)
+    period = 1.5e-12
+    uf = abi.make_ufield((0.1, 0.0, -1.0), scale=6000.0, func=abi.MRMG_FN_NEW, amp=1.0, gamma=period, param=0.4e-12,
+                         duty=0.4, moddepth=0.7)
+    s, gpu = pair(O, med, ufields=[uf] * med.n_lay)
+    head = head_table(med)
+    head.kw.update(func=abi.MRMG_FN_SIN, amp=0.8, gamma=4e12, phase=0.3, dc=0.2)
+    spot = laser(med, dtemp=300.0)
+    spot.kw.update(func=abi.MRMG_FN_STE, amp=1.0, gamma=period, param=0.5e-12, dc=1.0)
+    for k in range(med.n_lay):
+        both(s, gpu, k, 0, head)
+        both(s, gpu, k, 1, spot)
+    for k, m in enumerate(random_state(med, 9, tilt=0.3)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    n = 40
+    noise = O.mt_noise([61, 62], [L.ng for L in med.layers], [True] * med.n_lay, n)
+    gpu.set_llg(dt=5e-14, llg_type=ltype)
+    s.step(n, 5e-14, type=ltype, noise=noise)
+    gpu.step(n, noise=noise)
+    d = max(np.abs(s.get_state(k)[0] - gpu.get_state(k)[0]).max() for k in range(med.n_lay))
+    dh = max(np.abs(s.get_state(k)[1] - gpu.get_state(k)[1]).max() for k in range(med.n_lay))
+    say(f"time functions type {ltype}: |dm| {d:.2e} |dh| {dh:.2e}")
+    assert d <= 1e-10 and dh <= 1e-7
+
+
+def test_time_function_argument_checks():
+    from mrm_amd.llg import Llg
+    med = media.make_media(16, 16, dz=[4e-7], n_lay=1, pitch=6e-7, seed=3)
+    x = Llg(med.nx, med.ny, 1, med.dx, med.dy, med.dz)
+    try:
+        with pytest.raises(abi.MrmgError):
+            x.set_ufield(0, abi.make_ufield((0, 0, 1), func=9, amp=1.0, gamma=1.0))
+        with pytest.raises(abi.MrmgError):
+            x.set_ufield(0, abi.make_ufield((0, 0, 1), func=abi.MRMG_FN_STE, amp=1.0, gamma=1e-9, param=0.0))
+        x.set_ufield(0, abi.make_ufield((0, 0, 1), t_start=0.0, t_end=0.0, func=abi.MRMG_FN_SIN, amp=1.0, gamma=1e9))
+    finally:
+        x.close()
