@@ -110,6 +110,16 @@ typedef struct {
     double spin;                     /* spin S of the Brillouin/Langevin mean field         */
     double tc_nom;                   /* temperature scaling on when tc_nom <= 1e10          */
     double msexp, hkexp;             /* Ms, Hk ~ (1 - T/Tc)^exp when >= 0                   */
+    /* READ_SCALE tables of find_temp_scaling (temp_scaling.f90:365-431): n >= 2 nodes of
+     * increasing T/Tc (tab_t) and the scaling there (tab_s), interpolated linearly and held
+     * at the end values outside.  Ms uses its table when neither use_mag nor msexp >= 0 is
+     * set, Hk when hkexp < 0; alpha has only the table.  n = 0: no table.  With tc_nom <=
+     * 1e10 they set ms_scale, alpha_scale, hk_scale from the static temperature and, under a
+     * moving temperature source, follow the grain temperature every field evaluation.     */
+    int n_ms_tab, n_alpha_tab, n_hk_tab, pad3_;
+    const double *ms_tab_t, *ms_tab_s;
+    const double *alpha_tab_t, *alpha_tab_s;
+    const double *hk_tab_t, *hk_tab_s;
 } mrmg_layer;
 
 /* Time function of an applied field or source (funct, applied.f90:328-396):

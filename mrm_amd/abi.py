@@ -46,7 +46,10 @@ class Layer(C.Structure):
                 ("thm_alpha", C.c_double), ("floor_ms", C.c_double),
                 ("thermal", C.c_int), ("use_mag", C.c_int),
                 ("tc", _dp), ("lambda_ms", _dp), ("spin", C.c_double), ("tc_nom", C.c_double),
-                ("msexp", C.c_double), ("hkexp", C.c_double)]
+                ("msexp", C.c_double), ("hkexp", C.c_double),
+                ("n_ms_tab", C.c_int), ("n_alpha_tab", C.c_int), ("n_hk_tab", C.c_int), ("pad3_", C.c_int),
+                ("ms_tab_t", _dp), ("ms_tab_s", _dp), ("alpha_tab_t", _dp), ("alpha_tab_s", _dp),
+                ("hk_tab_t", _dp), ("hk_tab_s", _dp)]
 
 
 MRMG_FN_RAMP, MRMG_FN_SIN, MRMG_FN_EXP, MRMG_FN_COS, MRMG_FN_NEW, MRMG_FN_STE = 0, 1, 2, 3, 4, 5
@@ -168,6 +171,14 @@ class LayerArrays:
         self.tc_nom = float(kw.get("tc_nom", 1e22))
         self.msexp = float(kw.get("msexp", -1.0))
         self.hkexp = float(kw.get("hkexp", -1.0))
+        # READ_SCALE tables: {"ms"|"alpha"|"hk": (T/Tc nodes, scalings)}
+        self.tables = {}
+        for name, tab in (kw.get("tables") or {}).items():
+            if name not in ("ms", "alpha", "hk"):
+                raise ValueError(f"unknown scaling table {name}")
+            t = np.ascontiguousarray(tab[0], dtype=np.float64)
+            v = np.ascontiguousarray(tab[1], dtype=np.float64)
+            self.tables[name] = (t, v)
 
     def struct(self):
         s = Layer()
@@ -179,6 +190,10 @@ class LayerArrays:
         s.alpha, s.gamma, s.thm_alpha, s.floor_ms = self.alpha, self.gamma, self.thm_alpha, self.floor_ms
         s.thermal = self.thermal
         s.use_mag, s.spin, s.tc_nom, s.msexp, s.hkexp = self.use_mag, self.spin, self.tc_nom, self.msexp, self.hkexp
+        for name, (t, v) in self.tables.items():
+            setattr(s, f"n_{name}_tab", len(t))
+            setattr(s, f"{name}_tab_t", dptr(t))
+            setattr(s, f"{name}_tab_s", dptr(v))
         return s
 
 

@@ -84,7 +84,27 @@ struct LayerParams {
     int tscale;           // tc_nom <= 1e10
     SrcParams src[2];
     TFun uf_fn;           // time function of the uniform field
+    // READ_SCALE tables (0 Ms, 1 alpha, 2 Hk): n nodes of T/Tc at tabs + off and the
+    // scalings at tabs + off + n; n = 0 without a table
+    int tab_off[3], tab_n[3];
 };
+
+// the table branch of find_temp_scaling (temp_scaling.f90:397-416): linear between the
+// bracketing nodes, the end values outside
+__host__ __device__ inline double table_eval(const double* tt, const double* ss, int n, double x)
+{
+    if (x <= tt[0]) return ss[0];
+    if (x >= tt[n - 1]) return ss[n - 1];
+    int k1 = 0, k2 = n - 1;
+    while (k2 - k1 > 1) {
+        const int k = (k1 + k2) >> 1;
+        if (x < tt[k]) k2 = k;
+        else k1 = k;
+    }
+    double y = (x - tt[k1]) / (tt[k2] - tt[k1]);
+    y = y < 1.0 ? (y > 0.0 ? y : 0.0) : 1.0;
+    return ss[k1] + y * (ss[k2] - ss[k1]);
+}
 static_assert(sizeof(LayerParams) % 8 == 0, "layer parameters are copied to LDS in 8-byte words");
 
 // Per-stage axis tables of the moving sources, [layer][slot][time][n], time 0 = the
@@ -132,6 +152,7 @@ template <typename T> struct GrainData {
     const double* tc;
     const double* lam;
     const double* temp;
+    const double* tabs;   // READ_SCALE table nodes and values of every layer
     int any_src;          // some layer has a moving source
     SrcAxes ax;
 };

@@ -579,6 +579,31 @@ __device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& s
            (1.0 - fx) * yfc * v[nv * a11 + comp];
 }
 
+// (1 - T/Tc)^e clamped to [0, 1] (temp_scaling.f90:390-395)
+__device__ __forceinline__ double temp_power(double T, double Tc, double e)
+{
+    double x = 1.0 - T / Tc;
+    x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
+    return pow(x, e);
+}
+
+// find_temp_scaling of one quantity (0 Ms, 1 alpha, 2 Hk) at temperature T: the power law
+// for Ms and Hk when its exponent is >= 0, else the layer's table
+__device__ __forceinline__ bool scale_follows_t(const LayerParams& L, int q)
+{
+    if (!L.tscale) return false;
+    if (q == 0) return !L.use_mag && (L.msexp >= 0.0 || L.tab_n[0] > 1);
+    if (q == 1) return L.tab_n[1] > 1;
+    return L.hkexp >= 0.0 || L.tab_n[2] > 1;
+}
+__device__ __forceinline__ double scale_at_t(const LayerParams& L, const double* tabs, int q, double T, double Tc)
+{
+    const double e = q == 0 ? L.msexp : L.hkexp;
+    if (q != 1 && e >= 0.0) return temp_power(T, Tc, e);
+    const double* tt = tabs + L.tab_off[q];
+    return table_eval(tt, tt + L.tab_n[q], L.tab_n[q], T / Tc);
+}
+
 // tout (may be null): the grain temperature at the evaluation and at the next evaluation
 // from the layer's temperature source (field.f90:545-556); unchanged without one.
 // Loads are arranged for memory-level parallelism: the offsets and constants first, then
@@ -718,11 +743,7 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
     for (int i = 0; i < 3; i++) h[i] = happ[i] + hd[i];
     // find_temp_scaling of Hk at the evaluation's temperature (temp_scaling.f90:392-395)
     T hks = hk_s;
-    if (tsrc && L.tscale && L.hkexp >= 0.0 && tout) {
-        double x = 1.0 - tout[0] / gd.tc[g];
-        x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
-        hks = (T)pow(x, L.hkexp);
-    }
+    if (tsrc && tout && scale_follows_t(L, 2)) hks = (T)scale_at_t(L, gd.tabs, 2, tout[0], gd.tc[g]);
     // get_cubic_hk (field.f90:377-395)
     if (L.has_cub && hks > (T)0) {
         const T* A = gd.cub;
@@ -768,14 +789,6 @@ __device__ __forceinline__ void src_temp(const GrainData<T>& gd, const LayerPara
     const double dc = (double)(b1 - b0), tst = gd.temp[g];
     tout[0] = lane_sum<LPG>(ts0) / dc + tst;
     tout[1] = lane_sum<LPG>(ts1) / dc + tst;
-}
-
-// (1 - T/Tc)^e clamped to [0, 1] (temp_scaling.f90:390-395)
-__device__ __forceinline__ double temp_power(double T, double Tc, double e)
-{
-    double x = 1.0 - T / Tc;
-    x = x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0;
-    return pow(x, e);
 }
 
 // get_thermal's standard deviation (field.f90:407-423) for a grain whose temperature moves
@@ -893,10 +906,10 @@ __global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict_
     const LayerParams& L = gd.lp[gd.layer_of[g]];
     const T mm[3] = {m[g], m[gd.G + g], m[2 * gd.G + g]};
     T f = L.use_mag ? gd.ms[g] * m[(size_t)ST_MSC * gd.G + g] : gd.msf[g];
-    if (recompute && L.src[1].kind != MRMG_SRC_NONE && L.tscale && !L.use_mag && L.msexp >= 0.0) {
+    if (recompute && L.src[1].kind != MRMG_SRC_NONE && scale_follows_t(L, 0)) {
         double tt[2];
         src_temp<T, LPG>(gd, L, g, lane, tt);
-        const T msc = (T)temp_power(tt[0], gd.tc[g], L.msexp);
+        const T msc = (T)scale_at_t(L, gd.tabs, 0, tt[0], gd.tc[g]);
         if (lane == 0) m[(size_t)ST_MSC * gd.G + g] = msc;
         f = gd.ms[g] * msc;
     }
@@ -1170,7 +1183,8 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     T mfield[3] = {mf[g], mf[G + g], mf[2 * G + g]};
     const T mi[3] = {ma[g], ma[G + g], ma[2 * G + g]};
     const T msc = mf[(size_t)ST_MSC * G + g];
-    const T g_ms = gd.ms[g], g_msf = gd.msf[g], g_as = gd.alpha_scale[g], g_vol = gd.vol[g], g_sig = gd.sigma[g];
+    const T g_ms = gd.ms[g], g_msf = gd.msf[g], g_vol = gd.vol[g], g_sig = gd.sigma[g];
+    T g_as = gd.alpha_scale[g];
     const double g_temp = gd.temp[g];
     T dmv[3] = {0, 0, 0}, thv[3] = {0, 0, 0};
     if (MODE == GM_HEUN2)
@@ -1215,7 +1229,9 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
             }
         }
         // the Ms scale of the new state at the next evaluation follows the temperature
-        const bool dyn_ms = tsrc && L.tscale && !L.use_mag && L.msexp >= 0.0;
+        const bool dyn_ms = tsrc && scale_follows_t(L, 0);
+        // alpha_scale at this evaluation's temperature (push and thermal variance)
+        if (tsrc && scale_follows_t(L, 1)) g_as = (T)scale_at_t(L, gd.tabs, 1, Tev, gd.tc[g]);
         // thermal draw (get_thermal) happens with the first field of a step
         T th[3];
         if (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1) {
@@ -1279,7 +1295,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 3; c++) mo[c] = (T)mod[c];
             const T mscn = L.use_mag ? (T)(mago < 1.0 ? mago : 1.0)
-                                     : dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : gd.ms_scale[g];
+                                     : dyn_ms ? (T)scale_at_t(L, gd.tabs, 0, Tnx, gd.tc[g]) : gd.ms_scale[g];
             fsc = (L.use_mag || dyn_ms) ? g_ms * mscn : g_msf;
             T* dst = (MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_PUSH1) ? mb : (mc ? mc : ma);
             if (w0) {
@@ -1298,7 +1314,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         T H[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) H[c] = th[c] / sq + h[c];
-        const T mscn = dyn_ms ? (T)temp_power(Tnx, gd.tc[g], L.msexp) : msc;
+        const T mscn = dyn_ms ? (T)scale_at_t(L, gd.tabs, 0, Tnx, gd.tc[g]) : msc;
         fsc = (L.use_mag || dyn_ms) ? g_ms * mscn : g_msf;
         T d[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
         const T scale1 = (T)(-0.5 * L.gamma) * (T)a.dts;
@@ -1675,6 +1691,7 @@ struct HostLayer {
     std::vector<double> tc, lambda_ms;
     double spin = 0.5, msexp = -1.0, hkexp = -1.0;
     bool use_mag = false, tscale = false;
+    std::vector<double> tab_t[3], tab_s[3];   // READ_SCALE tables: 0 Ms, 1 alpha, 2 Hk
     // moving applied sources: slot 0 field table, slot 1 temperature (table or super-Gaussian)
     mrmg_source src[2] = {};
     std::vector<double> src_x[2], src_y[2], src_v[2];
@@ -1810,7 +1827,7 @@ struct Engine : EngineBase {
     DBuf<int> x_off, x_nb;
     DBuf<S> x_w;
     DBuf<S> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
-    DBuf<double> tc, lam, temp;
+    DBuf<double> tc, lam, temp, tabs;
     DBuf<LayerParams> lp;
     // state, ST_ROWS rows of G (mc: corrector output of the adaptive path until the step
     // is accepted)
@@ -2115,7 +2132,7 @@ struct Engine : EngineBase {
         std::vector<int> h_layer(G), h_gco(G + 1), h_gci(Ctot), h_c2g(nl * P, -1);
         std::vector<double> h_ms(G), h_msf(G), h_mss(G), h_as(G), h_hks(G), h_hk1(G), h_hk2(G), h_uni(3 * (size_t)G, 0.0),
             h_cub(9 * (size_t)G, 0.0), h_happ(3 * (size_t)G, 0.0), h_sig(G, 0.0), h_sigl(G, 0.0), h_vol(G);
-        std::vector<double> h_tc(G), h_lam(G), h_temp(G);
+        std::vector<double> h_tc(G), h_lam(G), h_temp(G), h_tabs;
         std::vector<double> h_m((size_t)ST_ROWS * G);
         has_happ = false;
         hlp.assign(nl, LayerParams{});
@@ -2170,6 +2187,12 @@ struct Engine : EngineBase {
             lpk.msexp = H.msexp;
             lpk.hkexp = H.hkexp;
             lpk.tscale = H.tscale;
+            for (int i = 0; i < 3; i++) {
+                lpk.tab_off[i] = (int)h_tabs.size();
+                lpk.tab_n[i] = (int)H.tab_t[i].size();
+                h_tabs.insert(h_tabs.end(), H.tab_t[i].begin(), H.tab_t[i].end());
+                h_tabs.insert(h_tabs.end(), H.tab_s[i].begin(), H.tab_s[i].end());
+            }
             lpk.has_uni = H.has_uni;
             lpk.has_cub = H.has_cub;
             lpk.thermal = H.thermal;
@@ -2224,6 +2247,8 @@ struct Engine : EngineBase {
         RCHK(tc.upload(h_tc.data(), G, c->stream));
         RCHK(lam.upload(h_lam.data(), G, c->stream));
         RCHK(temp.upload(h_temp.data(), G, c->stream));
+        h_tabs.push_back(0.0);   // never empty
+        RCHK(tabs.upload(h_tabs.data(), h_tabs.size(), c->stream));
         if (has_happ) RCHK(upT(happ, h_happ)); else happ.release();
         RCHK(upT(ma, h_m));
         mag_dirty = false;
@@ -2432,7 +2457,7 @@ struct Engine : EngineBase {
         d.x_off = x_off.p; d.x_nb = x_nb.p; d.x_w = x_w.p;
         d.nl = c->nl;
         d.lp = lp.p;
-        d.tc = tc.p; d.lam = lam.p; d.temp = temp.p;
+        d.tc = tc.p; d.lam = lam.p; d.temp = temp.p; d.tabs = tabs.p;
         if (cur_llb) d.sigma = sigma_llb.p;
         d.any_src = any_src;
         d.ax.xi = ax_xi.p; d.ax.xa = ax_xa.p; d.ax.yi = ax_yi.p; d.ax.ya = ax_ya.p; d.ax.yb = ax_yb.p;
@@ -3201,13 +3226,34 @@ int mrmg_set_layer(mrmg_ctx* c, int k, const mrmg_layer* in)
     H.tscale = in->tc_nom <= 1.e10;
     H.msexp = in->msexp;
     H.hkexp = in->hkexp;
+    {
+        const int nt[3] = {in->n_ms_tab, in->n_alpha_tab, in->n_hk_tab};
+        const double* tt[3] = {in->ms_tab_t, in->alpha_tab_t, in->hk_tab_t};
+        const double* ts[3] = {in->ms_tab_s, in->alpha_tab_s, in->hk_tab_s};
+        for (int i = 0; i < 3; i++) {
+            if (nt[i] == 0) continue;
+            if (nt[i] < 2 || !tt[i] || !ts[i]) return set_err(MRMG_ERR_ARG, "scaling table %d: need >= 2 nodes", i);
+            for (int j = 1; j < nt[i]; j++)
+                if (!(tt[i][j] > tt[i][j - 1])) return set_err(MRMG_ERR_ARG, "scaling table %d: T/Tc not increasing", i);
+            H.tab_t[i].assign(tt[i], tt[i] + nt[i]);
+            H.tab_s[i].assign(ts[i], ts[i] + nt[i]);
+        }
+    }
     if (in->tc_nom <= 1.e10) {
         H.use_mag = in->use_mag != 0;
         auto cl = [](double x) { return x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0; };
-        if (!H.use_mag && in->msexp >= 0.0)
-            for (int n = 0; n < ng; n++) H.ms_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->msexp);
-        if (in->hkexp >= 0.0)
-            for (int n = 0; n < ng; n++) H.hk_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->hkexp);
+        auto tab = [&](int i, int n) {
+            return table_eval(H.tab_t[i].data(), H.tab_s[i].data(), (int)H.tab_t[i].size(), H.temp[n] / H.tc[n]);
+        };
+        for (int n = 0; n < ng; n++) {
+            if (!H.use_mag) {
+                if (in->msexp >= 0.0) H.ms_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->msexp);
+                else if (!H.tab_t[0].empty()) H.ms_scale[n] = tab(0, n);
+            }
+            if (!H.tab_t[1].empty()) H.alpha_scale[n] = tab(1, n);
+            if (in->hkexp >= 0.0) H.hk_scale[n] = pow(cl(1.0 - H.temp[n] / H.tc[n]), in->hkexp);
+            else if (!H.tab_t[2].empty()) H.hk_scale[n] = tab(2, n);
+        }
     }
     // init_data_one: m = +z (data.f90:201-202), |m| = 1 (grain.f90:61)
     H.m.assign(3 * (size_t)ng, 0.0);

@@ -38,6 +38,8 @@ module mrmg_m
     integer(c_int) :: thermal, use_mag
     type(c_ptr) :: tc, lambda_ms
     real(c_double) :: spin, tc_nom, msexp, hkexp
+    integer(c_int) :: n_ms_tab, n_alpha_tab, n_hk_tab, pad3_     ! READ_SCALE tables
+    type(c_ptr) :: ms_tab_t, ms_tab_s, alpha_tab_t, alpha_tab_s, hk_tab_t, hk_tab_s
   end type
 
   type, bind(C) :: mrmg_ufield

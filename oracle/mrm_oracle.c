@@ -417,6 +417,8 @@ typedef struct {
     /* LLB and temperature scaling */
     double *tc, *lambda_ms, spin, tc_nom, msexp, hkexp;
     int use_mag;
+    int ntab[3];               /* READ_SCALE tables: 0 Ms, 1 alpha, 2 Hk */
+    double *tab_t[3], *tab_s[3];
     double *m;      /* (6,ng) double buffer, slot `in` */
     double *mag;    /* (2,ng) */
     double *h, *hthm, *dm, *h_app, *h_demag;
@@ -473,7 +475,8 @@ static void free_layer(olayer *L)
     void *ps[] = {L->g2gi, L->g2g, L->grid2grain, L->vol, L->ms, L->hk, L->uni, L->cub, L->hxi, L->hxg,
                   L->uhxi, L->uhxg, L->lhxi, L->lhxg, L->hx, L->uhx, L->lhx, L->ms_scale, L->alpha_scale,
                   L->hk_scale, L->temp, L->happ_s, L->m, L->mag, L->h, L->hthm, L->dm, L->h_app, L->h_demag,
-                  L->tc, L->lambda_ms, L->temp_s};
+                  L->tc, L->lambda_ms, L->temp_s, L->tab_t[0], L->tab_t[1], L->tab_t[2], L->tab_s[0], L->tab_s[1],
+                  L->tab_s[2]};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     memset(L, 0, sizeof(*L));
 }
@@ -502,19 +505,42 @@ static double *fill1(const double *p, int n)
 #define GMUB_KB 1.343427768e-4   /* g muB / kB (llg.f90:925) */
 
 /* find_temp_scaling (temp_scaling.f90:365-431) for the scale factors this path uses:
- * Ms from |m| (use_mag_p) or (1-T/Tc)^msexp, Hk from (1-T/Tc)^hkexp.  Tables are
- * per-grain arrays supplied by the caller (ms_scale, alpha_scale, hk_scale).  The
- * power laws depend on the static grain temperature only and are evaluated when the
- * layer is set (t_changed); the |m| form runs at every field evaluation. */
+ * Ms from |m| (use_mag_p), (1-T/Tc)^msexp or its READ_SCALE table, alpha from its table,
+ * Hk from (1-T/Tc)^hkexp or its table.  The power laws and tables follow the grain
+ * temperature (t_changed: at set time and whenever a temperature source moves); the |m|
+ * form runs at every field evaluation. */
 static double clamp01(double x) { return x < 1.0 ? (x > 0.0 ? x : 0.0) : 1.0; }
+/* the table branch of find_temp_scaling (temp_scaling.f90:397-416), 1-based bisection */
+static double tab_eval(const double *tt, const double *ss, int n, double x)
+{
+    int k1 = 1, k2 = n;
+    double y;
+    if (x <= tt[0]) {
+        y = 0.0;
+    } else if (x >= tt[k2 - 1]) {
+        y = 0.0; k1 = k2;
+    } else {
+        while (k2 - k1 > 1) {
+            int k = (k2 + k1) / 2;
+            if (x < tt[k - 1]) k2 = k; else k1 = k;
+        }
+        y = clamp01((x - tt[k1 - 1]) / (tt[k2 - 1] - tt[k1 - 1]));
+    }
+    return ss[k1 - 1] + y * (ss[k2 - 1] - ss[k1 - 1]);
+}
 static void temp_scaling(olayer *L, int t_changed)
 {
-    if (L->tc_nom > 1.e10) return;
-    if (!L->use_mag && t_changed && L->msexp >= 0.0) {
-        for (int n = 0; n < L->ng; n++) L->ms_scale[n] = pow(clamp01(1.0 - L->temp[n] / L->tc[n]), L->msexp);
+    if (L->tc_nom > 1.e10 || !t_changed) return;
+    for (int n = 0; n < L->ng; n++) {
+        const double x = L->temp[n] / L->tc[n];
+        if (!L->use_mag) {
+            if (L->msexp >= 0.0) L->ms_scale[n] = pow(clamp01(1.0 - x), L->msexp);
+            else if (L->ntab[0] > 1) L->ms_scale[n] = tab_eval(L->tab_t[0], L->tab_s[0], L->ntab[0], x);
+        }
+        if (L->ntab[1] > 1) L->alpha_scale[n] = tab_eval(L->tab_t[1], L->tab_s[1], L->ntab[1], x);
+        if (L->hkexp >= 0.0) L->hk_scale[n] = pow(clamp01(1.0 - x), L->hkexp);
+        else if (L->ntab[2] > 1) L->hk_scale[n] = tab_eval(L->tab_t[2], L->tab_s[2], L->ntab[2], x);
     }
-    if (t_changed && L->hkexp >= 0.0)
-        for (int n = 0; n < L->ng; n++) L->hk_scale[n] = pow(clamp01(1.0 - L->temp[n] / L->tc[n]), L->hkexp);
 }
 static void temp_scaling_mag(olayer *L, int in)
 {
@@ -571,6 +597,16 @@ int ora_set_layer(ora_sim *s, int k, const mrmg_layer *in)
     /* tc defaults to 1e22 (data.f90:201); lambdaMs = 3 Tc / (g muB/kB (S+1)) (grain.f90:61) */
     L->spin = in->spin; L->tc_nom = in->tc_nom; L->msexp = in->msexp; L->hkexp = in->hkexp;
     L->use_mag = in->use_mag;
+    {
+        const int nt[3] = {in->n_ms_tab, in->n_alpha_tab, in->n_hk_tab};
+        const double *tt[3] = {in->ms_tab_t, in->alpha_tab_t, in->hk_tab_t};
+        const double *ts[3] = {in->ms_tab_s, in->alpha_tab_s, in->hk_tab_s};
+        for (int i = 0; i < 3; i++) {
+            L->ntab[i] = (nt[i] > 0 && tt[i] && ts[i]) ? nt[i] : 0;
+            L->tab_t[i] = L->ntab[i] ? (double *)dupa(tt[i], sizeof(double) * nt[i]) : NULL;
+            L->tab_s[i] = L->ntab[i] ? (double *)dupa(ts[i], sizeof(double) * nt[i]) : NULL;
+        }
+    }
     L->tc = (double *)malloc(sizeof(double) * (ng ? ng : 1));
     L->lambda_ms = (double *)malloc(sizeof(double) * (ng ? ng : 1));
     for (int n = 0; n < ng; n++) {
@@ -651,6 +687,17 @@ static double ora_tscale(int func, double amp, double gamma, double phase, doubl
 double ora_funct_value(int func, double t, double gamm, double phase, double param, double duty, double moddepth)
 {
     return ora_funct(func, t, gamm, phase, param, duty, moddepth);
+}
+
+/* the per-grain scalings of a layer as the last find_temp_scaling left them (test access) */
+int ora_get_scales(ora_sim *s, int k, double *ms_scale, double *alpha_scale, double *hk_scale)
+{
+    if (k < 0 || k >= s->nl) return MRMG_ERR_ARG;
+    olayer *L = &s->L[k];
+    memcpy(ms_scale, L->ms_scale, sizeof(double) * L->ng);
+    memcpy(alpha_scale, L->alpha_scale, sizeof(double) * L->ng);
+    memcpy(hk_scale, L->hk_scale, sizeof(double) * L->ng);
+    return MRMG_OK;
 }
 
 int ora_set_ufield(ora_sim *s, int k, const mrmg_ufield *uf)

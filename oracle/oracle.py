@@ -65,6 +65,7 @@ def lib():
             "ora_lv": ([C.c_double], C.c_double),
             "ora_dlv": ([C.c_double], C.c_double),
             "ora_funct_value": ([C.c_int] + [C.c_double] * 6, C.c_double),
+            "ora_get_scales": ([_cp, C.c_int, _dp, _dp, _dp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -226,6 +227,12 @@ class Sim:
     def set_state(self, k, m):
         m = np.ascontiguousarray(m, dtype=np.float64)
         lib().ora_set_state(self.h, k, abi.dptr(m))
+
+    def get_scales(self, k):
+        """(ms_scale, alpha_scale, hk_scale) of layer k as find_temp_scaling last set them."""
+        a, b, c = (np.zeros(self.ngs[k]) for _ in range(3))
+        assert lib().ora_get_scales(self.h, k, abi.dptr(a), abi.dptr(b), abi.dptr(c)) == 0
+        return a, b, c
 
     def get_state(self, k):
         m = np.zeros((self.ngs[k], 3))

@@ -55,7 +55,7 @@ int main(void) {
   F(mrmg_geom, dx) F(mrmg_geom, dz) F(mrmg_geom, il_space)
   F(mrmg_layer, grid2grain) F(mrmg_layer, hx) F(mrmg_layer, lhx) F(mrmg_layer, h_app) F(mrmg_layer, alpha)
   F(mrmg_layer, floor_ms) F(mrmg_layer, thermal) F(mrmg_layer, use_mag) F(mrmg_layer, tc) F(mrmg_layer, lambda_ms)
-  F(mrmg_layer, spin) F(mrmg_layer, hkexp)
+  F(mrmg_layer, spin) F(mrmg_layer, hkexp) F(mrmg_layer, n_ms_tab) F(mrmg_layer, ms_tab_t) F(mrmg_layer, hk_tab_s)
   F(mrmg_ufield, t_start) F(mrmg_ufield, scale) F(mrmg_ufield, func) F(mrmg_ufield, amp) F(mrmg_ufield, dc)
   F(mrmg_source, scale) F(mrmg_source, func) F(mrmg_source, amp) F(mrmg_source, dc)
   printf("mrmg_source %zu\n", sizeof(mrmg_source));

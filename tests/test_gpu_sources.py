@@ -182,3 +182,41 @@ def test_time_function_argument_checks():
         x.set_ufield(0, abi.make_ufield((0, 0, 1), t_start=0.0, t_end=0.0, func=abi.MRMG_FN_SIN, amp=1.0, gamma=1e9))
     finally:
         x.close()
+
+
+def _scale_tables():
+    x = np.array([0.0, 0.3, 0.55, 0.8, 0.95, 1.1])
+    return {"ms": (x, np.array([1.0, 0.92, 0.75, 0.45, 0.2, 0.0])),
+            "alpha": (x, np.array([1.0, 1.1, 1.3, 1.8, 2.5, 3.0])),
+            "hk": (x[:4], np.array([1.0, 0.7, 0.4, 0.1]))}
+
+
+@pytest.mark.parametrize("ltype,moving", [(-3, True), (2, True), (-3, False)])
+def test_scale_tables_parity(oracle_mod, ltype, moving):
+    """READ_SCALE tables for Ms, alpha and Hk (temp_scaling.f90:397-416): at a static
+    temperature (set once) and under a moving spot (every evaluation: Hk and alpha at the
+    evaluation's temperature, the cells' Ms one stage ahead), thermal LLG."""
+    O = oracle_mod
+    med = llb_media(nx=48, ny=40, seed=71, tc_nom=700.0, temperature=300.0)
+    for L in med.layers:
+        L.tables = {k: (np.ascontiguousarray(v[0]), np.ascontiguousarray(v[1])) for k, v in _scale_tables().items()}
+        if not moving:
+            L.temp = np.random.default_rng(5).uniform(250.0, 800.0, L.ng)
+        else:
+            L.temp = np.full(L.ng, 300.0)
+    s, gpu = pair(O, med, ufields=[abi.make_ufield((0.1, 0.0, -1.0), scale=4000.0)] * med.n_lay)
+    if moving:
+        for k in range(med.n_lay):
+            both(s, gpu, k, 1, laser(med, dtemp=450.0))
+    for k, m in enumerate(random_state(med, 12, tilt=0.3)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    n = 40
+    noise = O.mt_noise([71, 72], [L.ng for L in med.layers], [True] * med.n_lay, n)
+    gpu.set_llg(dt=5e-14, llg_type=ltype)
+    s.step(n, 5e-14, type=ltype, noise=noise)
+    gpu.step(n, noise=noise)
+    d = max(np.abs(s.get_state(k)[0] - gpu.get_state(k)[0]).max() for k in range(med.n_lay))
+    dh = max(np.abs(s.get_state(k)[1] - gpu.get_state(k)[1]).max() for k in range(med.n_lay))
+    say(f"scale tables type {ltype} moving {moving}: |dm| {d:.2e} |dh| {dh:.2e}")
+    assert d <= 1e-10 and dh <= 1e-7

@@ -144,3 +144,54 @@ def test_temperature_source_scales_hk(oracle_mod):
     # the corrector's field is at the predicted m, a hair away from m: compare z parts
     want = hk * np.clip(1 - T / Tc, 0, 1) ** e * 0.8
     np.testing.assert_allclose(h[:, 2], want, rtol=1e-6)
+
+
+def _tables():
+    x = np.array([0.0, 0.3, 0.55, 0.8, 0.95, 1.1])
+    return {"ms": (x, np.array([1.0, 0.92, 0.75, 0.45, 0.2, 0.0])),
+            "alpha": (x, np.array([1.0, 1.1, 1.3, 1.8, 2.5, 3.0])),
+            "hk": (x[:4], np.array([1.0, 0.7, 0.4, 0.1]))}
+
+
+def test_scale_tables_static(oracle_mod):
+    """READ_SCALE tables (temp_scaling.f90:397-416) at a static temperature: each scaling is
+    the piecewise-linear table at T/Tc, held at the end values outside (numpy.interp)."""
+    O = oracle_mod
+    n = NX * NY
+    rng = np.random.default_rng(4)
+    T = rng.uniform(0.0, 900.0, n)
+    tc = np.full(n, 700.0)
+    tabs = _tables()
+    s, L, _ = _sim(O, temp=T, tc=tc, tc_nom=700.0, tables=tabs)
+    ms, al, hk = s.get_scales(0)
+    x = T / tc
+    np.testing.assert_allclose(ms, np.interp(x, *tabs["ms"]), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(al, np.interp(x, *tabs["alpha"]), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(hk, np.interp(x, *tabs["hk"]), rtol=1e-14, atol=1e-15)
+    # exact node hits and both ends
+    T2 = np.concatenate([tabs["ms"][0] * 700.0, [-50.0, 5000.0]])
+    T2 = np.resize(T2, n)
+    s2, _, _ = _sim(O, temp=T2, tc=tc, tc_nom=700.0, tables=tabs)
+    np.testing.assert_allclose(s2.get_scales(0)[0], np.interp(T2 / 700.0, *tabs["ms"]), rtol=1e-15, atol=0)
+
+
+def test_scale_tables_follow_moving_temperature(oracle_mod):
+    """Under a moving spot every field evaluation re-evaluates the tables at the grain
+    temperature; power laws take precedence for Ms and Hk when their exponent is >= 0."""
+    O = oracle_mod
+    n = NX * NY
+    tc = np.full(n, 700.0)
+    tabs = _tables()
+    s, L, _ = _sim(O, temp=np.full(n, 300.0), tc=tc, tc_nom=700.0, tables=tabs, hkexp=1.5)
+    spot = abi.SourceArrays(abi.MRMG_SRC_TEMP_SG, offset=(10e-7, 30e-7), vel=(3000.0, 0.0), sigma=(6e-7, 9e-7),
+                            expn=2, dtemp=500.0)
+    s.set_source(0, 1, spot)
+    s.set_state(0, np.tile([0.0, 0.0, 1.0], (n, 1)))
+    dt = 1e-15
+    s.step(1, dt)
+    T = 300.0 + s.source_cells(0, 1, dt)[0].ravel()     # temperature of the corrector's evaluation
+    ms, al, hk = s.get_scales(0)
+    np.testing.assert_allclose(ms, np.interp(T / 700.0, *tabs["ms"]), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(al, np.interp(T / 700.0, *tabs["alpha"]), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(hk, np.clip(1 - T / 700.0, 0, 1) ** 1.5, rtol=1e-13, atol=1e-15)
+    assert T.max() > 700.0 and T.min() < 400.0
