@@ -240,6 +240,15 @@ int mrmg_set_tensor(mrmg_ctx *ctx, const double *n_real);
  * images, plane_to_plane with num_images periodic images in x and y) and transform it.
  * When n_real_out is not NULL, the real-space tensor is also copied back to it. */
 int mrmg_build_tensor(mrmg_ctx *ctx, int num_images, double *n_real_out);
+/* As mrmg_build_tensor, with calculate_tensor's magnetic images (tensor.f90:103-200):
+ * sul_mu > 1 adds each source layer mirrored in the soft underlayer, space_to_sul below the
+ * bottom layer; head_mu > 1 adds its mirror in an infinite head, fly_height above the top
+ * layer.  An image is scaled by (mu-1)/(mu+1) with its x and y rows negated (so the tensor
+ * is no longer symmetric in its two indices).  num_images applies to both in-plane axes,
+ * as in the reference (plane_to_plane reads n_imgxi for y too, tensor_element.f90:230).
+ * mrmg_build_tensor is this with sul_mu = head_mu = 1. */
+int mrmg_build_tensor_images(mrmg_ctx *ctx, int num_images, double sul_mu, double head_mu, double fly_height,
+                             double space_to_sul, double *n_real_out);
 
 int mrmg_set_layer(mrmg_ctx *ctx, int k, const mrmg_layer *layer);
 int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);

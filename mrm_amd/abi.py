@@ -242,6 +242,7 @@ def load(path=None):
         ("mrmg_destroy", [C.c_void_p]),
         ("mrmg_set_tensor", [C.c_void_p, _dp]),
         ("mrmg_build_tensor", [C.c_void_p, C.c_int, _dp]),
+        ("mrmg_build_tensor_images", [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         ("mrmg_set_layer", [C.c_void_p, C.c_int, C.POINTER(Layer)]),
         ("mrmg_set_ufield", [C.c_void_p, C.c_int, C.POINTER(UField)]),
         ("mrmg_set_source", [C.c_void_p, C.c_int, C.c_int, C.POINTER(Source)]),

@@ -1513,8 +1513,12 @@ __device__ void van(double rx, double ry, double rz, double xs, double ys, doubl
 // plane_to_plane for one layer pair: one thread per output cell; the offsets (i,j) that
 // alias onto it (i = +-nx/2, j = +-ny/2 for even sizes) are summed in the reference's
 // j-outer, i-inner order (tensor_element.f90:233-259).  Writes the normalised block.
+// mode 0: blk = t / norm (a direct term alone); mode 1: blk = t (a direct term that images
+// follow); mode 2: blk += f t with f = fxy for the x and y rows (a = c % 3 < 2), fz for z
+// (an image term, tensor.f90:163-181).
 __global__ void k_plane_to_plane(int nx, int ny, double dx, double dy, double dzs, double dzo, double rz,
-                                 int nimg, double norm, double* __restrict__ blk)
+                                 int nimg, double norm, double* __restrict__ blk, int mode = 0, double fxy = 0.0,
+                                 double fz = 0.0)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nx * ny) return;
@@ -1542,8 +1546,23 @@ __global__ void k_plane_to_plane(int nx, int ny, double dx, double dy, double dz
         }
     }
     const size_t plane = (size_t)nx * ny;
+    if (mode == 0) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) blk[p + plane * c] = acc[c] / norm;
+        for (int c = 0; c < 9; c++) blk[p + plane * c] = acc[c] / norm;
+    } else if (mode == 1) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) blk[p + plane * c] = acc[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) blk[p + plane * c] = blk[p + plane * c] + ((c % 3) < 2 ? fxy : fz) * acc[c];
+    }
+}
+
+// the normalisation of a layer-pair block with images, n2d / (dx dy dz_ko nx ny)
+__global__ void k_tensor_div(double* __restrict__ blk, size_t n, double norm)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) blk[i] = blk[i] / norm;
 }
 
 // Tensor FFT in FP64, rows: one workgroup per (row pair, plane) of P real planes.
@@ -3107,14 +3126,29 @@ int mrmg_set_tensor(mrmg_ctx* c, const double* n_real)
 
 int mrmg_build_tensor(mrmg_ctx* c, int num_images, double* n_real_out)
 {
+    return mrmg_build_tensor_images(c, num_images, 1.0, 1.0, 8.e-7, 40.e-7, n_real_out);
+}
+
+int mrmg_build_tensor_images(mrmg_ctx* c, int num_images, double sul_mu, double head_mu, double fly_height,
+                             double space_to_sul, double* n_real_out)
+{
     if (!c) return set_err(MRMG_ERR_ARG, "null argument");
     if (num_images < 0) return set_err(MRMG_ERR_ARG, "num_images < 0");
     const int nx = c->nx, ny = c->ny, nl = c->nl;
     const size_t plane = (size_t)nx * ny, count = plane * 9 * nl * nl;
     DBuf<double> d;
     RCHK(d.alloc(count));
-    // calculate_tensor layer loop (tensor.f90:148-200), space_to_sul = 40e-7 cancels in rz
-    const double space_to_sul = 40.e-7;
+    // image strengths (mu-1)/(mu+1) of the soft underlayer and the head (tensor.f90:125-126)
+    const bool sul = sul_mu > 1.0, head = head_mu > 1.0;
+    const double mus = sul ? (sul_mu - 1.0) / (sul_mu + 1.0) : 0.0;
+    const double mush = head ? (head_mu - 1.0) / (head_mu + 1.0) : 0.0;
+    // distance from the head plane to the centre of each layer (tensor.f90:141-147)
+    std::vector<double> hdz(nl);
+    hdz[0] = fly_height;
+    for (int ks = 1; ks < nl; ks++) hdz[ks] = hdz[ks - 1] + c->dz[ks - 1] + c->il[ks];
+    for (int ks = 0; ks < nl; ks++) hdz[ks] = hdz[ks] + 0.5 * c->dz[ks];
+    // calculate_tensor layer loop (tensor.f90:148-200); space_to_sul cancels in the direct rz
+    const dim3 gp((unsigned)((plane + 127) / 128)), bp(128);
     double zsb = space_to_sul;
     for (int ks = nl - 1; ks >= 0; ks--) {
         const double zst = zsb + c->dz[ks];
@@ -3124,9 +3158,24 @@ int mrmg_build_tensor(mrmg_ctx* c, int num_images, double* n_real_out)
             const double rz = 0.5 * (zst - zot + zsb - zob);
             const double norm = c->dx * c->dy * c->dz[ko] * nx * ny;
             double* blk = d.p + plane * 9 * ((size_t)ko + (size_t)nl * ks);
-            hipLaunchKernelGGL(k_plane_to_plane, dim3((unsigned)((plane + 127) / 128)), dim3(128), 0, c->stream, nx, ny,
-                               c->dx, c->dy, c->dz[ks], c->dz[ko], rz, num_images, norm, blk);
+            hipLaunchKernelGGL(k_plane_to_plane, gp, bp, 0, c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko], rz,
+                               num_images, norm, blk, (sul || head) ? 1 : 0, 0.0, 0.0);
             HIPCHK(hipGetLastError());
+            if (sul) {
+                hipLaunchKernelGGL(k_plane_to_plane, gp, bp, 0, c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko],
+                                   0.5 * (-zst - zot - zsb - zob), num_images, norm, blk, 2, -mus, mus);
+                HIPCHK(hipGetLastError());
+            }
+            if (head) {
+                hipLaunchKernelGGL(k_plane_to_plane, gp, bp, 0, c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko],
+                                   hdz[ks] + hdz[ko], num_images, norm, blk, 2, -mush, mush);
+                HIPCHK(hipGetLastError());
+            }
+            if (sul || head) {
+                hipLaunchKernelGGL(k_tensor_div, dim3((unsigned)((plane * 9 + 255) / 256)), dim3(256), 0, c->stream,
+                                   blk, plane * 9, norm);
+                HIPCHK(hipGetLastError());
+            }
             zob = zot + c->il[ko];
         }
         zsb = zst + c->il[ks];

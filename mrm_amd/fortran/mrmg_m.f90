@@ -12,7 +12,8 @@ module mrmg_m
   implicit none
   private
   public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag
-  public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_set_layer, &
+  public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_build_tensor_images, &
+            mrmg_set_layer, &
             mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error
   public :: MRMG_FP64, MRMG_FP32, MRMG_NOISE_NONE, MRMG_NOISE_INJECT, MRMG_NOISE_PHILOX, MRMG_TYPE_HEUN
 
@@ -89,6 +90,14 @@ module mrmg_m
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: num_images
+      type(c_ptr), value :: n_real_out
+    end function
+    integer(c_int) function mrmg_build_tensor_images(ctx, num_images, sul_mu, head_mu, fly_height, space_to_sul, &
+                                                     n_real_out) bind(C, name='mrmg_build_tensor_images')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: num_images
+      real(c_double), value :: sul_mu, head_mu, fly_height, space_to_sul
       type(c_ptr), value :: n_real_out
     end function
     integer(c_int) function mrmg_set_layer(ctx, k, layer) bind(C, name='mrmg_set_layer')

@@ -52,11 +52,18 @@ class Llg:
         n_real = np.ascontiguousarray(n_real, dtype=np.float64)
         abi.check(self.lib, self.lib.mrmg_set_tensor(self.ctx, abi.dptr(n_real)))
 
-    def build_tensor(self, num_images=2, return_real=False):
+    def build_tensor(self, num_images=2, return_real=False, sul_mu=1.0, head_mu=1.0, fly_height=8e-7,
+                     space_to_sul=40e-7):
+        """calculate_tensor on the device (tensor.f90:82-216), with the SUL / head images
+        when sul_mu / head_mu > 1."""
         out = None
         if return_real:
             out = np.zeros(self.nx * self.ny * 9 * self.n_lay * self.n_lay)
-        abi.check(self.lib, self.lib.mrmg_build_tensor(self.ctx, num_images, abi.dptr(out)))
+        if sul_mu > 1.0 or head_mu > 1.0:
+            abi.check(self.lib, self.lib.mrmg_build_tensor_images(self.ctx, num_images, sul_mu, head_mu, fly_height,
+                                                                  space_to_sul, abi.dptr(out)))
+        else:
+            abi.check(self.lib, self.lib.mrmg_build_tensor(self.ctx, num_images, abi.dptr(out)))
         return out
 
     def set_layer(self, k, la):

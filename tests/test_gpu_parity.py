@@ -125,6 +125,34 @@ def test_tensor_build_parity(oracle_mod):
         assert np.array_equal(T[:, 3 * a + b], T[:, 3 * b + a])
 
 
+@pytest.mark.parametrize("sul_mu,head_mu", [(9.0, 1.0), (1.0, 50.0)])
+def test_tensor_images_parity(oracle_mod, sul_mu, head_mu):
+    """The device tensor with the SUL or head image (tensor.f90:157-181) against the oracle,
+    to the same bound as the direct tensor; the image makes xz != zx, so the library keeps
+    9 slots and the demag field still matches."""
+    say(f"tensor images sul_mu {sul_mu} head_mu {head_mu}")
+    nx, ny, nl = 32, 24, 2
+    dz = [2e-7, 3e-7]
+    il = [0.0, 1e-7]
+    gpu = Llg(nx, ny, nl, 2e-7, 2e-7, dz, il_space=il)
+    got = gpu.build_tensor(num_images=1, return_real=True, sul_mu=sul_mu, head_mu=head_mu, fly_height=6e-7,
+                           space_to_sul=30e-7)
+    ref = oracle_mod.calculate_tensor(nx, ny, nl, 2e-7, 2e-7, dz, il_space=il, num_images=1, sul_mu=sul_mu,
+                                      head_mu=head_mu, fly_height=6e-7, space_to_sul=30e-7)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-8 * scale
+    T = ref.reshape(nl * nl, 9, ny * nx)
+    assert np.abs(T[:, 2] - T[:, 6]).max() > 1e-6 * scale     # xz != zx with an image
+    med = media.make_media(nx, ny, dz=dz, n_lay=nl, pitch=6e-7, seed=77)
+    s, g2 = pair(oracle_mod, med, tensor=ref, il=il)
+    for k, m in enumerate(random_state(med, 3)):
+        s.set_state(k, m)
+        g2.set_state(k, m)
+    hc0, hg0 = s.demag_field()
+    hc1, hg1 = g2.demag_field()
+    assert np.abs(hc0 - hc1).max() <= 1e-12 * np.abs(hc0).max()
+
+
 def test_heun_step_parity_cfg0(oracle_mod):
     """Configuration 0 media, demag on, ramped tilted field, T = 0: Heun FP64 matches the
     oracle to 1e-10 after one step and after 100."""

@@ -479,3 +479,56 @@ def test_loop_statistics_closed_form(oracle_mod):
     np.testing.assert_allclose(d.mvar_all, (Ws[:, None] * np.array(vars_)).sum(0) / Ws.sum(), rtol=1e-9, atol=1e-15)
     np.testing.assert_allclose(d.have_all, (Ws[:, None] * np.array(hs)).sum(0) / Ws.sum(), rtol=1e-12, atol=1e-9)
     assert np.all(np.array(d.mvar[0]) > 1e-3)     # a spread state: the variance is not trivially zero
+
+
+def _flip_rows(blk, f):
+    """blk: Fortran (nx,ny,9) block as (9, ny, nx); the x and y rows (c % 3 < 2) times -f,
+    the z rows times f."""
+    out = blk.copy()
+    for c in range(9):
+        out[c] *= (-f if c % 3 < 2 else f)
+    return out
+
+
+@pytest.mark.parametrize("which", ["sul", "head"])
+def test_tensor_images_are_mirror_layers(oracle_mod, which):
+    """calculate_tensor's soft-underlayer and head images (tensor.f90:157-181): the image
+    term of a layer pair equals the direct term between the observation layer and the
+    source layer mirrored in the SUL (or head) plane, built as an extra layer of a longer
+    stack, with its x and y rows scaled by -(mu-1)/(mu+1) and its z rows by +(mu-1)/(mu+1)."""
+    O = oracle_mod
+    nx, ny, nl = 16, 12, 2
+    dx = dy = 2e-7
+    dz = [2e-7, 3e-7]
+    il = [0.0, 1e-7]           # il[k]: gap above layer k (layer 0 on top)
+    sts, fh, mu = 40e-7, 8e-7, 9.0
+    f = (mu - 1) / (mu + 1)
+    kw = dict(sul_mu=mu, head_mu=1.0) if which == "sul" else dict(sul_mu=1.0, head_mu=mu)
+    shape = (nl, nl, 9, ny, nx)
+    with_img = O.calculate_tensor(nx, ny, nl, dx, dy, dz, il, num_images=1, fly_height=fh, space_to_sul=sts,
+                                  **kw).reshape(shape)
+    direct = O.calculate_tensor(nx, ny, nl, dx, dy, dz, il, num_images=1).reshape(shape)
+    image = with_img - direct
+    if which == "sul":       # stack: real 0..nl-1 (top to bottom), then the mirrors of nl-1 .. 0
+        dz_e = dz + dz[::-1]
+        il_e = il + [2 * sts] + [il[nl - j] for j in range(1, nl)]
+        mirror = {ks: nl + (nl - 1 - ks) for ks in range(nl)}
+        real = {k: k for k in range(nl)}
+    else:                    # stack: the mirrors of nl-1 .. 0 above the head plane, then real 0..nl-1
+        dz_e = dz[::-1] + dz
+        il_e = [0.0] + [il[nl - j] for j in range(1, nl)] + [2 * fh] + il[1:]
+        mirror = {ks: nl - 1 - ks for ks in range(nl)}
+        real = {k: nl + k for k in range(nl)}
+    ne = 2 * nl
+    ext = O.calculate_tensor(nx, ny, ne, dx, dy, dz_e, il_e, num_images=1).reshape((ne, ne, 9, ny, nx))
+    scale = np.abs(direct).max()
+    for ko in range(nl):
+        for ks in range(nl):
+            # Fortran (nx,ny,9,nl,nl) flat = [ks][ko][c][y][x]
+            want = _flip_rows(ext[mirror[ks], real[ko]], f)
+            got = image[ks, ko]
+            # the far-field Newell sums cancel catastrophically: the two routes round rz
+            # differently, which moves the result by ~1e-9 of the tensor's largest value;
+            # a misplaced mirror (one layer thickness) or a wrong flip moves it by >1e-6
+            assert np.abs(got - want).max() <= 1e-8 * scale, (which, ko, ks)
+    assert np.abs(image).max() > 1e-5 * scale
