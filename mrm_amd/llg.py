@@ -163,7 +163,9 @@ class Llg:
             self.min_it = max(1, int(min_it))
 
     def steps_for(self, t):
-        """Steps an `LLG` command takes from time t with fixed dt (llg.f90:268-281)."""
+        """Steps an `LLG` command takes from time t with fixed dt (llg.f90:268-281).  The
+        convergence clause of cont_p is always true in the reference (get_applied returns
+        .true., applied.f90:1033), so the time/step rule alone ends the loop."""
         it, del_t = 0, 0.0
         while True:
             t = t + self.dt
