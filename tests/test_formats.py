@@ -1,0 +1,205 @@
+"""The reference's binary files (grain microstructure, tensor cache, magnetisation):
+record framing checked against scipy.io.FortranFile and against programs built with the
+Fortran compiler of this image (flang), and round trips of every format, including a
+simulation from a read-back microstructure that matches the original bit for bit."""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from mrm_amd import abi, formats, media
+
+
+def test_records_match_scipy(tmp_path):
+    from scipy.io import FortranFile
+    p = tmp_path / "a.bin"
+    a = np.arange(7, dtype=np.int32)
+    b = np.linspace(0, 1, 5)
+    with open(p, "wb") as f:
+        w = formats.RecordWriter(f)
+        w.write(a, b)
+        w.write(np.array([3.5]))
+    ff = FortranFile(p, "r", header_dtype=np.uint32)
+    rec = ff.read_record(np.uint8)
+    assert np.array_equal(np.frombuffer(rec[:28], "<i4"), a)
+    assert np.array_equal(np.frombuffer(rec[28:], "<f8"), b)
+    assert ff.read_reals(np.float64)[0] == 3.5
+    ff.close()
+    q = tmp_path / "b.bin"
+    ff = FortranFile(q, "w", header_dtype=np.uint32)
+    ff.write_record(np.array([1, 2, 3], dtype=np.int32), np.array([2.25, -1.0]))
+    ff.close()
+    with open(q, "rb") as f:
+        i, d = formats.RecordReader(f).read(("i", 3), ("d", 2))
+    assert list(i) == [1, 2, 3] and list(d) == [2.25, -1.0]
+
+
+def test_subrecords(tmp_path, monkeypatch):
+    """Records beyond the largest subrecord: every subrecord but the last carries a negative
+    length, and the reader joins them."""
+    monkeypatch.setattr(formats, "_MAXSUB", 24)
+    p = tmp_path / "s.bin"
+    data = np.arange(10, dtype=np.float64)        # 80 bytes -> 24 + 24 + 24 + 8
+    with open(p, "wb") as f:
+        formats.RecordWriter(f).write(data)
+    raw = p.read_bytes()
+    marks = []
+    pos = 0
+    while pos < len(raw):
+        n = int(np.frombuffer(raw[pos:pos + 4], "<i4")[0])
+        marks.append(n)
+        pos += 8 + abs(n)
+    assert marks == [-24, -24, -24, 8]
+    with open(p, "rb") as f:
+        assert np.array_equal(formats.RecordReader(f).read(("d", 10))[0], data)
+
+
+FLANG = shutil.which("flang") or ("/opt/rocm/lib/llvm/bin/flang" if os.path.exists("/opt/rocm/lib/llvm/bin/flang") else None)
+
+WRITER = """
+program w
+  implicit none
+  integer :: i(3) = (/ 4, -5, 6 /)
+  real(8) :: d(2) = (/ 1.5d0, -2.25d0 /)
+  logical :: t = .true., f = .false.
+  open(10, file='from_fortran.bin', form='unformatted', status='replace')
+  write(10) t, f, i, d
+  write(10) 7, 8.5d0
+  close(10)
+end program
+"""
+
+READER = """
+program r
+  implicit none
+  logical :: a, b
+  integer :: n, k(4)
+  real(8) :: x, y(3)
+  open(10, file='from_python.bin', form='unformatted', status='old')
+  read(10) a, b, n, x
+  read(10) k, y
+  close(10)
+  print '(l1,1x,l1,1x,i0,1x,f8.3)', a, b, n, x
+  print '(4(i0,1x),3(f8.3,1x))', k, y
+end program
+"""
+
+
+@pytest.mark.skipif(FLANG is None, reason="no Fortran compiler in this image")
+def test_records_against_fortran_compiler():
+    with tempfile.TemporaryDirectory() as d:
+        for name, src in (("w", WRITER), ("r", READER)):
+            open(os.path.join(d, f"{name}.f90"), "w").write(src)
+            subprocess.run([FLANG, "-o", os.path.join(d, name), os.path.join(d, f"{name}.f90")], check=True,
+                           capture_output=True)
+        subprocess.run([os.path.join(d, "w")], cwd=d, check=True)
+        with open(os.path.join(d, "from_fortran.bin"), "rb") as f:
+            r = formats.RecordReader(f)
+            t, fl, i, dd = r.read(("l", 1), ("l", 1), ("i", 3), ("d", 2))
+            n, x = r.read(("i", 1), ("d", 1))
+        assert t and not fl and list(i) == [4, -5, 6] and list(dd) == [1.5, -2.25] and n == 7 and x == 8.5
+        with open(os.path.join(d, "from_python.bin"), "wb") as f:
+            w = formats.RecordWriter(f)
+            w.write(True, False, 42, 3.25)
+            w.write(np.array([1, 2, 3, 4], np.int32), np.array([0.5, 1.5, 2.5]))
+        out = subprocess.run([os.path.join(d, "r")], cwd=d, check=True, capture_output=True, text=True).stdout.split()
+        assert out[:4] == ["T", "F", "42", "3.250"]
+        assert out[4:8] == ["1", "2", "3", "4"] and [float(v) for v in out[8:]] == [0.5, 1.5, 2.5]
+
+
+def _media_dicts(med):
+    out = []
+    for k, L in enumerate(med.layers):
+        ng = L.ng
+        d = dict(nx=med.nx, ny=med.ny, il=k + 1, dx=med.dx, dy=med.dy,
+                 grid2grain=L.grid2grain.reshape(med.ny, med.nx), grain2grid=L.grain2grid.reshape(-1, 2),
+                 grain2gridindex=L.grain2gridindex, grain_area=L.grain_vol / med.dz[k], ms=L.ms,
+                 tc=np.full(ng, 700.0), tau=np.linspace(0, 1, ng), hk=L.hk.reshape(ng, 2),
+                 uni_axis=None if L.uni_axis is None else L.uni_axis.reshape(ng, 3), cubic_axis=None)
+        for key, pre in (("hx", ""), ("uhx", "u"), ("lhx", "l")):
+            idx = getattr(L, f"{pre}hxgrainindex")
+            if idx is not None:
+                d[key] = (idx, getattr(L, f"{pre}hxgrain"), getattr(L, f"{pre}hx"))
+        out.append(d)
+    return out
+
+
+def test_grain_file_round_trip(tmp_path, oracle_mod):
+    """WRITE_GRAIN then READ_GRAIN of a two-physical-layer media (intralayer and
+    interlayer exchange CSRs, uniaxial axes): every array comes back, and a simulation
+    from the read-back media equals one from the original bit for bit."""
+    med = media.config_media("cfg1", nx=32, ny=32)
+    p = tmp_path / "g.bin"
+    src = _media_dicts(med)
+    src[0]["cubic_axis"] = np.tile(np.eye(3), (med.layers[0].ng, 1, 1))   # a cubic record too
+    formats.write_grain(p, src)
+    back = formats.read_grain(p, med.n_lay)
+    for a, b in zip(src, back):
+        for key in ("nx", "ny", "il", "dx", "dy"):
+            assert a[key] == b[key]
+        for key in ("grid2grain", "grain2grid", "grain2gridindex", "grain_area", "ms", "tc", "tau", "hk",
+                    "uni_axis", "cubic_axis"):
+            if a[key] is None:
+                assert b[key] is None
+            else:
+                assert np.array_equal(np.asarray(a[key]).reshape(np.shape(b[key])), b[key]), key
+        for key in ("hx", "uhx", "lhx"):
+            assert (a.get(key) is None) == (b[key] is None)
+            if a.get(key) is not None:
+                for x, y in zip(a[key], b[key]):
+                    assert np.array_equal(x, y)
+    # the media as the file holds it (no cubic axes) drives the same trajectory
+    formats.write_grain(p, _media_dicts(med))
+    back = formats.read_grain(p, med.n_lay)
+    O = oracle_mod
+    g = abi.make_geom(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, demag=False)
+    runs = []
+    phys = [dict(alpha=L.alpha, gamma=L.gamma, thm_alpha=L.thm_alpha, floor_ms=L.floor_ms, thermal=L.thermal,
+                 temp=L.temp) for L in med.layers]
+    for layers in (med.layers, [formats.layer_arrays(b, med.dz[k], **phys[k]) for k, b in enumerate(back)]):
+        s = O.Sim(g)
+        for k, L in enumerate(layers):
+            s.set_layer(k, L)
+        for k, u in enumerate(media.ramp_field(med.n_lay)):
+            s.set_ufield(k, u)
+        for k in range(med.n_lay):
+            s.set_state(k, np.tile([0.1, 0.0, 0.995], (med.layers[k].ng, 1)))
+        s.step(20, 5e-14)
+        runs.append([s.get_state(k)[0] for k in range(med.n_lay)])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_tensor_cache_round_trip(tmp_path, oracle_mod):
+    nx, ny, nl = 16, 12, 2
+    dz, il = [2e-7, 3e-7], [0.0, 1e-7]
+    N = oracle_mod.calculate_tensor(nx, ny, nl, 2e-7, 2e-7, dz, il, num_images=1)
+    p = tmp_path / "mrm_N.dat"
+    formats.write_tensor_cache(p, N, nx, ny, nl, 2e-7, 2e-7, dz, il, num_images=1)
+    got = formats.read_tensor_cache(p, nx, ny, nl, 2e-7, 2e-7, dz, il, num_images=1)
+    assert np.array_equal(got, N)
+    # any differing header value means recompute, as in the reference
+    assert formats.read_tensor_cache(p, nx, ny, nl, 2e-7, 2e-7, dz, il, num_images=2) is None
+    assert formats.read_tensor_cache(p, nx, ny, nl, 2e-7, 2e-7, [2e-7, 3.1e-7], il, num_images=1) is None
+    assert formats.read_tensor_cache(p, nx, ny, nl, 2e-7, 2e-7, dz, il, num_images=1, sul_mu=5.0) is None
+    raw = p.read_bytes()
+    assert int(np.frombuffer(raw[:4], "<i4")[0]) == 4 * 7 + 8 * 6 + 8 * 2 * nl
+
+
+def test_mag_file_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    layers = []
+    for n in (17, 9):
+        m = rng.standard_normal((n, 3))
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        layers.append((8, 6, 700.0, m, rng.uniform(300, 800, n)))
+    p = tmp_path / "m.bin"
+    formats.write_mag(p, layers)
+    back = formats.read_mag(p, 2)
+    for (nx, ny, msn, m, ms), (bx, by, bmsn, bm, bmag) in zip(layers, back):
+        assert (nx, ny, msn) == (bx, by, bmsn)
+        np.testing.assert_allclose(bm, m, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(bmag, ms, rtol=1e-15)
