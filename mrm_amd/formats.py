@@ -16,7 +16,12 @@ any nonzero read as true), REAL(8) is 8 bytes, arrays are in Fortran order.
   geometry and image options, then n2d (nx,ny,9,n_lay,n_lay).
 * The binary magnetisation of WRITE_MAG (mag.f90:285-290): per layer (nx, ny, n, ms_nom)
   and Ms * m as (3,n).
+* The checkpoint trio (checkpoint.f90:14-149): mrm_grain.chk (the grain microstructure),
+  mrm_0.chk / mrm_1.chk written in turn ((time, iline), the magnetisation, the random
+  generators of save_random, random.f90:38-64), and the formatted mrm.chk naming the
+  slot last written.
 """
+import os
 import struct
 
 import numpy as np
@@ -226,15 +231,32 @@ def read_tensor_cache(path, nx, ny, n_lay, dx, dy, dz, il_space, num_images=2, h
 
 
 # ------------------------------------------------------------------ magnetisation
+def _write_mag_records(w, layers):
+    for nx, ny, ms_nom, m, ms in layers:
+        m = np.asarray(m, np.float64).reshape(-1, 3)
+        w.write(int(nx), int(ny), len(m), float(ms_nom))
+        w.write(m * np.asarray(ms, np.float64)[:, None])
+
+
+def _read_mag_records(r, n_lay):
+    out = []
+    for _ in range(n_lay):
+        hd = r.read_bytes()
+        if len(hd) != 20:
+            raise ValueError("magnetisation header record of unexpected length")
+        nx, ny, n = struct.unpack_from("<3i", hd, 0)
+        ms_nom = struct.unpack_from("<d", hd, 12)[0]
+        mm = r.read(("d", 3 * n))[0].reshape(n, 3)
+        mag = np.sqrt((mm * mm).sum(1))
+        out.append((nx, ny, ms_nom, mm / np.where(mag > 0, mag, 1.0)[:, None], mag))
+    return out
+
+
 def write_mag(path, layers):
     """WRITE_MAG binary form (mag.f90:285-290): layers is a list of (nx, ny, ms_nom, m
     (ng,3), ms (ng)); each writes (nx, ny, ng, ms_nom) and Ms * m as (3,ng)."""
     with open(path, "wb") as f:
-        w = RecordWriter(f)
-        for nx, ny, ms_nom, m, ms in layers:
-            m = np.asarray(m, np.float64).reshape(-1, 3)
-            w.write(int(nx), int(ny), len(m), float(ms_nom))
-            w.write(m * np.asarray(ms, np.float64)[:, None])
+        _write_mag_records(RecordWriter(f), layers)
 
 
 def read_mag(path, n_lay):
@@ -242,14 +264,169 @@ def read_mag(path, n_lay):
     |Ms m| (ng)).  The reference's reader normalises m the same way; its Ms update writes
     ms(n) with n the grain count (mag.f90:246), i.e. only the last grain, which is not
     reproduced here: the per-grain magnitude is returned instead."""
+    with open(path, "rb") as f:
+        return _read_mag_records(RecordReader(f), n_lay)
+
+
+# ------------------------------------------------------------------ random generators
+MT_N = 624                 # MT19937 state words (random.f90:10)
+NAME_LEN = 40              # random_s%name, character(40)
+PHILOX_NAME = "philox4x32-10"
+
+
+def philox_generator(seed, counter):
+    """The device's Philox4x32-10 stream as a save_random entry: the 64-bit key in mt(0:1)
+    and the 64-bit step counter in mt(2:3), low word first; mti = 625 marks it as no
+    MT19937 state.  A reference run reading it finds no generator of that name and skips
+    it, as restore_random does for any unknown name (random.f90:100-103)."""
+    mt = np.zeros(MT_N, np.uint32)
+    mt[0], mt[1] = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+    mt[2], mt[3] = counter & 0xFFFFFFFF, (counter >> 32) & 0xFFFFFFFF
+    return dict(name=PHILOX_NAME, mti=MT_N + 1, mt=mt.view(np.int32), iset=0, gset=0.0)
+
+
+def philox_state(gen):
+    """(seed, counter) of a philox_generator entry."""
+    mt = np.asarray(gen["mt"], np.int32).view(np.uint32).astype(np.uint64)
+    return int(mt[0] | (mt[1] << np.uint64(32))), int(mt[2] | (mt[3] << np.uint64(32)))
+
+
+def _write_generators(w, gens):
+    # save_random (random.f90:38-64): the count, then one record per generator
+    w.write(len(gens))
+    for g in gens:
+        mt = np.asarray(g["mt"]).astype(np.int64)
+        if mt.shape != (MT_N,):
+            raise ValueError(f"generator state must hold {MT_N} words")
+        name = g["name"].encode("ascii")
+        if len(name) > NAME_LEN:
+            raise ValueError(f"generator name longer than {NAME_LEN} characters")
+        w.write_bytes(struct.pack("<i", int(g["mti"])) + (mt & 0xFFFFFFFF).astype("<u4").tobytes()
+                      + struct.pack("<id", int(g["iset"]), float(g["gset"])) + name.ljust(NAME_LEN))
+
+
+def _read_generators(r):
+    # restore_random (random.f90:66-107)
+    n = r.read(("i", 1))[0]
     out = []
+    for _ in range(n):
+        data = r.read_bytes()
+        if len(data) != 4 + 4 * MT_N + 12 + NAME_LEN:
+            raise ValueError("random generator record of unexpected length")
+        mti = struct.unpack_from("<i", data, 0)[0]
+        mt = np.frombuffer(data, "<i4", MT_N, 4).copy()
+        iset, gset = struct.unpack_from("<id", data, 4 + 4 * MT_N)
+        name = data[4 + 4 * MT_N + 12:].decode("ascii").strip()
+        out.append(dict(name=name, mti=mti, mt=mt, iset=iset, gset=gset))
+    return out
+
+
+# ------------------------------------------------------------------ checkpoints
+CHK_FILES = ("mrm.chk", "mrm_grain.chk", "mrm_0.chk", "mrm_1.chk")   # checkpoint.f90:8
+_EXACT_TAG = b"mrmgexct"
+
+
+def _last_slot(directory):
+    p = os.path.join(directory, CHK_FILES[0])
+    if not os.path.exists(p):
+        return 1                                   # idd starts at 1 (checkpoint.f90:10)
+    try:
+        return int(open(p).read().split()[2])
+    except (IndexError, ValueError):
+        return 1
+
+
+def write_checkpoint(directory, time, iline, mag_layers, generators, grain_layers=None, exact=None):
+    """One checkpoint save (checkpoint.f90:106-145).  grain_layers (write_grain's dicts)
+    writes mrm_grain.chk, which the reference saves once per run.  The slot after the
+    one mrm.chk names gets (time, iline), the magnetisation of mag_layers (write_mag's
+    tuples) and the generators; only then is mrm.chk rewritten, so a save cut short
+    leaves the previous slot valid.  exact: optional dict(layers=[(m (ng,3), |m| (ng))
+    per layer], dt=the integrator's current step) kept bit for bit in trailing records,
+    which the reference never reads (it closes the file after the generators); the
+    magnetisation records alone renormalise m and carry no step size.  Returns the slot
+    written (0 or 1)."""
+    if grain_layers is not None:
+        write_grain(os.path.join(directory, CHK_FILES[1]), grain_layers)
+    idd = 1 - _last_slot(directory)
+    with open(os.path.join(directory, CHK_FILES[2 + idd]), "wb") as f:
+        w = RecordWriter(f)
+        w.write(float(time), int(iline))
+        _write_mag_records(w, mag_layers)
+        _write_generators(w, generators)
+        if exact is not None:
+            w.write_bytes(_EXACT_TAG + struct.pack("<id", len(exact["layers"]), float(exact["dt"])))
+            for m, mag in exact["layers"]:
+                m = np.asarray(m, np.float64).reshape(-1, 3)
+                w.write(len(m), m, np.asarray(mag, np.float64).reshape(len(m)))
+    with open(os.path.join(directory, CHK_FILES[0]), "w") as f:
+        f.write(f" {float(time):.17E} {int(iline):11d} {idd:11d}\n")   # list-directed time, iline, idd
+    return idd
+
+
+def _read_slot(path, n_lay):
     with open(path, "rb") as f:
         r = RecordReader(f)
-        for _ in range(n_lay):
+        time, iline = r.read(("d", 1), ("i", 1))
+        mag = _read_mag_records(r, n_lay)
+        gens = _read_generators(r)
+        exact = None
+        try:
             hd = r.read_bytes()
-            nx, ny, n = struct.unpack_from("<3i", hd, 0)
-            ms_nom = struct.unpack_from("<d", hd, 12)[0]
-            mm = r.read(("d", 3 * n))[0].reshape(n, 3)
-            mag = np.sqrt((mm * mm).sum(1))
-            out.append((nx, ny, ms_nom, mm / np.where(mag > 0, mag, 1.0)[:, None], mag))
+        except EOFError:
+            hd = b""
+        if hd[:8] == _EXACT_TAG:
+            n, dt = struct.unpack_from("<id", hd, 8)
+            exact = dict(layers=[], dt=dt)
+            for _ in range(n):
+                data = r.read_bytes()
+                ng = struct.unpack_from("<i", data, 0)[0]
+                if len(data) != 4 + 32 * ng:
+                    raise ValueError("exact-state record of unexpected length")
+                m = np.frombuffer(data, "<f8", 3 * ng, 4).reshape(ng, 3).copy()
+                exact["layers"].append((m, np.frombuffer(data, "<f8", ng, 4 + 24 * ng).copy()))
+    return dict(time=float(time), iline=int(iline), mag=mag, generators=gens, exact=exact)
+
+
+def read_checkpoint(directory, n_lay):
+    """Checkpoint read as the reference does it (checkpoint.f90:58-99): None unless both
+    mrm.chk and mrm_grain.chk exist; the slot mrm.chk names is tried first, the other slot
+    if that one is missing or damaged.  Returns a dict with time, iline, slot, grain
+    (read_grain's layers), mag (read_mag's tuples), generators (dicts of name, mti, mt,
+    iset, gset) and exact (write_checkpoint's exact, or None)."""
+    if not all(os.path.exists(os.path.join(directory, n)) for n in CHK_FILES[:2]):
+        return None
+    grain = read_grain(os.path.join(directory, CHK_FILES[1]), n_lay)
+    first = _last_slot(directory)
+    for idd in (first, 1 - first):
+        p = os.path.join(directory, CHK_FILES[2 + idd])
+        if not os.path.exists(p):
+            continue
+        try:
+            out = _read_slot(p, n_lay)
+        except (ValueError, EOFError):
+            continue
+        out.update(slot=idd, grain=grain)
+        return out
+    return None
+
+
+def grain_dicts(layers, nx, ny, dx, dy, dz, tau=None):
+    """write_grain's layer dicts for mrm_amd.abi.LayerArrays layers (the inverse of
+    layer_arrays): grain_area = grain volume / dz; tc falls back to the layer's tc_nom."""
+    out = []
+    for k, la in enumerate(layers):
+        ng = la.ng
+        d = dict(nx=nx, ny=ny, il=k + 1, dx=dx, dy=dy, grid2grain=la.grid2grain.reshape(ny, nx),
+                 grain2grid=la.grain2grid.reshape(-1, 2), grain2gridindex=la.grain2gridindex,
+                 grain_area=la.grain_vol / np.broadcast_to(np.asarray(dz, float), (len(layers),))[k], ms=la.ms,
+                 tc=la.tc if la.tc is not None else np.full(ng, la.tc_nom),
+                 tau=np.zeros(ng) if tau is None else tau[k], hk=la.hk.reshape(ng, 2),
+                 uni_axis=None if la.uni_axis is None else la.uni_axis.reshape(ng, 3),
+                 cubic_axis=None if la.cubic_axis is None else la.cubic_axis.reshape(ng, 3, 3))
+        for key, pre in (("hx", ""), ("uhx", "u"), ("lhx", "l")):
+            idx = getattr(la, f"{pre}hxgrainindex")
+            if idx is not None:
+                d[key] = (idx, getattr(la, f"{pre}hxgrain"), getattr(la, f"{pre}hx"))
+        out.append(d)
     return out

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import abi
+from . import abi, formats
 
 
 class Llg:
@@ -200,6 +200,56 @@ class Llg:
         n = self.steps_for(t)
         d = self.step(n, t0=t, noise=noise, philox_seed=philox_seed)
         return d.t, d
+
+    # -- checkpoints -----------------------------------------------------------
+    def save_checkpoint(self, directory, t, iline, philox_seed=0, grain=True, ms_nom=None, generators=()):
+        """A checkpoint save (checkpoint.f90:106-145) into directory: mrm_grain.chk when
+        grain is true (the reference writes it once per run), the next of mrm_0.chk /
+        mrm_1.chk, then mrm.chk.  The Philox stream (philox_seed and the step counter)
+        goes in as one more random generator after generators (other save_random entries
+        to carry, e.g. MT19937 states); the state, |m| and dt are also kept bit for bit.
+        ms_nom: per-layer nominal Ms of the magnetisation header (default: mean grain Ms)."""
+        layers = [self._keep[("layer", k)] for k in range(self.n_lay)]
+        mag_layers, exact = [], []
+        for k, la in enumerate(layers):
+            m, _ = self.get_state(k)
+            nom = float(ms_nom[k]) if ms_nom is not None else float(np.mean(la.ms))
+            mag_layers.append((self.nx, self.ny, nom, m, la.ms))
+            exact.append((m, self.get_mag(k)))
+        gens = list(generators) + [formats.philox_generator(int(philox_seed), self.step_index)]
+        dz = self.geom._keep[0]
+        gl = formats.grain_dicts(layers, self.nx, self.ny, self.geom.dx, self.geom.dy, dz) if grain else None
+        return formats.write_checkpoint(directory, t, iline, mag_layers, gens, gl,
+                                        dict(layers=exact, dt=self.dt))
+
+    def load_checkpoint(self, directory, physics=None):
+        """A checkpoint read (checkpoint.f90:58-99): the state, the Philox counter and dt
+        from the newest readable slot.  physics (one dict of LayerArrays keywords per layer,
+        alpha, gamma, thermal, ...) also sets the layers from mrm_grain.chk, as the
+        reference re-reads its media.  Returns read_checkpoint's dict with philox_seed
+        added (None when the file holds no Philox stream); raises FileNotFoundError when no
+        checkpoint is readable."""
+        c = formats.read_checkpoint(directory, self.n_lay)
+        if c is None:
+            raise FileNotFoundError(f"no readable checkpoint in {directory}")
+        if physics is not None:
+            dz = self.geom._keep[0]
+            for k, L in enumerate(c["grain"]):
+                self.set_layer(k, formats.layer_arrays(L, dz[k], **physics[k]))
+        for k in range(self.n_lay):
+            if c["exact"] is not None:
+                m, mag = c["exact"]["layers"][k]
+                self.set_state(k, m)
+                self.set_mag(k, mag)
+            else:
+                self.set_state(k, c["mag"][k][3])
+        if c["exact"] is not None:
+            self.dt = c["exact"]["dt"]
+        c["philox_seed"] = None
+        for g in c["generators"]:
+            if g["name"] == formats.PHILOX_NAME:
+                c["philox_seed"], self.step_index = formats.philox_state(g)
+        return c
 
     def philox_normals(self, seed, step0, n_steps):
         out = np.zeros((n_steps, sum(self.ngs), 3))

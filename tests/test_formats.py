@@ -203,3 +203,176 @@ def test_mag_file_round_trip(tmp_path):
         assert (nx, ny, msn) == (bx, by, bmsn)
         np.testing.assert_allclose(bm, m, rtol=0, atol=1e-15)
         np.testing.assert_allclose(bmag, ms, rtol=1e-15)
+
+
+def test_generator_records(tmp_path):
+    """save_random / restore_random records (random.f90:38-107): the count, then per
+    generator mti, mt(0:623), iset, gset, name(40) in one record of 2552 bytes; the
+    Philox stream survives as one such entry with 64-bit key and counter."""
+    rng = np.random.default_rng(4)
+    mt_gen = dict(name="thermal layer 1", mti=17, mt=rng.integers(-2 ** 31, 2 ** 31, 624, dtype=np.int64),
+                  iset=1, gset=-0.625)
+    ph = formats.philox_generator(2 ** 63 + 5, 2 ** 40 + 3)
+    assert formats.philox_state(ph) == (2 ** 63 + 5, 2 ** 40 + 3)
+    p = tmp_path / "r.bin"
+    with open(p, "wb") as f:
+        formats._write_generators(formats.RecordWriter(f), [mt_gen, ph])
+    raw = p.read_bytes()
+    assert int(np.frombuffer(raw[12:16], "<i4")[0]) == 4 + 4 * 624 + 4 + 8 + 40
+    with open(p, "rb") as f:
+        back = formats._read_generators(formats.RecordReader(f))
+    assert [g["name"] for g in back] == ["thermal layer 1", formats.PHILOX_NAME]
+    assert back[0]["mti"] == 17 and back[0]["iset"] == 1 and back[0]["gset"] == -0.625
+    assert np.array_equal(back[0]["mt"], mt_gen["mt"].astype(np.int32))
+    assert formats.philox_state(back[1]) == (2 ** 63 + 5, 2 ** 40 + 3)
+
+
+def _ckpt_inputs(seed=0):
+    rng = np.random.default_rng(seed)
+    mags, exact = [], []
+    for n in (11, 7):
+        m = rng.standard_normal((n, 3))
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        ms = rng.uniform(300, 800, n)
+        mags.append((6, 4, 650.0, m, ms))
+        exact.append((m, rng.uniform(0.2, 1.0, n)))
+    return mags, exact
+
+
+def _grain(n_lay=2):
+    med = media.config_media("cfg1", nx=16, ny=16)
+    return _media_dicts(med)[:n_lay], med
+
+
+def test_checkpoint_slots(tmp_path):
+    """Saves alternate between mrm_0.chk and mrm_1.chk with mrm.chk naming the last
+    (checkpoint.f90:121-133); a read takes the named slot and falls back to the other one
+    when it is damaged; without mrm.chk or mrm_grain.chk there is no checkpoint
+    (checkpoint.f90:60)."""
+    d = str(tmp_path)
+    assert formats.read_checkpoint(d, 2) is None
+    grain, med = _grain()
+    mags, exact = _ckpt_inputs()
+    gens = [formats.philox_generator(9, 100)]
+    assert formats.write_checkpoint(d, 1e-9, 3, mags, gens, grain, dict(layers=exact, dt=5e-14)) == 0
+    c = formats.read_checkpoint(d, 2)
+    assert c["slot"] == 0 and c["time"] == 1e-9 and c["iline"] == 3 and c["exact"]["dt"] == 5e-14
+    assert formats.philox_state(c["generators"][0]) == (9, 100)
+    for (m, mag), (bm, bmag) in zip(exact, c["exact"]["layers"]):
+        assert np.array_equal(m, bm) and np.array_equal(mag, bmag)
+    for a, b in zip(mags, c["mag"]):
+        np.testing.assert_allclose(b[3], a[3], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(b[4], a[4], rtol=1e-15)
+    assert np.array_equal(c["grain"][1]["grid2grain"], grain[1]["grid2grain"])
+    gens2 = [formats.philox_generator(9, 200)]
+    assert formats.write_checkpoint(d, 2e-9, 4, mags, gens2) == 1          # grain kept from the first save
+    assert open(tmp_path / "mrm.chk").read().split()[1:] == ["4", "1"]
+    c = formats.read_checkpoint(d, 2)
+    assert c["slot"] == 1 and c["time"] == 2e-9 and formats.philox_state(c["generators"][0]) == (9, 200)
+    raw = (tmp_path / "mrm_1.chk").read_bytes()
+    (tmp_path / "mrm_1.chk").write_bytes(raw[:100])                        # a save cut short
+    c = formats.read_checkpoint(d, 2)
+    assert c["slot"] == 0 and c["iline"] == 3
+    assert formats.write_checkpoint(d, 3e-9, 5, mags, gens2) == 0          # the next save goes to the other slot
+    (tmp_path / "mrm_grain.chk").unlink()
+    assert formats.read_checkpoint(d, 2) is None
+    # without the exact records the state comes from the renormalised magnetisation
+    formats.write_grain(tmp_path / "mrm_grain.chk", grain)
+    formats.write_checkpoint(d, 4e-9, 6, mags, gens2)
+    c = formats.read_checkpoint(d, 2)
+    assert c["exact"] is None and c["iline"] == 6
+
+
+CKPT_READER = """
+program r
+  implicit none
+  real(8) :: time, ms_nom, gset
+  integer :: iline, idd, nx, ny, n, k, ngen, mti, mt(0:623), iset, g
+  real(8), allocatable :: mms(:,:)
+  character(40) :: name
+  open(10, file='mrm.chk', form='formatted', status='old')
+  read(10, *) time, iline, idd
+  close(10)
+  print '(es24.16,1x,i0,1x,i0)', time, iline, idd
+  if (idd == 0) open(10, file='mrm_0.chk', form='unformatted', status='old')
+  if (idd == 1) open(10, file='mrm_1.chk', form='unformatted', status='old')
+  read(10) time, iline
+  print '(es24.16,1x,i0)', time, iline
+  do k = 1, 2
+    read(10) nx, ny, n, ms_nom
+    allocate(mms(3,n))
+    read(10) mms
+    print '(i0,1x,i0,1x,i0,1x,f10.3,3(1x,es24.16))', nx, ny, n, ms_nom, mms(:,n)
+    deallocate(mms)
+  enddo
+  read(10) ngen
+  do g = 1, ngen
+    read(10) mti, mt, iset, gset, name
+    print '(i0,1x,i0,1x,i0,1x,i0,1x,f8.4,1x,a)', mti, mt(0), mt(623), iset, gset, trim(name)
+  enddo
+  close(10)
+end program
+"""
+
+CKPT_WRITER = """
+program w
+  implicit none
+  integer :: mt(0:623), i
+  real(8) :: mms(3,2)
+  character(40) :: name
+  do i = 0, 623
+    mt(i) = 1000 * i - 7
+  enddo
+  mms = reshape((/ 1d0, 2d0, 3d0, 0d0, 0d0, -5d0 /), (/ 3, 2 /))
+  open(10, file='mrm_1.chk', form='unformatted', status='replace')
+  write(10) 2.5d-9, 12
+  write(10) 4, 3, 2, 600d0
+  write(10) mms
+  write(10) 4, 3, 2, 600d0
+  write(10) mms
+  write(10) 1
+  name = 'mt one'
+  write(10) 311, mt, 1, 0.75d0, name
+  close(10)
+  open(10, file='mrm.chk', form='formatted', status='replace')
+  write(10, *) 2.5d-9, 12, 1
+  close(10)
+end program
+"""
+
+
+@pytest.mark.skipif(FLANG is None, reason="no Fortran compiler in this image")
+def test_checkpoint_against_fortran_compiler():
+    """A checkpoint written here read by the reference's read statements compiled with
+    flang (checkpoint.f90:76, 85; mag.f90:233-238; random.f90:77-80), and one written by its
+    write statements (checkpoint.f90:123, 133; mag.f90:285-290; random.f90:50-53) read here."""
+    with tempfile.TemporaryDirectory() as d:
+        for name, src in (("r", CKPT_READER), ("w", CKPT_WRITER)):
+            open(os.path.join(d, f"{name}.f90"), "w").write(src)
+            subprocess.run([FLANG, "-o", os.path.join(d, name), os.path.join(d, f"{name}.f90")], check=True,
+                           capture_output=True)
+        grain, _ = _grain()
+        mags, _ = _ckpt_inputs(1)
+        mt = np.arange(624, dtype=np.int64) * 3 - 2 ** 31
+        gens = [dict(name="thermal", mti=5, mt=mt, iset=0, gset=1.25), formats.philox_generator(7, 2 ** 33 + 1)]
+        formats.write_checkpoint(d, 1.5e-9, 8, mags, gens, grain)
+        out = subprocess.run([os.path.join(d, "r")], cwd=d, check=True, capture_output=True,
+                             text=True).stdout.splitlines()
+        assert float(out[0].split()[0]) == 1.5e-9 and out[0].split()[1:] == ["8", "0"]
+        assert float(out[1].split()[0]) == 1.5e-9 and out[1].split()[1] == "8"
+        for k, (nx, ny, nom, m, ms) in enumerate(mags):
+            f = out[2 + k].split()
+            assert [int(v) for v in f[:3]] == [nx, ny, len(m)] and float(f[3]) == nom
+            np.testing.assert_allclose([float(v) for v in f[4:]], m[-1] * ms[-1], rtol=1e-15)
+        assert out[4].split() == ["5", str(-2 ** 31), str(623 * 3 - 2 ** 31), "0", "1.2500", "thermal"]
+        assert out[5].split() == [str(formats.MT_N + 1), "7", "0", "0", "0.0000", formats.PHILOX_NAME]
+        for name in ("mrm.chk", "mrm_0.chk"):
+            os.unlink(os.path.join(d, name))
+        subprocess.run([os.path.join(d, "w")], cwd=d, check=True)
+        c = formats.read_checkpoint(d, 2)
+        assert c["slot"] == 1 and c["time"] == 2.5e-9 and c["iline"] == 12 and c["exact"] is None
+        g = c["generators"][0]
+        assert g["name"] == "mt one" and g["mti"] == 311 and g["iset"] == 1 and g["gset"] == 0.75
+        assert np.array_equal(g["mt"], 1000 * np.arange(624) - 7)
+        np.testing.assert_allclose(c["mag"][0][3], [[1, 2, 3] / np.sqrt(14), [0, 0, -1]], rtol=0, atol=1e-16)
+        np.testing.assert_allclose(c["mag"][0][4], [np.sqrt(14), 5])
