@@ -285,6 +285,37 @@ int mrmg_demag_field(mrmg_ctx *ctx, double *h_cell, double *h_grain);
 /* Device bytes held by the context. */
 int mrmg_footprint(mrmg_ctx *ctx, size_t *bytes);
 
+/* ---- outputs the reference does not compute: readback and the switching record.  Its
+ * nearest relative is off_field.f90 (the stray field on a plane above the media,
+ * plot_off_field 27-52), a plot-only path. ------------------------------------------- */
+typedef struct {
+    double sigma_x, sigma_y;  /* widths (cm) of the Gaussian sensitivity, > 0              */
+    double dir[3];            /* direction of the head's field at the media (reciprocity)   */
+    const double *w;          /* n_lay layer weights (e.g. a spacing loss per layer); NULL = 1 */
+} mrmg_sensor;
+
+/* Reciprocity readback of the current state at n_pos head positions:
+ *   out[p] = sum over cells of w[k] dz[k] dx dy (dir . M) gx gy,
+ *   gx = exp(-ddx^2 / (2 sigma_x^2)), gy = exp(-ddy^2 / (2 sigma_y^2)),
+ * with M the cell magnetisation as the demag convolution sees it (Ms m, emu/cm^3; 0 in
+ * non-magnetic cells) and ddx, ddy the periodic minimum-image distances from the cell
+ * centre ((i+1/2) dx, (j+1/2) dy) to (xpos[p], ypos[p]).  out is in emu. */
+int mrmg_readback(mrmg_ctx *ctx, const mrmg_sensor *s, int n_pos, const double *xpos, const double *ypos,
+                  double *out);
+
+/* Switching record.  enable != 0 clears it (every grain "not switched") and starts it:
+ * after each later completed step of mrmg_step, every grain not yet marked whose m
+ * satisfies dir . m > thr gets that step's counter (step0 + i), its end time and the
+ * applied field there (the grain's static field plus the uniform field of
+ * mrmg_set_ufield at that time; a moving source's field is not included).  enable 0
+ * stops marking; the record stays readable.  E.g. dir = (0,0,-1), thr = 0 under a
+ * reversing field gives each grain's first crossing of m_z = 0, and the switching-field
+ * distribution from h_app. */
+int mrmg_switch_arm(mrmg_ctx *ctx, int enable, const double dir[3], double thr);
+/* The record of layer k: step (ng; -1 = not marked), t (ng) and h_app (3,ng); any may be
+ * NULL.  t and h_app are meaningful only where step >= 0. */
+int mrmg_switch_read(mrmg_ctx *ctx, int k, int64_t *step, double *t, double *h_app);
+
 /* ---- instrumentation (no counterpart in the reference: its only timers are cpu_time
  * print-outs, demag.f90:280-283, main.f90:31-34) ------------------------------------- */
 typedef struct {

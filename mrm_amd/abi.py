@@ -197,6 +197,11 @@ class LayerArrays:
         return s
 
 
+class Sensor(C.Structure):
+    """mrmg_sensor: Gaussian reciprocity sensitivity of a read head."""
+    _fields_ = [("sigma_x", C.c_double), ("sigma_y", C.c_double), ("dir", C.c_double * 3), ("w", _dp)]
+
+
 def make_geom(nx, ny, n_lay, dx, dy, dz, il_space=None, precision=MRMG_FP64, demag=True):
     dz = np.ascontiguousarray(np.broadcast_to(np.asarray(dz, dtype=np.float64), (n_lay,)).copy())
     il = np.zeros(n_lay) if il_space is None else np.ascontiguousarray(il_space, dtype=np.float64)
@@ -259,6 +264,9 @@ def load(path=None):
         ("mrmg_copy_probe", [C.c_void_p, C.c_size_t, C.c_int, _dp]),
         ("mrmg_tune", [C.c_void_p, C.c_char_p, C.c_int]),
         ("mrmg_get_tune", [C.c_void_p, C.c_char_p, _ip]),
+        ("mrmg_readback", [C.c_void_p, C.POINTER(Sensor), C.c_int, _dp, _dp, _dp]),
+        ("mrmg_switch_arm", [C.c_void_p, C.c_int, _dp, C.c_double]),
+        ("mrmg_switch_read", [C.c_void_p, C.c_int, C.POINTER(C.c_int64), _dp, _dp]),
     ]:
         f = getattr(lib, name)
         f.argtypes = args

@@ -1449,6 +1449,93 @@ __global__ void __launch_bounds__(STATS_THREADS) k_layer_stats(const double* __r
     }
 }
 
+// ------------------------------------------------------------------------- readback
+// Reciprocity readback with a Gaussian sensitivity.  k_rb_plane folds the layers and the
+// components of the cell magnetisation into one plane P(x,y) = sum_k wk[k] (dir . M); then
+// one workgroup per head position sums P against the two Gaussians, whose values over the
+// cell centres it first puts in LDS (periodic minimum-image distances).
+struct RbWeights {
+    double w[MRMG_MAX_LAYERS];
+};
+template <typename TC>
+__global__ void __launch_bounds__(256) k_rb_plane(const TC* __restrict__ Mcell, size_t CP, int nxy, int nl,
+                                                  double d0, double d1, double d2, RbWeights wk,
+                                                  double* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxy) return;
+    double s = 0.0;
+    for (int k = 0; k < nl; k++) {
+        const size_t ci = (size_t)k * nxy + i;
+        s += wk.w[k] * (d0 * (double)Mcell[ci] + d1 * (double)Mcell[CP + ci] + d2 * (double)Mcell[2 * CP + ci]);
+    }
+    P[i] = s;
+}
+__global__ void __launch_bounds__(256) k_rb_scan(const double* __restrict__ P, int nx, int ny, double dx, double dy,
+                                                 double sx, double sy, const double* __restrict__ pos, int n_pos,
+                                                 double* __restrict__ out)
+{
+    extern __shared__ double gsh[];   // nx + ny
+    __shared__ double red[4];
+    const int p = blockIdx.x;
+    if (p >= n_pos) return;
+    const double x0 = pos[p], y0 = pos[n_pos + p];
+    const double Lx = nx * dx, Ly = ny * dy;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        double d = (i + 0.5) * dx - x0;
+        d -= Lx * rint(d / Lx);
+        gsh[i] = exp(-d * d / (2.0 * sx * sx));
+    }
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        double d = (j + 0.5) * dy - y0;
+        d -= Ly * rint(d / Ly);
+        gsh[nx + j] = exp(-d * d / (2.0 * sy * sy));
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int idx = threadIdx.x; idx < nx * ny; idx += blockDim.x) {
+        const int j = idx / nx, i = idx - j * nx;
+        s += P[idx] * gsh[i] * gsh[nx + j];
+    }
+    const double r = block_sum_fixed(s, red);
+    if (threadIdx.x == 0) out[p] = r;
+}
+
+// ------------------------------------------------------------------------- switching record
+// After a completed step: a grain not yet marked whose m satisfies dir . m > thr gets the
+// step's index, its end time and the applied field there (the grain's static field plus
+// the uniform ramped field at that time, as get_applied sums them; a moving table source
+// is not included).
+template <typename S>
+__global__ void __launch_bounds__(256) k_switch(GrainData<S> gd, const S* __restrict__ m, double d0, double d1,
+                                                double d2, double thr, long long step, double t,
+                                                long long* __restrict__ rs, double* __restrict__ rt,
+                                                double* __restrict__ rh)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = gd.G;
+    if (g >= G || rs[g] >= 0) return;
+    const double v = d0 * (double)m[g] + d1 * (double)m[(size_t)G + g] + d2 * (double)m[2 * (size_t)G + g];
+    if (v > thr) {
+        const LayerParams& L = gd.lp[gd.layer_of[g]];
+        double h[3] = {0, 0, 0};
+        if (gd.happ)
+            for (int i = 0; i < 3; i++) h[i] = (double)gd.happ[(size_t)i * G + g];
+        if (L.has_uf) {
+            const double sc = tscale_eval(L.uf_fn, L.uf_t0, L.uf_t1, L.uf_s0, L.uf_s1, L.uf_scale, t);
+            for (int i = 0; i < 3; i++) h[i] += sc * L.uf_dir[i];
+        }
+        rs[g] = step;
+        rt[g] = t;
+        for (int i = 0; i < 3; i++) rh[(size_t)i * G + g] = h[i];
+    }
+}
+__global__ void k_fill_i64(long long* __restrict__ a, size_t n, long long v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 // streaming copy (16-byte vectors), the known-size reference of the traffic counters
 __global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n)
 {
@@ -1788,6 +1875,9 @@ struct EngineBase {
     virtual int tune(const char* key, int v) = 0;
     virtual int get_tune(const char* key, int* v) = 0;
     virtual int kernel_stats(mrmg_kstat* out, int max, int* n) = 0;
+    virtual int readback(const mrmg_sensor* s, int n_pos, const double* xpos, const double* ypos, double* out) = 0;
+    virtual int switch_arm(int enable, const double* dir, double thr) = 0;
+    virtual int switch_read(int k, int64_t* step, double* t, double* hap) = 0;
 };
 
 struct mrmg_ctx {
@@ -1876,6 +1966,11 @@ struct Engine : EngineBase {
     int row_radix = 16, col_radix = 16;
     size_t lds_rowT = 0, lds_colT = 0;
     DBuf<double> noise_buf, diag_part, stats;
+    // switching record and readback buffers (allocated on first use)
+    DBuf<long long> sw_step;
+    DBuf<double> sw_t, sw_h, rb_plane, rb_pos, rb_out;
+    bool sw_on = false;
+    double sw_dir[3] = {0, 0, 0}, sw_thr = 0;
     // instrumentation: event pairs of the current mrmg_step call, per-class totals
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -2840,6 +2935,7 @@ struct Engine : EngineBase {
             dt = dt * max_dmdt;
             max_dmdt = max_dmdtnext;
             t = t + dt;
+            if (sw_on) RCHK(switch_mark((long long)a.step, t));
         }
         if (diag) {
             memset(diag, 0, sizeof(*diag));
@@ -2949,6 +3045,7 @@ struct Engine : EngineBase {
                 ma.swap_with(mb);
             }
             t = t + dt;
+            if (sw_on) RCHK(switch_mark((long long)a.step, t));
             if (prof && ev_used >= 512) {
                 HIPCHK(hipStreamSynchronize(c->stream));
                 RCHK(ev_collect());
@@ -2985,6 +3082,78 @@ struct Engine : EngineBase {
             strncpy(out[i].name, names[i], sizeof(out[i].name) - 1);
             out[i].launches = kt_n[i];
             out[i].total_ms = kt_ms[i];
+        }
+        return MRMG_OK;
+    }
+
+    int readback(const mrmg_sensor* sn, int n_pos, const double* xpos, const double* ypos, double* out) override
+    {
+        const int nx = c->nx, ny = c->ny, nl = c->nl, nxy = nx * ny;
+        const size_t CP = (size_t)nl * nxy;
+        RbWeights wk{};
+        for (int k = 0; k < nl; k++) wk.w[k] = (sn->w ? sn->w[k] : 1.0) * c->dz[k] * c->dx * c->dy;
+        if (rb_plane.n < (size_t)nxy) RCHK(rb_plane.alloc(nxy));
+        if (rb_pos.n < 2 * (size_t)n_pos) RCHK(rb_pos.alloc(2 * (size_t)n_pos));
+        if (rb_out.n < (size_t)n_pos) RCHK(rb_out.alloc(n_pos));
+        std::vector<double> pos(2 * (size_t)n_pos);
+        memcpy(pos.data(), xpos, sizeof(double) * n_pos);
+        memcpy(pos.data() + n_pos, ypos, sizeof(double) * n_pos);
+        HIPCHK(hipMemcpyAsync(rb_pos.p, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice, c->stream));
+        RCHK(scatter(ma.p));   // the cells hold the current state
+        hipLaunchKernelGGL((k_rb_plane<T>), dim3((nxy + 255) / 256), dim3(256), 0, c->stream, Mcell.p, CP, nxy, nl,
+                           sn->dir[0], sn->dir[1], sn->dir[2], wk, rb_plane.p);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_rb_scan, dim3(n_pos), dim3(256), sizeof(double) * (nx + ny), c->stream, rb_plane.p, nx,
+                           ny, c->dx, c->dy, sn->sigma_x, sn->sigma_y, rb_pos.p, n_pos, rb_out.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(out, rb_out.p, sizeof(double) * n_pos, hipMemcpyDeviceToHost));
+        return MRMG_OK;
+    }
+    int switch_arm(int enable, const double* dir, double thr) override
+    {
+        if (!enable) {
+            sw_on = false;
+            return MRMG_OK;
+        }
+        if (sw_step.n < (size_t)G) {
+            RCHK(sw_step.alloc(G));
+            RCHK(sw_t.alloc(G));
+            RCHK(sw_h.alloc(3 * (size_t)G));
+        }
+        hipLaunchKernelGGL(k_fill_i64, dim3((G + 255) / 256), dim3(256), 0, c->stream, sw_step.p, (size_t)G, -1LL);
+        HIPCHK(hipGetLastError());
+        for (int i = 0; i < 3; i++) sw_dir[i] = dir[i];
+        sw_thr = thr;
+        sw_on = true;
+        return MRMG_OK;
+    }
+    // after a completed step: first-time marks of the grains past the threshold
+    int switch_mark(long long step, double t)
+    {
+        hipLaunchKernelGGL((k_switch<S>), dim3((G + 255) / 256), dim3(256), 0, c->stream, gdata(), ma.p, sw_dir[0],
+                           sw_dir[1], sw_dir[2], sw_thr, step, t, sw_step.p, sw_t.p, sw_h.p);
+        HIPCHK(hipGetLastError());
+        return MRMG_OK;
+    }
+    int switch_read(int k, int64_t* step, double* t, double* hap) override
+    {
+        if (sw_step.n < (size_t)G) return set_err(MRMG_ERR_STATE, "switching record was never armed");
+        const int ng = c->layers[k].ng;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (step) {
+            std::vector<long long> v(ng);
+            HIPCHK(hipMemcpy(v.data(), sw_step.p + off[k], sizeof(long long) * ng, hipMemcpyDeviceToHost));
+            for (int n = 0; n < ng; n++) step[n] = (int64_t)v[n];
+        }
+        if (t) HIPCHK(hipMemcpy(t, sw_t.p + off[k], sizeof(double) * ng, hipMemcpyDeviceToHost));
+        if (hap) {
+            std::vector<double> v(3 * (size_t)ng);
+            for (int cc = 0; cc < 3; cc++)
+                HIPCHK(hipMemcpy(v.data() + (size_t)cc * ng, sw_h.p + (size_t)cc * G + off[k], sizeof(double) * ng,
+                                 hipMemcpyDeviceToHost));
+            for (int n = 0; n < ng; n++)
+                for (int cc = 0; cc < 3; cc++) hap[3 * (size_t)n + cc] = v[(size_t)cc * ng + n];
         }
         return MRMG_OK;
     }
@@ -3453,6 +3622,35 @@ int mrmg_demag_field(mrmg_ctx* c, double* h_cell, double* h_grain)
     if (c->demag && !c->eng->has_tensor()) return set_err(MRMG_ERR_STATE, "no tensor");
     RCHK(ensure_built(c));
     return c->eng->demag_field(h_cell, h_grain);
+}
+
+int mrmg_readback(mrmg_ctx* c, const mrmg_sensor* s, int n_pos, const double* xpos, const double* ypos,
+                  double* out)
+{
+    if (!c || !s) return set_err(MRMG_ERR_ARG, "null argument");
+    if (n_pos < 0) return set_err(MRMG_ERR_ARG, "n_pos < 0");
+    if (n_pos > 0 && (!xpos || !ypos || !out)) return set_err(MRMG_ERR_ARG, "null position or output array");
+    if (!(s->sigma_x > 0.0) || !(s->sigma_y > 0.0)) return set_err(MRMG_ERR_ARG, "sensor widths must be > 0");
+    if ((size_t)(c->nx + c->ny) * sizeof(double) > 64 * 1024)
+        return set_err(MRMG_ERR_UNSUPPORTED, "readback: nx + ny too large for one workgroup's table");
+    RCHK(ensure_built(c));
+    if (n_pos == 0) return MRMG_OK;
+    return c->eng->readback(s, n_pos, xpos, ypos, out);
+}
+
+int mrmg_switch_arm(mrmg_ctx* c, int enable, const double* dir, double thr)
+{
+    if (!c || (enable && !dir)) return set_err(MRMG_ERR_ARG, "null argument");
+    RCHK(ensure_built(c));
+    return c->eng->switch_arm(enable, dir, thr);
+}
+
+int mrmg_switch_read(mrmg_ctx* c, int k, int64_t* step, double* t, double* h_app)
+{
+    if (!c) return set_err(MRMG_ERR_ARG, "null argument");
+    if (k < 0 || k >= c->nl) return set_err(MRMG_ERR_ARG, "layer index %d out of range", k);
+    RCHK(ensure_built(c));
+    return c->eng->switch_read(k, step, t, h_app);
 }
 
 int mrmg_profile(mrmg_ctx* c, int enable)

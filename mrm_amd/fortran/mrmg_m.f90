@@ -11,10 +11,11 @@ module mrmg_m
   use, intrinsic :: iso_c_binding
   implicit none
   private
-  public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag
+  public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag, mrmg_sensor
   public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_build_tensor_images, &
             mrmg_set_layer, &
-            mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error
+            mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error, &
+            mrmg_readback, mrmg_switch_arm, mrmg_switch_read
   public :: MRMG_FP64, MRMG_FP32, MRMG_NOISE_NONE, MRMG_NOISE_INJECT, MRMG_NOISE_PHILOX, MRMG_TYPE_HEUN
 
   integer(c_int), parameter :: MRMG_FP64 = 0, MRMG_FP32 = 1
@@ -68,6 +69,13 @@ module mrmg_m
     real(c_double) :: max_dmdt
     real(c_double) :: mvar(3, 8), have(3, 8)
     real(c_double) :: mave_all(3), mvar_all(3), have_all(3)
+  end type
+
+  ! Gaussian reciprocity sensitivity of a read head (w: c_loc of n_lay weights or c_null_ptr)
+  type, bind(C) :: mrmg_sensor
+    real(c_double) :: sigma_x, sigma_y
+    real(c_double) :: dir(3)
+    type(c_ptr) :: w
   end type
 
   interface
@@ -144,6 +152,29 @@ module mrmg_m
       type(mrmg_step_params), intent(in) :: p
       type(c_ptr), value :: noise
       type(mrmg_diag), intent(out) :: diag
+    end function
+    integer(c_int) function mrmg_readback(ctx, sn, n_pos, xpos, ypos, out) bind(C, name='mrmg_readback')
+      import :: c_int, c_ptr, c_double, mrmg_sensor
+      type(c_ptr), value :: ctx
+      type(mrmg_sensor), intent(in) :: sn
+      integer(c_int), value :: n_pos
+      real(c_double), intent(in) :: xpos(*), ypos(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function mrmg_switch_arm(ctx, enable, dir, thr) bind(C, name='mrmg_switch_arm')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      real(c_double), intent(in) :: dir(3)
+      real(c_double), value :: thr
+    end function
+    ! step: integer(c_int64_t) (ng); t (ng); h_app (3,ng)
+    integer(c_int) function mrmg_switch_read(ctx, k, step, t, h_app) bind(C, name='mrmg_switch_read')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      integer(c_int64_t), intent(out) :: step(*)
+      real(c_double), intent(out) :: t(*), h_app(*)
     end function
     type(c_ptr) function mrmg_last_error() bind(C, name='mrmg_last_error')
       import :: c_ptr

@@ -201,6 +201,43 @@ class Llg:
         d = self.step(n, t0=t, noise=noise, philox_seed=philox_seed)
         return d.t, d
 
+    # -- readback and switching record ----------------------------------------
+    def readback(self, xpos, ypos, sigma, direction=(0.0, 0.0, 1.0), weights=None):
+        """Reciprocity readback of the current state at head positions (xpos[i], ypos[i])
+        with a Gaussian sensitivity of widths sigma = (sigma_x, sigma_y) (cm) along
+        direction; weights: per-layer factors (spacing loss), default 1.  Returns emu."""
+        xpos = np.ascontiguousarray(np.atleast_1d(xpos), dtype=np.float64)
+        ypos = np.ascontiguousarray(np.broadcast_to(np.asarray(ypos, np.float64), xpos.shape))
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and w.size != self.n_lay:
+            raise ValueError("weights needs one value per layer")
+        s = abi.Sensor(sigma_x=float(sigma[0]), sigma_y=float(sigma[1]), w=abi.dptr(w))
+        for i in range(3):
+            s.dir[i] = float(direction[i])
+        out = np.zeros(xpos.size)
+        abi.check(self.lib, self.lib.mrmg_readback(self.ctx, C.byref(s), xpos.size, abi.dptr(xpos), abi.dptr(ypos),
+                                                   abi.dptr(out)))
+        return out
+
+    def switch_arm(self, direction=(0.0, 0.0, -1.0), threshold=0.0):
+        """Start (and clear) the switching record: each later step marks the grains whose
+        direction . m exceeds threshold for the first time."""
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        abi.check(self.lib, self.lib.mrmg_switch_arm(self.ctx, 1, abi.dptr(d), float(threshold)))
+
+    def switch_stop(self):
+        abi.check(self.lib, self.lib.mrmg_switch_arm(self.ctx, 0, None, 0.0))
+
+    def switch_record(self, k):
+        """(step, t, h_app) of layer k's grains: the step counter, end time and applied
+        field (ng,3) of each grain's first crossing; step -1 where none happened."""
+        step = np.zeros(self.ngs[k], np.int64)
+        t = np.zeros(self.ngs[k])
+        h = np.zeros((self.ngs[k], 3))
+        abi.check(self.lib, self.lib.mrmg_switch_read(self.ctx, k, step.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                      abi.dptr(t), abi.dptr(h)))
+        return step, t, h
+
     # -- checkpoints -----------------------------------------------------------
     def save_checkpoint(self, directory, t, iline, philox_seed=0, grain=True, ms_nom=None, generators=()):
         """A checkpoint save (checkpoint.f90:106-145) into directory: mrm_grain.chk when

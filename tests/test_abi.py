@@ -3,6 +3,7 @@ ctypes structures match the header's layouts as the host C compiler sees them.""
 import ctypes as C
 import os
 import re
+import shutil
 import subprocess
 import tempfile
 
@@ -59,6 +60,8 @@ int main(void) {
   F(mrmg_ufield, t_start) F(mrmg_ufield, scale) F(mrmg_ufield, func) F(mrmg_ufield, amp) F(mrmg_ufield, dc)
   F(mrmg_source, scale) F(mrmg_source, func) F(mrmg_source, amp) F(mrmg_source, dc)
   printf("mrmg_source %zu\n", sizeof(mrmg_source));
+  printf("mrmg_sensor %zu\n", sizeof(mrmg_sensor));
+  F(mrmg_sensor, sigma_y) F(mrmg_sensor, dir) F(mrmg_sensor, w)
   F(mrmg_step_params, t0) F(mrmg_step_params, noise) F(mrmg_step_params, seed) F(mrmg_step_params, step0)
   F(mrmg_step_params, dt_frac) F(mrmg_step_params, max_dm) F(mrmg_step_params, llb)
   F(mrmg_diag, errmx) F(mrmg_diag, mmag) F(mrmg_diag, mave) F(mrmg_diag, rejected) F(mrmg_diag, max_dmdt)
@@ -85,9 +88,50 @@ def test_ctypes_layout_matches_header():
     assert C.sizeof(abi.StepParams) == want["mrmg_step_params"]
     assert C.sizeof(abi.Diag) == want["mrmg_diag"]
     assert C.sizeof(abi.Source) == want["mrmg_source"]
+    assert C.sizeof(abi.Sensor) == want["mrmg_sensor"]
     types = {"mrmg_geom": abi.Geom, "mrmg_layer": abi.Layer, "mrmg_ufield": abi.UField,
-             "mrmg_step_params": abi.StepParams, "mrmg_diag": abi.Diag, "mrmg_source": abi.Source}
+             "mrmg_step_params": abi.StepParams, "mrmg_diag": abi.Diag, "mrmg_source": abi.Source,
+             "mrmg_sensor": abi.Sensor}
     for k, v in want.items():
         if "." in k:
             t, f = k.split(".")
             assert getattr(types[t], f).offset == v, k
+
+
+FLANG = shutil.which("flang") or ("/opt/rocm/lib/llvm/bin/flang" if os.path.exists("/opt/rocm/lib/llvm/bin/flang")
+                                   else None)
+
+F_PROBE = """
+program p
+  use mrmg_m
+  use, intrinsic :: iso_c_binding
+  implicit none
+  type(mrmg_geom) :: g
+  type(mrmg_layer) :: l
+  type(mrmg_ufield) :: u
+  type(mrmg_step_params) :: s
+  type(mrmg_diag) :: d
+  type(mrmg_sensor) :: r
+  print '(6(i0,1x))', c_sizeof(g), c_sizeof(l), c_sizeof(u), c_sizeof(s), c_sizeof(d), c_sizeof(r)
+end program
+"""
+
+
+@pytest.mark.skipif(FLANG is None, reason="no Fortran compiler in this image")
+def test_fortran_binding_types_match_header():
+    """The bind(C) derived types of mrm_amd/fortran/mrmg_m.f90 have the sizes of the C
+    structs (the module and a program using it compile with the image's flang and link
+    against libmrmg.so)."""
+    with tempfile.TemporaryDirectory() as d:
+        mod = os.path.join(ROOT, "mrm_amd", "fortran", "mrmg_m.f90")
+        src = os.path.join(d, "p.f90")
+        open(src, "w").write(F_PROBE)
+        exe = os.path.join(d, "p")
+        if not os.path.exists(abi.LIB_PATH):
+            pytest.skip("libmrmg.so not built")
+        # linked against the library (the module procedure mrmg_error calls into it)
+        subprocess.run([FLANG, "-J", d, "-o", exe, mod, src, abi.LIB_PATH,
+                        "-Wl,-rpath," + os.path.dirname(abi.LIB_PATH)], check=True, capture_output=True, cwd=d)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    want = [C.sizeof(t) for t in (abi.Geom, abi.Layer, abi.UField, abi.StepParams, abi.Diag, abi.Sensor)]
+    assert [int(v) for v in out] == want
