@@ -161,8 +161,10 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
 // columns are contiguous and the loads coalesce), else [k][c][y][kx].
 // WV: each wave transforms its own run of columns (no workgroup barrier inside the FFTs);
 // E then bounds one lane's elements of its wave's run.
-template <typename T, int E, bool SYM, bool TL = false, bool WV = false>
-__global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+// BIG: 1024-thread workgroups (twice the lanes per FFT pass, at most 128 VGPRs), two
+// product items per thread.
+template <typename T, int E, bool SYM, bool TL = false, bool WV = false, bool BIG = false>
+__global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
                                                      const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
                                                      int nslot, int kg, int nb8, FFTPlan py,
                                                      const C2<T>* __restrict__ twy)
@@ -196,7 +198,7 @@ __global__ void __launch_bounds__(COL_MAX_THREADS) k_col(const C2<T>* __restrict
     // values of KSB source layers are loaded before any of them is used, so a thread waits
     // for memory once per KSB layers instead of once per layer.
     const int items = (ko1 - ko0) * ny;
-    constexpr int MAXI = 4, KSB = 2, NS = SYM ? 6 : 9;
+    constexpr int MAXI = BIG ? 2 : 4, KSB = 2, NS = SYM ? 6 : 9;
     C2<T> hv[MAXI][3];
 #pragma unroll
     for (int q = 0; q < MAXI; q++) {
@@ -2120,7 +2122,11 @@ struct Engine : EngineBase {
             return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs (threads %d/%d)",
                            c->nx, ny, nl, row_threads, col_threads);
         const int per = (nl + kg - 1) / kg;
-        if (per * ny > 4 * col_threads)
+        if (col_threads > 512 && wv)
+            return set_err(MRMG_ERR_UNSUPPORTED, "col_threads 1024 runs without wave-local FFTs");
+        if (col_threads > 512 && maxb_col > 16)   // more elements per lane spill at 128 VGPRs
+            return set_err(MRMG_ERR_UNSUPPORTED, "col_threads 1024 is for columns of up to 16 elements per lane");
+        if (per * ny > (col_threads > 512 ? 2 : 4) * col_threads)
             return set_err(MRMG_ERR_UNSUPPORTED, "column workgroup of %d threads cannot hold %d products", col_threads,
                            per * ny);
         return MRMG_OK;
@@ -2134,7 +2140,9 @@ struct Engine : EngineBase {
             old = row_threads; row_threads = v;
             if (shapes() != MRMG_OK) { row_threads = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else if (k == "col_threads") {
-            if (v != 256 && v != 512) return set_err(MRMG_ERR_ARG, "col_threads must be 256 or 512");
+            if (v != 256 && v != 512 && v != 1024) return set_err(MRMG_ERR_ARG, "col_threads must be 256, 512 or 1024");
+            if (v > 512 && nslot != 6)
+                return set_err(MRMG_ERR_UNSUPPORTED, "col_threads 1024 needs the symmetric (6-slot) tensor");
             old = col_threads; col_threads = v;
             if (shapes() != MRMG_OK) { col_threads = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else if (k == "col_groups") {
@@ -2207,6 +2215,10 @@ struct Engine : EngineBase {
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, false, false, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+        HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, false, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, cl));
         return MRMG_OK;
     }
     template <int MB> int lds_attr_split()
@@ -2523,6 +2535,10 @@ struct Engine : EngineBase {
     int set_slots(bool sym)
     {
         nslot = sym ? 6 : 9;
+        if (!sym && col_threads > 512) {   // the 1024-thread column kernel is built for 6 slots only
+            col_threads = 512;
+            RCHK(shapes());
+        }
         const int s6[6] = {0, 4, 8, 1, 2, 5};   // xx yy zz xy xz yz as c = 3a+b
         int s9[9];
         for (int i = 0; i < 9; i++) s9[i] = i;
@@ -2662,6 +2678,7 @@ struct Engine : EngineBase {
         const int nb8 = (nxh * kg + 7) / 8 * 8;
         RCHK(ev_begin(K_COL));
         if (nslot == 6) LAUNCH_E(maxb_col, launch_col_sym, nb8);
+        else if (col_threads > 512) return set_err(MRMG_ERR_STATE, "col_threads 1024 needs the 6-slot tensor");
         else LAUNCH_E(maxb_col, launch_col_gen, nb8);
         HIPCHK(hipGetLastError());
         RCHK(ev_end());
@@ -2743,7 +2760,14 @@ struct Engine : EngineBase {
     }
     template <int MB> void launch_col_sym(int nb8)
     {
-        if (tl && wv)
+        if (col_threads > 512) {
+            if (tl)
+                hipLaunchKernelGGL((k_col<T, MB, true, true, false, true>), dim3(nb8), dim3(col_threads), lds_col,
+                                   c->stream, S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
+            else
+                hipLaunchKernelGGL((k_col<T, MB, true, false, false, true>), dim3(nb8), dim3(col_threads), lds_col,
+                                   c->stream, S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
+        } else if (tl && wv)
             hipLaunchKernelGGL((k_col<T, MB, true, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
                                S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, kg, nb8, pcol, twy.p);
         else if (tl)

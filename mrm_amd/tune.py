@@ -16,7 +16,7 @@ SPACE = {
     "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
     "row_radix": ([4, 8, 16], ("row_fwd", "row_inv")),   # largest Stockham radix of the row FFTs
     "col_radix": ([4, 8, 16], ("col", "col_fwd", "col_inv")),
-    "col_threads": ([256, 512], ("col",)),
+    "col_threads": ([256, 512, 1024], ("col",)),
     "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)
     "lanes_per_grain": ([4, 8, 16], ("grain1", "grain2")),
 }

@@ -135,6 +135,30 @@ def test_column_contiguous_fused_path(oracle_mod, row_block):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("path,cr", [(0, 4), (2, 8), (2, 16)])
+def test_column_kernel_1024_threads(oracle_mod, path, cr):
+    """The 1024-thread column kernel (two product items per thread) gives the oracle's
+    demag field and the 512-thread kernel's, bit for bit, on both fused paths."""
+    med = media.make_media(64, 48, dz=[2e-7, 3e-7, 2e-7], n_lay=3, pitch=6e-7, seed=71)
+    s, gpu = pair(oracle_mod, med)
+    gpu.tune("col_path", path)
+    gpu.tune("col_radix", cr)
+    for k, m in enumerate(random_state(med, 2)):
+        s.set_state(k, m)
+        gpu.set_state(k, m)
+    hc5, hg5 = gpu.demag_field()
+    gpu.tune("col_threads", 1024)
+    assert gpu.get_tune("col_threads") == 1024
+    hc1, hg1 = gpu.demag_field()
+    hc0, hg0 = s.demag_field()
+    err = max(np.abs(hc0 - hc1).max(), np.abs(hg0 - hg1).max()) / np.abs(hc0).max()
+    say(f"col_threads 1024, path {path}, col radix {cr}: rel err {err:.2e}")
+    assert err <= 1e-12
+    assert np.array_equal(hc1, hc5) and np.array_equal(hg1, hg5)
+    with pytest.raises(abi.MrmgError):
+        gpu.tune("wave_fft", 1)          # the 1024-thread kernel has no wave-local form
+
+
 @pytest.mark.parametrize("path,rr,cr", [(0, 4, 8), (0, 8, 4), (1, 4, 4), (2, 8, 8), (2, 4, 16)])
 def test_fft_radix_choices(oracle_mod, path, rr, cr):
     """Row and column FFTs with a smaller largest radix (more, lighter Stockham passes)
