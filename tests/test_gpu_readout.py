@@ -200,3 +200,19 @@ def test_switching_fields_approach_stoner_wohlfarth():
     extrap = ratios[1] - (ratios[0] - ratios[1]) * q / (1.0 - q)
     say(f"extrapolated to zero ramp rate: {extrap.min():.4f} .. {extrap.max():.4f} of the SW field")
     assert np.abs(extrap - 1.0).max() < 5e-3
+
+
+def test_readout_argument_checks():
+    nx, ny, dx, dy, dz = 16, 16, 2e-7, 2e-7, 4e-7
+    g2g = np.ones((ny, nx), np.int32)
+    x = Llg(nx, ny, 1, dx, dy, [dz], demag=False)
+    x.set_layer(0, cell_layer(g2g, dx, dy, dz))
+    x.set_state(0, np.array([[0.0, 0.0, 1.0]]))
+    with pytest.raises(abi.MrmgError):
+        x.readback([0.0], [0.0], (0.0, 1e-7))          # widths must be > 0
+    with pytest.raises(ValueError):
+        x.readback([0.0], [0.0], (1e-7, 1e-7), weights=[1.0, 2.0])
+    assert x.readback(np.zeros(0), np.zeros(0), (1e-7, 1e-7)).size == 0
+    with pytest.raises(abi.MrmgError):
+        x.switch_record(0)                               # never armed
+    x.close()
