@@ -329,9 +329,13 @@ typedef struct {
 int mrmg_profile(mrmg_ctx *ctx, int enable);
 /* Launch shapes of the kernels, chosen by the caller or by mrm_amd's autotuner (the role
  * FFTW_EXHAUSTIVE planning plays for the reference, fftw.f90:25): "row_threads" (64, 128,
- * 256), "col_threads" (256, 512), "col_groups" (observation-layer groups per column
- * workgroup, 1..n_lay), "lanes_per_grain" (4, 8, 16).  Results do not depend on them
- * except "lanes_per_grain", which changes the order of the gather sums (last bits). */
+ * 256), "col_threads" (256, 512, or 1024 with the 6-slot tensor and without wave-local
+ * FFTs), "col_groups" (observation-layer groups per column workgroup, 1..n_lay),
+ * "col_path" (0 fused, 1 split, 2 fused on column-contiguous spectra), "row_block" (2, 4,
+ * 8 rows per transposing row workgroup), "row_radix" / "col_radix" (largest Stockham
+ * radix: 4, 8, 16), "wave_fft" (0, 1: wave-local FFTs), "lanes_per_grain" (4, 8, 16).
+ * Results do not depend on them except "lanes_per_grain", which changes the order of
+ * the gather sums (last bits).  A combination that does not fit the grid is refused. */
 int mrmg_tune(mrmg_ctx *ctx, const char *key, int value);
 int mrmg_get_tune(mrmg_ctx *ctx, const char *key, int *value);
 /* Copies up to max entries; *n receives the number of kernel classes. */
