@@ -334,8 +334,10 @@ int mrmg_profile(mrmg_ctx *ctx, int enable);
  * "col_path" (0 fused, 1 split, 2 fused on column-contiguous spectra), "row_block" (2, 4,
  * 8 rows per transposing row workgroup), "row_radix" / "col_radix" (largest Stockham
  * radix: 4, 8, 16), "wave_fft" (0, 1: wave-local FFTs), "lanes_per_grain" (4, 8, 16).
- * Results do not depend on them except "lanes_per_grain", which changes the order of
- * the gather sums (last bits).  A combination that does not fit the grid is refused. */
+ * The thread counts, "col_groups" and "row_block" leave results unchanged bit for bit;
+ * the others may change the order of floating-point operations (FFT factorisation,
+ * gather sums) and so the last bits.  A combination that does not fit the grid is
+ * refused. */
 int mrmg_tune(mrmg_ctx *ctx, const char *key, int value);
 int mrmg_get_tune(mrmg_ctx *ctx, const char *key, int *value);
 /* Copies up to max entries; *n receives the number of kernel classes. */
