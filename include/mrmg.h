@@ -308,7 +308,8 @@ int mrmg_readback(mrmg_ctx *ctx, const mrmg_sensor *s, int n_pos, const double *
  * satisfies dir . m > thr gets that step's counter (step0 + i), its end time and the
  * applied field there (the grain's static field plus the uniform field of
  * mrmg_set_ufield at that time; a moving source's field is not included).  enable 0
- * stops marking; the record stays readable.  E.g. dir = (0,0,-1), thr = 0 under a
+ * stops marking; the record stays readable until the next mrmg_set_layer, which ends
+ * and discards it (the grains change).  E.g. dir = (0,0,-1), thr = 0 under a
  * reversing field gives each grain's first crossing of m_z = 0, and the switching-field
  * distribution from h_app. */
 int mrmg_switch_arm(mrmg_ctx *ctx, int enable, const double dir[3], double thr);

@@ -2250,6 +2250,12 @@ struct Engine : EngineBase {
     {
         const int nl = c->nl, nx = c->nx, ny = c->ny;
         const size_t P = (size_t)nx * ny;
+        // a switching record refers to the grains of the previous build (and is sized for
+        // them): a rebuild ends it
+        sw_on = false;
+        sw_step.release();
+        sw_t.release();
+        sw_h.release();
         off.assign(nl + 1, 0);
         for (int k = 0; k < nl; k++) off[k + 1] = off[k] + L[k].ng;
         G = off[nl];

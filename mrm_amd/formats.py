@@ -331,7 +331,8 @@ def _last_slot(directory):
     if not os.path.exists(p):
         return 1                                   # idd starts at 1 (checkpoint.f90:10)
     try:
-        return int(open(p).read().split()[2])
+        with open(p) as f:
+            return int(f.read().split()[2])
     except (IndexError, ValueError):
         return 1
 
@@ -359,8 +360,11 @@ def write_checkpoint(directory, time, iline, mag_layers, generators, grain_layer
             for m, mag in exact["layers"]:
                 m = np.asarray(m, np.float64).reshape(-1, 3)
                 w.write(len(m), m, np.asarray(mag, np.float64).reshape(len(m)))
-    with open(os.path.join(directory, CHK_FILES[0]), "w") as f:
-        f.write(f" {float(time):.17E} {int(iline):11d} {idd:11d}\n")   # list-directed time, iline, idd
+    # list-directed time, iline, idd; replaced in one rename so it never names a slot half-written
+    tmp = os.path.join(directory, CHK_FILES[0] + ".tmp")
+    with open(tmp, "w") as f:
+        f.write(f" {float(time):.17E} {int(iline):11d} {idd:11d}\n")
+    os.replace(tmp, os.path.join(directory, CHK_FILES[0]))
     return idd
 
 

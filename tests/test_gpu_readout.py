@@ -215,4 +215,13 @@ def test_readout_argument_checks():
     assert x.readback(np.zeros(0), np.zeros(0), (1e-7, 1e-7)).size == 0
     with pytest.raises(abi.MrmgError):
         x.switch_record(0)                               # never armed
+    # a new layer (more grains) ends the record: stepping marks nothing and reading refuses
+    x.switch_arm()
+    x.step(2)
+    g2 = (np.arange(nx * ny, dtype=np.int32).reshape(ny, nx) + 1).astype(np.int32)
+    x.set_layer(0, cell_layer(g2, dx, dy, dz))
+    x.set_state(0, np.tile([0.0, 0.0, -1.0], (nx * ny, 1)))
+    x.step(2)
+    with pytest.raises(abi.MrmgError):
+        x.switch_record(0)
     x.close()
