@@ -3025,9 +3025,16 @@ struct Engine : EngineBase {
         HIPCHK(hipMemsetAsync(hthm.p, 0, hthm.bytes(), c->stream));
         // the cells must hold the current state before the first field evaluation
         if (demag_on()) RCHK(scatter(ma.p, &p->t0));
-        const int chunk = 64;
+        // injected normals go up in chunks of steps; Philox normals are drawn a chunk of
+        // steps ahead by one grid-wide kernel (the same values the grain kernel would draw)
+        // and then read like injected ones, which takes the draw off the grain kernel
+        const bool pre = p->noise == MRMG_NOISE_PHILOX && p->n_steps > 1;
+        const int chunk = pre ? 16 : 64;
         const size_t per = 3 * (size_t)G;
-        if (p->noise == MRMG_NOISE_INJECT) RCHK(noise_buf.alloc(per * std::min(chunk, std::max(p->n_steps, 1))));
+        if (p->noise == MRMG_NOISE_INJECT || pre) {
+            const size_t need = per * std::min(chunk, std::max(p->n_steps, 1));
+            if (noise_buf.n < need) RCHK(noise_buf.alloc(need));
+        }
         double t = p->t0;
         const double dt = p->dt;
         StepArgs a{};
@@ -3035,18 +3042,26 @@ struct Engine : EngineBase {
         a.dt = dt;
         a.sqrt_dt = sqrt(dt);
         a.type = type;
-        a.noise = p->noise;
+        a.noise = pre ? MRMG_NOISE_INJECT : p->noise;
         a.seed = p->seed;
         for (int it = 0; it < p->n_steps; it++) {
             const bool last = it == p->n_steps - 1;
             a.step = p->step0 + it;
             a.normals = nullptr;
-            if (p->noise == MRMG_NOISE_INJECT) {
+            if (p->noise == MRMG_NOISE_INJECT || pre) {
                 const int ci = it % chunk;
                 if (ci == 0) {
                     const int n = std::min(chunk, p->n_steps - it);
-                    HIPCHK(hipMemcpyAsync(noise_buf.p, noise + per * it, sizeof(double) * per * n,
-                                          hipMemcpyHostToDevice, c->stream));
+                    if (pre) {
+                        const size_t cnt = (size_t)n * G;
+                        hipLaunchKernelGGL(k_philox_dump, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream,
+                                           (unsigned long long)p->seed, (unsigned long long)(p->step0 + it), n, G,
+                                           noise_buf.p);
+                        HIPCHK(hipGetLastError());
+                    } else {
+                        HIPCHK(hipMemcpyAsync(noise_buf.p, noise + per * it, sizeof(double) * per * n,
+                                              hipMemcpyHostToDevice, c->stream));
+                    }
                 }
                 a.normals = noise_buf.p + per * ci;
             }

@@ -47,7 +47,8 @@ def footprint_estimate(med, precision):
     spectra = 2 * 3 * nl * kc * 2 * P
     cells = 2 * 3 * med.cells * P
     grains = 40 * 8 * med.G          # grain state and constants are FP64 in both forms
-    return tensor + spectra + cells + grains + (1 << 20)
+    noise = 16 * 3 * 8 * med.G       # Philox normals drawn 16 steps ahead
+    return tensor + spectra + cells + grains + noise + (1 << 20)
 
 
 class Server:
