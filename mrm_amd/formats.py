@@ -408,7 +408,7 @@ def read_checkpoint(directory, n_lay):
             continue
         try:
             out = _read_slot(p, n_lay)
-        except (ValueError, EOFError):
+        except (ValueError, EOFError, struct.error):
             continue
         out.update(slot=idd, grain=grain)
         return out
