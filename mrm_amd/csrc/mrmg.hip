@@ -3676,7 +3676,7 @@ int mrmg_readback(mrmg_ctx* c, const mrmg_sensor* s, int n_pos, const double* xp
     if (n_pos < 0) return set_err(MRMG_ERR_ARG, "n_pos < 0");
     if (n_pos > 0 && (!xpos || !ypos || !out)) return set_err(MRMG_ERR_ARG, "null position or output array");
     if (!(s->sigma_x > 0.0) || !(s->sigma_y > 0.0)) return set_err(MRMG_ERR_ARG, "sensor widths must be > 0");
-    if ((size_t)(c->nx + c->ny) * sizeof(double) > 64 * 1024)
+    if ((size_t)(c->nx + c->ny) * sizeof(double) > 64 * 1024 - 64)   // with the kernel's static reduction array
         return set_err(MRMG_ERR_UNSUPPORTED, "readback: nx + ny too large for one workgroup's table");
     RCHK(ensure_built(c));
     if (n_pos == 0) return MRMG_OK;
