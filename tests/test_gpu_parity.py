@@ -308,6 +308,28 @@ def test_cfg1_full_size_parity(oracle_mod):
     assert max_dm(s, g2, 4) <= 1e-10
 
 
+def test_cfg1_full_size_long_run(oracle_mod):
+    """Configuration 1 at full size for 100 Heun steps (5 ps) at T = 300 K with the
+    reference's MT noise, split over four calls: the trajectories stay within 1e-10."""
+    say("cfg1 full size, 100 steps")
+    med = media.config_media("cfg1", thm_alpha=1.0)
+    gpu = Llg(256, 256, 4, med.dx, med.dy, med.dz)
+    N = gpu.build_tensor(num_images=2, return_real=True)
+    s, g2 = pair(oracle_mod, med, tensor=N, ufields=media.ramp_field(4))
+    for k, m in enumerate(random_state(med, 12, tilt=0.2)):
+        s.set_state(k, m)
+        g2.set_state(k, m)
+    noise = oracle_mod.mt_noise([34, 35, 36, 37], med.ngs, [True] * 4, 100)
+    g2.set_llg(dt=5e-14)
+    t = 0.0
+    for c in range(4):
+        s.step(25, 5e-14, t0=t, noise=noise[25 * c:25 * (c + 1)])
+        t = g2.step(25, t0=t, noise=noise[25 * c:25 * (c + 1)]).t
+    d = max_dm(s, g2, 4)
+    say(f"  max|dm| after 100 steps: {d:.2e}")
+    assert d <= 1e-10
+
+
 def test_state_survives_layer_rebuild(oracle_mod):
     """set_layer after steps keeps the other layers' device state (the host copy is synced)."""
     med = media.config_media("cfg1", nx=32, ny=32)
