@@ -1880,6 +1880,7 @@ struct EngineBase {
     virtual int readback(const mrmg_sensor* s, int n_pos, const double* xpos, const double* ypos, double* out) = 0;
     virtual int switch_arm(int enable, const double* dir, double thr) = 0;
     virtual int switch_read(int k, int64_t* step, double* t, double* hap) = 0;
+    virtual int time_class(int cls, int reps, double* ms) = 0;
 };
 
 struct mrmg_ctx {
@@ -2670,63 +2671,90 @@ struct Engine : EngineBase {
         return MRMG_OK;
     }
 
+    // one launch of a demag kernel class (K_ROW_FWD, K_COL, K_ROW_INV on the fused paths;
+    // K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV on the split path)
+    int launch_class(int cls)
+    {
+        const int ny = c->ny, nl = c->nl, nxh = c->nxh;
+        const dim3 grow((ny + 1) / 2, nl), growT(rowT_blocks()), blk(row_threads), b256(256);
+        const int ncols = 3 * nl * nxh;
+        const dim3 gcol((ncols + CBc - 1) / CBc);
+        switch (cls) {
+        case K_ROW_FWD:
+            if (tl || split) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
+            else LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
+            break;
+        case K_COL: {
+            if (split) return set_err(MRMG_ERR_STATE, "the split path has no fused column kernel");
+            const int nb8 = (nxh * kg + 7) / 8 * 8;
+            if (nslot == 6) LAUNCH_E(maxb_col, launch_col_sym, nb8);
+            else if (col_threads > 512) return set_err(MRMG_ERR_STATE, "col_threads 1024 needs the 6-slot tensor");
+            else LAUNCH_E(maxb_col, launch_col_gen, nb8);
+            break;
+        }
+        case K_COLF:
+        case K_COLI:
+            if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate column FFTs");
+            if (cls == K_COLF) LAUNCH_E(maxb_colT, launch_colT_fwd, gcol, b256);
+            else LAUNCH_E(maxb_colT, launch_colT_inv, gcol, b256);
+            break;
+        case K_PROD: {
+            if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate product kernel");
+            const size_t Kc = (size_t)nxh * ny;
+            const dim3 gp((unsigned)((Kc + 255) / 256));
+            if (nslot == 6)
+                hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
+            else
+                hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh,
+                                   nslot);
+            break;
+        }
+        case K_ROW_INV:
+            if (tl || split) LAUNCH_E(maxb_rowT, launch_rowT_inv, growT, blk);
+            else LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
+            break;
+        default:
+            return set_err(MRMG_ERR_ARG, "not a demag kernel class");
+        }
+        HIPCHK(hipGetLastError());
+        return MRMG_OK;
+    }
+
     // get_demag: row r2c of the cell magnetisation, column FFT * tensor, row c2r
     int demag()
     {
-        if (split) return demag_split();
-        const int ny = c->ny, nl = c->nl, nxh = c->nxh;
-        dim3 grow((ny + 1) / 2, nl), growT(rowT_blocks()), blk(row_threads);
-        RCHK(ev_begin(K_ROW_FWD));
-        if (tl) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
-        else LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        const int nb8 = (nxh * kg + 7) / 8 * 8;
-        RCHK(ev_begin(K_COL));
-        if (nslot == 6) LAUNCH_E(maxb_col, launch_col_sym, nb8);
-        else if (col_threads > 512) return set_err(MRMG_ERR_STATE, "col_threads 1024 needs the 6-slot tensor");
-        else LAUNCH_E(maxb_col, launch_col_gen, nb8);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        RCHK(ev_begin(K_ROW_INV));
-        if (tl) LAUNCH_E(maxb_rowT, launch_rowT_inv, growT, blk);
-        else LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
+        static const int fused[3] = {K_ROW_FWD, K_COL, K_ROW_INV};
+        static const int splitc[5] = {K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV};
+        const int* seq = split ? splitc : fused;
+        const int n = split ? 5 : 3;
+        for (int i = 0; i < n; i++) {
+            RCHK(ev_begin(seq[i]));
+            RCHK(launch_class(seq[i]));
+            RCHK(ev_end());
+        }
         return MRMG_OK;
     }
-    // the split path over column-contiguous spectra
-    int demag_split()
+
+    // reps back-to-back launches of one demag kernel class between two events on the
+    // context's stream: the kernel's average duration without a per-launch event pair.
+    // Only the scratch spectra and cell fields are written; the state is untouched.
+    int time_class(int cls, int reps, double* ms) override
     {
-        const int ny = c->ny, nl = c->nl, nxh = c->nxh;
-        const int ncols = 3 * nl * nxh;
-        dim3 grow(rowT_blocks()), blk(row_threads);
-        RCHK(ev_begin(K_ROW_FWD));
-        LAUNCH_E(maxb_rowT, launch_rowT_fwd, grow, blk);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        const dim3 gcol((ncols + CBc - 1) / CBc), b256(256);
-        RCHK(ev_begin(K_COLF));
-        LAUNCH_E(maxb_colT, launch_colT_fwd, gcol, b256);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        const size_t Kc = (size_t)nxh * ny;
-        RCHK(ev_begin(K_PROD));
-        const dim3 gp((unsigned)((Kc + 255) / 256));
-        if (nslot == 6)
-            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
-        else
-            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        RCHK(ev_begin(K_COLI));
-        LAUNCH_E(maxb_colT, launch_colT_inv, gcol, b256);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
-        RCHK(ev_begin(K_ROW_INV));
-        LAUNCH_E(maxb_rowT, launch_rowT_inv, grow, blk);
-        HIPCHK(hipGetLastError());
-        RCHK(ev_end());
+        if (!demag_on()) return set_err(MRMG_ERR_STATE, "demag is off or has no tensor");
+        struct Ev {
+            hipEvent_t e = nullptr;
+            ~Ev() { if (e) (void)hipEventDestroy(e); }
+        } a, b;
+        HIPCHK(hipEventCreate(&a.e));
+        HIPCHK(hipEventCreate(&b.e));
+        RCHK(launch_class(cls));   // warm
+        HIPCHK(hipEventRecord(a.e, c->stream));
+        for (int r = 0; r < reps; r++) RCHK(launch_class(cls));
+        HIPCHK(hipEventRecord(b.e, c->stream));
+        HIPCHK(hipEventSynchronize(b.e));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, a.e, b.e));
+        *ms = (double)t / reps;
         return MRMG_OK;
     }
     // transposing row kernels: (ny/TY) * 3 n_lay blocks on a 1D grid padded to a multiple of 8
@@ -3720,6 +3748,19 @@ int mrmg_kernel_stats(mrmg_ctx* c, mrmg_kstat* out, int max, int* n)
 {
     if (!c || !out || !n) return set_err(MRMG_ERR_ARG, "null argument");
     return c->eng->kernel_stats(out, max, n);
+}
+
+int mrmg_time_kernel(mrmg_ctx* c, const char* name, int reps, double* ms)
+{
+    if (!c || !name || !ms) return set_err(MRMG_ERR_ARG, "null argument");
+    if (reps < 1) return set_err(MRMG_ERR_ARG, "reps must be >= 1");
+    static const char* names[] = {"row_fwd", "col", "row_inv", "", "", "col_fwd", "prod", "col_inv"};
+    int cls = -1;
+    for (int i = 0; i < 8; i++)
+        if (names[i][0] && strcmp(names[i], name) == 0) cls = i;
+    if (cls < 0) return set_err(MRMG_ERR_ARG, "unknown demag kernel class '%s'", name);
+    RCHK(ensure_built(c));
+    return c->eng->time_class(cls, reps, ms);
 }
 
 int mrmg_copy_probe(mrmg_ctx* c, size_t bytes, int reps, double* ms)

@@ -120,6 +120,12 @@ class Llg:
         abi.check(self.lib, self.lib.mrmg_copy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
         return ms.value
 
+    def time_kernel(self, name, reps=50):
+        """Average duration (ms) of `reps` back-to-back launches of one demag kernel class."""
+        ms = C.c_double()
+        abi.check(self.lib, self.lib.mrmg_time_kernel(self.ctx, name.encode(), int(reps), C.byref(ms)))
+        return ms.value
+
     def tune(self, key, value):
         abi.check(self.lib, self.lib.mrmg_tune(self.ctx, key.encode(), int(value)))
 

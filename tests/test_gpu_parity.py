@@ -423,3 +423,36 @@ def test_step_rejects_bad_arguments():
     y.set_llg(dt=5e-14, llg_type=7)
     with pytest.raises(abi.MrmgError):
         y.step(1)
+
+
+def test_time_kernel_leaves_state_alone():
+    """mrmg_time_kernel times each demag class of the active path by back-to-back launches
+    and does not change the trajectory; classes of the other path are refused."""
+    med = media.config_media("cfg1", nx=64, ny=64)
+    runs = []
+    for timed in (False, True):
+        x = Llg(64, 64, 4, med.dx, med.dy, med.dz)
+        x.build_tensor(num_images=1)
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(4)):
+            x.set_ufield(k, u)
+        x.set_llg(dt=5e-14)
+        x.tune("col_path", 2)
+        x.step(5, philox_seed=3)
+        if timed:
+            for name in ("row_fwd", "col", "row_inv"):
+                ms = x.time_kernel(name, reps=5)
+                say(f"  {name}: {ms * 1e3:.2f} us")
+                assert 0.0 < ms < 10.0
+            with pytest.raises(abi.MrmgError):
+                x.time_kernel("prod")
+            with pytest.raises(abi.MrmgError):
+                x.time_kernel("grain1")
+            x.tune("col_path", 1)
+            for name in ("col_fwd", "prod", "col_inv"):
+                assert x.time_kernel(name, reps=3) > 0.0
+            x.tune("col_path", 2)
+        x.step(5, t0=5 * 5e-14, philox_seed=3)
+        runs.append([x.get_state(k)[0] for k in range(4)])
+        x.close()
+    assert all(np.array_equal(p, q) for p, q in zip(*runs))
