@@ -177,6 +177,20 @@ def track_form(log, steps=100, warmup=10):
         el, _ = timed(x, steps, warmup * DT)
         out[name] = {"value": med.cells * steps / el, "ms_per_step": el * 1e3 / steps, "launch_shapes": shapes}
         log(f"track grid {name}: {el * 1e3 / steps:.3f} ms/step, {med.cells * steps / el:.4e} cell-updates/s")
+        # per-kernel event-pair durations and algorithmic bandwidth in this bandwidth-bound regime
+        x.profile(True)
+        x.step(20, t0=(warmup + steps) * DT, philox_seed=SEED, diag=False)
+        st = x.kernel_stats()
+        x.profile(False)
+        kb = kernel_bytes(med, 8 if name == "fp64" else 4)
+        ks = {}
+        for k, (n, ms) in st.items():
+            if n > 0:
+                us = 1e3 * ms / n
+                ks[k] = {"avg_us": round(us, 2), "gbs": round(kb[k] / (us * 1e-6) / 1e9, 1),
+                         "frac": round(kb[k] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3)}
+        out[name]["kernels"] = ks
+        log(f"track grid {name} kernels: " + ", ".join(f"{k} {v['avg_us']}us {v['frac']}" for k, v in ks.items()))
     out["speedup_fp32"] = out["fp32"]["value"] / out["fp64"]["value"]
     dmax, drms = max_rms(x64, x32, med.n_lay)
     out["max_dm_fp32_vs_fp64"], out["rms_dm_fp32_vs_fp64"] = dmax, drms
