@@ -24,5 +24,29 @@ def build(verbose=False):
     return OUT
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "mrmg_drive")
+
+
+def build_fortran_driver(verbose=False):
+    """The Fortran binding module and its example driver, linked against libmrmg.so with
+    an rpath relative to the executable.  Skipped (returns None) without flang."""
+    if not os.path.exists(FLANG):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [FLANG, "-O2", "-J", d, "-o", DRIVER + ".tmp", os.path.join(FORTRAN, "mrmg_m.f90"),
+               os.path.join(FORTRAN, "mrmg_drive.f90"), "-L" + HERE, "-lmrmg", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=d)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stderr[-4000:])
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran_driver(verbose=True)

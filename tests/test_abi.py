@@ -139,3 +139,15 @@ def test_fortran_binding_types_match_header():
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     want = [C.sizeof(t) for t in (abi.Geom, abi.Layer, abi.UField, abi.StepParams, abi.Diag, abi.Sensor)]
     assert [int(v) for v in out] == want
+
+
+def test_fortran_driver_reports_errors(tmp_path):
+    """Without a device the Fortran driver's first call fails cleanly: the status comes
+    back through the binding and mrmg_error() gives the library's message."""
+    drv = os.path.join(ROOT, "mrm_amd", "fortran", "mrmg_drive")
+    if not os.path.exists(drv):
+        pytest.skip("the Fortran driver is not built (no flang)")
+    r = subprocess.run([drv], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    if r.returncode == 0:
+        pytest.skip("a device is present: the driver ran (covered by the GPU tests)")
+    assert "mrmg_create failed" in r.stdout and "device" in r.stdout
