@@ -15,7 +15,8 @@ any nonzero read as true), REAL(8) is 8 bytes, arrays are in Fortran order.
 * The magnetostatic tensor cache mrm_N.dat (tensor.f90:52-80): a header record with the
   geometry and image options, then n2d (nx,ny,9,n_lay,n_lay).
 * The binary magnetisation of WRITE_MAG (mag.f90:285-290): per layer (nx, ny, n, ms_nom)
-  and Ms * m as (3,n).
+  and Ms * m as (3,n); its formatted (tecplot) form and the old REAL(4) per-cell form
+  (mag.f90:254-283).
 * The checkpoint trio (checkpoint.f90:14-149): mrm_grain.chk (the grain microstructure),
   mrm_0.chk / mrm_1.chk written in turn ((time, iline), the magnetisation, the random
   generators of save_random, random.f90:38-64), and the formatted mrm.chk naming the
@@ -266,6 +267,102 @@ def read_mag(path, n_lay):
     reproduced here: the per-grain magnitude is returned instead."""
     with open(path, "rb") as f:
         return _read_mag_records(RecordReader(f), n_lay)
+
+
+def write_mag_ascii(path, layers, old_tran=0):
+    """WRITE_MAG to a formatted file (mag.f90:254-270): per layer a tecplot ZONE header
+    and one line per cell, x fastest: i, j, layer, then Ms*ms_scale*m of the cell's grain
+    (zeros in non-magnetic cells) in 1PE12.5.  layers: list of (grid2grain (ny,nx), 1-based
+    grain ids with 0 for none; m (ng,3); ms_eff (ng) = Ms * ms_scale)."""
+    with open(path, "w") as f:
+        for k, (g2g, m, ms_eff) in enumerate(layers, start=1):
+            g2g = np.asarray(g2g)
+            ny, nx = g2g.shape
+            v = np.asarray(m, np.float64).reshape(-1, 3) * np.asarray(ms_eff, np.float64)[:, None]
+            f.write(f'ZONE T="{float(old_tran):9.2f}", I={nx:5d}, J={ny:5d}, K={1:3d} F=POINT\n')
+            for j in range(ny):
+                for i in range(nx):
+                    n = int(g2g[j, i])
+                    c = v[n - 1] if n > 0 else (0.0, 0.0, 0.0)
+                    f.write(f"{i + 1:5d}{j + 1:5d}{k:5d}" + "".join(f" {float(x):12.5E}" for x in c) + "\n")
+
+
+def read_mag_ascii(path, n_lay):
+    """The formatted magnetisation as the reference reads it (mag.f90:191-209): per layer
+    (nx, ny, cells (ny,nx,3)), the three values of each cell line as written."""
+    out = []
+    with open(path) as f:
+        for _ in range(n_lay):
+            hd = f.readline()
+            nx, ny = int(hd[22:27]), int(hd[31:36])     # "(22x,i5,4x,i5)"
+            cells = np.zeros((ny, nx, 3))
+            for j in range(ny):
+                for i in range(nx):
+                    line = f.readline()
+                    cells[j, i] = [float(line[15 + 13 * c + 1:15 + 13 * (c + 1)]) for c in range(3)]
+            out.append((nx, ny, cells))
+    return out
+
+
+def grain_directions(g2g, cells, ng):
+    """Unit m per grain from a cell array, as the reference's ASCII reader sets it: every
+    cell of a grain overwrites the grain's m with its normalised value, so the last cell
+    in reading order (x fastest) wins; a grain whose cells are all zero keeps NaN."""
+    m = np.full((ng, 3), np.nan)
+    g2g = np.asarray(g2g)
+    for j in range(g2g.shape[0]):
+        for i in range(g2g.shape[1]):
+            n = int(g2g[j, i])
+            if n > 0:
+                v = cells[j, i]
+                mg = float(v @ v)
+                if mg != 0.0:
+                    m[n - 1] = v / np.sqrt(mg)
+    return m
+
+
+def write_mag_old(path, layers):
+    """The old binary magnetisation (mag.f90:272-283): per layer one record of REAL(4)
+    (3,nx,ny), the grain's unit m in every cell and zeros elsewhere.  layers: list of
+    (grid2grain (ny,nx), m (ng,3))."""
+    with open(path, "wb") as f:
+        w = RecordWriter(f)
+        for g2g, m in layers:
+            g2g = np.asarray(g2g)
+            m = np.asarray(m, np.float64).reshape(-1, 3)
+            a = np.zeros(g2g.shape + (3,), np.float32)
+            on = g2g > 0
+            a[on] = m[g2g[on] - 1].astype(np.float32)
+            w.write_bytes(a.astype("<f4").tobytes())
+
+
+def read_mag_old(path, grid2grains):
+    """The old binary magnetisation as the reference reads it (mag.f90:212-229): each
+    magnetic cell sets its grain's m to the REAL(4) vector divided by its squared norm
+    (dot_product, not its norm: a quirk of the reference, harmless for unit vectors to
+    within REAL(4) rounding); the last cell of a grain wins.  Returns per layer m (ng,3)."""
+    out = []
+    with open(path, "rb") as f:
+        r = RecordReader(f)
+        for g2g in grid2grains:
+            g2g = np.asarray(g2g)
+            ny, nx = g2g.shape
+            data = r.read_bytes()
+            if len(data) != 12 * nx * ny:
+                raise ValueError("old magnetisation record of unexpected length")
+            a = np.frombuffer(data, "<f4").reshape(ny, nx, 3).astype(np.float64)
+            ng = int(g2g.max())
+            m = np.full((ng, 3), np.nan)
+            for j in range(ny):
+                for i in range(nx):
+                    n = int(g2g[j, i])
+                    if n > 0:
+                        v = a[j, i]
+                        mg = float(v @ v)
+                        if mg > 0.0:
+                            m[n - 1] = v / mg
+            out.append(m)
+    return out
 
 
 # ------------------------------------------------------------------ random generators

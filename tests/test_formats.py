@@ -376,3 +376,112 @@ def test_checkpoint_against_fortran_compiler():
         assert np.array_equal(g["mt"], 1000 * np.arange(624) - 7)
         np.testing.assert_allclose(c["mag"][0][3], [[1, 2, 3] / np.sqrt(14), [0, 0, -1]], rtol=0, atol=1e-16)
         np.testing.assert_allclose(c["mag"][0][4], [np.sqrt(14), 5])
+
+
+ASCII_WRITER = """
+program w
+  implicit none
+  integer :: i, j, k
+  real(8) :: v(3)
+  open(10, file='mag_f.txt', form='formatted', status='replace')
+  k = 1
+  write(10, "('ZONE T=""',f9.2,'"", I=',i5,', J=',i5,', K=',i3,' F=POINT')") real(0), 3, 2, 1
+  do j = 1, 2
+    do i = 1, 3
+      v = (/ 100.d0 * i - 0.125d0 * j, -3.5d-2 * i, 6.d2 + j /)
+      if (i == 2 .and. j == 1) v = 0.d0
+      write(10, "(3(i5),1p,7(1x,e12.5))") i, j, k, v
+    end do
+  end do
+  close(10)
+end program
+"""
+
+ASCII_READER = """
+program r
+  implicit none
+  integer :: i, j, n, ll, nx, ny
+  real(8) :: mm(3)
+  open(10, file='mag_p.txt', form='formatted', status='old')
+  read(10, "(22x,i5,4x,i5)") nx, ny
+  print '(i0,1x,i0)', nx, ny
+  do j = 1, ny
+    do i = 1, nx
+      read(10, "((3(i5),3(x,e12.5)))") n, ll, ll, mm
+      print '(3(i0,1x),3(es14.6,1x))', n, ll, i, mm
+    end do
+  end do
+  close(10)
+end program
+"""
+
+OLD_READER = """
+program r
+  implicit none
+  real(4) :: a(3, 3, 2)
+  open(10, file='mag_old.bin', form='unformatted', status='old')
+  read(10) a
+  close(10)
+  print '(18(es14.6,1x))', a
+end program
+"""
+
+
+@pytest.mark.skipif(FLANG is None, reason="no Fortran compiler in this image")
+def test_mag_ascii_and_old_forms_against_fortran_compiler():
+    """The formatted WRITE_MAG form written by the reference's format statements reads
+    back here, one written here reads with the reference's read formats, and the old
+    REAL(4) form written here reads with a Fortran REAL(4) read."""
+    g2g = np.array([[1, 0, 2], [1, 3, 2]], np.int32)          # (ny=2, nx=3), 1-based grains
+    m = np.array([[0.6, 0.0, 0.8], [0.0, 1.0, 0.0], [0.0, 0.0, -1.0]])
+    ms_eff = np.array([700.0, 650.0, 512.5])
+    with tempfile.TemporaryDirectory() as d:
+        for name, src in (("w", ASCII_WRITER), ("r", ASCII_READER), ("o", OLD_READER)):
+            open(os.path.join(d, f"{name}.f90"), "w").write(src)
+            subprocess.run([FLANG, "-o", os.path.join(d, name), os.path.join(d, f"{name}.f90")], check=True,
+                           capture_output=True)
+        subprocess.run([os.path.join(d, "w")], cwd=d, check=True)
+        (nx, ny, cells), = formats.read_mag_ascii(os.path.join(d, "mag_f.txt"), 1)
+        assert (nx, ny) == (3, 2)
+        for j in range(2):
+            for i in range(3):
+                want = [100.0 * (i + 1) - 0.125 * (j + 1), -3.5e-2 * (i + 1), 600.0 + j + 1]
+                if (i, j) == (1, 0):
+                    want = [0.0, 0.0, 0.0]
+                np.testing.assert_allclose(cells[j, i], want, rtol=5e-6, atol=0)
+        formats.write_mag_ascii(os.path.join(d, "mag_p.txt"), [(g2g, m, ms_eff)])
+        out = subprocess.run([os.path.join(d, "r")], cwd=d, check=True, capture_output=True,
+                             text=True).stdout.splitlines()
+        assert out[0].split() == ["3", "2"]
+        vals = np.array([[float(v) for v in line.split()[3:]] for line in out[1:]]).reshape(2, 3, 3)
+        want = np.zeros((2, 3, 3))
+        on = g2g > 0
+        want[on] = (m * ms_eff[:, None])[g2g[on] - 1]
+        np.testing.assert_allclose(vals, want, rtol=5e-6, atol=0)
+        back = formats.grain_directions(g2g, vals, 3)
+        np.testing.assert_allclose(back, m, rtol=0, atol=1e-5)
+        formats.write_mag_old(os.path.join(d, "mag_old.bin"), [(g2g, m)])
+        out = subprocess.run([os.path.join(d, "o")], cwd=d, check=True, capture_output=True, text=True).stdout
+        a = np.array([float(v) for v in out.split()]).reshape(2, 3, 3)          # Fortran a(3,nx,ny)
+        np.testing.assert_array_equal(a, want / np.where(on, ms_eff[np.maximum(g2g, 1) - 1], 1.0)[..., None]
+                                      .astype(np.float32))
+        old = formats.read_mag_old(os.path.join(d, "mag_old.bin"), [g2g])[0]
+        np.testing.assert_allclose(old, m, rtol=0, atol=1e-7)
+
+
+def test_mag_ascii_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    g2g = rng.integers(0, 6, (5, 7)).astype(np.int32)
+    g2g[0, 0] = 5
+    ng = 5
+    m = rng.standard_normal((ng, 3))
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    ms_eff = rng.uniform(300, 800, ng)
+    p = tmp_path / "m.txt"
+    formats.write_mag_ascii(p, [(g2g, m, ms_eff), (g2g, -m, ms_eff)])
+    layers = formats.read_mag_ascii(p, 2)
+    for sign, (nx, ny, cells) in zip((1, -1), layers):
+        assert (nx, ny) == (7, 5)
+        back = formats.grain_directions(g2g, cells, ng)
+        present = np.isin(np.arange(1, ng + 1), g2g)
+        np.testing.assert_allclose(back[present], sign * m[present], rtol=0, atol=2e-5)
