@@ -196,7 +196,9 @@ typedef struct {
     uint64_t step0;           /* Philox counter of the first step                       */
     double dt_frac;           /* 0: fixed dt (fixed_dt_p, llg.f90:1484-1486); else the
                                  gyro-period fraction of get_fields (llg.f90:384-386)    */
-    double dt_max;            /* upper bound of the adaptive dt (s)                     */
+    double dt_max;            /* upper bound of dt (s): get_fields applies it to the
+                                 adaptive step and to a fixed one (llg.f90:383-386);
+                                 > 0 with dt_frac != 0, <= 0 means none with a fixed dt  */
     double max_dm;            /* max rotation per step (degrees) before push_m rejects
                                  and retries the step (llg.f90:1229-1236)               */
     int llb;                  /* llgp%llb: Landau-Lifshitz-Bloch push (Heun)              */

@@ -3064,7 +3064,9 @@ struct Engine : EngineBase {
             if (noise_buf.n < need) RCHK(noise_buf.alloc(need));
         }
         double t = p->t0;
-        const double dt = p->dt;
+        // get_fields sets llgp%dt = min(dt, dt_max, dt) with a fixed step (fact1 = 0,
+        // fact0 = 1, llg.f90:383-386); dt_max <= 0 means no bound here
+        const double dt = p->dt_max > 0.0 ? std::min(p->dt, p->dt_max) : p->dt;
         StepArgs a{};
         a.llb = p->llb;
         a.dt = dt;

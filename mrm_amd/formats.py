@@ -307,17 +307,22 @@ def read_mag_ascii(path, n_lay):
 def grain_directions(g2g, cells, ng):
     """Unit m per grain from a cell array, as the reference's ASCII reader sets it: every
     cell of a grain overwrites the grain's m with its normalised value, so the last cell
-    in reading order (x fastest) wins; a grain whose cells are all zero keeps NaN."""
+    in reading order (x fastest) wins.  A zero vector on a magnetic cell, or a grain number
+    above ng, fails the read as the reference's does (ok_p false, mag.f90:203-205, which
+    then takes the error exit); a grain with no cells keeps NaN."""
     m = np.full((ng, 3), np.nan)
     g2g = np.asarray(g2g)
+    if g2g.size and int(g2g.max()) > ng:
+        raise ValueError(f"grain number {int(g2g.max())} above the layer's {ng} grains")
     for j in range(g2g.shape[0]):
         for i in range(g2g.shape[1]):
             n = int(g2g[j, i])
             if n > 0:
                 v = cells[j, i]
                 mg = float(v @ v)
-                if mg != 0.0:
-                    m[n - 1] = v / np.sqrt(mg)
+                if mg == 0.0:
+                    raise ValueError(f"zero magnetisation on magnetic cell ({i + 1},{j + 1}) of grain {n}")
+                m[n - 1] = v / np.sqrt(mg)
     return m
 
 
@@ -336,31 +341,37 @@ def write_mag_old(path, layers):
             w.write_bytes(a.astype("<f4").tobytes())
 
 
-def read_mag_old(path, grid2grains):
+def read_mag_old(path, grid2grains, ngs=None):
     """The old binary magnetisation as the reference reads it (mag.f90:212-229): each
     magnetic cell sets its grain's m to the REAL(4) vector divided by its squared norm
     (dot_product, not its norm: a quirk of the reference, harmless for unit vectors to
-    within REAL(4) rounding); the last cell of a grain wins.  Returns per layer m (ng,3)."""
+    within REAL(4) rounding); the last cell of a grain wins.  ngs: the grain count of each
+    layer (size(m,2)); default the largest grain number of its map.  A grain number above
+    it, or a zero vector on a magnetic cell, fails the read as in the reference (ok_p
+    false, then its error exit, mag.f90:219-226): ValueError.  Returns per layer m (ng,3)."""
     out = []
     with open(path, "rb") as f:
         r = RecordReader(f)
-        for g2g in grid2grains:
+        for il, g2g in enumerate(grid2grains):
             g2g = np.asarray(g2g)
             ny, nx = g2g.shape
             data = r.read_bytes()
             if len(data) != 12 * nx * ny:
                 raise ValueError("old magnetisation record of unexpected length")
             a = np.frombuffer(data, "<f4").reshape(ny, nx, 3).astype(np.float64)
-            ng = int(g2g.max())
+            ng = int(g2g.max()) if ngs is None else int(ngs[il])
             m = np.full((ng, 3), np.nan)
             for j in range(ny):
                 for i in range(nx):
                     n = int(g2g[j, i])
+                    if n > ng:
+                        raise ValueError(f"layer {il + 1}: grain number {n} above its {ng} grains")
                     if n > 0:
                         v = a[j, i]
                         mg = float(v @ v)
-                        if mg > 0.0:
-                            m[n - 1] = v / mg
+                        if not mg > 0.0:
+                            raise ValueError(f"layer {il + 1}: zero magnetisation on magnetic cell ({i + 1},{j + 1})")
+                        m[n - 1] = v / mg
             out.append(m)
     return out
 

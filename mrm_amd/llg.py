@@ -168,14 +168,19 @@ class Llg:
         if min_it is not None:
             self.min_it = max(1, int(min_it))
 
+    def fixed_dt(self):
+        """The fixed step get_fields takes: min(dt, dt_max) (fact0 = 1, llg.f90:383-386)."""
+        return min(self.dt, self.dt_max) if self.dt_max > 0.0 else self.dt
+
     def steps_for(self, t):
         """Steps an `LLG` command takes from time t with fixed dt (llg.f90:268-281).  The
         convergence clause of cont_p is always true in the reference (get_applied returns
         .true., applied.f90:1033), so the time/step rule alone ends the loop."""
         it, del_t = 0, 0.0
+        dt = self.fixed_dt()
         while True:
-            t = t + self.dt
-            del_t = del_t + self.dt
+            t = t + dt
+            del_t = del_t + dt
             it += 1
             if not ((del_t < self.del_t and t < max(self.t_max, self.t_min)) or it < self.min_it):
                 return it
@@ -199,6 +204,8 @@ class Llg:
         self.step_index += int(n_steps)
         if d is not None and self.dt_frac != 0.0:
             self.dt = d.dt          # llgp%dt carries over to the next call
+        elif self.dt_frac == 0.0 and n_steps > 0:
+            self.dt = self.fixed_dt()   # get_fields overwrote llgp%dt with the bounded step
         return d
 
     def llg(self, t, noise=None, philox_seed=None):

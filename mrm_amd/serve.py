@@ -36,6 +36,7 @@ class Request:
     # filled in by the server
     start: float = 0.0
     end: float = 0.0
+    footprint: int = 0          # device bytes its context held
     m: list = field(default_factory=list)
 
 
@@ -52,27 +53,104 @@ def footprint_estimate(med, precision):
 
 
 class Server:
+    """Admission by device footprint.  `used` counts every live context: the requests in
+    flight and the idle contexts kept in the pool for reuse.  A request that does not fit
+    first closes idle contexts, least recently used first, then waits for a running one
+    to finish."""
+
     def __init__(self, slots=8, mem_budget=64 << 30):
         self.slots = slots
         self.budget = mem_budget
         self.used = 0
         self.peak = 0
         self.cv = threading.Condition()
-        self.pool = {}            # (nx, ny, nl, precision, dz) -> [idle contexts]
+        self.pool = {}            # context key -> [(last use, estimate, context)] idle
+        self.clock = 0
+        self.admissions = []      # (bytes of every live context) at each admission
 
-    def _context(self, r):
-        key = (r.med.nx, r.med.ny, r.med.n_lay, r.precision, tuple(r.med.dz))
+    @staticmethod
+    def key(r):
+        """The grid, cell size, layer stack and precision decide a context's tensor and
+        buffers; the tensor's identity covers anything else it was built from."""
+        m = r.med
+        il = tuple(getattr(m, "il_space", None) or ())
+        return (m.nx, m.ny, m.n_lay, r.precision, float(m.dx), float(m.dy), tuple(m.dz), il, id(r.tensor))
+
+    def _idle(self):
+        return sum(len(v) for v in self.pool.values())
+
+    def _evict_lru(self, keep=None):
+        """Close the least recently used idle context other than `keep` (under the lock);
+        False if there is none."""
+        best = None
+        for k, lst in self.pool.items():
+            for i, (tl, need, x) in enumerate(lst):
+                if x is keep:
+                    continue
+                if best is None or tl < best[0]:
+                    best = (tl, k, i)
+        if best is None:
+            return False
+        _, k, i = best
+        _, need, x = self.pool[k].pop(i)
+        if not self.pool[k]:
+            del self.pool[k]
+        x.close()
+        self.used -= need
+        return True
+
+    def _admit(self, r, need):
+        """Take a pooled context of r's key (its bytes are already counted) or reserve
+        `need` bytes for a new one.  Returns the pooled context or None."""
+        key = self.key(r)
         with self.cv:
-            lst = self.pool.get(key)
-            if lst:
-                return key, lst.pop()
+            while True:
+                lst = self.pool.get(key)
+                if lst:
+                    # a pooled context counts its own estimate; the request's may be larger
+                    # (more grains), so the difference must fit as well
+                    _, old, x = lst[-1]
+                    if self.used + max(0, need - old) <= self.budget:
+                        lst.pop()
+                        if not lst:
+                            del self.pool[key]
+                        self.used += need - old
+                        self.peak = max(self.peak, self.used)
+                        self.admissions.append(self.used)
+                        return x
+                    if not self._evict_lru(keep=x):
+                        self.cv.wait()
+                    continue
+                if self.used + need <= self.budget:
+                    self.used += need
+                    self.peak = max(self.peak, self.used)
+                    self.admissions.append(self.used)
+                    return None
+                if not self._evict_lru():
+                    self.cv.wait()
+
+    def _context(self, r, x):
+        if x is not None:
+            return x
         med = r.med
         x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=r.precision)
         x.set_tensor(r.tensor)
-        return key, x
+        return x
 
-    def _run(self, r, t0):
-        key, x = self._context(r)
+    def _release(self, r, x, need):
+        """Keep the context idle in the pool (still counted) or close it."""
+        with self.cv:
+            if x is not None and self._idle() < self.slots:
+                self.clock += 1
+                self.pool.setdefault(self.key(r), []).append((self.clock, need, x))
+                x = None
+            else:
+                self.used -= need
+            self.cv.notify_all()
+        if x is not None:
+            x.close()
+
+    def _run(self, r, x, t0):
         x.set_media(r.med)
         for k, u in enumerate(r.ufields or media.ramp_field(r.med.n_lay)):
             x.set_ufield(k, u)
@@ -83,15 +161,9 @@ class Server:
         x.step_index = 0
         x.set_llg(dt=r.dt)
         x.step(r.n_steps, philox_seed=r.seed, diag=False)
+        r.footprint = x.footprint()
         r.m = [x.get_state(k)[0] for k in range(r.med.n_lay)]
         r.end = time.perf_counter() - t0
-        with self.cv:
-            # keep at most `slots` idle contexts resident; the rest give their memory back
-            if sum(len(v) for v in self.pool.values()) < self.slots:
-                self.pool.setdefault(key, []).append(x)
-                x = None
-        if x is not None:
-            x.close()
 
     def run(self, requests):
         """Serve the requests (sorted by arrival); returns wall seconds of the run."""
@@ -106,28 +178,27 @@ class Server:
                     if nxt[0] >= len(reqs):
                         return
                     r = reqs[nxt[0]]
-                    need = footprint_estimate(r.med, r.precision)
-                    if need > self.budget:
-                        nxt[0] += 1
-                        errors.append((r.rid, "larger than the memory budget"))
-                        continue
-                    while self.used + need > self.budget:
-                        self.cv.wait()
                     nxt[0] += 1
-                    self.used += need
-                    self.peak = max(self.peak, self.used)
+                need = footprint_estimate(r.med, r.precision)
+                if need > self.budget:
+                    errors.append((r.rid, "larger than the memory budget"))
+                    continue
                 wait = r.arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
+                pooled = self._admit(r, need)
                 r.start = time.perf_counter() - t0
+                x = None
                 try:
-                    self._run(r, t0)
+                    x = self._context(r, pooled)
+                    self._run(r, x, t0)
                 except Exception as e:  # the request fails, the server goes on
                     errors.append((r.rid, str(e)))
+                    if x is not None:
+                        x.close()
+                        x = None
                 finally:
-                    with self.cv:
-                        self.used -= need
-                        self.cv.notify_all()
+                    self._release(r, x, need)
 
         th = [threading.Thread(target=worker) for _ in range(self.slots)]
         for t in th:
@@ -139,10 +210,12 @@ class Server:
         return time.perf_counter() - t0
 
     def close(self):
-        for lst in self.pool.values():
-            for x in lst:
-                x.close()
-        self.pool = {}
+        with self.cv:
+            for lst in self.pool.values():
+                for _, need, x in lst:
+                    x.close()
+                    self.used -= need
+            self.pool = {}
 
 
 FFT_SIZES = sorted({2 ** a * 3 ** b * 5 ** c for a in range(11) for b in range(4) for c in range(3)

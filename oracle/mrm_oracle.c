@@ -1432,7 +1432,9 @@ static void loop_stats(ora_sim *s, mrmg_diag *d)
 int ora_step(ora_sim *s, const mrmg_step_params *p, const double *noise, mrmg_diag *diag)
 {
     int G = 0; for (int k = 0; k < s->nl; k++) G += s->L[k].ng;
-    const double dt_frac = p->dt_frac, dt_max = p->dt_frac != 0.0 ? p->dt_max : 1.e22;
+    /* get_fields bounds dt by dt_max with a fixed step too (fact0 = 1, llg.f90:383-386);
+     * dt_max <= 0 means no bound with a fixed step */
+    const double dt_frac = p->dt_frac, dt_max = (p->dt_frac != 0.0 || p->dt_max > 0.0) ? p->dt_max : 1.e22;
     int fixed = (dt_frac == 0.0);
     if (!fixed && p->max_dm > 0.0)
         for (int k = 0; k < s->nl; k++) s->L[k].max_dm = p->max_dm * acos(-1.0) / 180.0;

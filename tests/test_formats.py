@@ -485,3 +485,29 @@ def test_mag_ascii_round_trip(tmp_path):
         back = formats.grain_directions(g2g, cells, ng)
         present = np.isin(np.arange(1, ng + 1), g2g)
         np.testing.assert_allclose(back[present], sign * m[present], rtol=0, atol=2e-5)
+
+
+def test_mag_reads_fail_like_the_reference(tmp_path):
+    """A zero vector on a magnetic cell, or a grain number beyond the layer's grains, fails
+    the ASCII and old binary reads (mag.f90:203-205, 219-226) instead of leaving NaN."""
+    g2g = np.array([[1, 0, 2], [2, 2, 1]], np.int32)
+    cells = np.zeros((2, 3, 3))
+    cells[g2g > 0] = [0.0, 0.0, 1.0]
+    np.testing.assert_array_equal(formats.grain_directions(g2g, cells, 2), [[0, 0, 1], [0, 0, 1]])
+    bad = cells.copy()
+    bad[1, 2] = 0.0                       # grain 1's last cell
+    with pytest.raises(ValueError, match="zero magnetisation"):
+        formats.grain_directions(g2g, bad, 2)
+    with pytest.raises(ValueError, match="above"):
+        formats.grain_directions(g2g, cells, 1)
+    m = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0]])
+    p = tmp_path / "old.bin"
+    formats.write_mag_old(p, [(g2g, m)])
+    np.testing.assert_allclose(formats.read_mag_old(p, [g2g])[0], m, atol=1e-7)
+    with pytest.raises(ValueError, match="above"):
+        formats.read_mag_old(p, [g2g], ngs=[1])
+    # a grain map that marks a cell magnetic where the file holds zeros
+    g2g_more = g2g.copy()
+    g2g_more[0, 1] = 1
+    with pytest.raises(ValueError, match="zero magnetisation"):
+        formats.read_mag_old(p, [g2g_more])

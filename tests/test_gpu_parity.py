@@ -262,6 +262,57 @@ def test_philox_noise_is_what_the_step_uses(oracle_mod):
     assert abs(flat.mean()) < 4 / np.sqrt(flat.size) and abs(flat.var() - 1) < 4 * np.sqrt(2 / flat.size)
 
 
+def test_philox_in_kernel_draw_equals_the_chunked_draw():
+    """A run split into one-step calls draws its normals inside the grain kernel
+    (philox_normals3); one multi-step call draws them ahead with k_philox_dump.  Both must
+    give the same trajectory bit for bit, so results do not depend on how a run is split
+    into calls; with demag on, since the cells are refreshed at each call's start."""
+    say("philox split: 40 x step(1) vs step(40)")
+    med = media.config_media("cfg1", nx=32, ny=32)
+    N = None
+    out = []
+    for split in (False, True):
+        x = Llg(32, 32, 4, med.dx, med.dy, med.dz)
+        if N is None:
+            N = x.build_tensor(num_images=1, return_real=True)
+        else:
+            x.set_tensor(N)
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(4)):
+            x.set_ufield(k, u)
+        x.set_llg(dt=5e-14)
+        if split:
+            t = 0.0
+            for i in range(40):
+                t = x.step(1, t0=t, philox_seed=77).t
+        else:
+            x.step(40, philox_seed=77)
+        out.append([x.get_state(k)[0] for k in range(4)])
+        x.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_adaptive_philox_equals_injected_dump():
+    """The adaptive path always draws Philox normals inside the grain kernel: it must equal
+    the same adaptive run fed the normals mrmg_philox_normals dumps, rejections included."""
+    say("adaptive philox vs injected dump")
+    med = media.config_media("cfg1", nx=32, ny=32)
+    a = Llg(32, 32, 4, med.dx, med.dy, med.dz, demag=False)
+    b = Llg(32, 32, 4, med.dx, med.dy, med.dz, demag=False)
+    for x in (a, b):
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(4)):
+            x.set_ufield(k, u)
+        x.set_llg(dt=1e-14, dt_frac=0.2, dt_max=1e-12, max_dm=2.0)
+    nz = b.philox_normals(4321, 0, 30)
+    da = a.step(30, philox_seed=4321)
+    db = b.step(30, noise=nz)
+    assert da.rejected == db.rejected and da.t == db.t and da.dt == db.dt
+    for k in range(4):
+        assert np.array_equal(a.get_state(k)[0], b.get_state(k)[0]), k
+
+
 def test_philox_normals_match_numpy_restatement():
     from tests.philox_np import philox_normals
     say("philox vs numpy")

@@ -54,6 +54,11 @@ def test_admission_respects_the_memory_budget():
     srv.close()
     budget = int(need * 1.5)
     assert srv.peak <= budget
+    # every live context (in flight or idle in the pool) is counted at each admission,
+    # and the estimate bounds what a context really holds
+    assert max(srv.admissions) <= budget
+    for r in reqs:
+        assert 0 < r.footprint <= serve.footprint_estimate(r.med, r.precision), r.rid
     # at every request's start, the requests then in flight fit the budget together
     for r in reqs:
         live = [q for q in reqs if q.start <= r.start < q.end]

@@ -403,6 +403,33 @@ def test_adaptive_dt_closed_form(oracle_mod):
     assert d.dt == 2e-13          # dt_max caps it
 
 
+def test_fixed_dt_bounded_by_dt_max(oracle_mod):
+    """get_fields sets llgp%dt = min(dt, dt_max, fact1/max(1,|h|) + fact0 dt) every step,
+    and with a fixed step fact1 = 0, fact0 = 1 (llg.f90:383-386): a fixed dt above dt_max
+    runs at dt_max.  The run must equal one at dt = dt_max step for step, and the time
+    must advance by dt_max per step."""
+    O = oracle_mod
+    g, L = _single_grain(alpha=0.1)
+    runs = []
+    for dt in (1e-13, 4e-14):
+        s = O.Sim(g)
+        s.set_layer(0, L)
+        s.set_ufield(0, abi.make_ufield((0, 0, 1), scale=8000.0))
+        s.set_state(0, np.array([[0.6, 0.0, 0.8]]))
+        d = s.step(25, dt, t0=0.0, dt_max=4e-14)
+        runs.append((d.t, d.dt, s.get_state(0)[0]))
+    t = 0.0
+    for _ in range(25):
+        t = t + 4e-14
+    assert runs[0][0] == t and runs[0][1] == 4e-14
+    assert np.array_equal(runs[0][2], runs[1][2])
+    # unbounded: the 1e-13 step is taken as given
+    s = O.Sim(g)
+    s.set_layer(0, L)
+    d = s.step(3, 1e-13, t0=0.0)
+    assert d.dt == 1e-13
+
+
 def test_adaptive_rejection_bounds_rotation(oracle_mod):
     """push_m (llg.f90:1229-1236) retries with a smaller step until no layer rotates more
     than max_dm; every accepted step stays within it."""
