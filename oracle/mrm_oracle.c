@@ -1241,7 +1241,11 @@ static void push_llb(ora_sim *s, int k, int type, int iter, double dt, double dt
                 aperp = alpha * (1.0 - lambdam * lv(meq) * T / (3.0 * mhtot * Tc));
                 apar = alpha * 2.0 * T / (3.0 * Tc);
                 scale2 = -aperp / (fabs(mag) + 1.e-20);
+#if defined(ORA_PLANT_LLB_STH_NO_MAG)
+                double sth = sqrt(fabs(scale2 / (alpha * L->alpha_scale[n] * dt0)));
+#else
                 double sth = sqrt(fabs(scale2 / (alpha * L->alpha_scale[n] * dt0 * (fabs(mag) + 1.e-20))));
+#endif
                 for (int c = 0; c < 3; c++) h[c] = ph[c] + th[c] * sth;
                 for (int c = 0; c < 3; c++) m[c] = min_[c];
                 cross3(m, h, t); y = dot3(t, t);
@@ -1266,7 +1270,11 @@ static void push_llb(ora_sim *s, int k, int type, int iter, double dt, double dt
                 meq = mag + (L->mag[2 * n + mi_] - mag) *
                       exp(scale1 * 4.0 * alpha * lambdam * lambdam * T * T * lv(meq) /
                           (3.0 * Tc * Tc * dlv(meq) * (mhtot + 1.e-20)));
+#if defined(ORA_PLANT_LLB_ROT_SPAR_APERP)
+                double spar = sqrt(fabs(aperp / (alpha * L->alpha_scale[n] * dt0))), tp[3];
+#else
                 double spar = sqrt(fabs(apar / (alpha * L->alpha_scale[n] * dt0))), tp[3];
+#endif
                 for (int c = 0; c < 3; c++) tp[c] = th[c] * spar;
                 meq = meq - 2.0 * scale11 * dot3(tp, mh);
                 L->mag[2 * n + mo_] = fabs(meq);
@@ -1302,7 +1310,13 @@ static void push_llb(ora_sim *s, int k, int type, int iter, double dt, double dt
                 for (int c = 0; c < 3; c++) h[c] = th[c] / sq;
                 double hm = dot3(h, m);
                 for (int c = 0; c < 3; c++) { hpar[c] = hm * m[c]; hperp[c] = h[c] - hpar[c]; }
+#if defined(ORA_PLANT_LLB_SPAR_NO_SQRT)   /* planted mistakes for the thermal pins' own test */
+                double sp = apar, sr = sqrt((aperp - apar > 0.0 ? aperp - apar : 0.0) / (aperp * aperp));
+#elif defined(ORA_PLANT_LLB_SPERP_NO_SQUARE)
+                double sp = sqrt(apar), sr = sqrt((aperp - apar > 0.0 ? aperp - apar : 0.0) / aperp);
+#else
                 double sp = sqrt(apar), sr = sqrt((aperp - apar > 0.0 ? aperp - apar : 0.0) / (aperp * aperp));
+#endif
                 for (int c = 0; c < 3; c++) { hpar[c] = hpar[c] * sp; hperp[c] = hperp[c] * sr; }
                 scale11 = 2.0 * scale1;
                 meq = apar * (lambdam / mhtot) * lambdam * 2.0 * scale1 * T * lv(meq) / (3.0 * dlv(meq) * Tc) *

@@ -29,12 +29,13 @@ def build():
 _lib = None
 
 
-def lib():
+def lib(path=None):
+    """The oracle library (cached), or the one built at path."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+    if _lib is None or path is not None:
+        if path is None and not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path or LIB)
         sig = {
             "ora_mt_size": ([], C.c_size_t),
             "ora_mt_seed": ([_cp, C.c_int], None),
@@ -71,8 +72,19 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
+        if path is not None:
+            return L
         _lib = L
     return _lib
+
+
+def planted(define, directory):
+    """The oracle compiled with -D<define> into directory: a copy with a planted mistake
+    (the ORA_PLANT_* switches of mrm_oracle.c), which a pin must tell from the real one."""
+    out = os.path.join(directory, f"liboracle_{define.lower()}.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-fno-fast-math", "-std=c11", "-D_GNU_SOURCE", "-D" + define,
+                    "-shared", "-o", out, os.path.join(HERE, "mrm_oracle.c"), "-lm"], check=True)
+    return lib(out)
 
 
 def ref_available():
@@ -182,16 +194,17 @@ def funct(func, t, gamma, phase=0.0, param=0.0, duty=0.0, moddepth=0.0):
 class Sim:
     """The restated llg/field/demag path on the CPU, driven like the C ABI."""
 
-    def __init__(self, geom):
+    def __init__(self, geom, library=None):
         self._g = geom
-        self.h = lib().ora_create(C.byref(geom))
+        self.L = library if library is not None else lib()
+        self.h = self.L.ora_create(C.byref(geom))
         self.nx, self.ny, self.nl = geom.nx, geom.ny, geom.n_lay
         self.ngs = [0] * self.nl
         self._keep = []
 
     def close(self):
         if self.h:
-            lib().ora_destroy(self.h)
+            self.L.ora_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -199,54 +212,54 @@ class Sim:
 
     def set_tensor(self, n_real):
         n_real = np.ascontiguousarray(n_real, dtype=np.float64)
-        lib().ora_set_tensor(self.h, abi.dptr(n_real))
+        self.L.ora_set_tensor(self.h, abi.dptr(n_real))
 
     def set_layer(self, k, la):
         s = la.struct()
         self._keep.append(la)
-        assert lib().ora_set_layer(self.h, k, C.byref(s)) == 0
+        assert self.L.ora_set_layer(self.h, k, C.byref(s)) == 0
         self.ngs[k] = la.ng
 
     def set_ufield(self, k, uf):
-        assert lib().ora_set_ufield(self.h, k, C.byref(uf)) == 0
+        assert self.L.ora_set_ufield(self.h, k, C.byref(uf)) == 0
 
     def set_source(self, k, slot, src):
         self._keep.append(src)
         st = src.struct() if src is not None else abi.Source(kind=abi.MRMG_SRC_NONE)
-        assert lib().ora_set_source(self.h, k, slot, C.byref(st)) == 0
+        assert self.L.ora_set_source(self.h, k, slot, C.byref(st)) == 0
 
     def source_cells(self, k, slot, t):
         nv = 3 if slot == 0 else 1
         out = np.zeros((nv, self.ny, self.nx))
-        assert lib().ora_source_cells(self.h, k, slot, t, abi.dptr(out)) == 0
+        assert self.L.ora_source_cells(self.h, k, slot, t, abi.dptr(out)) == 0
         return out
 
     def set_max_dm(self, k, deg):
-        lib().ora_set_max_dm(self.h, k, deg)
+        self.L.ora_set_max_dm(self.h, k, deg)
 
     def set_state(self, k, m):
         m = np.ascontiguousarray(m, dtype=np.float64)
-        lib().ora_set_state(self.h, k, abi.dptr(m))
+        self.L.ora_set_state(self.h, k, abi.dptr(m))
 
     def get_scales(self, k):
         """(ms_scale, alpha_scale, hk_scale) of layer k as find_temp_scaling last set them."""
         a, b, c = (np.zeros(self.ngs[k]) for _ in range(3))
-        assert lib().ora_get_scales(self.h, k, abi.dptr(a), abi.dptr(b), abi.dptr(c)) == 0
+        assert self.L.ora_get_scales(self.h, k, abi.dptr(a), abi.dptr(b), abi.dptr(c)) == 0
         return a, b, c
 
     def get_state(self, k):
         m = np.zeros((self.ngs[k], 3))
         h = np.zeros((self.ngs[k], 3))
-        lib().ora_get_state(self.h, k, abi.dptr(m), abi.dptr(h))
+        self.L.ora_get_state(self.h, k, abi.dptr(m), abi.dptr(h))
         return m, h
 
     def set_mag(self, k, mag):
         mag = np.ascontiguousarray(mag, dtype=np.float64)
-        lib().ora_set_mag(self.h, k, abi.dptr(mag))
+        self.L.ora_set_mag(self.h, k, abi.dptr(mag))
 
     def get_mag(self, k):
         mag = np.zeros(self.ngs[k])
-        lib().ora_get_mag(self.h, k, abi.dptr(mag))
+        self.L.ora_get_mag(self.h, k, abi.dptr(mag))
         return mag
 
     def step(self, n_steps, dt, t0=0.0, type=abi.MRMG_TYPE_HEUN, noise=None, dt_frac=0.0, dt_max=1e22,
@@ -256,12 +269,12 @@ class Sim:
                            dt_frac=dt_frac, dt_max=dt_max, max_dm=max_dm, llb=int(bool(llb)))
         d = abi.Diag()
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
-        lib().ora_step(self.h, C.byref(p), abi.dptr(nz), C.byref(d))
+        self.L.ora_step(self.h, C.byref(p), abi.dptr(nz), C.byref(d))
         return d
 
     def demag_field(self):
         G = sum(self.ngs)
         hc = np.zeros((3, self.nl, self.ny, self.nx))
         hg = np.zeros((G, 3))
-        lib().ora_demag_field(self.h, abi.dptr(hc), abi.dptr(hg))
+        self.L.ora_demag_field(self.h, abi.dptr(hc), abi.dptr(hg))
         return hc, hg
