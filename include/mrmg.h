@@ -305,6 +305,7 @@ typedef struct {
 int mrmg_readback(mrmg_ctx *ctx, const mrmg_sensor *s, int n_pos, const double *xpos, const double *ypos,
                   double *out);
 
+
 /* Switching record.  enable != 0 clears it (every grain "not switched") and starts it:
  * after each later completed step of mrmg_step, every grain not yet marked whose m
  * satisfies dir . m > thr gets that step's counter (step0 + i), its end time and the
@@ -342,6 +343,10 @@ int mrmg_profile(mrmg_ctx *ctx, int enable);
  * gather sums) and so the last bits.  A combination that does not fit the grid is
  * refused. */
 int mrmg_tune(mrmg_ctx *ctx, const char *key, int value);
+/* The value of a tunable, or of the read-only key "tensor_offsets": 1 when the stored
+ * tensor spectra are the 2 n_lay - 1 z-offset blocks (the tensor depends on the layer pair
+ * only through ko - ks: equal thicknesses and spacings, no magnetic images above or
+ * below), 0 when they are the n_lay^2 layer-pair blocks. */
 int mrmg_get_tune(mrmg_ctx *ctx, const char *key, int *value);
 /* Copies up to max entries; *n receives the number of kernel classes. */
 int mrmg_kernel_stats(mrmg_ctx *ctx, mrmg_kstat *out, int max, int *n);

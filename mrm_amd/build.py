@@ -1,23 +1,59 @@
-"""Build recipe of libmrmg.so (hipcc, gfx950).  Run by __graft_entry__.build()."""
+"""Build recipe of libmrmg.so (hipcc, gfx950).  Run by __graft_entry__.build().
+
+The library is several translation units compiled in parallel and linked into one shared
+object; the demag units are compiled once per precision form."""
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "mrmg.hip")
+CSRC = os.path.join(HERE, "csrc")
+# (source, object, extra flags): the demag units once per precision form
+UNITS = [("mrmg.hip", "mrmg.o", []), ("grain.hip", "grain.o", []), ("tensor.hip", "tensor.o", []),
+         ("demag.hip", "demag_f32.o", ["-DMRMG_FORM=float"]), ("demag.hip", "demag_f64.o", ["-DMRMG_FORM=double"]),
+         ("column.hip", "column_f32.o", ["-DMRMG_FORM=float"]), ("column.hip", "column_f64.o", ["-DMRMG_FORM=double"])]
+OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libmrmg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 arithmetic
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose=False):
+def _sources_newer(obj, src):
+    """True when obj is missing or older than its unit or any header of csrc/ or include/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps.append(os.path.join(HERE, "..", "include", "mrmg.h"))
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=False, reuse=False):
+    """Compile every unit (or, with reuse, only those whose sources changed) and link."""
+    os.makedirs(OBJ, exist_ok=True)
+
+    def compile_unit(unit):
+        u, o, extra = unit
+        src = os.path.join(CSRC, u)
+        obj = os.path.join(OBJ, o)
+        if reuse and not _sources_newer(obj, src):
+            return obj
+        r = subprocess.run([HIPCC] + FLAGS + extra + ["-c", "-o", obj + ".tmp", src], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {u}:\n" + r.stderr[-4000:])
+        os.replace(obj + ".tmp", obj)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        objs = list(ex.map(compile_unit, UNITS))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
+                       capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+        raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
@@ -48,5 +84,6 @@ def build_fortran_driver(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import sys
+    build(verbose=True, reuse="--reuse" in sys.argv)
     build_fortran_driver(verbose=True)

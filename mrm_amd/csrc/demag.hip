@@ -1,0 +1,433 @@
+// demag.hip - the demag FFT kernels (get_mag scatter into the row r2c, column FFTs, the
+// k-space product of demag.f90:117-121, row c2r) and the Engine members that launch them.
+#include "engine.h"
+#include "device_common.h"
+
+#ifndef MRMG_FORM
+#error "compile with -DMRMG_FORM=float or -DMRMG_FORM=double"
+#endif
+
+// ------------------------------------------------------------------------- demag kernels
+// Row r2c: one workgroup per (row pair, layer).  Two rows of each component of the cell
+// magnetisation (written by the grain kernels, get_mag data.f90:344-353) are packed as
+// z = M(y0) + i M(y1), transformed with one complex FFT and separated.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+                                                 const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(3 * nx);
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
+    const bool has1 = y1 < ny;
+    const size_t CP = (size_t)nl * ny * nx;
+    const T* r0 = Mcell + ((size_t)k * ny + y0) * nx;
+    staged_load<(E < 8 ? E : 8)>(
+        3 * nx,
+        [&](int idx) {
+            const int c = idx / nx, x = idx - c * nx;
+            return C2<T>{r0[c * CP + x], r0[c * CP + (has1 ? nx : 0) + x]};
+        },
+        [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = has1 ? v.y : (T)0; }, tw, twx, nx);
+    __syncthreads();
+    lds_fft<T, E, false>(re, im, 3, px, tw);
+    for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
+        const int c = idx / nxh, kx = idx - c * nxh;
+        const int kn = kx == 0 ? 0 : nx - kx;
+        const T zr = re[LP(c * nx + kx)], zi = im[LP(c * nx + kx)], nr = re[LP(c * nx + kn)], ni = im[LP(c * nx + kn)];
+        C2<T> A = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
+        C2<T> B = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
+        C2<T>* row = S + (((size_t)k * 3 + c) * ny + y0) * nxh;
+        row[kx] = A;
+        if (has1) row[nxh + kx] = B;
+    }
+}
+
+
+// Row c2r: FFTW's 1D c2r semantics (imaginary parts of the DC and Nyquist bins ignored).
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T* __restrict__ Hcell,
+                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+                                                 const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(3 * nx);
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
+    const bool has1 = y1 < ny;
+    staged_load<(E < 4 ? E : 4)>(
+        3 * nxh,
+        [&](int idx) {
+            const int c = idx / nxh, kx = idx - c * nxh;
+            const C2<T>* row = S2 + (((size_t)k * 3 + c) * ny + y0) * nxh;
+            return C4<T>{row[kx], row[(has1 ? nxh : 0) + kx]};
+        },
+        [&](int idx, C4<T> v) {
+            const int c = idx / nxh, kx = idx - c * nxh;
+            C2<T> A = v.a, B = has1 ? v.b : C2<T>{0, 0};
+            if (kx == 0 || 2 * kx == nx) { A.y = 0; B.y = 0; }
+            re[LP(c * nx + kx)] = A.x - B.y;
+            im[LP(c * nx + kx)] = A.y + B.x;
+            if (kx > 0 && nx - kx != kx) {
+                re[LP(c * nx + nx - kx)] = A.x + B.y;
+                im[LP(c * nx + nx - kx)] = B.x - A.y;
+            }
+        },
+        tw, twx, nx);
+    __syncthreads();
+    lds_fft<T, E, true>(re, im, 3, px, tw);
+    const size_t plane = (size_t)nx * ny;
+    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+        const int c = idx / nx, x = idx - c * nx;
+        T* hc = Hcell + ((size_t)c * nl + k) * plane;
+        hc[(size_t)y0 * nx + x] = re[LP(idx)];
+        if (has1) hc[(size_t)y1 * nx + x] = im[LP(idx)];
+    }
+}
+
+// ------------------------------------------------------------------------- split column path
+// For grids whose 3*n_lay columns of one kx do not fit one workgroup's LDS, the demag
+// runs as five kernels over column-contiguous spectra ST[k][c][kx][y]:
+//   k_rowT_fwd  r2c of TY rows of one (component, layer), written as TY-long y runs
+//   k_colT      y-FFT of whole columns (CB columns per workgroup), in place
+//   k_prod      the k-space product (demag.f90:117-121), one thread per (kx, ky)
+//   k_colT      inverse y-FFT, in place
+//   k_rowT_inv  c2r of TY rows, read back from TY-long y runs
+template <typename T, int E, bool WV = false>
+__global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+                                                  const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    const int B = TY / 2;
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(B * nx);
+    // 1D grid padded to a multiple of 8, XCD-contiguous: the row blocks that share the
+    // cache lines of one kx run (TY*sizeof(C2<T>) < 128 bytes) are read/written in one L2
+    const int nyb = (ny + TY - 1) / TY;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    if (L >= nyb * 3 * nl) return;
+    const int y0 = (L % nyb) * TY, kc = L / nyb;        // kc = k*3 + c
+    const int k = kc / 3, c = kc - 3 * k;
+    const size_t CP = (size_t)nl * ny * nx;
+    const T* src = Mcell + c * CP + (size_t)k * ny * nx;
+    staged_load<(E < 8 ? E : 8)>(
+        B * nx,
+        [&](int idx) {
+            const int p = idx / nx, x = idx - p * nx;
+            const int ya = y0 + 2 * p, yb = ya + 1;
+            return C2<T>{src[(size_t)(ya < ny ? ya : ny - 1) * nx + x], src[(size_t)(yb < ny ? yb : ny - 1) * nx + x]};
+        },
+        [&](int idx, C2<T> v) {
+            const int p = idx / nx, ya = y0 + 2 * p;
+            re[LP(idx)] = ya < ny ? v.x : (T)0;
+            im[LP(idx)] = ya + 1 < ny ? v.y : (T)0;
+        },
+        tw, twx, nx);
+    __syncthreads();
+    if (WV) {
+        lds_fft_wave<T, E, false>(re, im, B, px, tw);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, false>(re, im, B, px, tw);
+    }
+    C2<T>* dst = S + (size_t)kc * nxh * ny;
+    for (int idx = threadIdx.x; idx < nxh * TY; idx += blockDim.x) {
+        const int kx = idx / TY, yy = idx - kx * TY;
+        const int y = y0 + yy;
+        if (y >= ny) continue;
+        const int p = yy >> 1;
+        const int kn = kx == 0 ? 0 : nx - kx;
+        const T zr = re[LP(p * nx + kx)], zi = im[LP(p * nx + kx)], nr = re[LP(p * nx + kn)], ni = im[LP(p * nx + kn)];
+        C2<T> v;
+        if ((yy & 1) == 0) v = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
+        else v = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
+        dst[(size_t)kx * ny + y] = v;
+    }
+}
+
+// y-FFT of ncols contiguous columns of length ny, CB columns per workgroup, in place.
+// Twiddles are read from the global table (cached), not staged.
+template <typename T, int E, bool INV>
+__global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int ncols, int CB, FFTPlan py,
+                                              const C2<T>* __restrict__ twy)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* re = reinterpret_cast<T*>(smem_raw);
+    const int q0 = blockIdx.x * CB;
+    const int nb = min(CB, ncols - q0);
+    T* im = re + LPN(CB * ny);
+    C2<T>* base = S + (size_t)q0 * ny;
+    staged_load<(E < 8 ? E : 8), T>(
+        nb * ny, [&](int idx) { return base[idx]; }, [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
+    __syncthreads();
+    lds_fft<T, E, INV>(re, im, nb, py, twy);
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
+}
+
+// The k-space product over column-contiguous spectra: thread per (kx, ky), observation
+// layers in chunks of OC, each output summed over (ks, b) in the reference's order.
+template <typename T, bool SYM, int OC>
+__global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+                                              const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot,
+                                              int toep)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t Kc = (size_t)nxh * ny;
+    if (idx >= Kc) return;
+    const int kx = (int)(idx / ny), ky = (int)(idx - (size_t)kx * ny);
+    for (int kc0 = 0; kc0 < nl; kc0 += OC) {
+        C2<T> h[OC][3];
+#pragma unroll
+        for (int j = 0; j < OC; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) h[j][a] = {0, 0};
+        for (int ks = 0; ks < nl; ks++) {
+            const C2<T> mx = S1[((size_t)(3 * ks + 0)) * Kc + idx];
+            const C2<T> my = S1[((size_t)(3 * ks + 1)) * Kc + idx];
+            const C2<T> mz = S1[((size_t)(3 * ks + 2)) * Kc + idx];
+#pragma unroll
+            for (int j = 0; j < OC; j++) {
+                const int ko = kc0 + j;
+                if (ko >= nl) break;
+                const C2<T>* t = Tspec + tensor_block(kx, ko, ks, nl, toep) * nslot * ny + ky;
+                C2<T> a;
+                if (SYM) {
+                    const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
+                    a = cmul(txx, mx); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txy, my); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txz, mz); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(txy, mx); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(tyy, my); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(tyz, mz); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(txz, mx); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(tyz, my); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(tzz, mz); h[j][2].x += a.x; h[j][2].y += a.y;
+                } else {
+                    a = cmul(t[0 * ny], mx); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[1 * ny], my); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[2 * ny], mz); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(t[3 * ny], mx); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[4 * ny], my); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[5 * ny], mz); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(t[6 * ny], mx); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(t[7 * ny], my); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(t[8 * ny], mz); h[j][2].x += a.x; h[j][2].y += a.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < OC; j++) {
+            const int ko = kc0 + j;
+            if (ko >= nl) break;
+#pragma unroll
+            for (int a = 0; a < 3; a++) S2[((size_t)(3 * ko + a)) * Kc + idx] = h[j][a];
+        }
+    }
+}
+
+// c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
+// imaginary parts of the DC and Nyquist bins ignored).
+template <typename T, int E, bool WV = false>
+__global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T* __restrict__ Hcell,
+                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+                                                  const C2<T>* __restrict__ twx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
+    const int B = TY / 2;
+    T* re = reinterpret_cast<T*>(tw + nx);
+    T* im = re + LPN(B * nx);
+    const int nyb = (ny + TY - 1) / TY;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    if (L >= nyb * 3 * nl) return;
+    const int y0 = (L % nyb) * TY, kc = L / nyb;
+    const int k = kc / 3, c = kc - 3 * k;
+    const C2<T>* src = S + (size_t)kc * nxh * ny;
+    // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
+    staged_load<(E < 4 ? E : 4)>(
+        nxh * B,
+        [&](int idx) {
+            const int kx = idx / B, p = idx - kx * B;
+            const int ya = y0 + 2 * p, yb = ya + 1;
+            return C4<T>{src[(size_t)kx * ny + (ya < ny ? ya : ny - 1)], src[(size_t)kx * ny + (yb < ny ? yb : ny - 1)]};
+        },
+        [&](int idx, C4<T> v) {
+            const int kx = idx / B, p = idx - kx * B;
+            const int ya = y0 + 2 * p;
+            C2<T> A = ya < ny ? v.a : C2<T>{0, 0}, Bv = ya + 1 < ny ? v.b : C2<T>{0, 0};
+            if (kx == 0 || 2 * kx == nx) { A.y = 0; Bv.y = 0; }
+            re[LP(p * nx + kx)] = A.x - Bv.y;
+            im[LP(p * nx + kx)] = A.y + Bv.x;
+            if (kx > 0 && nx - kx != kx) {
+                re[LP(p * nx + nx - kx)] = A.x + Bv.y;
+                im[LP(p * nx + nx - kx)] = Bv.x - A.y;
+            }
+        },
+        tw, twx, nx);
+    __syncthreads();
+    if (WV) {
+        lds_fft_wave<T, E, true>(re, im, B, px, tw);
+        __syncthreads();
+    } else {
+        lds_fft<T, E, true>(re, im, B, px, tw);
+    }
+    T* hc = Hcell + ((size_t)c * nl + k) * (size_t)nx * ny;
+    for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
+        const int p = idx / nx, x = idx - p * nx;
+        const int ya = y0 + 2 * p, yb = ya + 1;
+        if (ya < ny) hc[(size_t)ya * nx + x] = re[LP(idx)];
+        if (yb < ny) hc[(size_t)yb * nx + x] = im[LP(idx)];
+    }
+}
+
+template <typename T>
+template <int MB> int Engine<T>::lds_attr_mb()
+{
+    const int r = (int)lds_row;
+    HIPCHK(hipFuncSetAttribute((const void*)k_row_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    RCHK(lds_attr_col<MB>());
+    return MRMG_OK;
+}
+
+template <typename T>
+template <int MB> int Engine<T>::lds_attr_split()
+{
+    const int r = (int)lds_rowT, cl = (int)lds_colT;
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    return MRMG_OK;
+}
+
+// dynamic LDS above 64 KiB must be requested per kernel
+template <typename T>
+int Engine<T>::lds_attr()
+{
+    if (fused_ok && (lds_row > 64 * 1024 || lds_col > 64 * 1024)) {
+        RCHK(lds_attr_mb<4>()); RCHK(lds_attr_mb<8>()); RCHK(lds_attr_mb<16>());
+        RCHK(lds_attr_mb<32>()); RCHK(lds_attr_mb<64>());
+    }
+    if (split_ok && (lds_rowT > 64 * 1024 || lds_colT > 64 * 1024)) {
+        RCHK(lds_attr_split<4>()); RCHK(lds_attr_split<8>()); RCHK(lds_attr_split<16>());
+        RCHK(lds_attr_split<32>()); RCHK(lds_attr_split<64>());
+    }
+    return MRMG_OK;
+}
+
+// one launch of a demag kernel class (K_ROW_FWD, K_COL, K_ROW_INV on the fused paths;
+// K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV on the split path)
+template <typename T>
+int Engine<T>::launch_class(int cls)
+{
+    const int ny = c->ny, nl = c->nl, nxh = c->nxh;
+    const dim3 grow((ny + 1) / 2, nl), growT(rowT_blocks()), blk(row_threads), b256(256);
+    const int ncols = 3 * nl * nxh;
+    const dim3 gcol((ncols + CBc - 1) / CBc);
+    switch (cls) {
+    case K_ROW_FWD:
+        if (tl || split) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
+        else LAUNCH_E(maxb_row, launch_row_fwd, grow, blk);
+        break;
+    case K_COL: {
+        if (split) return set_err(MRMG_ERR_STATE, "the split path has no fused column kernel");
+        const int nb8 = (nxh * kg + 7) / 8 * 8;
+        if (nslot == 6) LAUNCH_E(maxb_col, launch_col_sym, nb8);
+        else if (col_threads > 512) return set_err(MRMG_ERR_STATE, "col_threads 1024 needs the 6-slot tensor");
+        else LAUNCH_E(maxb_col, launch_col_gen, nb8);
+        break;
+    }
+    case K_COLF:
+    case K_COLI:
+        if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate column FFTs");
+        if (cls == K_COLF) LAUNCH_E(maxb_colT, launch_colT_fwd, gcol, b256);
+        else LAUNCH_E(maxb_colT, launch_colT_inv, gcol, b256);
+        break;
+    case K_PROD: {
+        if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate product kernel");
+        const size_t Kc = (size_t)nxh * ny;
+        const dim3 gp((unsigned)((Kc + 255) / 256));
+        if (nslot == 6)
+            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot,
+                               (int)toep);
+        else
+            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh,
+                               nslot, (int)toep);
+        break;
+    }
+    case K_ROW_INV:
+        if (tl || split) LAUNCH_E(maxb_rowT, launch_rowT_inv, growT, blk);
+        else LAUNCH_E(maxb_row, launch_row_inv, grow, blk);
+        break;
+    default:
+        return set_err(MRMG_ERR_ARG, "not a demag kernel class");
+    }
+    HIPCHK(hipGetLastError());
+    return MRMG_OK;
+}
+
+template <typename T>
+template <int MB> void Engine<T>::launch_rowT_fwd(dim3 g, dim3 b)
+{
+    if (wv)
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
+                           c->nxh, TYr, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
+                           c->nxh, TYr, prow, twx.p);
+}
+
+template <typename T>
+template <int MB> void Engine<T>::launch_colT_fwd(dim3 g, dim3 b)
+{
+    hipLaunchKernelGGL((k_colT<T, MB, false>), g, b, lds_colT, c->stream, S1.p, c->ny, 3 * c->nl * c->nxh, CBc,
+                       pcol, twy.p);
+}
+
+template <typename T>
+template <int MB> void Engine<T>::launch_colT_inv(dim3 g, dim3 b)
+{
+    hipLaunchKernelGGL((k_colT<T, MB, true>), g, b, lds_colT, c->stream, S2.p, c->ny, 3 * c->nl * c->nxh, CBc,
+                       pcol, twy.p);
+}
+
+template <typename T>
+template <int MB> void Engine<T>::launch_rowT_inv(dim3 g, dim3 b)
+{
+    if (wv)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
+                           c->nxh, TYr, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
+                           c->nxh, TYr, prow, twx.p);
+}
+
+template <typename T>
+template <int MB> void Engine<T>::launch_row_fwd(dim3 g, dim3 b)
+{
+    hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
+                       prow, twx.p);
+}
+
+
+
+template <typename T>
+template <int MB> void Engine<T>::launch_row_inv(dim3 g, dim3 b)
+{
+    hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
+                       prow, twx.p);
+}
+
+
+// each precision form is its own object (the unit is compiled with MRMG_FORM=float and
+// with MRMG_FORM=double)
+template int Engine<MRMG_FORM>::lds_attr();
+template int Engine<MRMG_FORM>::launch_class(int);

@@ -8,7 +8,7 @@
 //   cell2grain         [k][y][x] -> global grain or -1               (get_mag scatter, data.f90:344-353)
 //   spectra S1, S2     [k][c][y][kx], kx = 0..nx/2                   (x = halved axis, fftw.f90:66-84)
 //   Hcell              [c][k][y][x]                                   (rx2d, ry2d, rz2d of demag.f90)
-//   Tspec              [kx][ko][ks][slot][ky]                         (n2d of demag.f90:255 regrouped so
+//   Tspec              [kx][q][slot][ky], q = tensor_block            (n2d of demag.f90:255 regrouped so
 //                                                                     one column workgroup streams one
 //                                                                     contiguous block)
 #pragma once
@@ -119,6 +119,15 @@ struct SrcAxes {
     const double* vals;   // table values
     int nx, ny;
 };
+
+// Index of the stored tensor block that maps layer ks's magnetisation onto layer ko's
+// field at column kx of Tspec[kx][q][slot][ky]: the layer pair q = ko*nl + ks, or, when the
+// tensor depends on the layers only through their z offset (equal thicknesses and spacings,
+// no magnetic images), q = ko - ks + nl - 1 of the 2 nl - 1 offsets.
+__host__ __device__ inline size_t tensor_block(int kx, int ko, int ks, int nl, int toep)
+{
+    return toep ? (size_t)kx * (2 * nl - 1) + (size_t)(ko - ks + nl - 1) : ((size_t)kx * nl + ko) * nl + ks;
+}
 
 // rows of the grain state buffers
 #define ST_MAG 3
