@@ -305,6 +305,21 @@ typedef struct {
 int mrmg_readback(mrmg_ctx *ctx, const mrmg_sensor *s, int n_pos, const double *xpos, const double *ypos,
                   double *out);
 
+/* mrmg_readback without device allocation.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc).  The result equals mrmg_readback's bit for
+ * bit (the same kernels); the folded plane, the head positions and the outputs are staged
+ * in one fixed device scratch array of MRMG_NOALLOC_PLANE_MAX + 3 * MRMG_NOALLOC_POS_MAX
+ * doubles (33.5 MB on each GPU the library runs on, held until the process ends) instead
+ * of buffers the library allocates.  Limits: nx * ny <= MRMG_NOALLOC_PLANE_MAX and
+ * n_pos <= MRMG_NOALLOC_POS_MAX, else MRMG_ERR_ARG; the context must have been built by an
+ * earlier call (mrmg_step, mrmg_readback, ...), else MRMG_ERR_STATE, because building it
+ * allocates.  Every call on one GPU uses that GPU's one scratch array: calls that overlap
+ * on one GPU (from several host threads) race, and the caller must serialise them. */
+#define MRMG_NOALLOC_PLANE_MAX (4 * 1024 * 1024)
+#define MRMG_NOALLOC_POS_MAX 65536
+int mrmg_readback_noalloc(mrmg_ctx *ctx, const mrmg_sensor *s, int n_pos, const double *xpos, const double *ypos,
+                          double *out);
 
 /* Switching record.  enable != 0 clears it (every grain "not switched") and starts it:
  * after each later completed step of mrmg_step, every grain not yet marked whose m

@@ -266,6 +266,7 @@ def load(path=None):
         ("mrmg_tune", [C.c_void_p, C.c_char_p, C.c_int]),
         ("mrmg_get_tune", [C.c_void_p, C.c_char_p, _ip]),
         ("mrmg_readback", [C.c_void_p, C.POINTER(Sensor), C.c_int, _dp, _dp, _dp]),
+        ("mrmg_readback_noalloc", [C.c_void_p, C.POINTER(Sensor), C.c_int, _dp, _dp, _dp]),
         ("mrmg_switch_arm", [C.c_void_p, C.c_int, _dp, C.c_double]),
         ("mrmg_switch_read", [C.c_void_p, C.c_int, C.POINTER(C.c_int64), _dp, _dp]),
     ]:

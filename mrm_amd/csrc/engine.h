@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,7 @@ struct EngineBase {
 
 struct mrmg_ctx {
     int nx = 0, ny = 0, nl = 0, nxh = 0, precision = 0, demag = 1;
+    int device = 0;   // the device current at mrmg_create; the stream and buffers live there
     double dx = 0, dy = 0;
     std::vector<double> dz, il;
     hipStream_t stream = nullptr;

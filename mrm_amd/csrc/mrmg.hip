@@ -24,14 +24,40 @@ int set_err(int code, const char* fmt, ...)
     return code;
 }
 
-
+// Scratch of the allocation-free readback: the plane, then the head positions (x, y), then
+// the outputs.  Host code never passes this name to a kernel (it is a host-side stand-in);
+// it uses the device address hipGetSymbolAddress returns for the current device.
+#define RB_SCRATCH_DOUBLES ((size_t)MRMG_NOALLOC_PLANE_MAX + 3 * (size_t)MRMG_NOALLOC_POS_MAX)
+static __device__ __align__(256) double scratch[RB_SCRATCH_DOUBLES];
 
 // ------------------------------------------------------------------------- C ABI
 // (the definitions take C linkage from their declarations in mrmg.h)
 
 const char* mrmg_last_error(void) { return g_err.c_str(); }
 
+// Device address of the readback scratch array on each device, taken once per device by
+// mrmg_create with that device current.  Each device holds its own copy of the array.
+#define SCRATCH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
 
+static int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return set_err(MRMG_ERR_UNSUPPORTED, "device %d: too many devices", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return MRMG_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+    return MRMG_OK;
+}
+
+static double* scratch_of(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 
 int mrmg_device_count(int* n)
 {
@@ -60,6 +86,8 @@ int mrmg_create(const mrmg_geom* g, mrmg_ctx** out)
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return set_err(MRMG_ERR_HIP, "no HIP device");
+    HIPCHK(hipGetDevice(&c->device));
+    RCHK(scratch_setup(c->device));
     HIPCHK(hipStreamCreate(&c->stream));
     c->layers.resize(c->nl);
     if (c->precision == MRMG_FP64) c->eng.reset(new Engine<double>(c.get()));
@@ -540,7 +568,30 @@ int mrmg_readback(mrmg_ctx* c, const mrmg_sensor* s, int n_pos, const double* xp
     return c->eng->readback(s, n_pos, xpos, ypos, out);
 }
 
-
+int mrmg_readback_noalloc(mrmg_ctx* c, const mrmg_sensor* s, int n_pos, const double* xpos, const double* ypos,
+                          double* out)
+{
+    if (!c || !s) return set_err(MRMG_ERR_ARG, "null argument");
+    if (n_pos < 0) return set_err(MRMG_ERR_ARG, "n_pos < 0");
+    if (n_pos > 0 && (!xpos || !ypos || !out)) return set_err(MRMG_ERR_ARG, "null position or output array");
+    if (!(s->sigma_x > 0.0) || !(s->sigma_y > 0.0)) return set_err(MRMG_ERR_ARG, "sensor widths must be > 0");
+    if ((size_t)(c->nx + c->ny) * sizeof(double) > 64 * 1024 - 64)
+        return set_err(MRMG_ERR_UNSUPPORTED, "readback: nx + ny too large for one workgroup's table");
+    if ((size_t)c->nx * c->ny > (size_t)MRMG_NOALLOC_PLANE_MAX)
+        return set_err(MRMG_ERR_ARG, "readback_noalloc: nx*ny = %zu exceeds the scratch plane of %d",
+                       (size_t)c->nx * c->ny, MRMG_NOALLOC_PLANE_MAX);
+    if (n_pos > MRMG_NOALLOC_POS_MAX)
+        return set_err(MRMG_ERR_ARG, "readback_noalloc: n_pos = %d exceeds %d", n_pos, MRMG_NOALLOC_POS_MAX);
+    // building the engine allocates: this entry point runs only on a built context
+    if (c->dirty) return set_err(MRMG_ERR_STATE, "readback_noalloc: context not built yet (run mrmg_step first)");
+    if (n_pos == 0) return MRMG_OK;
+    double* sp = scratch_of(c->device);
+    if (!sp) return set_err(MRMG_ERR_STATE, "readback_noalloc: no scratch address for device %d", c->device);
+    double* plane = sp;
+    double* pos = sp + MRMG_NOALLOC_PLANE_MAX;
+    double* dout = pos + 2 * (size_t)MRMG_NOALLOC_POS_MAX;
+    return c->eng->readback_into(s, n_pos, xpos, ypos, out, plane, pos, dout);
+}
 
 int mrmg_switch_arm(mrmg_ctx* c, int enable, const double* dir, double thr)
 {

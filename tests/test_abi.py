@@ -45,6 +45,7 @@ def test_library_loads_without_gpu():
     assert lib.mrmg_create(None, None) == 1   # MRMG_ERR_ARG, no device call made
     # the readback and switching-record entry points reject a null context the same way
     assert lib.mrmg_readback(None, None, 0, None, None, None) == 1
+    assert lib.mrmg_readback_noalloc(None, None, 0, None, None, None) == 1
     assert lib.mrmg_switch_arm(None, 1, None, 0.0) == 1
     assert lib.mrmg_switch_read(None, 0, None, None, None) == 1
 
