@@ -1,16 +1,22 @@
 """Benchmark of the granular-media time step on one MI355X.
 
-Workload (configuration 1): 256 x 256 x 4 cells of 2 nm, two exchange-coupled 4 nm
-physical layers (Ku 2e6 top, 1e7 bottom, Ms 700, Voronoi grains of 8 nm pitch with
-one-cell boundaries), T = 300 K thermal field from the on-device Philox source, uniform
-field tilted 5 deg from -z ramped 0 -> 20 kOe over 2 ns, Heun at dt = 50 fs, periodic FFT
-demag.  A step is one full Heun step (two field evaluations, two pushes) of all cells.
+Headline workload (the LLB heat-assisted write, the largest single-GPU configuration):
+2048 x 2048 x 8 cells of 1 nm, FePt-like grains (Voronoi, 8 nm pitch, 1 nm boundaries,
+Ms 1100 emu/cc, Ku 5e7 erg/cc, Tc = 700 K, Ms(T) from the LLB |m| and Hk(T) ~ m_e(T)),
+300 K ambient plus a Gaussian laser spot (FWHM 40 nm, peak 750 K) moving down-track at
+15 m/s, an 8 kOe write field along -z, the LLB Heun push at dt = 1 fs with the on-device
+Philox thermal field, and the periodic FFT demag with the tensor built on the device in
+FP64.  A step is one full Heun step (two field evaluations, two pushes) of all cells.
 
-value = cells * steps / wall seconds of the FP64 plain form with every input resident on
-the device.  The FP32 precision form is timed on the same workload and compared with it.
-The JSON line also carries the roofline of the dominant kernel (HIP-event durations on
-the library's stream, algorithmic bytes per launch) and a CPU baseline (the oracle, the
-C/OpenMP restatement of the reference path, on a bounded sample of the same workload).
+value = cells * steps / wall seconds of the FP32 precision form (FFT, convolution and cell
+fields in FP32, grain state and push in FP64) with every input resident on the device;
+the FP64 plain form is timed beside it on the same workload, and the two are compared
+after 1e4 steps on a 512 x 512 x 8 crop of the same recipe.  The JSON line carries the
+roofline of the dominant kernel (HIP-event durations on the library's stream,
+algorithmic bytes per launch), a CPU baseline (the oracle, the C/OpenMP restatement of
+the reference path, on a bounded sample: steps of the 512 x 512 x 8 crop), and extra
+keys for the 256 x 256 x 4 thermal LLG workload, the 512 x 4096 x 4 recording track and
+the serving loop.
 """
 import argparse
 import json
@@ -29,7 +35,8 @@ from mrm_amd.tune import autotune  # noqa: E402
 
 METRIC = "LLG cell-updates/s FP32 vs FP64 at max|Δm|≤1e-4; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
-DT = 5e-14
+DT = 5e-14          # the LLG workloads
+DT_LLB = 1e-15      # the LLB heat-assisted write
 SEED = 1
 
 
@@ -53,25 +60,8 @@ def timed(x, steps, t0):
     return time.perf_counter() - t, d
 
 
-def col_bytes(med, P):
-    """Algorithmic bytes of one column launch: spectra in and out (12 n_lay Kc P) plus the
-    unique tensor (T_u Kc 2P, T_u = 6 n_lay (n_lay+1)/2)."""
-    nl = med.n_lay
-    kc = (med.nx // 2 + 1) * med.ny
-    tu = 6 * nl * (nl + 1) // 2
-    return 12 * nl * kc * P + tu * kc * 2 * P
-
-
-def kernel_bytes(med, P):
-    """Algorithmic bytes per launch of each kernel class: the row r2c reads the cell
-    magnetisation and writes the spectra; the column kernel see col_bytes; the row c2r
-    reads the spectra and writes the cell field; a grain stage reads ~25 P per grain
-    (state, constants, dm, thermal), the exchange CSRs (4 + P per entry), the cell index
-    map (4 per magnetic cell) and gathers/scatters the cell field/magnetisation (6 P per
-    magnetic cell)."""
-    nl, C = med.n_lay, med.cells
-    kc = (med.nx // 2 + 1) * med.ny
-    G = med.G
+def media_counts(med):
+    """Magnetic cells and exchange CSR entries of a media."""
     cm = sum(int(L.grain2gridindex[-1]) for L in med.layers)
     E = 0
     for L in med.layers:
@@ -79,19 +69,58 @@ def kernel_bytes(med, P):
             v = getattr(L, f)
             if v is not None:
                 E += int(v[-1])
+    return cm, E
+
+
+def kernel_bytes(med, P, offsets):
+    """Algorithmic bytes per launch of each kernel class.  Row r2c: the cell magnetisation
+    in (3 C P), the spectra out (6 n_lay Kc P); row c2r the reverse.  Column FFTs of the
+    split path: the spectra in and out, in place (12 n_lay Kc P).  Product (split path)
+    and fused column kernel: the spectra in and out plus the unique tensor, 2 P per complex
+    element, T_u = 6 n_lay (n_lay+1)/2 per (kx, ky) (index symmetry and layer reciprocity),
+    or 6 (2 n_lay - 1) when the stored tensor is the z-offset form and that is fewer.  A
+    grain stage: ~25 P per grain (state, constants, dm, thermal), the exchange CSRs (4 + P
+    per entry), the cell index map (4 per magnetic cell) and the cell field gathered and
+    magnetisation scattered (6 P per magnetic cell)."""
+    nl, C = med.n_lay, med.cells
+    kc = (med.nx // 2 + 1) * med.ny
+    G = med.G
+    cm, E = media_counts(med)
     grain = 25 * P * G + E * (4 + P) + 4 * cm + 6 * cm * P
     tu = 6 * nl * (nl + 1) // 2
+    if offsets:
+        tu = min(tu, 6 * (2 * nl - 1))
+    spectra = 12 * nl * kc * P
     return {
         "row_fwd": 3 * C * P + 6 * nl * kc * P,
-        "col": col_bytes(med, P),
+        "col": spectra + 2 * tu * kc * P,
         "row_inv": 6 * nl * kc * P + 3 * C * P,
         "grain1": grain,
         "grain2": grain,
-        # split column path: column FFTs in place, product streaming spectra and tensor
-        "col_fwd": 12 * nl * kc * P,
-        "prod": 12 * nl * kc * P + 2 * tu * kc * P,
-        "col_inv": 12 * nl * kc * P,
+        "col_fwd": spectra,
+        "prod": spectra + 2 * tu * kc * P,
+        "col_inv": spectra,
     }
+
+
+def kernel_table(x, med, P, steps, t0, log, tag):
+    """Per-kernel HIP-event durations over `steps` steps and their algorithmic GB/s."""
+    x.profile(True)
+    x.step(steps, t0=t0, philox_seed=SEED, diag=False)
+    st = x.kernel_stats()
+    x.profile(False)
+    kb = kernel_bytes(med, P, x.get_tune("tensor_offsets"))
+    tot = sum(ms for n, ms in st.values() if n > 0)
+    kern = {}
+    for k, (n, ms) in st.items():
+        if n > 0:
+            us = 1e3 * ms / n
+            kern[k] = {"launches": n, "avg_us": round(us, 3), "share": round(ms / tot, 4),
+                       "gbs": round(kb[k] / (us * 1e-6) / 1e9, 1),
+                       "frac": round(kb[k] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "bytes": kb[k]}
+    log(f"{tag} kernels: " + ", ".join(f"{k} {v['avg_us']:.1f}us {v['share'] * 100:.0f}% {v['frac']:.3f}"
+                                       for k, v in kern.items()))
+    return kern
 
 
 def pmc_traffic(kernel, tag):
@@ -162,42 +191,50 @@ def accuracy(med, N, steps, log):
     return out
 
 
-def track_form(log, steps=100, warmup=10):
-    """The 512 x 4096 x 4 recording-track grid (1.5 nm cells, the two-layer media, T=300 K):
-    both forms timed, and compared after the same steps."""
+def track_write(log, bits=2, dt=1e-13, bit_length=12e-7, velocity=2000.0):
+    """The recording track: 512 x 4096 x 4 cells of 1.5 nm, the two-layer media at T = 300 K
+    (Philox), a Karlqvist head (gap 20 nm, fly height 5 nm, peak 15 kOe) moving at 20 m/s
+    writes the first `bits` bits of the seeded NRZ sequence (12 nm each) in both forms with
+    the same noise; then a readback scan (Gaussian sensitivity, FWHM 15 nm) over the written
+    stretch.  Reports each form's rate over the write, the FP32 readback signal's error
+    against FP64's and max/rms |dm|."""
+    from mrm_amd import recording
     med = media.config_media("cfg2")
-    x64, N = build_ctx(med, abi.MRMG_FP64)
-    x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
+    y0 = 0.25 * med.ny * med.dy
+    pattern = recording.nrz_bits(bits, seed=7)
+    out = {"workload": f"{med.nx}x{med.ny}x{med.n_lay} cells of 1.5 nm, {med.G} grains, T=300 K, Karlqvist head "
+                       f"gap 20 nm fly 5 nm peak 15 kOe at 20 m/s, NRZ bits {pattern.tolist()} of 12 nm "
+                       f"(the first {bits} of the seeded sequence), Heun dt={dt:g} s",
+           "steps": bits * int(round(bit_length / velocity / dt))}
+    N, xs = None, {}
+    for name, prec in (("fp64", abi.MRMG_FP64), ("fp32", abi.MRMG_FP32)):
+        x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=prec)
+        if N is None:
+            N = x.build_tensor(num_images=2, return_real=True)
+        else:
+            x.set_tensor(N)
+        x.set_media(med)
+        src = lambda pol: recording.head_sources(med.nx, med.ny, med.dx, med.dy, med.dz, start=y0, polarity=pol)
+        x.step(0, diag=False)          # build the context before the clock starts
+        t = time.perf_counter()
+        tend = recording.write_nrz(x, pattern, bit_length, velocity, dt, src, t0=0.0, philox_seed=SEED)
+        el = time.perf_counter() - t
+        ys, v = recording.readback_scan(x, y0 - 2e-6, y0 + velocity * tend + 2e-6, 1e-7, 0.5 * med.nx * med.dx)
+        out[name] = {"value": med.cells * out["steps"] / el, "ms_per_step": el * 1e3 / out["steps"],
+                     "readback_peak_emu": float(np.abs(v).max())}
+        log(f"track write {name}: {el * 1e3 / out['steps']:.3f} ms/step over {out['steps']} steps")
+        xs[name] = (x, v)
     del N
-    out = {"workload": f"{med.nx}x{med.ny}x{med.n_lay} cells of 1.5 nm, {med.G} grains, T=300 K, Heun dt=50 fs",
-           "steps": steps}
-    for name, x in (("fp64", x64), ("fp32", x32)):
-        shapes = autotune(x, steps=3)
-        x.step(warmup, philox_seed=SEED, diag=False)
-        el, _ = timed(x, steps, warmup * DT)
-        out[name] = {"value": med.cells * steps / el, "ms_per_step": el * 1e3 / steps, "launch_shapes": shapes}
-        log(f"track grid {name}: {el * 1e3 / steps:.3f} ms/step, {med.cells * steps / el:.4e} cell-updates/s")
-        # per-kernel event-pair durations and algorithmic bandwidth in this bandwidth-bound regime
-        x.profile(True)
-        x.step(20, t0=(warmup + steps) * DT, philox_seed=SEED, diag=False)
-        st = x.kernel_stats()
-        x.profile(False)
-        kb = kernel_bytes(med, 8 if name == "fp64" else 4)
-        ks = {}
-        for k, (n, ms) in st.items():
-            if n > 0:
-                us = 1e3 * ms / n
-                ks[k] = {"avg_us": round(us, 2), "gbs": round(kb[k] / (us * 1e-6) / 1e9, 1),
-                         "frac": round(kb[k] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3)}
-        out[name]["kernels"] = ks
-        log(f"track grid {name} kernels: " + ", ".join(f"{k} {v['avg_us']}us {v['frac']}" for k, v in ks.items()))
-    out["speedup_fp32"] = out["fp32"]["value"] / out["fp64"]["value"]
-    dmax, drms = max_rms(x64, x32, med.n_lay)
+    v64, v32 = xs["fp64"][1], xs["fp32"][1]
+    out["readback_error_fp32_vs_fp64"] = float(np.abs(v32 - v64).max() / np.abs(v64).max())
+    dmax, drms = max_rms(xs["fp64"][0], xs["fp32"][0], med.n_lay)
     out["max_dm_fp32_vs_fp64"], out["rms_dm_fp32_vs_fp64"] = dmax, drms
-    out["steps_compared"] = warmup + steps
-    log(f"track grid fp32 x{out['speedup_fp32']:.2f}, max|dm| {dmax:.2e} rms {drms:.2e}")
-    x64.close()
-    x32.close()
+    out["speedup_fp32"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["readback_positions"] = int(v64.size)
+    log(f"track write: fp32 x{out['speedup_fp32']:.2f}, readback error {out['readback_error_fp32_vs_fp64']:.2e}, "
+        f"max|dm| {dmax:.2e} rms {drms:.2e}")
+    for x, _ in xs.values():
+        x.close()
     return out
 
 
@@ -216,43 +253,136 @@ def serving(log, n_requests=40, rate=40.0):
     return st
 
 
-def cpu_baseline(med, tensor, target_s=15.0):
-    """The oracle (C restatement, OpenMP) on the same workload: a bounded sample of steps."""
+def llb_context(med, precision, tensor=None, num_images=2):
+    """A context of the LLB heat-assisted write: the media, the moving laser spot and the
+    write field on every layer, |m| at the 300 K mean-field value, LLB Heun at DT_LLB."""
+    x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=precision)
+    if tensor is None:
+        x.build_tensor(num_images=num_images)
+    else:
+        x.set_tensor(tensor)
+    x.set_media(med)
+    srcs, ufs = media.laser_write(med.n_lay, med.nx, med.ny, med.dx, med.dy)
+    x._keep["laser"] = srcs
+    for k in range(med.n_lay):
+        x.set_source(k, 1, srcs[k])
+        x.set_ufield(k, ufs[k])
+    me = media.equilibrium_magnetisation(300.0 / 700.0)
+    for k in range(med.n_lay):
+        x.set_mag(k, np.full(med.ngs[k], me))
+    x.set_llg(dt=DT_LLB, llb=True)
+    return x
+
+
+def llb_compare(a, b, n_lay):
+    """max/rms |dm| of the directions, max |d|m||, and grains whose m_z sign differs."""
+    dmax, drms = max_rms(a, b, n_lay)
+    dmag = max(float(np.abs(a.get_mag(k) - b.get_mag(k)).max()) for k in range(n_lay))
+    sw = sum(int((np.sign(a.get_state(k)[0][:, 2]) != np.sign(b.get_state(k)[0][:, 2])).sum()) for k in range(n_lay))
+    return {"max_dm": dmax, "rms_dm": drms, "max_dmag": dmag, "switched": sw}
+
+
+def crop_accuracy(steps, log):
+    """FP32 against FP64 on a 512 x 512 x 8 crop of the LLB write after `steps` steps from
+    the same start with the same Philox noise, and, as the control for thermal chaos, two
+    FP64 runs that differ only in the summation order of the grain gathers."""
+    med = media.config_media("cfg4", nx=512, ny=512)
+    runs = {}
+    for name, prec, lanes in (("fp64", abi.MRMG_FP64, None), ("fp32", abi.MRMG_FP32, None),
+                              ("fp64_reordered", abi.MRMG_FP64, 4)):
+        x = llb_context(med, prec)
+        if lanes:
+            x.tune("lanes_per_grain", lanes)
+        x.step(steps, philox_seed=SEED, diag=False)
+        runs[name] = x
+    out = {"crop": "512x512x8 of the same recipe, laser centred", "steps": steps, "grains": med.G,
+           "fp32_vs_fp64": llb_compare(runs["fp64"], runs["fp32"], med.n_lay),
+           "fp64_vs_fp64_reordered": llb_compare(runs["fp64"], runs["fp64_reordered"], med.n_lay)}
+    for x in runs.values():
+        x.close()
+    log(f"crop accuracy after {steps} steps: fp32 vs fp64 {out['fp32_vs_fp64']}, "
+        f"fp64 vs fp64 reordered {out['fp64_vs_fp64_reordered']}")
+    return out
+
+
+def cpu_baseline(log, target_s=15.0, n=256):
+    """The oracle (C restatement, OpenMP over the host cores) on a bounded sample of the
+    headline workload: LLB Heun steps of its n x n x 8 crop with injected normals (the
+    tensor is built on the device and handed over; only the steps are timed)."""
     from oracle import oracle as O
+    med = media.config_media("cfg4", nx=n, ny=n)
+    x = Llg(n, n, med.n_lay, med.dx, med.dy, med.dz)
+    N = x.build_tensor(num_images=2, return_real=True)
+    x.close()
     g = abi.make_geom(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz)
     s = O.Sim(g)
-    s.set_tensor(tensor)
+    s.set_tensor(N)
+    del N
+    srcs, ufs = media.laser_write(med.n_lay, med.nx, med.ny, med.dx, med.dy)
     for k, L in enumerate(med.layers):
         s.set_layer(k, L)
-    for k, u in enumerate(media.ramp_field(med.n_lay)):
-        s.set_ufield(k, u)
+        s.set_source(k, 1, srcs[k])
+        s.set_ufield(k, ufs[k])
+        s.set_mag(k, np.full(med.ngs[k], media.equilibrium_magnetisation(300.0 / 700.0)))
+    log(f"cpu baseline: oracle set up on the {n}x{n}x8 crop")
     rng = np.random.default_rng(3)
-    probe = 2
-    noise = rng.standard_normal((probe, med.G, 3))
+    probe = 1
     t = time.perf_counter()
-    s.step(probe, DT, noise=noise)
+    s.step(probe, DT_LLB, noise=rng.standard_normal((probe, med.G, 3)), llb=True)
     per = (time.perf_counter() - t) / probe
-    n = int(max(2, min(400, target_s / max(per, 1e-6))))
-    noise = rng.standard_normal((n, med.G, 3))
+    steps = int(max(2, min(200, target_s / max(per, 1e-6))))
+    log(f"cpu baseline: {per:.2f} s per step, timing {steps} steps")
+    noise = rng.standard_normal((steps, med.G, 3))
     t = time.perf_counter()
-    s.step(n, DT, t0=probe * DT, noise=noise)
+    s.step(steps, DT_LLB, t0=probe * DT_LLB, noise=noise, llb=True)
     el = time.perf_counter() - t
-    return {"value": med.cells * n / el, "unit": "cell-updates/s", "cores": int(O.lib().ora_num_threads()),
+    return {"value": med.cells * steps / el, "unit": "cell-updates/s", "cores": int(O.lib().ora_num_threads()),
             "kind": "port",
-            "sample": f"{n} FP64 Heun steps of the same 256x256x4 workload (injected normals), {el:.1f} s"}
+            "sample": f"{steps} FP64 LLB Heun steps of the {n}x{n}x8 crop of the headline workload (injected "
+                      f"normals, laser and write field), {el:.1f} s; a smaller grid than the headline's, so its "
+                      f"FFTs are shorter per cell"}
+
+
+def llg_cfg1(a, log):
+    """The 256 x 256 x 4 thermal LLG workload: both forms timed, and the FP32 form's
+    accuracy against FP64 after accuracy_steps steps (T = 0 and T = 300 K)."""
+    med = media.config_media("cfg1")
+    x64, N = build_ctx(med, abi.MRMG_FP64)
+    shapes64 = autotune(x64)
+    x64.step(50, philox_seed=SEED)
+    el64, _ = timed(x64, 1000, 50 * DT)
+    v64 = med.cells * 1000 / el64
+    kern = kernel_table(x64, med, 8, 200, 1050 * DT, log, "cfg1 fp64")
+    x64.close()
+    x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
+    shapes32 = autotune(x32)
+    x32.step(50, philox_seed=SEED)
+    el32, _ = timed(x32, 1000, 50 * DT)
+    v32 = med.cells * 1000 / el32
+    x32.close()
+    log(f"cfg1: fp64 {v64:.4e}, fp32 {v32:.4e} cell-updates/s (x{v32 / v64:.2f})")
+    out = {"workload": "256x256x4 cells of 2 nm, two coupled 4 nm layers, T=300 K, Heun dt=50 fs, 1000 steps",
+           "fp64": {"value": v64, "ms_per_step": el64, "launch_shapes": shapes64, "kernels": kern},
+           "fp32": {"value": v32, "ms_per_step": el32, "launch_shapes": shapes32},
+           "speedup_fp32": v32 / v64}
+    out["fp64"]["ms_per_step"] = el64 * 1e3 / 1000
+    out["fp32"]["ms_per_step"] = el32 * 1e3 / 1000
+    if a.accuracy_steps > 0:
+        out["accuracy"] = accuracy(med, N, a.accuracy_steps, log)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
-    ap.add_argument("--no-track", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the cfg1, track and serving legs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -260,84 +390,76 @@ def main():
         return 2
     log = (lambda *x: None) if a.quiet else (lambda *x: print(*x, file=sys.stderr, flush=True))
 
-    med = media.config_media("cfg1")
-    log(f"workload cfg1: {med.nx}x{med.ny}x{med.n_lay} cells, {med.G} grains")
-    x64, N = build_ctx(med, abi.MRMG_FP64)
-    shapes64 = autotune(x64)
-    log(f"fp64 launch shapes: {shapes64}")
-    x64.step(a.warmup, philox_seed=SEED)
-    t0 = a.warmup * DT
-    el64, _ = timed(x64, a.steps, t0)
-    v64 = med.cells * a.steps / el64
-    log(f"fp64: {el64 * 1e3 / a.steps:.4f} ms/step, {v64:.4e} cell-updates/s")
-
-    # per-kernel durations with HIP events on the library's stream, same workload
-    x64.profile(True)
-    x64.step(a.steps, t0=t0 + a.steps * DT, philox_seed=SEED)
-    st = x64.kernel_stats()
-    x64.profile(False)
-    kb = kernel_bytes(med, 8)
-    kern = {k: {"launches": n, "avg_us": 1e3 * ms / max(n, 1), "share": 0.0} for k, (n, ms) in st.items() if n > 0}
-    tot = sum(ms for n, ms in st.values() if n > 0)
-    for k in kern:
-        kern[k]["share"] = st[k][1] / tot if tot else 0.0
-        kern[k]["gbs"] = kb[k] / (kern[k]["avg_us"] * 1e-6) / 1e9 if kern[k]["avg_us"] > 0 else 0.0
+    med = media.config_media("cfg4")
+    log(f"workload cfg4: {med.nx}x{med.ny}x{med.n_lay} cells, {med.G} grains")
+    forms = {}
+    for name, prec in (("fp32", abi.MRMG_FP32), ("fp64", abi.MRMG_FP64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        t = time.perf_counter()
+        x = llb_context(med, prec)
+        setup = time.perf_counter() - t
+        shapes = {} if a.no_tune else autotune(x, steps=3)
+        x.step(a.warmup, philox_seed=SEED)
+        el, _ = timed(x, a.steps, a.warmup * DT_LLB)
+        v = med.cells * a.steps / el
+        log(f"cfg4 {name}: {el * 1e3 / a.steps:.3f} ms/step, {v:.4e} cell-updates/s (setup {setup:.1f} s)")
+        P = 4 if name == "fp32" else 8
+        kern = kernel_table(x, med, P, 10, (a.warmup + a.steps) * DT_LLB, log, f"cfg4 {name}")
+        forms[name] = {"x": x, "value": v, "el": el, "shapes": shapes, "kernels": kern, "setup_s": setup,
+                       "footprint_bytes": x.footprint(), "offsets": x.get_tune("tensor_offsets")}
+        if name == "fp64":
+            x.close()
+    f32 = forms["fp32"]
+    x = f32["x"]
+    kern = f32["kernels"]
     dom = max(kern, key=lambda k: kern[k]["share"])
-    log("kernels: " + ", ".join(f"{k} {v['avg_us']:.2f}us {v['share'] * 100:.0f}% {v['gbs']:.0f}GB/s" for k, v in kern.items()))
-    # avg_launch_us: the event-pair average over the profiled steps of the workload (it
-    # includes the pairs' own overhead, so it errs long).  back_to_back_us: the same kernel
-    # launched 200 times between two events, its operands warm in the caches (errs short).
     avg_us = kern[dom]["avg_us"]
     b2b = None
     try:
-        b2b = round(1e3 * x64.time_kernel(dom, reps=200), 3)
+        b2b = round(1e3 * x.time_kernel(dom, reps=50), 3)
     except Exception as e:  # a grain class is not re-launchable on its own
         log("back-to-back timing unavailable for", dom, e)
-    ach = kb[dom] / (avg_us * 1e-6) / 1e9
+    ach = kern[dom]["bytes"] / (avg_us * 1e-6) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, "cfg1_fp64"),
-            "bytes_per_launch": kb[dom], "avg_launch_us": round(avg_us, 3), "back_to_back_us": b2b,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, "cfg4_fp32"),
+            "bytes_per_launch": kern[dom]["bytes"], "avg_launch_us": avg_us, "back_to_back_us": b2b,
             "copy_ceiling_gbs": None}
     try:
         nb = 1 << 30
-        ms = x64.copy_probe(nb, reps=10)
+        ms = x.copy_probe(nb, reps=10)
         roof["copy_ceiling_gbs"] = round(2 * nb / (ms * 1e-3) / 1e9, 1)
     except Exception as e:  # pragma: no cover
         log("copy probe failed:", e)
+    x.close()
 
-    out = {"metric": METRIC, "value": v64, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": el64 * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64",
+    el = f32["el"]
+    out = {"metric": METRIC, "value": f32["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": el * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded Voronoi media, Philox thermal field)",
-           "config": {"workload": "cfg1: 256x256x4 cells of 2 nm, two coupled 4 nm layers, T=300 K, "
-                                  "Heun dt=50 fs, periodic FFT demag, FP64 plain form",
+           "config": {"workload": "cfg4: LLB heat-assisted write, 2048x2048x8 cells of 1 nm, FePt-like Tc=700 K, "
+                                  "Gaussian laser FWHM 40 nm peak 750 K at 15 m/s, 8 kOe, 300 K ambient, "
+                                  "LLB Heun dt=1 fs, periodic FFT demag (tensor built in FP64), FP32 form",
                       "grid": [med.nx, med.ny, med.n_lay], "cells": med.cells, "grains": med.G,
-                      "launch_shapes": shapes64,
-                      "grain_updates_per_s": med.G * a.steps / el64},
+                      "launch_shapes": f32["shapes"], "tensor_offsets": f32["offsets"],
+                      "footprint_bytes": f32["footprint_bytes"], "setup_s": round(f32["setup_s"], 2),
+                      "grain_updates_per_s": med.G * a.steps / el},
            "roofline": roof,
-           "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()}}
-
-    if not a.no_fp32:
-        x32, _ = build_ctx(med, abi.MRMG_FP32, tensor=N)
-        shapes32 = autotune(x32)
-        log(f"fp32 launch shapes: {shapes32}")
-        x32.step(a.warmup, philox_seed=SEED)
-        el32, _ = timed(x32, a.steps, t0)
-        v32 = med.cells * a.steps / el32
-        # accuracy of the precision form: same start, same noise, same number of steps
-        x32.close()
-        log(f"fp32: {el32 * 1e3 / a.steps:.4f} ms/step, {v32:.4e} cell-updates/s, x{v32 / v64:.2f}")
-        acc = accuracy(med, N, a.accuracy_steps, log)
-        out["fp32_form"] = {"value": v32, "unit": "cell-updates/s", "ms_per_step": el32 * 1e3 / a.steps,
-                            "speedup_vs_fp64": v32 / v64, "max_dm_vs_fp64_T0": acc["T0_fp32_vs_fp64"]["max_dm"],
-                            "accuracy": acc, "launch_shapes": shapes32}
-    if not a.no_track:
-        out["track_grid"] = track_form(log)
-    if not a.no_serve:
+           "kernels": kern}
+    if "fp64" in forms:
+        f64 = forms["fp64"]
+        out["fp64_form"] = {"value": f64["value"], "unit": "cell-updates/s", "ms_per_step": f64["el"] * 1e3 / a.steps,
+                            "fp32_speedup": f32["value"] / f64["value"], "launch_shapes": f64["shapes"],
+                            "kernels": f64["kernels"], "footprint_bytes": f64["footprint_bytes"]}
+    if a.accuracy_steps > 0:
+        out["fp32_accuracy"] = crop_accuracy(a.accuracy_steps, log)
+    if not a.no_extra:
+        out["llg_cfg1"] = llg_cfg1(a, log)
+        out["track_write"] = track_write(log)
         out["serving"] = serving(log)
-
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(med, N)
+        out["cpu_baseline"] = cpu_baseline(log)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
     return 0

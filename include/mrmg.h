@@ -368,8 +368,8 @@ int mrmg_kernel_stats(mrmg_ctx *ctx, mrmg_kstat *out, int max, int *n);
 /* `reps` back-to-back launches of one demag kernel class ("row_fwd", "col", "row_inv" on
  * the fused paths; "row_fwd", "col_fwd", "prod", "col_inv", "row_inv" on the split path)
  * between two events on the context's stream; *ms receives the average duration, free of
- * the per-launch event pair that mrmg_profile adds.  Writes only scratch spectra and
- * cell fields. */
+ * the per-launch event pair that mrmg_profile adds.  Writes only the spectra and the
+ * field's run sums, which every field evaluation rewrites before it reads them. */
 int mrmg_time_kernel(mrmg_ctx *ctx, const char *name, int reps, double *ms);
 /* Device-to-device copy of `bytes` bytes on the context's stream with the library's own
  * streaming copy kernel, `reps` times; *ms receives the average duration.  Used as the

@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(const C2<T
     const int per = (nl + kg - 1) / kg;
     const int ko0 = grp * per, ko1 = ko0 + per < nl ? ko0 + per : nl;
     if (ko0 >= ko1) return;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int ncol = 3 * nl;
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + ny);

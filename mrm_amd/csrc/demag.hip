@@ -7,30 +7,83 @@
 #error "compile with -DMRMG_FORM=float or -DMRMG_FORM=double"
 #endif
 
+// ------------------------------------------------------------------------- runs <-> LDS rows
+// The cell magnetisation of rows y0 .. y0+nrows-1 of layer k into LDS, component c0+ci of
+// NC at row pair p = (y - y0)/2: re holds the even rows, im the odd ones, at element
+// (ci*np + p)*nx + x (get_mag, data.f90:344-353: Ms ms_scale m on a grain's cells, 0 on
+// non-magnetic cells).  Mg[c][g] is the grain's cell value.  Also stages the row
+// twiddles.  roff (shared, nrows+1 entries) keeps the block's run offsets for the store.
+template <typename T, int NC>
+__device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restrict__ Mg, int G, int c0, int k, int ny,
+                                            int y0, int nrows, int np, int nx, T* re, T* im, int* roff, C2<T>* tw,
+                                            const C2<T>* __restrict__ twx)
+{
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    if (tid <= nrows) roff[tid] = R.row_off[k * ny + min(y0 + tid, ny)];
+    for (int i = tid; i < nx; i += nt) tw[i] = twx[i];
+    const int n = NC * np * nx;
+    for (int i = tid; i < n; i += nt) {
+        re[LP(i)] = (T)0;
+        im[LP(i)] = (T)0;
+    }
+    __syncthreads();
+    const int q1 = roff[nrows];
+    for (int q = roff[0] + tid; q < q1; q += nt) {
+        int yy = 0;
+        while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
+        const int geo = R.geo[q], g = R.grain[q];
+        const int i0 = geo & 0xffff, len = geo >> 16;
+        T* dst = (yy & 1) ? im : re;
+#pragma unroll
+        for (int ci = 0; ci < NC; ci++) {
+            const T v = Mg[(size_t)(c0 + ci) * G + g];
+            const int b = (ci * np + (yy >> 1)) * nx + i0;
+            for (int t = 0; t < len; t++) dst[LP(b + t)] = v;
+        }
+    }
+}
+
+// The cell field of the same rows, summed over each run (in x order, in FP64) into
+// Hr[c][run] (c = c0+ci), with the layout of runs_to_lds.
+template <typename T, int NC>
+__device__ __forceinline__ void lds_to_runs(const RowRuns& R, double* __restrict__ Hr, int c0, int nrows, int np,
+                                            int nx, const T* re, const T* im, const int* roff)
+{
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    const int q1 = roff[nrows];
+    for (int q = roff[0] + tid; q < q1; q += nt) {
+        int yy = 0;
+        while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
+        const int geo = R.geo[q], r = R.gm[q];
+        const int i0 = geo & 0xffff, len = geo >> 16;
+        const T* src = (yy & 1) ? im : re;
+#pragma unroll
+        for (int ci = 0; ci < NC; ci++) {
+            const int b = (ci * np + (yy >> 1)) * nx + i0;
+            double s = 0.0;
+            for (int t = 0; t < len; t++) s += (double)src[LP(b + t)];
+            Hr[(size_t)(c0 + ci) * R.nruns + r] = s;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------- demag kernels
 // Row r2c: one workgroup per (row pair, layer).  Two rows of each component of the cell
 // magnetisation (written by the grain kernels, get_mag data.f90:344-353) are packed as
 // z = M(y0) + i M(y1), transformed with one complex FFT and separated.
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mg, int G, RowRuns R, C2<T>* __restrict__ S,
                                                  int nx, int ny, int nl, int nxh, FFTPlan px,
                                                  const C2<T>* __restrict__ twx)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int roff[3];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(3 * nx);
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
-    const size_t CP = (size_t)nl * ny * nx;
-    const T* r0 = Mcell + ((size_t)k * ny + y0) * nx;
-    staged_load<(E < 8 ? E : 8)>(
-        3 * nx,
-        [&](int idx) {
-            const int c = idx / nx, x = idx - c * nx;
-            return C2<T>{r0[c * CP + x], r0[c * CP + (has1 ? nx : 0) + x]};
-        },
-        [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = has1 ? v.y : (T)0; }, tw, twx, nx);
+    runs_to_lds<T, 3>(R, Mg, G, 0, k, ny, y0, has1 ? 2 : 1, 1, nx, re, im, roff, tw, twx);
     __syncthreads();
     lds_fft<T, E, false>(re, im, 3, px, tw);
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
@@ -47,12 +100,15 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mcell, C2
 
 
 // Row c2r: FFTW's 1D c2r semantics (imaginary parts of the DC and Nyquist bins ignored).
-template <typename T, int E>
+// CELLS: the cell field into Hcell [c][k][y][x] (mrmg_demag_field), else its run sums into Hr.
+template <typename T, int E, bool CELLS = false>
 __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T* __restrict__ Hcell,
+                                                 double* __restrict__ Hr, RowRuns R,
                                                  int nx, int ny, int nl, int nxh, FFTPlan px,
                                                  const C2<T>* __restrict__ twx)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int roff[3];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     T* re = reinterpret_cast<T*>(tw + nx);
     T* im = re + LPN(3 * nx);
@@ -78,7 +134,12 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
         },
         tw, twx, nx);
     __syncthreads();
+    if (!CELLS && threadIdx.x <= (has1 ? 2 : 1)) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
     lds_fft<T, E, true>(re, im, 3, px, tw);
+    if (!CELLS) {
+        lds_to_runs<T, 3>(R, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
+        return;
+    }
     const size_t plane = (size_t)nx * ny;
     for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
         const int c = idx / nx, x = idx - c * nx;
@@ -97,11 +158,12 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
 //   k_colT      inverse y-FFT, in place
 //   k_rowT_inv  c2r of TY rows, read back from TY-long y runs
 template <typename T, int E, bool WV = false>
-__global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C2<T>* __restrict__ S,
+__global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mg, int G, RowRuns R, C2<T>* __restrict__ S,
                                                   int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int roff[9];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
@@ -113,21 +175,7 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mcell, C
     if (L >= nyb * 3 * nl) return;
     const int y0 = (L % nyb) * TY, kc = L / nyb;        // kc = k*3 + c
     const int k = kc / 3, c = kc - 3 * k;
-    const size_t CP = (size_t)nl * ny * nx;
-    const T* src = Mcell + c * CP + (size_t)k * ny * nx;
-    staged_load<(E < 8 ? E : 8)>(
-        B * nx,
-        [&](int idx) {
-            const int p = idx / nx, x = idx - p * nx;
-            const int ya = y0 + 2 * p, yb = ya + 1;
-            return C2<T>{src[(size_t)(ya < ny ? ya : ny - 1) * nx + x], src[(size_t)(yb < ny ? yb : ny - 1) * nx + x]};
-        },
-        [&](int idx, C2<T> v) {
-            const int p = idx / nx, ya = y0 + 2 * p;
-            re[LP(idx)] = ya < ny ? v.x : (T)0;
-            im[LP(idx)] = ya + 1 < ny ? v.y : (T)0;
-        },
-        tw, twx, nx);
+    runs_to_lds<T, 1>(R, Mg, G, c, k, ny, y0, min(TY, ny - y0), B, nx, re, im, roff, tw, twx);
     __syncthreads();
     if (WV) {
         lds_fft_wave<T, E, false>(re, im, B, px, tw);
@@ -156,7 +204,7 @@ template <typename T, int E, bool INV>
 __global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int ncols, int CB, FFTPlan py,
                                               const C2<T>* __restrict__ twy)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T* re = reinterpret_cast<T*>(smem_raw);
     const int q0 = blockIdx.x * CB;
     const int nb = min(CB, ncols - q0);
@@ -232,12 +280,14 @@ __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T
 
 // c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
 // imaginary parts of the DC and Nyquist bins ignored).
-template <typename T, int E, bool WV = false>
+template <typename T, int E, bool WV = false, bool CELLS = false>
 __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T* __restrict__ Hcell,
+                                                  double* __restrict__ Hr, RowRuns R,
                                                   int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int roff[9];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
     const int B = TY / 2;
     T* re = reinterpret_cast<T*>(tw + nx);
@@ -270,11 +320,17 @@ __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T
         },
         tw, twx, nx);
     __syncthreads();
+    const int nrows = min(TY, ny - y0);
+    if (!CELLS && (int)threadIdx.x <= nrows) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
     if (WV) {
         lds_fft_wave<T, E, true>(re, im, B, px, tw);
         __syncthreads();
     } else {
         lds_fft<T, E, true>(re, im, B, px, tw);
+    }
+    if (!CELLS) {
+        lds_to_runs<T, 1>(R, Hr, c, nrows, B, nx, re, im, roff);
+        return;
     }
     T* hc = Hcell + ((size_t)c * nl + k) * (size_t)nx * ny;
     for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
@@ -291,6 +347,7 @@ template <int MB> int Engine<T>::lds_attr_mb()
     const int r = (int)lds_row;
     HIPCHK(hipFuncSetAttribute((const void*)k_row_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     RCHK(lds_attr_col<MB>());
     return MRMG_OK;
 }
@@ -303,6 +360,10 @@ template <int MB> int Engine<T>::lds_attr_split()
     HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               r));
     HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     return MRMG_OK;
@@ -378,10 +439,10 @@ template <typename T>
 template <int MB> void Engine<T>::launch_rowT_fwd(dim3 g, dim3 b)
 {
     if (wv)
-        hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
-                           c->nxh, TYr, prow, twx.p);
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny,
+                           c->nl, c->nxh, TYr, prow, twx.p);
     else
-        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl,
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny, c->nl,
                            c->nxh, TYr, prow, twx.p);
 }
 
@@ -402,30 +463,38 @@ template <int MB> void Engine<T>::launch_colT_inv(dim3 g, dim3 b)
 template <typename T>
 template <int MB> void Engine<T>::launch_rowT_inv(dim3 g, dim3 b)
 {
-    if (wv)
-        hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
-                           c->nxh, TYr, prow, twx.p);
+    const RowRuns R = runs();
+    if (wv && cells_out)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, true, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx,
+                           c->ny, c->nl, c->nxh, TYr, prow, twx.p);
+    else if (wv)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx, c->ny,
+                           c->nl, c->nxh, TYr, prow, twx.p);
+    else if (cells_out)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, false, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx,
+                           c->ny, c->nl, c->nxh, TYr, prow, twx.p);
     else
-        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl,
+        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx, c->ny, c->nl,
                            c->nxh, TYr, prow, twx.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_row_fwd(dim3 g, dim3 b)
 {
-    hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mcell.p, S1.p, c->nx, c->ny, c->nl, c->nxh,
-                       prow, twx.p);
+    hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny, c->nl,
+                       c->nxh, prow, twx.p);
 }
-
-
 
 template <typename T>
 template <int MB> void Engine<T>::launch_row_inv(dim3 g, dim3 b)
 {
-    hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, c->nx, c->ny, c->nl, c->nxh,
-                       prow, twx.p);
+    if (cells_out)
+        hipLaunchKernelGGL((k_row_inv<T, MB, true>), g, b, lds_row, c->stream, S2.p, Hcell.p, Hr.p, runs(), c->nx,
+                           c->ny, c->nl, c->nxh, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, Hr.p, runs(), c->nx, c->ny,
+                           c->nl, c->nxh, prow, twx.p);
 }
-
 
 // each precision form is its own object (the unit is compiled with MRMG_FORM=float and
 // with MRMG_FORM=double)

@@ -178,6 +178,7 @@ struct EngineBase {
     virtual int set_mag(int k, const double* mag) = 0;
     virtual int get_mag(int k, double* mag) = 0;
     virtual int update_ufield(std::vector<HostLayer>& L) = 0;
+    virtual int update_sources(std::vector<HostLayer>& L) = 0;
     virtual int step(const mrmg_step_params* p, const double* noise, mrmg_diag* diag) = 0;
     virtual int demag_field(double* h_cell, double* h_grain) = 0;
     virtual size_t footprint() const = 0;
@@ -260,7 +261,11 @@ struct Engine : EngineBase {
     int G = 0, Ctot = 0, nslot = 0;
     std::vector<int> off;
     // grain data
-    DBuf<int> layer_of, gc_off, gc_idx, cell2grain;
+    DBuf<int> layer_of, gc_off, cell2grain;
+    // runs (RowRuns): grain by grain (run_off per grain, run_geo, run_row) and row by row
+    // (rrow_off per layer row, rrm_geo, rrm_grain, rrm_gm)
+    DBuf<int> run_off, run_geo, run_row, rrow_off, rrm_geo, rrm_grain, rrm_gm;
+    int nruns = 0;
     DBuf<int> x_off, x_nb;
     DBuf<S> x_w;
     DBuf<S> ms, msf, ms_scale, alpha_scale, hk_scale, hk1, hk2, uni, cub, happ, sigma, sigma_llb, vol;
@@ -278,7 +283,12 @@ struct Engine : EngineBase {
     bool any_src = false;
     // demag
     DBuf<C2<T>> S1, S2, Tspec, twx, twy;
-    DBuf<T> Hcell, Mcell;
+    // Mg: the cell value Ms ms_scale m of every grain, [3][G]; Hr: the cell field summed
+    // over every run, [3][nruns], FP64; Hcell: the cell field [c][k][y][x], only for
+    // mrmg_demag_field (cells_out)
+    DBuf<T> Mg, Hcell;
+    DBuf<double> Hr;
+    bool cells_out = false;
     DBuf<int> slot2c;
     // launch shapes (mrmg_tune): threads of the row and column workgroups, observation-layer
     // groups per column workgroup, lanes per grain in the grain kernels
@@ -360,13 +370,15 @@ struct Engine : EngineBase {
     size_t footprint() const override
     {
         size_t b = 0;
-        b += layer_of.bytes() + gc_off.bytes() + gc_idx.bytes() + cell2grain.bytes();
+        b += layer_of.bytes() + gc_off.bytes() + cell2grain.bytes();
+        b += run_off.bytes() + run_geo.bytes() + run_row.bytes() + rrow_off.bytes() + rrm_geo.bytes() + rrm_grain.bytes() +
+             rrm_gm.bytes() + Mg.bytes() + Hr.bytes();
         b += x_off.bytes() + x_nb.bytes() + x_w.bytes();
         b += ms.bytes() + msf.bytes() + ms_scale.bytes() + alpha_scale.bytes() + hk_scale.bytes() + hk1.bytes() +
              hk2.bytes() + uni.bytes() + cub.bytes() + happ.bytes() + sigma.bytes() + sigma_llb.bytes() + vol.bytes() +
              lp.bytes() + tc.bytes() + lam.bytes() + temp.bytes();
         b += ma.bytes() + mb.bytes() + mc.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
-        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() + Mcell.bytes() +
+        b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() +
              slot2c.bytes();
         b += noise_buf.bytes() + diag_part.bytes();
         return b;
@@ -412,10 +424,6 @@ struct Engine : EngineBase {
         RCHK(lds_attr());
         RCHK(S1.alloc((size_t)nl * 3 * ny * nxh));
         RCHK(S2.alloc((size_t)nl * 3 * ny * nxh));
-        RCHK(Hcell.alloc((size_t)3 * nl * nx * ny));
-        HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
-        RCHK(Mcell.alloc((size_t)3 * nl * nx * ny));
-        HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
         kg = 1;
         return MRMG_OK;
     }
@@ -661,8 +669,7 @@ struct Engine : EngineBase {
         auto upT = [&](DBuf<S>& d, const std::vector<double>& v) -> int { return d.upload(v.data(), v.size(), c->stream); };
         RCHK(layer_of.upload(h_layer.data(), G, c->stream));
         RCHK(gc_off.upload(h_gco.data(), G + 1, c->stream));
-        h_gci.push_back(0);   // spare entry
-        RCHK(gc_idx.upload(h_gci.data(), h_gci.size(), c->stream));
+        RCHK(build_runs(h_gco, h_gci, h_layer));
         RCHK(cell2grain.upload(h_c2g.data(), h_c2g.size(), c->stream));
         RCHK(upT(ms, h_ms)); RCHK(upT(msf, h_msf)); RCHK(upT(ms_scale, h_mss)); RCHK(upT(alpha_scale, h_as));
         RCHK(upT(hk_scale, h_hks)); RCHK(upT(hk1, h_hk1)); RCHK(upT(hk2, h_hk2)); RCHK(upT(uni, h_uni));
@@ -682,9 +689,83 @@ struct Engine : EngineBase {
         HIPCHK(hipMemsetAsync(h.p, 0, h.bytes(), c->stream));
         RCHK(diag_part.alloc((size_t)(((size_t)G * 16 + 255) / 256) * DIAG_STRIDE));
         RCHK(stats.alloc((size_t)10 * nl));
-        HIPCHK(hipMemsetAsync(Mcell.p, 0, Mcell.bytes(), c->stream));
+        RCHK(Mg.alloc(3 * (size_t)G));
+        HIPCHK(hipMemsetAsync(Mg.p, 0, Mg.bytes(), c->stream));
+        RCHK(Hr.alloc(3 * (size_t)nruns));
+        HIPCHK(hipMemsetAsync(Hr.p, 0, Hr.bytes(), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return MRMG_OK;
+    }
+
+    // Runs from the grains' cells (gci: global cell k*P + j*nx + i, grain by grain): each
+    // grain's cells in (row, x) order, split where the row ends or x jumps.  A cell that
+    // two grains claim is refused.
+    int build_runs(const std::vector<int>& gco, const std::vector<int>& gci, const std::vector<int>& layer)
+    {
+        const int nx = c->nx, ny = c->ny, nl = c->nl;
+        const long long P = (long long)nx * ny;
+        std::vector<int> roff(G + 1, 0), geo, row;
+        std::vector<long long> key;
+        for (int g = 0; g < G; g++) {
+            key.assign(gci.begin() + gco[g], gci.begin() + gco[g + 1]);
+            std::sort(key.begin(), key.end());
+            for (size_t a = 0; a < key.size();) {
+                size_t b = a + 1;
+                while (b < key.size() && key[b] == key[b - 1] + 1 && key[b] % nx != 0) b++;
+                if (b < key.size() && key[b] == key[b - 1])
+                    return set_err(MRMG_ERR_ARG, "layer %d grain %d lists cell %lld twice", layer[g], g + 1, key[b] % P);
+                const int cell = (int)(key[a] % P);
+                geo.push_back((cell % nx) | ((int)(b - a) << 16));
+                row.push_back(cell / nx);
+                a = b;
+            }
+            roff[g + 1] = (int)geo.size();
+        }
+        nruns = (int)geo.size();
+        // the same runs row by row, each row's runs in grain order
+        std::vector<int> rowoff((size_t)nl * ny + 1, 0);
+        std::vector<int> grain_of(nruns);
+        for (int g = 0; g < G; g++)
+            for (int r = roff[g]; r < roff[g + 1]; r++) {
+                grain_of[r] = g;
+                rowoff[(size_t)layer[g] * ny + row[r] + 1]++;
+            }
+        for (size_t i = 1; i < rowoff.size(); i++) rowoff[i] += rowoff[i - 1];
+        std::vector<int> cur(rowoff.begin(), rowoff.end() - 1), rgeo(nruns), rgrain(nruns), rgm(nruns);
+        std::vector<char> seen((size_t)nl * P, 0);
+        for (int r = 0; r < nruns; r++) {
+            const int g = grain_of[r];
+            const size_t rw = (size_t)layer[g] * ny + row[r];
+            const int i0 = geo[r] & 0xffff, len = geo[r] >> 16;
+            for (int i = i0; i < i0 + len; i++) {
+                char& f = seen[rw * nx + i];
+                if (f) return set_err(MRMG_ERR_ARG, "layer %d: cell (%d,%d) belongs to two grains", layer[g], i + 1, row[r] + 1);
+                f = 1;
+            }
+            const int q = cur[rw]++;
+            rgeo[q] = geo[r];
+            rgrain[q] = g;
+            rgm[q] = r;
+        }
+        geo.push_back(0);   // no array is empty
+        row.push_back(0);
+        rgeo.push_back(0); rgrain.push_back(0); rgm.push_back(0);
+        RCHK(run_off.upload(roff.data(), roff.size(), c->stream));
+        RCHK(run_geo.upload(geo.data(), geo.size(), c->stream));
+        RCHK(run_row.upload(row.data(), row.size(), c->stream));
+        RCHK(rrow_off.upload(rowoff.data(), rowoff.size(), c->stream));
+        RCHK(rrm_geo.upload(rgeo.data(), rgeo.size(), c->stream));
+        RCHK(rrm_grain.upload(rgrain.data(), rgrain.size(), c->stream));
+        RCHK(rrm_gm.upload(rgm.data(), rgm.size(), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+        return MRMG_OK;
+    }
+
+    RowRuns runs() const
+    {
+        RowRuns R;
+        R.row_off = rrow_off.p; R.geo = rrm_geo.p; R.grain = rrm_grain.p; R.gm = rrm_gm.p; R.nruns = nruns;
+        return R;
     }
 
     // the moving sources of every layer into LayerParams and the device tables
@@ -756,6 +837,11 @@ struct Engine : EngineBase {
     }
 
     int update_ufield(std::vector<HostLayer>& L) override { return fill_ufield(L); }
+    int update_sources(std::vector<HostLayer>& L) override
+    {
+        RCHK(fill_sources(L));
+        return fill_ufield(L);
+    }
 
     int set_tensor_planes(const double* d_real, bool toeplitz) override;
 
@@ -810,7 +896,8 @@ struct Engine : EngineBase {
         d.layer_of = layer_of.p;
         d.ms = ms.p; d.msf = msf.p; d.ms_scale = ms_scale.p; d.alpha_scale = alpha_scale.p; d.hk_scale = hk_scale.p;
         d.hk1 = hk1.p; d.hk2 = hk2.p; d.uni = uni.p; d.cub = cub.p; d.happ = has_happ ? happ.p : nullptr;
-        d.sigma = sigma.p; d.vol = vol.p; d.gc_off = gc_off.p; d.gc_idx = gc_idx.p;
+        d.sigma = sigma.p; d.vol = vol.p; d.gc_off = gc_off.p;
+        d.run_off = run_off.p; d.run_geo = run_geo.p; d.run_row = run_row.p; d.nruns = nruns;
         d.x_off = x_off.p; d.x_nb = x_nb.p; d.x_w = x_w.p;
         d.nl = c->nl;
         d.lp = lp.p;
@@ -899,7 +986,7 @@ struct Engine : EngineBase {
 
     // reps back-to-back launches of one demag kernel class between two events on the
     // context's stream: the kernel's average duration without a per-launch event pair.
-    // Only the scratch spectra and cell fields are written; the state is untouched.
+    // Only the spectra and the run sums are written; the state is untouched.
     int time_class(int cls, int reps, double* ms) override
     {
         if (!demag_on()) return set_err(MRMG_ERR_STATE, "demag is off or has no tensor");
@@ -1267,21 +1354,23 @@ struct Engine : EngineBase {
     int demag_field(double* h_cell, double* h_grain) override
     {
         const size_t CP = (size_t)c->nl * c->nx * c->ny;
+        // the cell field is written to memory for this call only (the step keeps run sums)
+        if (Hcell.n < 3 * CP) RCHK(Hcell.alloc(3 * CP));
         if (demag_on()) {
             RCHK(scatter(ma.p));
-            RCHK(demag());
+            cells_out = true;
+            const int rc = demag();
+            cells_out = false;
+            RCHK(rc);
         } else {
             HIPCHK(hipMemsetAsync(Hcell.p, 0, Hcell.bytes(), c->stream));
         }
+        std::vector<T> t(3 * CP);
+        HIPCHK(hipMemcpyAsync(t.data(), Hcell.p, t.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (h_cell) {
-            std::vector<T> t(3 * CP);
-            HIPCHK(hipMemcpy(t.data(), Hcell.p, t.size() * sizeof(T), hipMemcpyDeviceToHost));
+        if (h_cell)
             for (size_t i = 0; i < t.size(); i++) h_cell[i] = (double)t[i];
-        }
         if (h_grain) {
-            std::vector<T> t(3 * CP);
-            HIPCHK(hipMemcpy(t.data(), Hcell.p, t.size() * sizeof(T), hipMemcpyDeviceToHost));
             int g = 0;
             for (int k = 0; k < c->nl; k++) {
                 const HostLayer& H = c->layers[k];

@@ -107,12 +107,8 @@ __device__ __forceinline__ double scale_at_t(const LayerParams& L, const double*
 // spare entry at the end) and add nothing.
 // Per-grain offsets and field constants, loaded by the caller with the grain's state.
 template <typename T> struct GrainPre {
-    int b0, b1, x0, x1;   // cell range (grain2gridindex) and exchange CSR range
+    int b0, b1, r0, r1, x0, x1;   // cell range (grain2gridindex: its count), run range, exchange CSR range
     T hk_s, hk1, hk2, u[3], happ[3];
-    // the lane's cells of the first gather round, for the scatter (valid when the grain
-    // needed a single round)
-    int ca, cb;
-    bool one_round;
 };
 
 template <typename T>
@@ -120,6 +116,7 @@ __device__ __forceinline__ void grain_pre(const GrainData<T>& gd, int g, GrainPr
 {
     const int G = gd.G;
     P.b0 = gd.gc_off[g]; P.b1 = gd.gc_off[g + 1];
+    P.r0 = gd.run_off[g]; P.r1 = gd.run_off[g + 1];
     P.x0 = gd.x_off[g]; P.x1 = gd.x_off[g + 1];
     P.hk_s = gd.hk_scale[g]; P.hk1 = gd.hk1[g]; P.hk2 = gd.hk2[g];
     P.u[0] = gd.uni[g]; P.u[1] = gd.uni[G + g]; P.u[2] = gd.uni[2 * G + g];
@@ -127,13 +124,11 @@ __device__ __forceinline__ void grain_pre(const GrainData<T>& gd, int g, GrainPr
     if (gd.happ) {
         P.happ[0] = gd.happ[g]; P.happ[1] = gd.happ[G + g]; P.happ[2] = gd.happ[2 * G + g];
     }
-    P.ca = P.cb = 0;
-    P.one_round = false;
 }
 
 template <typename T, int LPG, typename TC>
 __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerParams& L, int k, int g, int lane,
-                                         const T* __restrict__ mf, const TC* __restrict__ Hcell, size_t CP,
+                                         const T* __restrict__ mf, const double* __restrict__ Hr,
                                          double t, const T* m, T* h, double* tout, GrainPre<T>& P)
 {
     const int G = gd.G;
@@ -143,28 +138,29 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
     const T u[3] = {P.u[0], P.u[1], P.u[2]};
     T happ[3] = {P.happ[0], P.happ[1], P.happ[2]};
     const T* __restrict__ msc = mf + (size_t)ST_MSC * G;
-    // get_magnetostatic gather (field.f90:463-468) and the exchange CSR (field.f90:326-363:
-    // intralayer, then the layers above and below, merged per grain)
+    // get_magnetostatic gather (field.f90:463-468), summed over the grain's runs (each run
+    // the sum of its cells, written by the row c2r), and the exchange CSR
+    // (field.f90:326-363: intralayer, then the layers above and below, merged per grain)
+    const int r0 = P.r0, r1 = P.r1;
+    const size_t NR = (size_t)gd.nruns;
     T hd[3] = {0, 0, 0}, hx[3] = {0, 0, 0};
     double hs[3] = {0, 0, 0}, ts0 = 0, ts1 = 0;
     if (!fsrc && !tsrc) {
         T s0 = 0, s1 = 0, s2 = 0, e0 = 0, e1 = 0, e2 = 0;
-        const int nc = b1 - b0, ne = x1 - x0;
-        const int ncr = (nc + 2 * LPG - 1) / (2 * LPG), ner = (ne + 2 * LPG - 1) / (2 * LPG);
+        const int nrun = r1 - r0, ne = x1 - x0;
+        const int ncr = (nrun + 2 * LPG - 1) / (2 * LPG), ner = (ne + 2 * LPG - 1) / (2 * LPG);
         const int nr = ncr > ner ? ncr : ner;
-        P.one_round = ncr <= 1;
         for (int r = 0; r < nr; r++) {
-            const int qa = b0 + lane + 2 * r * LPG, qb = qa + LPG;
+            const int qa = r0 + lane + 2 * r * LPG, qb = qa + LPG;
             const int ja = x0 + lane + 2 * r * LPG, jb = ja + LPG;
-            const bool hqa = qa < b1, hqb = qb < b1, hja = ja < x1, hjb = jb < x1;
-            const int ca = gd.gc_idx[hqa ? qa : b0], cb = gd.gc_idx[hqb ? qb : b0];
-            if (r == 0) { P.ca = ca; P.cb = cb; }
+            const bool hqa = qa < r1, hqb = qb < r1, hja = ja < x1, hjb = jb < x1;
             const int na = gd.x_nb[hja ? ja : x0], nbb = gd.x_nb[hjb ? jb : x0];
             const T wa = gd.x_w[hja ? ja : x0], wb = gd.x_w[hjb ? jb : x0];
             T c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;
-            if (Hcell) {
-                c0 = (T)Hcell[ca]; c1 = (T)Hcell[CP + ca]; c2 = (T)Hcell[2 * CP + ca];
-                d0 = (T)Hcell[cb]; d1 = (T)Hcell[CP + cb]; d2 = (T)Hcell[2 * CP + cb];
+            if (Hr) {
+                const int ra = hqa ? qa : r0, rb = hqb ? qb : r0;
+                c0 = (T)Hr[ra]; c1 = (T)Hr[NR + ra]; c2 = (T)Hr[2 * NR + ra];
+                d0 = (T)Hr[rb]; d1 = (T)Hr[NR + rb]; d2 = (T)Hr[2 * NR + rb];
             }
             const T pa = msc[na], pb = msc[nbb];
             const T ax = mf[na], ay = mf[G + na], az = mf[2 * G + na];
@@ -180,8 +176,8 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
                 e0 += w * bx; e1 += w * by; e2 += w * bz;
             }
         }
-        if (Hcell) {
-            const T cnt = (T)nc;
+        if (Hr) {
+            const T cnt = (T)(b1 - b0);
             hd[0] = lane_sum<LPG>(s0) / cnt;
             hd[1] = lane_sum<LPG>(s1) / cnt;
             hd[2] = lane_sum<LPG>(s2) / cnt;
@@ -190,25 +186,28 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         hx[1] = lane_sum<LPG>(e1);
         hx[2] = lane_sum<LPG>(e2);
     } else {
+        // moving sources: the source is sampled at every cell of the grain's runs
         T s0 = 0, s1 = 0, s2 = 0;
-        for (int q = b0 + lane; q < b1; q += LPG) {
-            const int ci = gd.gc_idx[q];
-            if (Hcell) {
-                s0 += (T)Hcell[ci];
-                s1 += (T)Hcell[CP + ci];
-                s2 += (T)Hcell[2 * CP + ci];
+        for (int q = r0 + lane; q < r1; q += LPG) {
+            if (Hr) {
+                s0 += (T)Hr[q];
+                s1 += (T)Hr[NR + q];
+                s2 += (T)Hr[2 * NR + q];
             }
-            const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
-            if (fsrc)
+            const int geo = gd.run_geo[q], j = gd.run_row[q];
+            const int i0 = geo & 0xffff, len = geo >> 16;
+            for (int i = i0; i < i0 + len; i++) {
+                if (fsrc)
 #pragma unroll
-                for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
-            if (tsrc) {
-                ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
-                ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+                    for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
+                if (tsrc) {
+                    ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+                    ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+                }
             }
         }
         const T cnt = (T)(b1 - b0);
-        if (Hcell) {
+        if (Hr) {
             hd[0] = lane_sum<LPG>(s0) / cnt;
             hd[1] = lane_sum<LPG>(s1) / cnt;
             hd[2] = lane_sum<LPG>(s2) / cnt;
@@ -273,13 +272,15 @@ template <typename T, int LPG>
 __device__ __forceinline__ void src_temp(const GrainData<T>& gd, const LayerParams& L, int g, int lane, double* tout)
 {
     const int k = gd.layer_of[g];
-    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1];
+    const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1], r0 = gd.run_off[g], r1 = gd.run_off[g + 1];
     double ts0 = 0, ts1 = 0;
-    for (int q = b0 + lane; q < b1; q += LPG) {
-        const int ci = gd.gc_idx[q];
-        const int i = ci % gd.ax.nx, j = (ci / gd.ax.nx) % gd.ax.ny;
-        ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
-        ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+    for (int q = r0 + lane; q < r1; q += LPG) {
+        const int geo = gd.run_geo[q], j = gd.run_row[q];
+        const int i0 = geo & 0xffff, len = geo >> 16;
+        for (int i = i0; i < i0 + len; i++) {
+            ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+            ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+        }
     }
     const double dc = (double)(b1 - b0), tst = gd.temp[g];
     tout[0] = lane_sum<LPG>(ts0) / dc + tst;
@@ -368,32 +369,21 @@ __global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, i
     }
 }
 
-// get_mag (data.f90:344-353): M = Ms*ms_scale*m on every cell of grain g, lanes split the
-// grain's cells.  Non-magnetic cells are never written and stay 0.
-template <typename T, int LPG, typename TC>
-__device__ __forceinline__ void scatter_cells(const GrainData<T>& gd, TC* __restrict__ Mcell, size_t CP, int g,
-                                              int lane, T f, const T* m, const GrainPre<T>* P = nullptr)
+// get_mag (data.f90:344-353): the value Ms*ms_scale*m that every cell of grain g takes,
+// kept per grain in Mg[c][g] (in the demag's type); the row r2c spreads it over the grain's
+// runs, and non-magnetic cells stay 0.
+template <typename T, typename TC>
+__device__ __forceinline__ void store_mg(TC* __restrict__ Mg, int G, int g, T f, const T* m)
 {
-    const TC v0 = (TC)(f * m[0]), v1 = (TC)(f * m[1]), v2 = (TC)(f * m[2]);
-    if (P && P->one_round) {
-        // the cells this lane gathered: b0+lane and b0+lane+LPG
-        const int qa = P->b0 + lane, qb = qa + LPG;
-        if (qa < P->b1) { Mcell[P->ca] = v0; Mcell[CP + P->ca] = v1; Mcell[2 * CP + P->ca] = v2; }
-        if (qb < P->b1) { Mcell[P->cb] = v0; Mcell[CP + P->cb] = v1; Mcell[2 * CP + P->cb] = v2; }
-        return;
-    }
-    for (int q = gd.gc_off[g] + lane; q < gd.gc_off[g + 1]; q += LPG) {
-        const int ci = gd.gc_idx[q];
-        Mcell[ci] = v0;
-        Mcell[CP + ci] = v1;
-        Mcell[2 * CP + ci] = v2;
-    }
+    Mg[g] = (TC)(f * m[0]);
+    Mg[G + g] = (TC)(f * m[1]);
+    Mg[2 * G + g] = (TC)(f * m[2]);
 }
 
 // recompute: layers whose Ms follows a moving temperature take it at the axes' first time
 template <typename T, int LPG, typename TC>
-__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict__ m, TC* __restrict__ Mcell,
-                                                 size_t CP, int recompute)
+__global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict__ m, TC* __restrict__ Mg,
+                                                 int recompute)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = tid / LPG, lane = tid % LPG;
@@ -408,7 +398,7 @@ __global__ void __launch_bounds__(256) k_scatter(GrainData<T> gd, T* __restrict_
         if (lane == 0) m[(size_t)ST_MSC * gd.G + g] = msc;
         f = gd.ms[g] * msc;
     }
-    scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, f, mm);
+    if (lane == 0) store_mg<T, TC>(Mg, gd.G, g, f, mm);
 }
 
 // ------------------------------------------------------------------------- LLB push
@@ -652,7 +642,7 @@ __device__ __forceinline__ void rot_push(int type, T scale1, T scale2, const T* 
 template <typename T, int MODE, int LPG, typename TC>
 __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ ma, T* __restrict__ mb,
                                                T* __restrict__ dm, T* __restrict__ hthm, T* __restrict__ hout,
-                                               const TC* __restrict__ Hcell, TC* __restrict__ Mcell, size_t CP,
+                                               const double* __restrict__ Hr, TC* __restrict__ Mg,
                                                StepArgs a, double* __restrict__ diag_part, T* __restrict__ mc)
 {
     // grid padded to a multiple of 8: each XCD takes a contiguous range of grains, so the
@@ -702,7 +692,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     T h[3] = {0, 0, 0};
     double tev[2] = {0, 0};   // grain temperature now and at the next evaluation (moving sources)
     if (MODE != GM_PUSH1) {
-        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hcell, CP, a.t, mfield, h, tev, P);
+        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hr, a.t, mfield, h, tev, P);
     } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) h[c] = hout[(size_t)c * G + g];
@@ -872,9 +862,8 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
         }
         }
     }
-    // the new magnetisation of the grain onto its cells, for the next field evaluation
-    // (every lane of the grain holds mo and fsc)
-    if (MODE != GM_FIELD1 && Mcell && act) scatter_cells<T, LPG, TC>(gd, Mcell, CP, g, lane, fsc, mo, &P);
+    // the new magnetisation of the grain's cells, for the next field evaluation
+    if (MODE != GM_FIELD1 && Mg && w0) store_mg<T, TC>(Mg, G, g, fsc, mo);
     if (a.diag) {
         __syncthreads();
         for (int i = threadIdx.x; i < DIAG_STRIDE; i += blockDim.x) diag_part[(size_t)bl * DIAG_STRIDE + i] = red[i];
@@ -953,16 +942,16 @@ struct RbWeights {
     double w[MRMG_MAX_LAYERS];
 };
 template <typename TC>
-__global__ void __launch_bounds__(256) k_rb_plane(const TC* __restrict__ Mcell, size_t CP, int nxy, int nl,
-                                                  double d0, double d1, double d2, RbWeights wk,
+__global__ void __launch_bounds__(256) k_rb_plane(const TC* __restrict__ Mg, int G, const int* __restrict__ c2g,
+                                                  int nxy, int nl, double d0, double d1, double d2, RbWeights wk,
                                                   double* __restrict__ P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nxy) return;
     double s = 0.0;
     for (int k = 0; k < nl; k++) {
-        const size_t ci = (size_t)k * nxy + i;
-        s += wk.w[k] * (d0 * (double)Mcell[ci] + d1 * (double)Mcell[CP + ci] + d2 * (double)Mcell[2 * CP + ci]);
+        const int g = c2g[(size_t)k * nxy + i];
+        if (g >= 0) s += wk.w[k] * (d0 * (double)Mg[g] + d1 * (double)Mg[G + g] + d2 * (double)Mg[2 * G + g]);
     }
     P[i] = s;
 }
@@ -1051,16 +1040,15 @@ int Engine<T>::src_axes(double t0, double t1)
     return MRMG_OK;
 }
 
-// get_mag for the state m (used where no grain kernel has written the cells yet); with
-// t0 given, Ms scales that follow a moving temperature are taken at t0
+// get_mag for the state m (used where no grain kernel has written the grains' cell values
+// yet); with t0 given, Ms scales that follow a moving temperature are taken at t0
 template <typename T>
 int Engine<T>::scatter(S* m, const double* t0 )
 {
-    const size_t CP = (size_t)c->nl * c->nx * c->ny;
     const size_t threads = (size_t)G * 8;
     if (t0) RCHK(src_axes(*t0, *t0));
     hipLaunchKernelGGL((k_scatter<S, 8, T>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
-                       gdata(), m, Mcell.p, CP, t0 && any_src ? 1 : 0);
+                       gdata(), m, Mg.p, t0 && any_src ? 1 : 0);
     HIPCHK(hipGetLastError());
     return MRMG_OK;
 }
@@ -1068,13 +1056,11 @@ int Engine<T>::scatter(S* m, const double* t0 )
 template <typename T>
 template <int MODE, int LPG> void Engine<T>::launch_grain(const StepArgs& a, S* mout)
 {
-    const size_t CP = (size_t)c->nl * c->nx * c->ny;
-    const T* hc = demag_on() ? Hcell.p : nullptr;
     const size_t threads = (size_t)G * LPG;
     const unsigned nb8 = (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);
     hipLaunchKernelGGL((k_grain<S, MODE, LPG, T>), dim3(nb8), dim3(256), 0, c->stream,
-                       gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, hc, demag_on() ? Mcell.p : nullptr, CP, a,
-                       diag_part.p, mout);
+                       gdata(), ma.p, mb.p, dm.p, hthm.p, h.p, demag_on() ? Hr.p : nullptr,
+                       demag_on() ? Mg.p : nullptr, a, diag_part.p, mout);
 }
 
 // mout: destination of the corrector (null: ma); t_next: time of the next field
@@ -1109,14 +1095,13 @@ template <typename T>
 int Engine<T>::readback_into(const mrmg_sensor* sn, int n_pos, const double* xpos, const double* ypos, double* out, double* plane, double* pos, double* dout)
 {
     const int nx = c->nx, ny = c->ny, nl = c->nl, nxy = nx * ny;
-    const size_t CP = (size_t)nl * nxy;
     RbWeights wk{};
     for (int k = 0; k < nl; k++) wk.w[k] = (sn->w ? sn->w[k] : 1.0) * c->dz[k] * c->dx * c->dy;
     HIPCHK(hipMemcpyAsync(pos, xpos, sizeof(double) * n_pos, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(pos + n_pos, ypos, sizeof(double) * n_pos, hipMemcpyHostToDevice, c->stream));
-    RCHK(scatter(ma.p));   // the cells hold the current state
-    hipLaunchKernelGGL((k_rb_plane<T>), dim3((nxy + 255) / 256), dim3(256), 0, c->stream, Mcell.p, CP, nxy, nl,
-                       sn->dir[0], sn->dir[1], sn->dir[2], wk, plane);
+    RCHK(scatter(ma.p));   // the grains' cell values of the current state
+    hipLaunchKernelGGL((k_rb_plane<T>), dim3((nxy + 255) / 256), dim3(256), 0, c->stream, Mg.p, G, cell2grain.p, nxy,
+                       nl, sn->dir[0], sn->dir[1], sn->dir[2], wk, plane);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_rb_scan, dim3(n_pos), dim3(256), sizeof(double) * (nx + ny), c->stream, plane, nx, ny,
                        c->dx, c->dy, sn->sigma_x, sn->sigma_y, pos, n_pos, dout);

@@ -129,6 +129,20 @@ __host__ __device__ inline size_t tensor_block(int kx, int ko, int ks, int nl, i
     return toep ? (size_t)kx * (2 * nl - 1) + (size_t)(ko - ks + nl - 1) : ((size_t)kx * nl + ko) * nl + ks;
 }
 
+// Runs: the cells of one grain that follow each other along x in one row of a layer.
+// The row FFT kernels read the grains' cell magnetisation through them (get_mag,
+// data.f90:344-353) and write the cell field's sums over them, which the grain kernels add
+// up (get_magnetostatic, field.f90:463-468); no cell array goes to memory.  Runs are
+// numbered grain by grain (each grain's runs contiguous, in the order of its cells) and
+// listed a second time row by row for the row kernels.
+struct RowRuns {
+    const int* row_off;   // [nl*ny + 1]: the runs of row (k, y) are [row_off[k*ny + y], row_off[k*ny + y + 1])
+    const int* geo;       // row by row: first x | length << 16
+    const int* grain;     // row by row: global grain
+    const int* gm;        // row by row: the run's number in the grain-by-grain order
+    int nruns;
+};
+
 // rows of the grain state buffers
 #define ST_MAG 3
 #define ST_MSC 4
@@ -150,10 +164,14 @@ template <typename T> struct GrainData {
     const T* happ;        // [3][G] static per-grain applied field, or null
     const T* sigma;       // thermal std sqrt(x)*tsigma (field.f90:407-423)
     const T* vol;
-    const int* gc_off;    // grain -> cells CSR (grain2gridindex)
-    const int* gc_idx;    //   global cell index k*P + j*nx + i
+    const int* gc_off;    // grain -> cells CSR (grain2gridindex): the cell count of grain g
+                          // is gc_off[g+1] - gc_off[g]
+    const int* run_off;   // grain -> runs CSR (RowRuns' grain-by-grain numbering)
+    const int* run_geo;   //   first x | length << 16
+    const int* run_row;   //   row y within the grain's layer
+    int nruns;
     // exchange CSR, per grain the intralayer entries, then those to the layer above, then
-    // to the layer below; x_nb/x_w and gc_idx carry one spare entry at the end
+    // to the layer below; x_nb/x_w carry one spare entry at the end
     const int* x_off; const int* x_nb; const T* x_w;
     const LayerParams* lp;
     int nl;

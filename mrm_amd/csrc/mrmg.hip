@@ -456,7 +456,9 @@ int mrmg_set_source(mrmg_ctx* c, int k, int slot, const mrmg_source* src)
                 if (!(src->ypos[i] > src->ypos[i - 1])) return set_err(MRMG_ERR_ARG, "table y nodes must increase");
         }
     }
-    RCHK(save_device_state(c));
+    // a built context takes the new source without a rebuild: only the source tables and
+    // the layer parameters change
+    const bool built = !c->dirty;
     HostLayer& H = c->layers[k];
     H.src[slot] = *src;
     H.src_x[slot].clear(); H.src_y[slot].clear(); H.src_v[slot].clear();
@@ -467,7 +469,7 @@ int mrmg_set_source(mrmg_ctx* c, int k, int slot, const mrmg_source* src)
         H.src_v[slot].assign(src->vals, src->vals + (size_t)nv * src->nxp * src->nyp);
     }
     H.src[slot].xpos = H.src[slot].ypos = H.src[slot].vals = nullptr;
-    c->dirty = true;
+    if (built) return c->eng->update_sources(c->layers);
     return MRMG_OK;
 }
 

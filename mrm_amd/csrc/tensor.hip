@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_tensor_rows(const double* __restrict__ 
                                                      int nx, int ny, int nxh, FFTPlan px,
                                                      const C2<double>* __restrict__ twx)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<double>* tw = reinterpret_cast<C2<double>*>(smem_raw);
     double* re = reinterpret_cast<double*>(tw + nx);
     double* im = re + LPN(nx);
@@ -146,7 +146,7 @@ template <int E>
 __global__ void __launch_bounds__(256) k_tensor_cols(C2<double>* __restrict__ TC, int ny, int nxh, FFTPlan py,
                                                      const C2<double>* __restrict__ twy)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<double>* tw = reinterpret_cast<C2<double>*>(smem_raw);
     double* re = reinterpret_cast<double*>(tw + ny);
     double* im = re + LPN(ny);

@@ -11,16 +11,24 @@ module mrmg_m
   use, intrinsic :: iso_c_binding
   implicit none
   private
-  public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_step_params, mrmg_diag, mrmg_sensor
+  public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_source, mrmg_step_params, mrmg_diag, mrmg_sensor
   public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_build_tensor_images, &
             mrmg_set_layer, &
             mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error, &
-            mrmg_readback, mrmg_switch_arm, mrmg_switch_read
+            mrmg_readback, mrmg_switch_arm, mrmg_switch_read, mrmg_set_source, mrmg_tune, mrmg_get_tune, &
+            mrmg_philox_normals, mrmg_footprint
   public :: MRMG_FP64, MRMG_FP32, MRMG_NOISE_NONE, MRMG_NOISE_INJECT, MRMG_NOISE_PHILOX, MRMG_TYPE_HEUN
+  public :: MRMG_SRC_NONE, MRMG_SRC_FIELD_TABLE, MRMG_SRC_TEMP_TABLE, MRMG_SRC_TEMP_SG
+  public :: MRMG_FN_RAMP, MRMG_FN_SIN, MRMG_FN_EXP, MRMG_FN_COS, MRMG_FN_NEW, MRMG_FN_STE
 
   integer(c_int), parameter :: MRMG_FP64 = 0, MRMG_FP32 = 1
   integer(c_int), parameter :: MRMG_NOISE_NONE = 0, MRMG_NOISE_INJECT = 1, MRMG_NOISE_PHILOX = 2
   integer(c_int), parameter :: MRMG_TYPE_HEUN = -3
+  ! moving sources of get_appl (applied.f90:827-1034) and the time functions of set_fld
+  integer(c_int), parameter :: MRMG_SRC_NONE = 0, MRMG_SRC_FIELD_TABLE = 1, MRMG_SRC_TEMP_TABLE = 2, &
+                               MRMG_SRC_TEMP_SG = 3
+  integer(c_int), parameter :: MRMG_FN_RAMP = 0, MRMG_FN_SIN = 1, MRMG_FN_EXP = 2, MRMG_FN_COS = 3, &
+                               MRMG_FN_NEW = 4, MRMG_FN_STE = 5
 
   type, bind(C) :: mrmg_geom
     integer(c_int) :: nx, ny, n_lay, precision, demag
@@ -48,6 +56,20 @@ module mrmg_m
     real(c_double) :: dir(3)
     real(c_double) :: t_start, t_end, s_start, s_end, scale
     integer(c_int) :: func, pad_                ! MRMG_FN_*: 0 ramp, 1 SIN, 2 EXP, 3 COS, 4 NEW, 5 STE
+    real(c_double) :: amp, gamma, phase, param, duty, moddepth, dc
+  end type
+
+  ! A moving field table (READ_FIELD data with FLD_VELOCITY), temperature table (READ_TEMP
+  ! with TEMP_VELOCITY) or super-Gaussian spot of one layer: the field of get_appl's
+  ! moving branches (applied.f90:884-903).  xpos(nxp), ypos(nyp) and vals (3,nxp,nyp) for a
+  ! field or (nxp,nyp) for a temperature are passed with c_loc.
+  type, bind(C) :: mrmg_source
+    integer(c_int) :: kind, nxp, nyp, expn
+    type(c_ptr) :: xpos, ypos, vals
+    real(c_double) :: vel_x, vel_y, offset_x, offset_y, ref_x, ref_y, t_origin
+    real(c_double) :: isigx, isigy, dtemp
+    real(c_double) :: t_start, t_end, s_start, s_end, scale
+    integer(c_int) :: func, pad_
     real(c_double) :: amp, gamma, phase, param, duty, moddepth, dc
   end type
 
@@ -175,6 +197,40 @@ module mrmg_m
       integer(c_int), value :: k
       integer(c_int64_t), intent(out) :: step(*)
       real(c_double), intent(out) :: t(*), h_app(*)
+    end function
+    ! slot 0: the field table of layer k; slot 1: its temperature table or spot
+    integer(c_int) function mrmg_set_source(ctx, k, slot, src) bind(C, name='mrmg_set_source')
+      import :: c_int, c_ptr, mrmg_source
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k, slot
+      type(mrmg_source), intent(in) :: src
+    end function
+    ! launch shapes (the FFTW_EXHAUSTIVE planning role, fftw.f90:25); key: a C string,
+    ! e.g. 'lanes_per_grain'//c_null_char
+    integer(c_int) function mrmg_tune(ctx, key, value) bind(C, name='mrmg_tune')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      integer(c_int), value :: value
+    end function
+    integer(c_int) function mrmg_get_tune(ctx, key, value) bind(C, name='mrmg_get_tune')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      integer(c_int), intent(out) :: value
+    end function
+    ! the device's thermal normals for steps step0.., laid out (3,G,n_steps)
+    integer(c_int) function mrmg_philox_normals(ctx, seed, step0, n_steps, out) bind(C, name='mrmg_philox_normals')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: seed, step0
+      integer(c_int), value :: n_steps
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function mrmg_footprint(ctx, bytes) bind(C, name='mrmg_footprint')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), intent(out) :: bytes
     end function
     type(c_ptr) function mrmg_last_error() bind(C, name='mrmg_last_error')
       import :: c_ptr

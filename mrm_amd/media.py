@@ -148,14 +148,15 @@ def make_media(nx, ny, dx=2e-7, dy=2e-7, dz=8e-7, n_lay=1, pitch=8e-7, ms=700.0,
                cone_deg=3.0, a_ex=1e-6, intergrain_frac=0.05, boundary_width=2e-7, layer_ku=None,
                physical=None, alpha=0.02, gamma=1.76e7, thm_alpha=None, temperature=0.0, thermal=False,
                seed=1, boundary=True, exchange=True, tc_nom=None, tc_sigma=0.0, spin=0.5, msexp=-1.0,
-               hkexp=-1.0, use_mag=False):
+               hkexp=-1.0, use_mag=False, hk_table=None):
     """Build media following the recipe of the benchmark configurations.
 
     layer_ku: per layer Ku (erg/cc); physical: physical-layer id of each layer (sub-layers
     of one physical layer are coupled with 2*A/dz, neighbouring physical layers with
     intergrain_frac of that).  tc_nom (K): per-grain Curie temperatures tc_nom (1 +
     tc_sigma z) for LLB and temperature scaling (Ms, Hk ~ (1-T/Tc)^exp, or Ms from |m|
-    with use_mag); None leaves Tc at 1e22 and scaling off.
+    with use_mag); None leaves Tc at 1e22 and scaling off.  hk_table: (T/Tc nodes,
+    Hk scalings), a READ_SCALE table for Hk (temp_scaling.f90:397-416).
     """
     g2g = voronoi_grid(nx, ny, dx, dy, pitch, seed, boundary=boundary)
     g2gi, cells, counts = grain_cells(g2g)
@@ -177,6 +178,8 @@ def make_media(nx, ny, dx=2e-7, dy=2e-7, dz=8e-7, n_lay=1, pitch=8e-7, ms=700.0,
         zt = np.random.default_rng(seed + 3).standard_normal(ng)
         llb = dict(tc=tc_nom * (1.0 + tc_sigma * zt), spin=spin, tc_nom=tc_nom, msexp=msexp, hkexp=hkexp,
                    use_mag=use_mag)
+        if hk_table is not None:
+            llb["tables"] = {"hk": hk_table}
     one = np.arange(0, ng + 1, dtype=np.int32)
     self_nb = np.arange(1, ng + 1, dtype=np.int32)
     for k in range(n_lay):
@@ -211,10 +214,59 @@ def config_media(name, **over):
     elif name == "cfg2":     # 512x4096x4 of 1.5 nm, config-1 media
         kw = dict(nx=512, ny=4096, dx=1.5e-7, dy=1.5e-7, dz=1.5e-7, n_lay=4, layer_ku=[2e6, 2e6, 1e7, 1e7],
                   physical=[0, 0, 1, 1], temperature=300.0, thermal=True)
+    elif name == "cfg4":     # 2048x2048x8 of 1 nm, FePt-like LLB media, Tc = 700 K
+        kw = dict(nx=2048, ny=2048, dx=1e-7, dy=1e-7, dz=1e-7, n_lay=8, pitch=8e-7, boundary_width=1e-7,
+                  ms=1100.0, ku=5e7, physical=[0] * 8, alpha=0.1, temperature=300.0, thermal=True,
+                  tc_nom=700.0, spin=0.5, use_mag=True, hk_table=equilibrium_table())
     else:
         raise ValueError(name)
     kw.update(over)
     return make_media(**kw)
+
+
+def equilibrium_magnetisation(t_over_tc):
+    """m_e at T/Tc: the zero-field fixed point m = L(3 m Tc/T) of the mean field that the
+    reference's LLB relaxes |m| to (L the Langevin function of llg.f90:704-710; lambda c =
+    3 Tc/T for any spin S, grain.f90:61 with llg.f90:932); 0 at and above Tc."""
+    tau = float(t_over_tc)
+    if tau >= 1.0:
+        return 0.0
+    if tau <= 0.0:
+        return 1.0
+
+    def lang(x):
+        return 1.0 / np.tanh(x) - 1.0 / x if abs(x) > 1e-6 else x / 3.0
+
+    lo, hi = 1e-12, 1.0
+    for _ in range(200):            # m - L(3m/tau) < 0 below the root, > 0 above
+        mid = 0.5 * (lo + hi)
+        if mid - lang(3.0 * mid / tau) > 0.0:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+def equilibrium_table(n=41):
+    """(T/Tc nodes, m_e) on [0, 1.05]: the READ_SCALE form of Hk(T)/Hk(0) = m_e(T) for
+    Ku(T) ~ m_e^2 with Ms(T) ~ m_e (Hk = 2 Ku/Ms)."""
+    t = np.concatenate([np.linspace(0.0, 0.95, n - 6), np.linspace(0.96, 1.0, 5), [1.05]])
+    return t, np.array([equilibrium_magnetisation(x) for x in t])
+
+
+def laser_write(n_lay, nx, ny, dx, dy, peak_dtemp=450.0, fwhm=40e-7, velocity=1500.0, h_write=8000.0,
+                centre=None):
+    """The heat-assisted write of the LLB configuration: a Gaussian laser spot (the
+    super-Gaussian source with power 2, set_sg/get_sg, applied.f90:1778-1811) of the given
+    FWHM and peak temperature rise over the ambient, moving down-track (y) at velocity
+    (cm/s) from centre at t = 0, on every layer, and a uniform write field h_write along -z.
+    Returns (sources, ufields), one per layer."""
+    s = fwhm / (2.0 * np.sqrt(np.log(2.0)))       # exp(-(u/s)^2) has this FWHM
+    cx, cy = centre if centre is not None else (0.5 * nx * dx, 0.5 * ny * dy)
+    srcs = [abi.SourceArrays(abi.MRMG_SRC_TEMP_SG, vel=(0.0, velocity), offset=(cx, cy), sigma=(s, s), expn=2,
+                             dtemp=peak_dtemp, t_start=-1e22, t_end=1e22) for _ in range(n_lay)]
+    ufs = [abi.make_ufield((0.0, 0.0, -1.0), scale=h_write) for _ in range(n_lay)]
+    return srcs, ufs
 
 
 def ramp_field(n_lay, tilt_deg=5.0, h_max=20000.0, t_ramp=2e-9):

@@ -1,7 +1,9 @@
 """The Fortran binding end to end on the device: mrm_amd/fortran/mrmg_drive (built by
 __graft_entry__.build() with flang against mrmg_m.f90 and libmrmg.so) runs one LLG
-command through the iso_c_binding interfaces, with the reference's own array layouts;
-the same run made through the Python binding gives the same m bit for bit."""
+command through the iso_c_binding interfaces, with the reference's own array layouts,
+then the same command under a moving write-head field table (mrmg_set_source, the
+FLD_VELOCITY table of applied.f90:884-903) with a launch shape set by mrmg_tune; the same
+runs made through the Python binding give the same m bit for bit."""
 import math
 import os
 import subprocess
@@ -19,7 +21,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "mrm_amd", "fortran", "mrmg_drive")
 
 
-def python_run():
+def head_table():
+    """The driver's table: nodes xp (5), yp (4) and the field (yp, xp, 3) = Fortran (3,nxp,nyp)."""
+    xp = np.array([-2e-6 + 1.2e-6 * i for i in range(5)])
+    yp = np.array([-1e-6 + 1.5e-6 * j for j in range(4)])
+    tab = np.zeros((4, 5, 3))
+    for j in range(4):
+        for i in range(5):
+            tab[j, i] = (300.0 * (i - 2), 50.0 * (j - 1) * i, -2e3 - 500.0 * i + 100.0 * j * j)
+    return xp, yp, tab
+
+
+def python_run(source=False):
     nx = ny = 16
     dx = dy = 2e-7
     dz = 4e-7
@@ -41,6 +54,11 @@ def python_run():
     x.set_layer(0, la)
     d = np.array([0.1, 0.0, -1.0]) / math.sqrt(1.01)
     x.set_ufield(0, abi.make_ufield(d, t_start=0.0, t_end=1e-10, s_start=0.0, s_end=1e4))
+    if source:
+        xp, yp, tab = head_table()
+        x.set_source(0, 0, abi.SourceArrays(abi.MRMG_SRC_FIELD_TABLE, xpos=xp, ypos=yp, vals=tab, vel=(0.0, 2000.0),
+                                            t_start=0.0, t_end=2e-12, s_start=0.0, s_end=1.0))
+        x.tune("lanes_per_grain", 4)
     x.set_state(0, m)
     x.set_llg(dt=5e-14, llg_type=abi.MRMG_TYPE_HEUN)
     x.step(50)
@@ -56,9 +74,12 @@ def test_fortran_driver_matches_python_binding(tmp_path):
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     say("  " + r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
-    mf = np.fromfile(tmp_path / "m_fortran.bin", dtype="<f8").reshape(-1, 3)   # Fortran m(3,ng)
-    mp = python_run()
-    assert mf.shape == mp.shape
-    say(f"  max|dm| Fortran vs Python binding: {np.abs(mf - mp).max():.2e}")
-    assert np.array_equal(mf, mp)
-    assert np.abs(mp[:, 2] - mp[0, 2]).max() > 0      # the grains moved differently
+    for name, source in (("m_fortran.bin", False), ("m_fortran_src.bin", True)):
+        mf = np.fromfile(tmp_path / name, dtype="<f8").reshape(-1, 3)   # Fortran m(3,ng)
+        mp = python_run(source)
+        assert mf.shape == mp.shape
+        say(f"  {name}: max|dm| Fortran vs Python binding: {np.abs(mf - mp).max():.2e}")
+        assert np.array_equal(mf, mp)
+        assert np.abs(mp[:, 2] - mp[0, 2]).max() > 0      # the grains moved differently
+    # the head table changed the run
+    assert not np.array_equal(python_run(False), python_run(True))
