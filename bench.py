@@ -61,40 +61,53 @@ def timed(x, steps, t0):
 
 
 def media_counts(med):
-    """Magnetic cells and exchange CSR entries of a media."""
-    cm = sum(int(L.grain2gridindex[-1]) for L in med.layers)
-    E = 0
+    """Magnetic cells, runs (a grain's cells that follow each other along x in one row) and
+    exchange CSR entries of a media."""
+    cm = runs = E = 0
     for L in med.layers:
+        g2gi = np.asarray(L.grain2gridindex)
+        cm += int(g2gi[-1])
+        cells = np.asarray(L.grain2grid).reshape(-1, 2).astype(np.int64) - 1
+        gid = np.full((med.ny, med.nx), -1, np.int64)
+        gid[cells[:, 1], cells[:, 0]] = np.repeat(np.arange(L.ng), np.diff(g2gi))
+        start = gid >= 0
+        start[:, 1:] &= gid[:, 1:] != gid[:, :-1]
+        runs += int(start.sum())
         for f in ("hxgrainindex", "uhxgrainindex", "lhxgrainindex"):
             v = getattr(L, f)
             if v is not None:
                 E += int(v[-1])
-    return cm, E
+    return cm, runs, E
 
 
 def kernel_bytes(med, P, offsets):
-    """Algorithmic bytes per launch of each kernel class.  Row r2c: the cell magnetisation
-    in (3 C P), the spectra out (6 n_lay Kc P); row c2r the reverse.  Column FFTs of the
-    split path: the spectra in and out, in place (12 n_lay Kc P).  Product (split path)
-    and fused column kernel: the spectra in and out plus the unique tensor, 2 P per complex
-    element, T_u = 6 n_lay (n_lay+1)/2 per (kx, ky) (index symmetry and layer reciprocity),
-    or 6 (2 n_lay - 1) when the stored tensor is the z-offset form and that is fewer.  A
-    grain stage: ~25 P per grain (state, constants, dm, thermal), the exchange CSRs (4 + P
-    per entry), the cell index map (4 per magnetic cell) and the cell field gathered and
-    magnetisation scattered (6 P per magnetic cell)."""
-    nl, C = med.n_lay, med.cells
+    """Algorithmic bytes per launch of each kernel class (P: bytes of the demag path's real
+    type; the grain state is FP64 in both forms).  Row r2c: the grains' magnetisation
+    (3 P per grain), the row-major run lists (geometry and grain, 8 per run, plus the row
+    offsets) and the spectra out (6 n_lay Kc P).  Row c2r: the spectra in, the run
+    geometry (4 per run) and the FP64 run sums of the cell field out (24 per run).  Column
+    FFTs of the split path: the spectra in and out, in place (12 n_lay Kc P).  Product
+    (split path) and fused column kernel: the spectra in and out plus the unique tensor,
+    2 P per complex element, T_u = 6 n_lay (n_lay+1)/2 per (kx, ky) (index symmetry and
+    layer reciprocity), or 6 (2 n_lay - 1) when the stored tensor is the z-offset form
+    and that is fewer.  A grain stage: ~25 doubles per grain (state, constants, dm,
+    thermal, field), the exchange CSRs (4 + 8 per entry), the grain's run offsets (4), its
+    runs' FP64 field sums, geometry and rows (32 per run) and its magnetisation out to the
+    row kernels (3 P)."""
+    nl = med.n_lay
     kc = (med.nx // 2 + 1) * med.ny
     G = med.G
-    cm, E = media_counts(med)
-    grain = 25 * P * G + E * (4 + P) + 4 * cm + 6 * cm * P
+    cm, runs, E = media_counts(med)
+    rows = 4 * (nl * med.ny + 1)
+    grain = 25 * 8 * G + E * (4 + 8) + 4 * G + 32 * runs + 3 * P * G
     tu = 6 * nl * (nl + 1) // 2
     if offsets:
         tu = min(tu, 6 * (2 * nl - 1))
     spectra = 12 * nl * kc * P
     return {
-        "row_fwd": 3 * C * P + 6 * nl * kc * P,
+        "row_fwd": 3 * P * G + 8 * runs + rows + 6 * nl * kc * P,
         "col": spectra + 2 * tu * kc * P,
-        "row_inv": 6 * nl * kc * P + 3 * C * P,
+        "row_inv": 6 * nl * kc * P + 4 * runs + rows + 24 * runs,
         "grain1": grain,
         "grain2": grain,
         "col_fwd": spectra,
@@ -137,6 +150,119 @@ def pmc_traffic(kernel, tag):
             except ValueError:
                 return None
     return None
+
+
+def kernel_class(name):
+    """The class of kernel_table of a demangled kernel name in rocprofv3's output, or None."""
+    n = (name[5:] if name.startswith("void ") else name).split("(")[0]
+    base = n.split("<")[0].strip()
+    args = [s.strip() for s in n[n.find("<") + 1:n.rfind(">")].split(",")] if "<" in n else []
+    if base in ("k_rowT_fwd", "k_row_fwd"):
+        return "row_fwd"
+    if base in ("k_rowT_inv", "k_row_inv"):
+        return "row_inv"
+    if base == "k_colT":
+        return "col_inv" if args[2] == "true" else "col_fwd"
+    if base in ("k_prod", "k_prod_offsets"):
+        return "prod"
+    if base == "k_col":
+        return "col"
+    if base == "k_grain":
+        return "grain2" if args[1] in ("1", "3") else "grain1"
+    if base == "k_copy16":
+        return "copy16"
+    return None
+
+
+def _prof_csv(d, suffix):
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def summarize(kind, dirs, tag, commands, workload):
+    """Write profiles/kernels_<tag>.txt (kind 'trace': the kernel_stats.csv of a rocprofv3
+    --kernel-trace --stats -f csv run) or profiles/pmc_traffic_<tag>.txt (kind 'pmc': the
+    counter_collection.csv of a FETCH_SIZE run and of a WRITE_SIZE run, scaled by what the
+    same counters read for k_copy16 over 1 GiB)."""
+    out = []
+    if kind == "trace":
+        rows = _prof_csv(dirs[0], "kernel_stats.csv")
+        out.append(f"# rocprofv3 --kernel-trace --stats summary, workload {workload}")
+        out.append(f"# command: {commands[0]}")
+        out.append("# class: the bench's kernel class of the kernel; the run includes the device tensor build, the")
+        out.append("# launch-shape trials and the copy-ceiling probe (k_copy16 over 1 GiB)")
+        out.append(f"{'kernel':<72} {'class':>8} {'calls':>7} {'total_us':>12} {'avg_us':>10} {'pct':>6}")
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+            name = r["Name"]
+            out.append(f"{name[:72]:<72} {kernel_class(name) or '-':>8} {int(r['Calls']):>7} "
+                       f"{float(r['TotalDurationNs']) / 1e3:>12.1f} {float(r['AverageNs']) / 1e3:>10.3f} "
+                       f"{float(r['Percentage']):>6.2f}")
+        # the timed steps: per class, the trailing dispatches that ran the class's last variant
+        disp = sorted(_prof_csv(dirs[0], "kernel_trace.csv"), key=lambda r: int(r["Dispatch_Id"]))
+        seq = {}
+        for r in disp:
+            c = kernel_class(r["Kernel_Name"])
+            if c:
+                seq.setdefault(c, []).append((r["Kernel_Name"].split("(")[0],
+                                              (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+        out.append("# timed variants: per class, the dispatches after its last change of variant (the launch-shape")
+        out.append("# trials come first): class, dispatches, avg_us, variant")
+        for c in sorted(seq):
+            last = seq[c][-1][0]
+            tail = []
+            for nm, us in reversed(seq[c]):
+                if nm != last:
+                    break
+                tail.append(us)
+            out.append(f"timed {c} {len(tail)} {sum(tail) / len(tail):.3f} {last}")
+        path = os.path.join(ROOT, "profiles", f"kernels_{tag}.txt")
+    else:
+        per = []
+        for d, counter in zip(dirs, ("FETCH_SIZE", "WRITE_SIZE")):
+            acc = {}
+            for r in _prof_csv(d, "counter_collection.csv"):
+                if r["Counter_Name"] != counter:
+                    continue
+                a = acc.setdefault(r["Kernel_Name"], [0.0, 0, 0])
+                a[0] += float(r["Counter_Value"])
+                a[1] += 1
+                a[2] = max(a[2], int(r["Dispatch_Id"]))
+            per.append({k: (v[0] / v[1], v[1], v[2]) for k, v in acc.items()})
+        fetch, write = per
+        copy = [k for k in fetch if kernel_class(k) == "copy16"]
+        kib = (1 << 30) / 1024.0
+        ff = kib / fetch[copy[0]][0]
+        fw = kib / write[copy[0]][0]
+        out.append(f"# HBM traffic per dispatch from rocprofv3 --pmc, workload {workload}")
+        out.append(f"# FETCH_SIZE pass: {commands[0]}")
+        out.append(f"# WRITE_SIZE pass: {commands[1]}")
+        out.append(f"# calibration, k_copy16 over 1 GiB in the same runs: FETCH_SIZE {fetch[copy[0]][0]:.1f} KiB -> "
+                   f"factor {ff:.3f}; WRITE_SIZE {write[copy[0]][0]:.1f} KiB -> factor {fw:.3f}")
+        out.append("# traffic_bytes = (FETCH_SIZE * factor + WRITE_SIZE * factor) * 1024, per class for the variant")
+        out.append("# of its last dispatch (the timed steps run after the launch-shape trials)")
+        out.append("# class         fetch_kib    write_kib  traffic_bytes   variant, dispatches")
+        best = {}
+        for k, (v, n, last) in fetch.items():
+            c = kernel_class(k)
+            if c and k in write and (c not in best or last > best[c][2]):
+                best[c] = (k, n, last)
+        for c in sorted(best):
+            k, n, _ = best[c]
+            t = (fetch[k][0] * ff + write[k][0] * fw) * 1024
+            out.append(f"{c} {fetch[k][0]:.1f} {write[k][0]:.1f} {int(t)}")
+            out.append(f"#   {k}, {n} dispatches")
+        path = os.path.join(ROOT, "profiles", f"pmc_traffic_{tag}.txt")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    print(f"wrote {path}")
 
 
 def max_rms(a, b, n_lay):
@@ -383,7 +509,16 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the cfg1, track and serving legs")
+    ap.add_argument("--summarize", choices=("trace", "pmc"),
+                    help="write a profiles/ summary from rocprofv3 CSV output (no GPU work)")
+    ap.add_argument("--prof-dirs", nargs="+", help="rocprofv3 output folders (pmc: the FETCH_SIZE one, then WRITE_SIZE)")
+    ap.add_argument("--prof-tag", default="cfg4_fp32")
+    ap.add_argument("--prof-commands", nargs="+", default=[], help="the commands that made the folders")
+    ap.add_argument("--prof-workload", default="2048x2048x8 LLB heat-assisted write, FP32 form (bench.py headline)")
     a = ap.parse_args()
+    if a.summarize:
+        summarize(a.summarize, a.prof_dirs, a.prof_tag, a.prof_commands + ["", ""], a.prof_workload)
+        return 0
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)

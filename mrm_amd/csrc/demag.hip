@@ -278,6 +278,65 @@ __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T
     }
 }
 
+// The product with the z-offset tensor (tensor_block's 2 nl - 1 blocks): thread per
+// (kx, ky), the NL layers' spectra and field sums all in registers.  The offsets run from
+// NL-1 down to -(NL-1), so every field sum H_ko still adds its sources in ascending ks, as
+// k_prod and the reference do (demag.f90:117-121), while each spectrum and each stored
+// tensor value is read from memory once.
+template <typename T, bool SYM, int NL>
+__global__ void __launch_bounds__(256) k_prod_offsets(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
+                                                      const C2<T>* __restrict__ Tspec, int ny, int nxh, int nslot)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t Kc = (size_t)nxh * ny;
+    if (idx >= Kc) return;
+    const int kx = (int)(idx / ny), ky = (int)(idx - (size_t)kx * ny);
+    C2<T> m[NL][3], h[NL][3];
+#pragma unroll
+    for (int k = 0; k < NL; k++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            m[k][b] = S1[((size_t)(3 * k + b)) * Kc + idx];
+            h[k][b] = {0, 0};
+        }
+#pragma unroll
+    for (int d = NL - 1; d >= -(NL - 1); d--) {
+        const C2<T>* t = Tspec + tensor_block(kx, d > 0 ? d : 0, d > 0 ? 0 : -d, NL, 1) * nslot * ny + ky;
+        C2<T> tv[SYM ? 6 : 9];
+#pragma unroll
+        for (int sl = 0; sl < (SYM ? 6 : 9); sl++) tv[sl] = t[sl * ny];
+#pragma unroll
+        for (int ko = 0; ko < NL; ko++) {
+            const int ks = ko - d;
+            if (ks < 0 || ks >= NL) continue;
+            const C2<T> mx = m[ks][0], my = m[ks][1], mz = m[ks][2];
+            C2<T> a;
+            if (SYM) {
+                a = cmul(tv[0], mx); h[ko][0].x += a.x; h[ko][0].y += a.y;
+                a = cmul(tv[3], my); h[ko][0].x += a.x; h[ko][0].y += a.y;
+                a = cmul(tv[4], mz); h[ko][0].x += a.x; h[ko][0].y += a.y;
+                a = cmul(tv[3], mx); h[ko][1].x += a.x; h[ko][1].y += a.y;
+                a = cmul(tv[1], my); h[ko][1].x += a.x; h[ko][1].y += a.y;
+                a = cmul(tv[5], mz); h[ko][1].x += a.x; h[ko][1].y += a.y;
+                a = cmul(tv[4], mx); h[ko][2].x += a.x; h[ko][2].y += a.y;
+                a = cmul(tv[5], my); h[ko][2].x += a.x; h[ko][2].y += a.y;
+                a = cmul(tv[2], mz); h[ko][2].x += a.x; h[ko][2].y += a.y;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    a = cmul(tv[3 * r + 0], mx); h[ko][r].x += a.x; h[ko][r].y += a.y;
+                    a = cmul(tv[3 * r + 1], my); h[ko][r].x += a.x; h[ko][r].y += a.y;
+                    a = cmul(tv[3 * r + 2], mz); h[ko][r].x += a.x; h[ko][r].y += a.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S2[((size_t)(3 * k + b)) * Kc + idx] = h[k][b];
+}
+
 // c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
 // imaginary parts of the DC and Nyquist bins ignored).
 template <typename T, int E, bool WV = false, bool CELLS = false>
@@ -416,7 +475,26 @@ int Engine<T>::launch_class(int cls)
         if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate product kernel");
         const size_t Kc = (size_t)nxh * ny;
         const dim3 gp((unsigned)((Kc + 255) / 256));
-        if (nslot == 6)
+        if (toep && nl <= 8) {
+#define KPO(NLV)                                                                                                  \
+    if (nslot == 6)                                                                                               \
+        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nxh,  \
+                           nslot);                                                                                \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nxh, \
+                           nslot);
+            switch (nl) {
+            case 1: KPO(1) break;
+            case 2: KPO(2) break;
+            case 3: KPO(3) break;
+            case 4: KPO(4) break;
+            case 5: KPO(5) break;
+            case 6: KPO(6) break;
+            case 7: KPO(7) break;
+            default: KPO(8) break;
+            }
+#undef KPO
+        } else if (nslot == 6)
             hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot,
                                (int)toep);
         else

@@ -420,6 +420,8 @@ struct Engine : EngineBase {
             return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: a row or column FFT does not fit one workgroup", nx, ny,
                            nl);
         split = !fused_ok;
+        // long rows run fastest with the widest row workgroups (measured at 2048 points)
+        if (nx >= 1024) row_threads = 256;
         RCHK(shapes());
         RCHK(lds_attr());
         RCHK(S1.alloc((size_t)nl * 3 * ny * nxh));
