@@ -14,9 +14,13 @@ the FP64 plain form is timed beside it on the same workload, and the two are com
 after 1e4 steps on a 512 x 512 x 8 crop of the same recipe.  The JSON line carries the
 roofline of the dominant kernel (HIP-event durations on the library's stream,
 algorithmic bytes per launch), a CPU baseline (the oracle, the C/OpenMP restatement of
-the reference path, on a bounded sample: steps of the 512 x 512 x 8 crop), and extra
-keys for the 256 x 256 x 4 thermal LLG workload, the 512 x 4096 x 4 recording track and
-the serving loop.
+the reference path, on a bounded sample: steps of a 256 x 256 x 8 crop), and extra keys
+for the 256 x 256 x 4 thermal LLG workload (with its switching-field distribution over 10
+seeds), the 512 x 4096 x 4 recording track, the serving loop and the write with open
+in-plane boundaries.
+
+`python bench.py --summarize trace|pmc --prof-dirs ...` writes the profiles/ summaries
+from rocprofv3's CSV output instead (no GPU work).
 """
 import argparse
 import json
@@ -80,7 +84,7 @@ def media_counts(med):
     return cm, runs, E
 
 
-def kernel_bytes(med, P, offsets):
+def kernel_bytes(med, P, offsets, fft=None):
     """Algorithmic bytes per launch of each kernel class (P: bytes of the demag path's real
     type; the grain state is FP64 in both forms).  Row r2c: the grains' magnetisation
     (3 P per grain), the row-major run lists (geometry and grain, 8 per run, plus the row
@@ -90,12 +94,15 @@ def kernel_bytes(med, P, offsets):
     (split path) and fused column kernel: the spectra in and out plus the unique tensor,
     2 P per complex element, T_u = 6 n_lay (n_lay+1)/2 per (kx, ky) (index symmetry and
     layer reciprocity), or 6 (2 n_lay - 1) when the stored tensor is the z-offset form
-    and that is fewer.  A grain stage: ~25 doubles per grain (state, constants, dm,
+    and that is fewer.  With open boundaries (fft = the padded grid) Kc counts the padded
+    grid, and the row kernels and the column kernels' media side move only the ny media
+    rows of each column.  A grain stage: ~25 doubles per grain (state, constants, dm,
     thermal, field), the exchange CSRs (4 + 8 per entry), the grain's run offsets (4), its
     runs' FP64 field sums, geometry and rows (32 per run) and its magnetisation out to the
     row kernels (3 P)."""
     nl = med.n_lay
-    kc = (med.nx // 2 + 1) * med.ny
+    fx, fy = fft or (med.nx, med.ny)     # the FFT grid (2nx x 2ny for open boundaries)
+    kc = (fx // 2 + 1) * fy
     G = med.G
     cm, runs, E = media_counts(med)
     rows = 4 * (nl * med.ny + 1)
@@ -103,16 +110,17 @@ def kernel_bytes(med, P, offsets):
     tu = 6 * nl * (nl + 1) // 2
     if offsets:
         tu = min(tu, 6 * (2 * nl - 1))
-    spectra = 12 * nl * kc * P
+    half = 6 * nl * kc * P                                 # one pass over the spectra
+    media_rows = 6 * nl * (fx // 2 + 1) * med.ny * P       # the rows the row kernels touch
     return {
-        "row_fwd": 3 * P * G + 8 * runs + rows + 6 * nl * kc * P,
-        "col": spectra + 2 * tu * kc * P,
-        "row_inv": 6 * nl * kc * P + 4 * runs + rows + 24 * runs,
+        "row_fwd": 3 * P * G + 8 * runs + rows + media_rows,
+        "col": media_rows + media_rows + 2 * tu * kc * P,
+        "row_inv": media_rows + 4 * runs + rows + 24 * runs,
         "grain1": grain,
         "grain2": grain,
-        "col_fwd": spectra,
-        "prod": spectra + 2 * tu * kc * P,
-        "col_inv": spectra,
+        "col_fwd": media_rows + half,
+        "prod": 2 * half + 2 * tu * kc * P,
+        "col_inv": half + media_rows,
     }
 
 
@@ -379,12 +387,13 @@ def serving(log, n_requests=40, rate=40.0):
     return st
 
 
-def llb_context(med, precision, tensor=None, num_images=2):
+def llb_context(med, precision, tensor=None, num_images=2, open=False):
     """A context of the LLB heat-assisted write: the media, the moving laser spot and the
-    write field on every layer, |m| at the 300 K mean-field value, LLB Heun at DT_LLB."""
-    x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=precision)
+    write field on every layer, |m| at the 300 K mean-field value, LLB Heun at DT_LLB.
+    open: open in-plane boundaries (the demag on the zero-padded 2nx x 2ny grid)."""
+    x = Llg(med.nx, med.ny, med.n_lay, med.dx, med.dy, med.dz, precision=precision, open=open)
     if tensor is None:
-        x.build_tensor(num_images=num_images)
+        x.build_tensor(num_images=0 if open else num_images)
     else:
         x.set_tensor(tensor)
     x.set_media(med)
@@ -495,7 +504,88 @@ def llg_cfg1(a, log):
     out["fp32"]["ms_per_step"] = el32 * 1e3 / 1000
     if a.accuracy_steps > 0:
         out["accuracy"] = accuracy(med, N, a.accuracy_steps, log)
+    if a.sfd_seeds > 0:
+        out["sfd"] = sfd_cfg1(med, N, a.sfd_seeds, log)
     return out
+
+
+def open_write(a, log, steps=20):
+    """The heat-assisted write with open in-plane boundaries: the same 2048 x 2048 x 8
+    problem, its demag FFTs on the zero-padded 4096 x 4096 grid (z coupling through the 15
+    layer-offset tensor blocks), FP32 form, default launch shapes."""
+    med = media.config_media("cfg4")
+    t = time.perf_counter()
+    x = llb_context(med, abi.MRMG_FP32, open=True)
+    setup = time.perf_counter() - t
+    x.step(a.warmup, philox_seed=SEED)
+    el, d = timed(x, steps, a.warmup * DT_LLB)
+    v = med.cells * steps / el
+    kern = kernel_table(x, med, 4, 5, (a.warmup + steps) * DT_LLB, log, "open write fp32")
+    kb = kernel_bytes(med, 4, x.get_tune("tensor_offsets"), fft=(2 * med.nx, 2 * med.ny))
+    for k in kern:
+        kern[k]["bytes"] = kb[k]
+        kern[k]["gbs"] = round(kb[k] / (kern[k]["avg_us"] * 1e-6) / 1e9, 1)
+        kern[k]["frac"] = round(kern[k]["gbs"] / HBM_PEAK_GBS, 4)
+    out = {"workload": "2048x2048x8 LLB heat-assisted write, open in-plane boundaries (demag FFTs on the zero-padded "
+                       "4096x4096 grid), FP32 form",
+           "value": v, "unit": "cell-updates/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
+           "setup_s": round(setup, 2), "footprint_bytes": x.footprint(), "finite": bool(np.isfinite(d.mmag)),
+           "tensor_offsets": x.get_tune("tensor_offsets"), "kernels": kern}
+    x.close()
+    log(f"open write fp32: {out['ms_per_step']:.3f} ms/step, {v:.4e} cell-updates/s (setup {setup:.1f} s)")
+    return out
+
+
+def sfd_cfg1(med, N, n_seeds, log, steps=40000):
+    """The per-grain switching-field distribution of the 256 x 256 x 4 thermal workload:
+    m from +z, the field tilted 5 degrees from -z ramped 0 -> 20 kOe over 2 ns, Heun at
+    dt = 50 fs for 40 000 steps at T = 300 K, one run per Philox seed; a grain's switching
+    field is the ramp field at the step its m_z first turns negative (the device's
+    switching record).  Reported per form: the switched fraction, quantiles of the
+    switching field per layer and over all grains, and the two-sample KS p-value of the
+    FP32 form's sample against the FP64 form's (one value per grain column, the mean over
+    its layers; unswitched columns censored at +inf)."""
+    from scipy import stats
+    h_max, t_ramp = 20000.0, 2e-9
+    res = {}
+    cols = {}
+    for name, prec in (("fp64", abi.MRMG_FP64), ("fp32", abi.MRMG_FP32)):
+        x, _ = build_ctx(med, prec, tensor=N)
+        hs = np.full((n_seeds, med.G), np.nan)
+        off = np.concatenate([[0], np.cumsum(med.ngs)])
+        t0 = time.perf_counter()
+        for i in range(n_seeds):
+            for k in range(med.n_lay):
+                x.set_state(k, np.tile([0.0, 0.0, 1.0], (med.ngs[k], 1)))
+            x.step_index = 0
+            x.switch_arm((0.0, 0.0, -1.0), 0.0)
+            x.step(steps, philox_seed=100 + i, diag=False)
+            for k in range(med.n_lay):
+                st, _, _ = x.switch_record(k)
+                v = h_max * np.minimum((st + 1) * DT / t_ramp, 1.0)
+                hs[i, off[k]:off[k + 1]] = np.where(st >= 0, v, np.nan)
+            log(f"sfd {name} seed {i}: switched {np.isfinite(hs[i]).mean():.3f}")
+        el = time.perf_counter() - t0
+        x.close()
+        q = (5, 25, 50, 75, 95)
+        per_layer = []
+        for k in range(med.n_lay):
+            v = hs[:, off[k]:off[k + 1]]
+            f = v[np.isfinite(v)]
+            per_layer.append({"switched": float(np.isfinite(v).mean()),
+                              "quantiles_oe": [round(float(z), 1) for z in np.percentile(f, q)] if f.size else None})
+        f = hs[np.isfinite(hs)]
+        res[name] = {"switched": float(np.isfinite(hs).mean()), "mean_oe": float(f.mean()), "std_oe": float(f.std()),
+                     "quantiles_oe": [round(float(z), 1) for z in np.percentile(f, q)], "per_layer": per_layer,
+                     "wall_s": round(el, 1)}
+        hc = np.where(np.isfinite(hs), hs, 1e9).reshape(n_seeds, med.n_lay, -1).mean(axis=1).ravel()
+        cols[name] = hc
+        log(f"sfd {name}: {res[name]}")
+    res["ks_fp32_vs_fp64_p"] = float(stats.ks_2samp(cols["fp32"], cols["fp64"]).pvalue)
+    res["protocol"] = (f"{n_seeds} Philox seeds, m from +z, field 5 deg from -z ramped 0 -> 20 kOe over 2 ns, "
+                       f"Heun dt=50 fs, {steps} steps, T=300 K; quantiles (5, 25, 50, 75, 95 %) of the switched grains")
+    log(f"sfd: KS p fp32 vs fp64 {res['ks_fp32_vs_fp64_p']:.3g}")
+    return res
 
 
 def main():
@@ -508,6 +598,7 @@ def main():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--sfd-seeds", type=int, default=10, help="Philox seeds of the cfg1 switching-field leg (0: skip)")
     ap.add_argument("--no-extra", action="store_true", help="skip the cfg1, track and serving legs")
     ap.add_argument("--summarize", choices=("trace", "pmc"),
                     help="write a profiles/ summary from rocprofv3 CSV output (no GPU work)")
@@ -593,6 +684,7 @@ def main():
         out["llg_cfg1"] = llg_cfg1(a, log)
         out["track_write"] = track_write(log)
         out["serving"] = serving(log)
+        out["open_write"] = open_write(a, log)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(log)
         log(f"cpu baseline: {out['cpu_baseline']}")

@@ -231,16 +231,28 @@ typedef struct {
 const char *mrmg_last_error(void);
 int mrmg_device_count(int *n);
 int mrmg_create(const mrmg_geom *g, mrmg_ctx **out);
+/* No counterpart in the reference, whose demag is periodic in x and y (demag.f90:113-127):
+ * a context with open in-plane boundaries.  The demag is the aperiodic convolution over the
+ * nx x ny media, computed with FFTs of the grid zero-padded to 2nx x 2ny (the media in the
+ * first nx x ny cells, zeros elsewhere, every displacement of the media grid held once).
+ * The tensor calls below then work on that padded grid: (2nx,2ny,9,n_lay,n_lay) arrays
+ * divided by dx*dy*dz(ko)*2nx*2ny, num_images 0.  Everything else (layers, states, sources,
+ * readback, which then uses plain instead of minimum-image distances) is on the media grid.
+ * 2nx and 2ny must factor into 2, 3 and 5. */
+int mrmg_create_open(const mrmg_geom *g, mrmg_ctx **out);
 int mrmg_destroy(mrmg_ctx *ctx);
 
 /* Demag tensor, real space, the (nx,ny,9,n_lay,n_lay) FP64 array of calculate_tensor
  * (tensor.f90:82-216) already divided by dx*dy*dz(ko)*nx*ny; component c = 3(a-1)+b maps
  * source component b to field component a (demag.f90:117-121).  Transformed on the
- * device in FP64 (demag.f90:259-267). */
+ * device in FP64 (demag.f90:259-267).  (2nx,2ny,...) in an open context. */
 int mrmg_set_tensor(mrmg_ctx *ctx, const double *n_real);
 /* Build the same tensor on the device in FP64 (tensor.f90:136-183 without SUL/head
  * images, plane_to_plane with num_images periodic images in x and y) and transform it.
- * When n_real_out is not NULL, the real-space tensor is also copied back to it. */
+ * When n_real_out is not NULL, the real-space tensor is also copied back to it.  In an
+ * open context num_images must be 0 and the tensor is the padded grid's: at index i of
+ * 2nx the displacement -i (i < nx) or 2nx - i (i > nx) of the media grid, zero at i = nx
+ * (likewise in y). */
 int mrmg_build_tensor(mrmg_ctx *ctx, int num_images, double *n_real_out);
 /* As mrmg_build_tensor, with calculate_tensor's magnetic images (tensor.f90:103-200):
  * sul_mu > 1 adds each source layer mirrored in the soft underlayer, space_to_sul below the
@@ -251,6 +263,11 @@ int mrmg_build_tensor(mrmg_ctx *ctx, int num_images, double *n_real_out);
  * mrmg_build_tensor is this with sul_mu = head_mu = 1. */
 int mrmg_build_tensor_images(mrmg_ctx *ctx, int num_images, double sul_mu, double head_mu, double fly_height,
                              double space_to_sul, double *n_real_out);
+/* No counterpart in the reference (its program builds one tensor per run): give dst the
+ * tensor spectra of src, a context of the same precision, grid, boundaries, cell size and
+ * layer stack on the same device, as a device-to-device copy instead of a rebuild (a
+ * request loop starting many simulations of one grid). */
+int mrmg_copy_tensor(mrmg_ctx *dst, const mrmg_ctx *src);
 
 int mrmg_set_layer(mrmg_ctx *ctx, int k, const mrmg_layer *layer);
 int mrmg_set_ufield(mrmg_ctx *ctx, int k, const mrmg_ufield *uf);
@@ -275,6 +292,19 @@ int mrmg_get_mag(mrmg_ctx *ctx, int k, double *mag);
  * layer order, in the order get_thermal consumes them (field.f90:405-431).  diag may be
  * NULL.  The call returns after the device work is complete. */
 int mrmg_step(mrmg_ctx *ctx, const mrmg_step_params *p, const double *noise, mrmg_diag *diag);
+
+/* No counterpart in the reference (its program runs one simulation): fixed-dt steps of n
+ * contexts together, for a request loop that serves many simulations on one GPU.  The
+ * contexts must have equal grids (media and FFT), precision, layer count, launch shapes
+ * and tensor storage, and each params[b] the same n_steps, dt, dt_max, llg_type, llb and
+ * noise (MRMG_NOISE_NONE or MRMG_NOISE_PHILOX; seed, step0 and t0 are per member); no
+ * adaptive dt, moving sources, switching record or profiling.  Each demag kernel runs once
+ * for all members; with graph != 0 (llg_type < 0) one step is captured as a hipGraph and
+ * replayed n_steps times.  Every member ends where its own mrmg_step with the same
+ * parameters would end, bit for bit; t_end (n values, may be NULL) receives each member's
+ * end time (its t0 plus n_steps additions of dt, the next call's t0).  The work runs on
+ * ctxs[0]'s stream; no other call may use these contexts meanwhile. */
+int mrmg_step_batch(mrmg_ctx *const *ctxs, const mrmg_step_params *params, int n, int graph, double *t_end);
 
 /* The normals the Philox source draws for steps step0 .. step0+n_steps-1, same (3,G,n)
  * layout as injected noise, so a CPU path can be fed the identical thermal field. */

@@ -244,6 +244,7 @@ def load(path=None):
     for name, args in [
         ("mrmg_device_count", [_ip]),
         ("mrmg_create", [C.POINTER(Geom), C.POINTER(C.c_void_p)]),
+        ("mrmg_create_open", [C.POINTER(Geom), C.POINTER(C.c_void_p)]),
         ("mrmg_destroy", [C.c_void_p]),
         ("mrmg_set_tensor", [C.c_void_p, _dp]),
         ("mrmg_build_tensor", [C.c_void_p, C.c_int, _dp]),
@@ -256,6 +257,8 @@ def load(path=None):
         ("mrmg_set_mag", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_get_mag", [C.c_void_p, C.c_int, _dp]),
         ("mrmg_step", [C.c_void_p, C.POINTER(StepParams), _dp, C.POINTER(Diag)]),
+        ("mrmg_step_batch", [C.POINTER(C.c_void_p), C.POINTER(StepParams), C.c_int, C.c_int, _dp]),
+        ("mrmg_copy_tensor", [C.c_void_p, C.c_void_p]),
         ("mrmg_philox_normals", [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
         ("mrmg_demag_field", [C.c_void_p, _dp, _dp]),
         ("mrmg_footprint", [C.c_void_p, C.POINTER(C.c_size_t)]),

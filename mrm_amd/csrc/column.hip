@@ -20,12 +20,19 @@
 // E then bounds one lane's elements of its wave's run.
 // BIG: 1024-thread workgroups (twice the lanes per FFT pass, at most 128 VGPRs), two
 // product items per thread.
-template <typename T, int E, bool SYM, bool TL = false, bool WV = false, bool BIG = false>
-__global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
-                                                     const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh,
+// BAT: one launch over a batch of contexts (blockIdx.z the member, operands from tab).
+// ny: the column length (the FFT grid); nyv: the rows the row kernels write and read
+// (the media rows; below ny for open boundaries, whose padding rows are zero in S1 and
+// not needed in S2).
+template <typename T, int E, bool SYM, bool TL = false, bool WV = false, bool BIG = false, bool BAT = false>
+__global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
+                                                     int ny, int nyv, int nl, int nxh,
                                                      int nslot, int toep, int kg, int nb8, FFTPlan py,
                                                      const C2<T>* __restrict__ twy)
 {
+    const C2<T>* __restrict__ S1 = BAT ? tab[blockIdx.z].S1 : one.S1;
+    C2<T>* __restrict__ S2 = BAT ? tab[blockIdx.z].S2 : one.S2;
+    const C2<T>* __restrict__ Tspec = BAT ? tab[blockIdx.z].Tspec : one.Tspec;
     const int L = xcd_remap(blockIdx.x, nb8);
     const int kx = L / kg, grp = L - kx * kg;
     if (kx >= nxh) return;
@@ -41,6 +48,7 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(const C2<T
         ncol * ny,
         [&](int idx) {
             const int col = idx / ny, y = idx - col * ny;
+            if (y >= nyv) return C2<T>{0, 0};
             return TL ? S1[((size_t)col * nxh + kx) * ny + y] : S1[((size_t)col * ny + y) * nxh + kx];
         },
         [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; }, tw, twy, ny);
@@ -134,46 +142,55 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(const C2<T
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const int idx = e0 + i;
         const int col = idx / ny, y = idx - col * ny;
+        if (y >= nyv) continue;
         const C2<T> v = {re[LP(idx)], im[LP(idx)]};
         if (TL) S2[((size_t)col * nxh + kx) * ny + y] = v;
         else S2[((size_t)col * ny + y) * nxh + kx] = v;
     }
 }
 
+#define COL_ARGS o, btab, c->fy, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p
 template <typename T>
 template <int MB> void Engine<T>::launch_col_sym(int nb8)
 {
-    if (col_threads > 512) {
+    const DemagOps<T> o = ops();
+    const dim3 g(nb8, 1, btab ? (unsigned)bn : 1u), b(col_threads);
+    if (btab) {
         if (tl)
-            hipLaunchKernelGGL((k_col<T, MB, true, true, false, true>), dim3(nb8), dim3(col_threads), lds_col,
-                               c->stream, S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+            hipLaunchKernelGGL((k_col<T, MB, true, true, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
         else
-            hipLaunchKernelGGL((k_col<T, MB, true, false, false, true>), dim3(nb8), dim3(col_threads), lds_col,
-                               c->stream, S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+            hipLaunchKernelGGL((k_col<T, MB, true, false, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
+    } else if (col_threads > 512) {
+        if (tl)
+            hipLaunchKernelGGL((k_col<T, MB, true, true, false, true>), g, b, lds_col, c->stream, COL_ARGS);
+        else
+            hipLaunchKernelGGL((k_col<T, MB, true, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
     } else if (tl && wv)
-        hipLaunchKernelGGL((k_col<T, MB, true, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
-                           S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, true, true, true>), g, b, lds_col, c->stream, COL_ARGS);
     else if (tl)
-        hipLaunchKernelGGL((k_col<T, MB, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                           Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, true, true>), g, b, lds_col, c->stream, COL_ARGS);
     else
-        hipLaunchKernelGGL((k_col<T, MB, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                           Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, true>), g, b, lds_col, c->stream, COL_ARGS);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_col_gen(int nb8)
 {
-    if (tl && wv)
-        hipLaunchKernelGGL((k_col<T, MB, false, true, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream,
-                           S1.p, S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+    const DemagOps<T> o = ops();
+    const dim3 g(nb8, 1, btab ? (unsigned)bn : 1u), b(col_threads);
+    if (btab) {
+        if (tl)
+            hipLaunchKernelGGL((k_col<T, MB, false, true, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
+        else
+            hipLaunchKernelGGL((k_col<T, MB, false, false, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
+    } else if (tl && wv)
+        hipLaunchKernelGGL((k_col<T, MB, false, true, true>), g, b, lds_col, c->stream, COL_ARGS);
     else if (tl)
-        hipLaunchKernelGGL((k_col<T, MB, false, true>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p,
-                           S2.p, Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, false, true>), g, b, lds_col, c->stream, COL_ARGS);
     else
-        hipLaunchKernelGGL((k_col<T, MB, false>), dim3(nb8), dim3(col_threads), lds_col, c->stream, S1.p, S2.p,
-                           Tspec.p, c->ny, c->nl, c->nxh, nslot, (int)toep, kg, nb8, pcol, twy.p);
+        hipLaunchKernelGGL((k_col<T, MB, false>), g, b, lds_col, c->stream, COL_ARGS);
 }
+#undef COL_ARGS
 
 template <typename T>
 template <int MB> int Engine<T>::lds_attr_col()
@@ -188,6 +205,14 @@ template <int MB> int Engine<T>::lds_attr_col()
     HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, false, false, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, false, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, false, false, false, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     return MRMG_OK;
 }

@@ -71,11 +71,20 @@ __device__ __forceinline__ void lds_to_runs(const RowRuns& R, double* __restrict
 // Row r2c: one workgroup per (row pair, layer).  Two rows of each component of the cell
 // magnetisation (written by the grain kernels, get_mag data.f90:344-353) are packed as
 // z = M(y0) + i M(y1), transformed with one complex FFT and separated.
-template <typename T, int E>
-__global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mg, int G, RowRuns R, C2<T>* __restrict__ S,
-                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+// nx: the FFT row length; ny: the media rows of a layer; sy: the spectra's rows (the FFT
+// column length, 2 ny with open boundaries, whose padding rows the kernel leaves alone).
+// BAT: one launch over a batch of contexts, blockIdx.z the member (operands from tab);
+// else the context's own operands (one).  The same holds for every demag kernel below.
+template <typename T, int E, bool BAT = false>
+__global__ void __launch_bounds__(256) k_row_fwd(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
+                                                 int nx, int ny, int sy, int nl, int nxh, FFTPlan px,
                                                  const C2<T>* __restrict__ twx)
 {
+    const DemagOps<T> o = BAT ? tab[blockIdx.z] : one;
+    const T* __restrict__ Mg = o.Mg;
+    const int G = o.G;
+    const RowRuns R = o.R;
+    C2<T>* __restrict__ S = o.S1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int roff[3];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
@@ -92,7 +101,7 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mg, int G
         const T zr = re[LP(c * nx + kx)], zi = im[LP(c * nx + kx)], nr = re[LP(c * nx + kn)], ni = im[LP(c * nx + kn)];
         C2<T> A = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
         C2<T> B = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
-        C2<T>* row = S + (((size_t)k * 3 + c) * ny + y0) * nxh;
+        C2<T>* row = S + (((size_t)k * 3 + c) * sy + y0) * nxh;
         row[kx] = A;
         if (has1) row[nxh + kx] = B;
     }
@@ -100,13 +109,18 @@ __global__ void __launch_bounds__(256) k_row_fwd(const T* __restrict__ Mg, int G
 
 
 // Row c2r: FFTW's 1D c2r semantics (imaginary parts of the DC and Nyquist bins ignored).
-// CELLS: the cell field into Hcell [c][k][y][x] (mrmg_demag_field), else its run sums into Hr.
-template <typename T, int E, bool CELLS = false>
-__global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T* __restrict__ Hcell,
-                                                 double* __restrict__ Hr, RowRuns R,
-                                                 int nx, int ny, int nl, int nxh, FFTPlan px,
+// CELLS: the cell field into Hcell [c][k][y][x] (mrmg_demag_field; mx: the media row
+// length), else its run sums into Hr.
+template <typename T, int E, bool CELLS = false, bool BAT = false>
+__global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
+                                                 int nx, int ny, int sy, int mx, int nl, int nxh, FFTPlan px,
                                                  const C2<T>* __restrict__ twx)
 {
+    const DemagOps<T> o = BAT ? tab[blockIdx.z] : one;
+    const C2<T>* __restrict__ S2 = o.S2;
+    T* __restrict__ Hcell = o.Hcell;
+    double* __restrict__ Hr = o.Hr;
+    const RowRuns R = o.R;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int roff[3];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
@@ -118,7 +132,7 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
         3 * nxh,
         [&](int idx) {
             const int c = idx / nxh, kx = idx - c * nxh;
-            const C2<T>* row = S2 + (((size_t)k * 3 + c) * ny + y0) * nxh;
+            const C2<T>* row = S2 + (((size_t)k * 3 + c) * sy + y0) * nxh;
             return C4<T>{row[kx], row[(has1 ? nxh : 0) + kx]};
         },
         [&](int idx, C4<T> v) {
@@ -140,12 +154,12 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
         lds_to_runs<T, 3>(R, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
         return;
     }
-    const size_t plane = (size_t)nx * ny;
-    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
-        const int c = idx / nx, x = idx - c * nx;
+    const size_t plane = (size_t)mx * ny;
+    for (int idx = threadIdx.x; idx < 3 * mx; idx += blockDim.x) {
+        const int c = idx / mx, x = idx - c * mx;
         T* hc = Hcell + ((size_t)c * nl + k) * plane;
-        hc[(size_t)y0 * nx + x] = re[LP(idx)];
-        if (has1) hc[(size_t)y1 * nx + x] = im[LP(idx)];
+        hc[(size_t)y0 * mx + x] = re[LP(c * nx + x)];
+        if (has1) hc[(size_t)y1 * mx + x] = im[LP(c * nx + x)];
     }
 }
 
@@ -157,11 +171,16 @@ __global__ void __launch_bounds__(256) k_row_inv(const C2<T>* __restrict__ S2, T
 //   k_prod      the k-space product (demag.f90:117-121), one thread per (kx, ky)
 //   k_colT      inverse y-FFT, in place
 //   k_rowT_inv  c2r of TY rows, read back from TY-long y runs
-template <typename T, int E, bool WV = false>
-__global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mg, int G, RowRuns R, C2<T>* __restrict__ S,
-                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+template <typename T, int E, bool WV = false, bool BAT = false>
+__global__ void __launch_bounds__(256) k_rowT_fwd(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
+                                                  int nx, int ny, int sy, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
 {
+    const DemagOps<T> o = BAT ? tab[blockIdx.z] : one;
+    const T* __restrict__ Mg = o.Mg;
+    const int G = o.G;
+    const RowRuns R = o.R;
+    C2<T>* __restrict__ S = o.S1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int roff[9];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
@@ -183,7 +202,7 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mg, int 
     } else {
         lds_fft<T, E, false>(re, im, B, px, tw);
     }
-    C2<T>* dst = S + (size_t)kc * nxh * ny;
+    C2<T>* dst = S + (size_t)kc * nxh * sy;
     for (int idx = threadIdx.x; idx < nxh * TY; idx += blockDim.x) {
         const int kx = idx / TY, yy = idx - kx * TY;
         const int y = y0 + yy;
@@ -194,36 +213,47 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(const T* __restrict__ Mg, int 
         C2<T> v;
         if ((yy & 1) == 0) v = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
         else v = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
-        dst[(size_t)kx * ny + y] = v;
+        dst[(size_t)kx * sy + y] = v;
     }
 }
 
 // y-FFT of ncols contiguous columns of length ny, CB columns per workgroup, in place.
-// Twiddles are read from the global table (cached), not staged.
-template <typename T, int E, bool INV>
-__global__ void __launch_bounds__(256) k_colT(C2<T>* __restrict__ S, int ny, int ncols, int CB, FFTPlan py,
-                                              const C2<T>* __restrict__ twy)
+// Twiddles are read from the global table (cached), not staged.  Rows y >= nyv (the
+// padding of open boundaries) are read as zero by the forward transform and not stored by
+// the inverse one (no row kernel reads them).
+template <typename T, int E, bool INV, bool BAT = false>
+__global__ void __launch_bounds__(256) k_colT(DemagOps<T> one, const DemagOps<T>* __restrict__ tab, int ny, int nyv,
+                                              int ncols, int CB, FFTPlan py, const C2<T>* __restrict__ twy)
 {
+    C2<T>* __restrict__ S = INV ? (BAT ? tab[blockIdx.z].S2 : one.S2) : (BAT ? tab[blockIdx.z].S1 : one.S1);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* re = reinterpret_cast<T*>(smem_raw);
     const int q0 = blockIdx.x * CB;
     const int nb = min(CB, ncols - q0);
     T* im = re + LPN(CB * ny);
     C2<T>* base = S + (size_t)q0 * ny;
-    staged_load<(E < 8 ? E : 8), T>(
-        nb * ny, [&](int idx) { return base[idx]; }, [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
+    if (INV || nyv == ny)
+        staged_load<(E < 8 ? E : 8), T>(
+            nb * ny, [&](int idx) { return base[idx]; }, [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
+    else
+        staged_load<(E < 8 ? E : 8), T>(
+            nb * ny, [&](int idx) { return idx % ny < nyv ? base[idx] : C2<T>{0, 0}; },
+            [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
     __syncthreads();
     lds_fft<T, E, INV>(re, im, nb, py, twy);
-    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x)
+        if (!INV || nyv == ny || idx % ny < nyv) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
 }
 
 // The k-space product over column-contiguous spectra: thread per (kx, ky), observation
 // layers in chunks of OC, each output summed over (ks, b) in the reference's order.
-template <typename T, bool SYM, int OC>
-__global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
-                                              const C2<T>* __restrict__ Tspec, int ny, int nl, int nxh, int nslot,
-                                              int toep)
+template <typename T, bool SYM, int OC, bool BAT = false>
+__global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>* __restrict__ tab, int ny, int nl,
+                                              int nxh, int nslot, int toep)
 {
+    const C2<T>* __restrict__ S1 = BAT ? tab[blockIdx.z].S1 : one.S1;
+    C2<T>* __restrict__ S2 = BAT ? tab[blockIdx.z].S2 : one.S2;
+    const C2<T>* __restrict__ Tspec = BAT ? tab[blockIdx.z].Tspec : one.Tspec;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t Kc = (size_t)nxh * ny;
     if (idx >= Kc) return;
@@ -283,10 +313,13 @@ __global__ void __launch_bounds__(256) k_prod(const C2<T>* __restrict__ S1, C2<T
 // NL-1 down to -(NL-1), so every field sum H_ko still adds its sources in ascending ks, as
 // k_prod and the reference do (demag.f90:117-121), while each spectrum and each stored
 // tensor value is read from memory once.
-template <typename T, bool SYM, int NL>
-__global__ void __launch_bounds__(256) k_prod_offsets(const C2<T>* __restrict__ S1, C2<T>* __restrict__ S2,
-                                                      const C2<T>* __restrict__ Tspec, int ny, int nxh, int nslot)
+template <typename T, bool SYM, int NL, bool BAT = false>
+__global__ void __launch_bounds__(256) k_prod_offsets(DemagOps<T> one, const DemagOps<T>* __restrict__ tab, int ny,
+                                                      int nxh, int nslot)
 {
+    const C2<T>* __restrict__ S1 = BAT ? tab[blockIdx.z].S1 : one.S1;
+    C2<T>* __restrict__ S2 = BAT ? tab[blockIdx.z].S2 : one.S2;
+    const C2<T>* __restrict__ Tspec = BAT ? tab[blockIdx.z].Tspec : one.Tspec;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t Kc = (size_t)nxh * ny;
     if (idx >= Kc) return;
@@ -339,12 +372,16 @@ __global__ void __launch_bounds__(256) k_prod_offsets(const C2<T>* __restrict__ 
 
 // c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
 // imaginary parts of the DC and Nyquist bins ignored).
-template <typename T, int E, bool WV = false, bool CELLS = false>
-__global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T* __restrict__ Hcell,
-                                                  double* __restrict__ Hr, RowRuns R,
-                                                  int nx, int ny, int nl, int nxh, int TY, FFTPlan px,
+template <typename T, int E, bool WV = false, bool CELLS = false, bool BAT = false>
+__global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
+                                                  int nx, int ny, int sy, int mx, int nl, int nxh, int TY, FFTPlan px,
                                                   const C2<T>* __restrict__ twx)
 {
+    const DemagOps<T> o = BAT ? tab[blockIdx.z] : one;
+    const C2<T>* __restrict__ S = o.S2;
+    T* __restrict__ Hcell = o.Hcell;
+    double* __restrict__ Hr = o.Hr;
+    const RowRuns R = o.R;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int roff[9];
     C2<T>* tw = reinterpret_cast<C2<T>*>(smem_raw);
@@ -356,14 +393,14 @@ __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T
     if (L >= nyb * 3 * nl) return;
     const int y0 = (L % nyb) * TY, kc = L / nyb;
     const int k = kc / 3, c = kc - 3 * k;
-    const C2<T>* src = S + (size_t)kc * nxh * ny;
+    const C2<T>* src = S + (size_t)kc * nxh * sy;
     // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
     staged_load<(E < 4 ? E : 4)>(
         nxh * B,
         [&](int idx) {
             const int kx = idx / B, p = idx - kx * B;
             const int ya = y0 + 2 * p, yb = ya + 1;
-            return C4<T>{src[(size_t)kx * ny + (ya < ny ? ya : ny - 1)], src[(size_t)kx * ny + (yb < ny ? yb : ny - 1)]};
+            return C4<T>{src[(size_t)kx * sy + (ya < ny ? ya : ny - 1)], src[(size_t)kx * sy + (yb < ny ? yb : ny - 1)]};
         },
         [&](int idx, C4<T> v) {
             const int kx = idx / B, p = idx - kx * B;
@@ -391,12 +428,12 @@ __global__ void __launch_bounds__(256) k_rowT_inv(const C2<T>* __restrict__ S, T
         lds_to_runs<T, 1>(R, Hr, c, nrows, B, nx, re, im, roff);
         return;
     }
-    T* hc = Hcell + ((size_t)c * nl + k) * (size_t)nx * ny;
-    for (int idx = threadIdx.x; idx < B * nx; idx += blockDim.x) {
-        const int p = idx / nx, x = idx - p * nx;
+    T* hc = Hcell + ((size_t)c * nl + k) * (size_t)mx * ny;
+    for (int idx = threadIdx.x; idx < B * mx; idx += blockDim.x) {
+        const int p = idx / mx, x = idx - p * mx;
         const int ya = y0 + 2 * p, yb = ya + 1;
-        if (ya < ny) hc[(size_t)ya * nx + x] = re[LP(idx)];
-        if (yb < ny) hc[(size_t)yb * nx + x] = im[LP(idx)];
+        if (ya < ny) hc[(size_t)ya * mx + x] = re[LP(p * nx + x)];
+        if (yb < ny) hc[(size_t)yb * mx + x] = im[LP(p * nx + x)];
     }
 }
 
@@ -407,6 +444,9 @@ template <int MB> int Engine<T>::lds_attr_mb()
     HIPCHK(hipFuncSetAttribute((const void*)k_row_fwd<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
     HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_row_fwd<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_row_inv<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               r));
     RCHK(lds_attr_col<MB>());
     return MRMG_OK;
 }
@@ -425,6 +465,12 @@ template <int MB> int Engine<T>::lds_attr_split()
                                r));
     HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_fwd<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_rowT_inv<T, MB, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, r));
+    HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_colT<T, MB, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     return MRMG_OK;
 }
 
@@ -444,14 +490,19 @@ int Engine<T>::lds_attr()
 }
 
 // one launch of a demag kernel class (K_ROW_FWD, K_COL, K_ROW_INV on the fused paths;
-// K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV on the split path)
+// K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV on the split path); with a batch table set
+// (btab, bn members), one launch for all members, blockIdx.z the member
 template <typename T>
 int Engine<T>::launch_class(int cls)
 {
-    const int ny = c->ny, nl = c->nl, nxh = c->nxh;
-    const dim3 grow((ny + 1) / 2, nl), growT(rowT_blocks()), blk(row_threads), b256(256);
+    // row kernels: the media rows; column and product kernels: the FFT grid
+    const int ny = c->ny, fy = c->fy, nl = c->nl, nxh = c->nxh;
+    const unsigned z = btab ? (unsigned)bn : 1u;
+    const dim3 grow((ny + 1) / 2, nl, z), growT(rowT_blocks(), 1, z), blk(row_threads), b256(256);
     const int ncols = 3 * nl * nxh;
-    const dim3 gcol((ncols + CBc - 1) / CBc);
+    const dim3 gcol((ncols + CBc - 1) / CBc, 1, z);
+    if (btab && (wv || cells_out))
+        return set_err(MRMG_ERR_STATE, "batched demag launches run without wave-local FFTs or cell output");
     switch (cls) {
     case K_ROW_FWD:
         if (tl || split) LAUNCH_E(maxb_rowT, launch_rowT_fwd, growT, blk);
@@ -473,16 +524,19 @@ int Engine<T>::launch_class(int cls)
         break;
     case K_PROD: {
         if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate product kernel");
-        const size_t Kc = (size_t)nxh * ny;
-        const dim3 gp((unsigned)((Kc + 255) / 256));
+        const size_t Kc = (size_t)nxh * fy;
+        const dim3 gp((unsigned)((Kc + 255) / 256), 1, z);
+        const DemagOps<T> o = ops();
         if (toep && nl <= 8) {
-#define KPO(NLV)                                                                                                  \
-    if (nslot == 6)                                                                                               \
-        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nxh,  \
-                           nslot);                                                                                \
-    else                                                                                                          \
-        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nxh, \
-                           nslot);
+#define KPO(NLV)                                                                                                     \
+    if (nslot == 6 && btab)                                                                                          \
+        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV, true>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);   \
+    else if (nslot == 6)                                                                                             \
+        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);         \
+    else if (btab)                                                                                                   \
+        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV, true>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);  \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);
             switch (nl) {
             case 1: KPO(1) break;
             case 2: KPO(2) break;
@@ -494,12 +548,17 @@ int Engine<T>::launch_class(int cls)
             default: KPO(8) break;
             }
 #undef KPO
-        } else if (nslot == 6)
-            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh, nslot,
+        } else if (nslot == 6 && btab)
+            hipLaunchKernelGGL((k_prod<T, true, 4, true>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
+                               (int)toep);
+        else if (nslot == 6)
+            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot, (int)toep);
+        else if (btab)
+            hipLaunchKernelGGL((k_prod<T, false, 4, true>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
                                (int)toep);
         else
-            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, S1.p, S2.p, Tspec.p, ny, nl, nxh,
-                               nslot, (int)toep);
+            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
+                               (int)toep);
         break;
     }
     case K_ROW_INV:
@@ -516,62 +575,88 @@ int Engine<T>::launch_class(int cls)
 template <typename T>
 template <int MB> void Engine<T>::launch_rowT_fwd(dim3 g, dim3 b)
 {
-    if (wv)
-        hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny,
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB, false, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy,
                            c->nl, c->nxh, TYr, prow, twx.p);
-    else
-        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny, c->nl,
+    else if (wv)
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy, c->nl,
                            c->nxh, TYr, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_rowT_fwd<T, MB>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy, c->nl, c->nxh,
+                           TYr, prow, twx.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_colT_fwd(dim3 g, dim3 b)
 {
-    hipLaunchKernelGGL((k_colT<T, MB, false>), g, b, lds_colT, c->stream, S1.p, c->ny, 3 * c->nl * c->nxh, CBc,
-                       pcol, twy.p);
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_colT<T, MB, false, true>), g, b, lds_colT, c->stream, o, btab, c->fy, c->ny,
+                           3 * c->nl * c->nxh, CBc, pcol, twy.p);
+    else
+        hipLaunchKernelGGL((k_colT<T, MB, false>), g, b, lds_colT, c->stream, o, btab, c->fy, c->ny,
+                           3 * c->nl * c->nxh, CBc, pcol, twy.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_colT_inv(dim3 g, dim3 b)
 {
-    hipLaunchKernelGGL((k_colT<T, MB, true>), g, b, lds_colT, c->stream, S2.p, c->ny, 3 * c->nl * c->nxh, CBc,
-                       pcol, twy.p);
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_colT<T, MB, true, true>), g, b, lds_colT, c->stream, o, btab, c->fy, c->ny,
+                           3 * c->nl * c->nxh, CBc, pcol, twy.p);
+    else
+        hipLaunchKernelGGL((k_colT<T, MB, true>), g, b, lds_colT, c->stream, o, btab, c->fy, c->ny,
+                           3 * c->nl * c->nxh, CBc, pcol, twy.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_rowT_inv(dim3 g, dim3 b)
 {
-    const RowRuns R = runs();
-    if (wv && cells_out)
-        hipLaunchKernelGGL((k_rowT_inv<T, MB, true, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx,
-                           c->ny, c->nl, c->nxh, TYr, prow, twx.p);
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, false, false, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny,
+                           c->fy, c->nx, c->nl, c->nxh, TYr, prow, twx.p);
+    else if (wv && cells_out)
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, true, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy,
+                           c->nx, c->nl, c->nxh, TYr, prow, twx.p);
     else if (wv)
-        hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx, c->ny,
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy, c->nx,
                            c->nl, c->nxh, TYr, prow, twx.p);
     else if (cells_out)
-        hipLaunchKernelGGL((k_rowT_inv<T, MB, false, true>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx,
-                           c->ny, c->nl, c->nxh, TYr, prow, twx.p);
+        hipLaunchKernelGGL((k_rowT_inv<T, MB, false, true>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy,
+                           c->nx, c->nl, c->nxh, TYr, prow, twx.p);
     else
-        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, S2.p, Hcell.p, Hr.p, R, c->nx, c->ny, c->nl,
+        hipLaunchKernelGGL((k_rowT_inv<T, MB>), g, b, lds_rowT, c->stream, o, btab, c->fx, c->ny, c->fy, c->nx, c->nl,
                            c->nxh, TYr, prow, twx.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_row_fwd(dim3 g, dim3 b)
 {
-    hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, Mg.p, G, runs(), S1.p, c->nx, c->ny, c->nl,
-                       c->nxh, prow, twx.p);
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_row_fwd<T, MB, true>), g, b, lds_row, c->stream, o, btab, c->fx, c->ny, c->fy, c->nl,
+                           c->nxh, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_row_fwd<T, MB>), g, b, lds_row, c->stream, o, btab, c->fx, c->ny, c->fy, c->nl, c->nxh,
+                           prow, twx.p);
 }
 
 template <typename T>
 template <int MB> void Engine<T>::launch_row_inv(dim3 g, dim3 b)
 {
-    if (cells_out)
-        hipLaunchKernelGGL((k_row_inv<T, MB, true>), g, b, lds_row, c->stream, S2.p, Hcell.p, Hr.p, runs(), c->nx,
-                           c->ny, c->nl, c->nxh, prow, twx.p);
-    else
-        hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, S2.p, Hcell.p, Hr.p, runs(), c->nx, c->ny,
+    const DemagOps<T> o = ops();
+    if (btab)
+        hipLaunchKernelGGL((k_row_inv<T, MB, false, true>), g, b, lds_row, c->stream, o, btab, c->fx, c->ny, c->fy,
+                           c->nx, c->nl, c->nxh, prow, twx.p);
+    else if (cells_out)
+        hipLaunchKernelGGL((k_row_inv<T, MB, true>), g, b, lds_row, c->stream, o, btab, c->fx, c->ny, c->fy, c->nx,
                            c->nl, c->nxh, prow, twx.p);
+    else
+        hipLaunchKernelGGL((k_row_inv<T, MB>), g, b, lds_row, c->stream, o, btab, c->fx, c->ny, c->fy, c->nx, c->nl,
+                           c->nxh, prow, twx.p);
 }
 
 // each precision form is its own object (the unit is compiled with MRMG_FORM=float and

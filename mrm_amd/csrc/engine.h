@@ -193,10 +193,16 @@ struct EngineBase {
     virtual int switch_arm(int enable, const double* dir, double thr) = 0;
     virtual int switch_read(int k, int64_t* step, double* t, double* hap) = 0;
     virtual int time_class(int cls, int reps, double* ms) = 0;
+    // the tensor spectra of another context of the same form, grid and layer stack
+    virtual int copy_tensor(EngineBase* src) = 0;
 };
 
 struct mrmg_ctx {
+    // nx, ny: the media grid; fx, fy: the FFT grid of the demag, equal to it for periodic
+    // boundaries, 2 nx x 2 ny (zero-padded) for open ones; nxh = fx/2 + 1
     int nx = 0, ny = 0, nl = 0, nxh = 0, precision = 0, demag = 1;
+    int fx = 0, fy = 0;
+    bool open = false;
     int device = 0;   // the device current at mrmg_create; the stream and buffers live there
     double dx = 0, dy = 0;
     std::vector<double> dz, il;
@@ -247,8 +253,10 @@ static inline int pick_e(int batch, const FFTPlan& p, int threads)
 int launch_philox_dump(hipStream_t s, unsigned long long seed, unsigned long long step0, int nsteps, int G,
                        double* out);
 int launch_copy16(hipStream_t s, unsigned grid, const uint4* a, uint4* b, size_t n);
+// onx, ony > 0: open boundaries, the (nx, ny) = (2 onx, 2 ony) padded grid's displacements
 int launch_plane_to_plane(hipStream_t s, int nx, int ny, double dx, double dy, double dzs, double dzo, double rz,
-                          int nimg, double norm, double* blk, int mode, double fxy, double fz);
+                          int nimg, double norm, double* blk, int mode, double fxy, double fz, int onx = 0,
+                          int ony = 0);
 int launch_tensor_div(hipStream_t s, double* blk, size_t n, double norm);
 // z-offset (Toeplitz) and index-symmetry checks of a real tensor of (nx,ny) planes on the device
 int tensor_toeplitz_l1(hipStream_t s, const double* R, size_t block_elems, int nl, double* d_out);
@@ -278,7 +286,8 @@ struct Engine : EngineBase {
     bool mag_dirty = false;
     // moving sources: node positions and values of every table, per-stage axis tables,
     // and the grain temperature of the last field evaluation (adaptive path)
-    DBuf<double> src_pos, src_vals, ax_xa, ax_ya, ax_yb, tlast;
+    DBuf<double> src_pos, src_vals, ax_xa, ax_xc, ax_ya, ax_yb, tlast;
+    bool any_sg = false;   // a super-Gaussian temperature source on some layer
     DBuf<int> ax_xi, ax_yi;
     bool any_src = false;
     // demag
@@ -380,14 +389,33 @@ struct Engine : EngineBase {
         b += ma.bytes() + mb.bytes() + mc.bytes() + dm.bytes() + hthm.bytes() + h.bytes();
         b += S1.bytes() + S2.bytes() + Tspec.bytes() + twx.bytes() + twy.bytes() + Hcell.bytes() +
              slot2c.bytes();
-        b += noise_buf.bytes() + diag_part.bytes();
+        b += noise_buf.bytes() + diag_part.bytes() + stats.bytes();
+        b += src_pos.bytes() + src_vals.bytes() + ax_xa.bytes() + ax_xc.bytes() + ax_ya.bytes() + ax_yb.bytes() +
+             ax_xi.bytes() + ax_yi.bytes() + tlast.bytes();
+        b += sw_step.bytes() + sw_t.bytes() + sw_h.bytes() + rb_plane.bytes() + rb_pos.bytes() + rb_out.bytes();
+        b += btab_buf.bytes() + clk_buf.bytes();
         return b;
     }
     bool has_tensor() const override { return tensor_ok; }
 
+    int copy_tensor(EngineBase* srcb) override
+    {
+        Engine<T>* s = static_cast<Engine<T>*>(srcb);   // the caller checked the form
+        if (!s->tensor_ok) return set_err(MRMG_ERR_STATE, "the source context has no tensor");
+        HIPCHK(hipStreamSynchronize(s->c->stream));
+        RCHK(set_slots(s->nslot == 6));
+        RCHK(Tspec.alloc(s->Tspec.n));
+        HIPCHK(hipMemcpyAsync(Tspec.p, s->Tspec.p, Tspec.bytes(), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        toep = s->toep;
+        tensor_ok = true;
+        return MRMG_OK;
+    }
+
     int init_fft()
     {
-        const int nx = c->nx, ny = c->ny, nxh = c->nxh, nl = c->nl;
+        // the FFT grid (padded to 2 nx x 2 ny for open boundaries)
+        const int nx = c->fx, ny = c->fy, nxh = c->nxh, nl = c->nl;
         auto wx = twiddle_table(nx), wy = twiddle_table(ny);
         std::vector<C2<T>> tx(nx), ty(ny);
         for (int i = 0; i < nx; i++) tx[i] = {(T)wx[i].x, (T)wx[i].y};
@@ -426,6 +454,10 @@ struct Engine : EngineBase {
         RCHK(lds_attr());
         RCHK(S1.alloc((size_t)nl * 3 * ny * nxh));
         RCHK(S2.alloc((size_t)nl * 3 * ny * nxh));
+        // open boundaries: the row r2c writes the media rows only; the padding rows of S1
+        // stay zero for the fused column kernel (which does not write S1), and the split
+        // path's column kernel reads them as zero
+        if (c->open) HIPCHK(hipMemsetAsync(S1.p, 0, S1.bytes(), c->stream));
         kg = 1;
         return MRMG_OK;
     }
@@ -433,7 +465,7 @@ struct Engine : EngineBase {
     // derived launch parameters of the current shapes
     int shapes()
     {
-        const int nl = c->nl, ny = c->ny;
+        const int nl = c->nl, ny = c->fy;
         // wave-local transforms: a lane's bound covers its wave's run of ceil(B / waves)
         auto row_e = [&]() {
             const int nw = row_threads / 64, b = TYr / 2;
@@ -442,13 +474,13 @@ struct Engine : EngineBase {
         if (split || tl) {
             maxb_rowT = row_e();
             if (maxb_rowT < 0)
-                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: row FFT too large for %d threads", c->nx, ny, nl,
+                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: row FFT too large for %d threads", c->fx, ny, nl,
                                row_threads);
         }
         if (split) {
             maxb_colT = pick_e(CBc, c->py, 256);
             if (maxb_rowT < 0 || maxb_colT < 0)
-                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: split-path FFT too large for %d threads", c->nx, ny,
+                return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: split-path FFT too large for %d threads", c->fx, ny,
                                nl, row_threads);
             return MRMG_OK;
         }
@@ -457,7 +489,7 @@ struct Engine : EngineBase {
         maxb_col = tl && wv ? pick_e((3 * nl + ncw - 1) / ncw, c->py, 64) : pick_e(3 * nl, c->py, col_threads);
         if (maxb_row < 0 || maxb_col < 0)
             return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d too large for one-workgroup LDS FFTs (threads %d/%d)",
-                           c->nx, ny, nl, row_threads, col_threads);
+                           c->fx, ny, nl, row_threads, col_threads);
         const int per = (nl + kg - 1) / kg;
         if (col_threads > 512 && wv)
             return set_err(MRMG_ERR_UNSUPPORTED, "col_threads 1024 runs without wave-local FFTs");
@@ -506,13 +538,13 @@ struct Engine : EngineBase {
         } else if (k == "row_radix" || k == "col_radix") {
             if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "%s must be 4, 8 or 16", k.c_str());
             FFTPlan q;
-            if (!factorize(k == "row_radix" ? c->nx : c->ny, q, v) || q.nfac > MRMG_MAX_FAC)
+            if (!factorize(k == "row_radix" ? c->fx : c->fy, q, v) || q.nfac > MRMG_MAX_FAC)
                 return set_err(MRMG_ERR_UNSUPPORTED, "%s %d: no plan", k.c_str(), v);
             if (k == "row_radix") { prow = q; row_radix = v; }
             else { pcol = q; col_radix = v; }
         } else if (k == "row_block") {
             if (v != 2 && v != 4 && v != 8) return set_err(MRMG_ERR_ARG, "row_block must be 2, 4 or 8");
-            const size_t l = sizeof(C2<T>) * c->nx + sizeof(T) * 2 * LPN((v / 2) * c->nx);
+            const size_t l = sizeof(C2<T>) * c->fx + sizeof(T) * 2 * LPN((v / 2) * c->fx);
             if (l > 160 * 1024 || pick_e(v / 2, c->px, 256) < 0)
                 return set_err(MRMG_ERR_UNSUPPORTED, "row_block %d does not fit LDS", v);
             const int ot = TYr;
@@ -802,8 +834,11 @@ struct Engine : EngineBase {
                 }
             }
         }
+        any_sg = false;
+        for (int k = 0; k < nl; k++)
+            for (int sl = 0; sl < 2; sl++) any_sg = any_sg || hlp[k].src[sl].kind == MRMG_SRC_TEMP_SG;
         if (!any_src) {
-            src_pos.release(); src_vals.release(); ax_xa.release(); ax_ya.release(); ax_yb.release();
+            src_pos.release(); src_vals.release(); ax_xa.release(); ax_xc.release(); ax_ya.release(); ax_yb.release();
             ax_xi.release(); ax_yi.release(); tlast.release();
             return MRMG_OK;
         }
@@ -813,6 +848,8 @@ struct Engine : EngineBase {
         RCHK(src_vals.upload(vals.data(), vals.size(), c->stream));
         const size_t nax = (size_t)nl * 4 * nx, nay = (size_t)nl * 4 * ny;
         RCHK(ax_xa.alloc(nax)); RCHK(ax_xi.alloc(nax));
+        if (any_sg) RCHK(ax_xc.alloc((size_t)nl * 4 * (nx + 1)));
+        else ax_xc.release();
         RCHK(ax_ya.alloc(nay)); RCHK(ax_yb.alloc(nay)); RCHK(ax_yi.alloc(nay));
         HIPCHK(hipMemsetAsync(ax_xi.p, 0, ax_xi.bytes(), c->stream));
         HIPCHK(hipMemsetAsync(ax_yi.p, 0, ax_yi.bytes(), c->stream));
@@ -906,7 +943,8 @@ struct Engine : EngineBase {
         d.tc = tc.p; d.lam = lam.p; d.temp = temp.p; d.tabs = tabs.p;
         if (cur_llb) d.sigma = sigma_llb.p;
         d.any_src = any_src;
-        d.ax.xi = ax_xi.p; d.ax.xa = ax_xa.p; d.ax.yi = ax_yi.p; d.ax.ya = ax_ya.p; d.ax.yb = ax_yb.p;
+        d.ax.xi = ax_xi.p; d.ax.xa = ax_xa.p; d.ax.xc = ax_xc.p; d.ax.yi = ax_yi.p; d.ax.ya = ax_ya.p;
+        d.ax.yb = ax_yb.p;
         d.ax.pos = src_pos.p; d.ax.vals = src_vals.p; d.ax.nx = c->nx; d.ax.ny = c->ny;
         return d;
     }
@@ -1008,6 +1046,21 @@ struct Engine : EngineBase {
         *ms = (double)t / reps;
         return MRMG_OK;
     }
+    // this context's operands of the demag kernels
+    DemagOps<T> ops() const
+    {
+        DemagOps<T> o{};
+        o.Mg = Mg.p; o.S1 = S1.p; o.S2 = S2.p; o.Tspec = Tspec.p; o.Hcell = Hcell.p; o.Hr = Hr.p;
+        o.R = runs();
+        o.G = G;
+        return o;
+    }
+    // batched demag launches (serving): while btab is set, launch_class runs one launch
+    // for the bn contexts of the device table btab (this context's shapes, their operands)
+    const DemagOps<T>* btab = nullptr;
+    int bn = 1;
+    DBuf<DemagOps<T>> btab_buf;
+    DBuf<double> clk_buf;   // device clocks [t, step] of a batch's members
     // transposing row kernels: (ny/TY) * 3 n_lay blocks on a 1D grid padded to a multiple of 8
     unsigned rowT_blocks() const { return (unsigned)(((c->ny + TYr - 1) / TYr * 3 * c->nl + 7) / 8 * 8); }
     template <int MB> void launch_rowT_fwd(dim3 g, dim3 b);
@@ -1030,7 +1083,8 @@ struct Engine : EngineBase {
     {
         const int nb = (int)(((size_t)G * lpg + 255) / 256), nl = c->nl;
         std::vector<double> part((size_t)nb * DIAG_STRIDE);
-        HIPCHK(hipMemcpy(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(part.data(), diag_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         r.errmx.assign(nl, 0.0); r.errmx1.assign(nl, 0.0); r.mmag.assign(nl, 0.0); r.hmag.assign(nl, 0.0);
         for (int b = 0; b < nb; b++) {
             const double* q = &part[(size_t)b * DIAG_STRIDE];
@@ -1049,7 +1103,8 @@ struct Engine : EngineBase {
     {
         const int nl = c->nl;
         std::vector<double> o((size_t)10 * nl);
-        HIPCHK(hipMemcpy(o.data(), stats.p, o.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(o.data(), stats.p, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         double x = 0.0;
         for (int k = 0; k < nl; k++) {
             const double* q = &o[(size_t)10 * k];
@@ -1338,15 +1393,21 @@ struct Engine : EngineBase {
         HIPCHK(hipStreamSynchronize(c->stream));
         if (step) {
             std::vector<long long> v(ng);
-            HIPCHK(hipMemcpy(v.data(), sw_step.p + off[k], sizeof(long long) * ng, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpyAsync(v.data(), sw_step.p + off[k], sizeof(long long) * ng, hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
             for (int n = 0; n < ng; n++) step[n] = (int64_t)v[n];
         }
-        if (t) HIPCHK(hipMemcpy(t, sw_t.p + off[k], sizeof(double) * ng, hipMemcpyDeviceToHost));
+        if (t) {
+            HIPCHK(hipMemcpyAsync(t, sw_t.p + off[k], sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
         if (hap) {
             std::vector<double> v(3 * (size_t)ng);
             for (int cc = 0; cc < 3; cc++)
-                HIPCHK(hipMemcpy(v.data() + (size_t)cc * ng, sw_h.p + (size_t)cc * G + off[k], sizeof(double) * ng,
-                                 hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpyAsync(v.data() + (size_t)cc * ng, sw_h.p + (size_t)cc * G + off[k],
+                                      sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
             for (int n = 0; n < ng; n++)
                 for (int cc = 0; cc < 3; cc++) hap[3 * (size_t)n + cc] = v[(size_t)cc * ng + n];
         }
@@ -1390,3 +1451,173 @@ struct Engine : EngineBase {
         return MRMG_OK;
     }
 };
+
+// ---------------------------------------------------------------------------------------
+// Serving: fixed-dt steps of a batch of contexts with equal grids and launch shapes.  The
+// members' work runs on the first member's stream: per stage each member's grain kernel,
+// and each demag kernel class once for all members (their operands in a device table,
+// blockIdx.z the member).  The time and thermal step counter of every member live in a
+// device clock, so one step is the same launch sequence every time; with graph set it is
+// captured once as a hipGraph and replayed n_steps times.  Each member gets the values
+// its own mrmg_step with Philox noise (or none) would give: the same kernels, the same
+// thermal normals (drawn in the grain kernel from the member's seed and step) and the
+// host loop's time arithmetic, done on the device.
+int launch_clock_advance(hipStream_t s, double* clk, int n, double dt);
+
+template <typename T>
+int batch_step(std::vector<Engine<T>*>& M, const mrmg_step_params* P, int graph, double* t_end)
+{
+    const int B = (int)M.size();
+    if (B < 1) return set_err(MRMG_ERR_ARG, "empty batch");
+    Engine<T>& L = *M[0];
+    const mrmg_step_params& p0 = P[0];
+    const int type = p0.type;
+    if (!(type == -3 || type == 2 || type == -2 || type == 1 || type == -1))
+        return set_err(MRMG_ERR_UNSUPPORTED, "llg_type %d not implemented", type);
+    if (graph && type > 0) return set_err(MRMG_ERR_UNSUPPORTED, "graph replay needs llg_type < 0 (no buffer swap)");
+    if (p0.dt <= 0.0 || p0.n_steps < 0) return set_err(MRMG_ERR_ARG, "dt must be > 0 and n_steps >= 0");
+    for (int b = 0; b < B; b++) {
+        const Engine<T>& e = *M[b];
+        const mrmg_step_params& p = P[b];
+        if (p.n_steps != p0.n_steps || p.dt != p0.dt || p.type != type || p.llb != p0.llb || p.noise != p0.noise ||
+            p.dt_max != p0.dt_max)
+            return set_err(MRMG_ERR_ARG, "batch member %d: n_steps, dt, dt_max, type, llb and noise must be equal", b);
+        if (p.dt_frac != 0.0) return set_err(MRMG_ERR_UNSUPPORTED, "batch member %d: adaptive dt", b);
+        if (p.noise != MRMG_NOISE_NONE && p.noise != MRMG_NOISE_PHILOX)
+            return set_err(MRMG_ERR_UNSUPPORTED, "batch member %d: noise must be none or Philox", b);
+        if (e.c->nx != L.c->nx || e.c->ny != L.c->ny || e.c->nl != L.c->nl || e.c->fx != L.c->fx ||
+            e.c->fy != L.c->fy || e.c->device != L.c->device)
+            return set_err(MRMG_ERR_ARG, "batch member %d: grid or device differs", b);
+        if (e.demag_on() != L.demag_on() || e.split != L.split || e.tl != L.tl || e.wv || e.TYr != L.TYr ||
+            e.CBc != L.CBc || e.row_threads != L.row_threads || e.col_threads != L.col_threads || e.kg != L.kg ||
+            e.row_radix != L.row_radix || e.col_radix != L.col_radix || e.maxb_row != L.maxb_row ||
+            e.maxb_col != L.maxb_col || e.maxb_rowT != L.maxb_rowT || e.maxb_colT != L.maxb_colT ||
+            e.nslot != L.nslot || e.toep != L.toep || e.lds_row != L.lds_row || e.lds_col != L.lds_col ||
+            e.lds_rowT != L.lds_rowT || e.lds_colT != L.lds_colT)
+            return set_err(MRMG_ERR_ARG, "batch member %d: launch shapes or tensor storage differ", b);
+        if (e.col_threads > 512) return set_err(MRMG_ERR_UNSUPPORTED, "batched column launches need col_threads <= 512");
+        if (e.any_src || e.sw_on || e.prof)
+            return set_err(MRMG_ERR_UNSUPPORTED, "batch member %d: moving sources, switching record or profiling", b);
+        for (int k = 0; k < e.c->nl && p.llb; k++)
+            if (!(e.c->layers[k].spin > 0.0)) return set_err(MRMG_ERR_ARG, "batch member %d: LLB needs spin > 0", b);
+        for (int q = 0; q < b; q++)
+            if (M[q] == M[b]) return set_err(MRMG_ERR_ARG, "batch member %d appears twice", b);
+    }
+    // every member's work goes on the lead stream for the call
+    const hipStream_t s = L.c->stream;
+    std::vector<hipStream_t> own(B);
+    for (int b = 0; b < B; b++) {
+        own[b] = M[b]->c->stream;
+        if (own[b] != s) HIPCHK(hipStreamSynchronize(own[b]));
+        M[b]->c->stream = s;
+    }
+    struct Restore {
+        std::vector<Engine<T>*>& M;
+        std::vector<hipStream_t>& own;
+        ~Restore()
+        {
+            for (size_t b = 0; b < M.size(); b++) M[b]->c->stream = own[b];
+        }
+    } restore{M, own};
+    const double dt = p0.dt_max > 0.0 ? std::min(p0.dt, p0.dt_max) : p0.dt;
+    // the prologue of each member's llg() (step, step_fixed)
+    std::vector<DemagOps<T>> tab(B);
+    std::vector<double> clk(2 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        Engine<T>& e = *M[b];
+        e.cur_llb = p0.llb != 0;
+        RCHK(e.sync_aux());
+        if (!e.cur_llb && e.mag_dirty) RCHK(e.refresh_sigma());
+        HIPCHK(hipMemsetAsync(e.hthm.p, 0, e.hthm.bytes(), s));
+        if (e.demag_on()) RCHK(e.scatter(e.ma.p, &P[b].t0));
+        tab[b] = e.ops();
+        clk[2 * b] = P[b].t0;
+        clk[2 * b + 1] = (double)P[b].step0;
+    }
+    if (L.btab_buf.n < (size_t)B) RCHK(L.btab_buf.alloc(B));
+    if (L.clk_buf.n < 2 * (size_t)B) RCHK(L.clk_buf.alloc(2 * (size_t)B));
+    HIPCHK(hipMemcpyAsync(L.btab_buf.p, tab.data(), sizeof(DemagOps<T>) * B, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(L.clk_buf.p, clk.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, s));
+    std::vector<StepArgs> a(B);
+    for (int b = 0; b < B; b++) {
+        a[b] = StepArgs{};
+        a[b].llb = p0.llb;
+        a[b].dt = dt;
+        a[b].sqrt_dt = sqrt(dt);
+        a[b].type = type;
+        a[b].noise = p0.noise;
+        a[b].seed = P[b].seed;
+        a[b].clk = L.clk_buf.p + 2 * b;
+    }
+    auto demag_all = [&]() -> int {
+        if (!L.demag_on()) return MRMG_OK;
+        L.btab = L.btab_buf.p;
+        L.bn = B;
+        const int rc = L.demag();
+        L.btab = nullptr;
+        L.bn = 1;
+        return rc;
+    };
+    // one step of every member (step_fixed's loop body; stage times as offsets to the clock)
+    auto one_step = [&]() -> int {
+        if (type == -3) {
+            RCHK(demag_all());
+            for (int b = 0; b < B; b++) {
+                a[b].t = 0.0; a[b].dts = dt;
+                RCHK(M[b]->template grain<GM_HEUN1>(a[b], nullptr, 0.0));
+            }
+            RCHK(demag_all());
+            for (int b = 0; b < B; b++) {
+                a[b].t = dt;
+                RCHK(M[b]->template grain<GM_HEUN2>(a[b], nullptr, 0.0));
+            }
+        } else if (type < 0) {
+            RCHK(demag_all());
+            for (int b = 0; b < B; b++) {
+                a[b].t = 0.0; a[b].dts = dt * 0.5;
+                RCHK(M[b]->template grain<GM_ROT1>(a[b], nullptr, 0.0));
+            }
+            RCHK(demag_all());
+            for (int b = 0; b < B; b++) {
+                a[b].t = dt * 0.5; a[b].dts = dt;
+                RCHK(M[b]->template grain<GM_ROT2>(a[b], nullptr, 0.0));
+            }
+        } else {
+            RCHK(demag_all());
+            for (int b = 0; b < B; b++) {
+                a[b].t = 0.0; a[b].dts = dt;
+                RCHK(M[b]->template grain<GM_ROT1>(a[b], nullptr, 0.0));
+                M[b]->ma.swap_with(M[b]->mb);
+            }
+        }
+        return launch_clock_advance(s, L.clk_buf.p, B, dt);
+    };
+    int rc = MRMG_OK;
+    if (graph && p0.n_steps > 0) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = one_step();
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc == MRMG_OK && ec != hipSuccess) rc = set_err(MRMG_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        if (rc == MRMG_OK && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+            rc = set_err(MRMG_ERR_HIP, "graph instantiation failed");
+        for (int it = 0; rc == MRMG_OK && it < p0.n_steps; it++)
+            if (hipGraphLaunch(ge, s) != hipSuccess) rc = set_err(MRMG_ERR_HIP, "graph launch failed");
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    } else {
+        for (int it = 0; rc == MRMG_OK && it < p0.n_steps; it++) rc = one_step();
+    }
+    if (rc == MRMG_OK && t_end) {
+        HIPCHK(hipMemcpyAsync(clk.data(), L.clk_buf.p, sizeof(double) * 2 * B, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (rc == MRMG_OK && t_end)
+        for (int b = 0; b < B; b++) t_end[b] = clk[2 * b];
+    for (int b = 0; b < B; b++) {
+        if (M[b]->cur_llb) M[b]->mag_dirty = true;
+        M[b]->cur_llb = false;
+    }
+    return rc;
+}

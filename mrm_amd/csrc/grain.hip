@@ -74,6 +74,16 @@ __device__ __forceinline__ double src_cell(const SrcAxes& ax, const SrcParams& s
            (1.0 - fx) * yfc * v[nv * a11 + comp];
 }
 
+// The super-Gaussian temperature source (slot 1) of layer k summed over the run of cells
+// i0 .. i0+len-1 in row j at time ts: ya[j] times the run's share of the running sums of
+// xa (the cells' own values ya[j] xa[i], get_sg, applied.f90:1792-1811, added up).
+__device__ __forceinline__ double sg_run(const SrcAxes& ax, int k, int ts, int i0, int len, int j)
+{
+    const size_t b = (size_t)(k * 2 + 1) * 2 + ts;
+    const double* xc = ax.xc + b * (ax.nx + 1);
+    return ax.ya[b * ax.ny + j] * (xc[i0 + len] - xc[i0]);
+}
+
 // (1 - T/Tc)^e clamped to [0, 1] (temp_scaling.f90:390-395)
 __device__ __forceinline__ double temp_power(double T, double Tc, double e)
 {
@@ -186,8 +196,11 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
         hx[1] = lane_sum<LPG>(e1);
         hx[2] = lane_sum<LPG>(e2);
     } else {
-        // moving sources: the source is sampled at every cell of the grain's runs
+        // moving sources: a table is sampled at every cell of the grain's runs; the
+        // separable super-Gaussian temperature is summed per run from its running sums
         T s0 = 0, s1 = 0, s2 = 0;
+        const bool tsg = tsrc && L.src[1].kind == MRMG_SRC_TEMP_SG;
+        const bool per_cell = fsrc || (tsrc && !tsg);
         for (int q = r0 + lane; q < r1; q += LPG) {
             if (Hr) {
                 s0 += (T)Hr[q];
@@ -196,15 +209,20 @@ __device__ __forceinline__ void field_at(const GrainData<T>& gd, const LayerPara
             }
             const int geo = gd.run_geo[q], j = gd.run_row[q];
             const int i0 = geo & 0xffff, len = geo >> 16;
-            for (int i = i0; i < i0 + len; i++) {
-                if (fsrc)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
-                if (tsrc) {
-                    ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
-                    ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
-                }
+            if (tsg) {
+                ts0 += sg_run(gd.ax, k, 0, i0, len, j);
+                ts1 += sg_run(gd.ax, k, 1, i0, len, j);
             }
+            if (per_cell)
+                for (int i = i0; i < i0 + len; i++) {
+                    if (fsrc)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) hs[c] += src_cell(gd.ax, L.src[0], k, 0, 0, i, j, c, 3);
+                    if (tsrc && !tsg) {
+                        ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
+                        ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
+                    }
+                }
         }
         const T cnt = (T)(b1 - b0);
         if (Hr) {
@@ -274,9 +292,15 @@ __device__ __forceinline__ void src_temp(const GrainData<T>& gd, const LayerPara
     const int k = gd.layer_of[g];
     const int b0 = gd.gc_off[g], b1 = gd.gc_off[g + 1], r0 = gd.run_off[g], r1 = gd.run_off[g + 1];
     double ts0 = 0, ts1 = 0;
+    const bool tsg = L.src[1].kind == MRMG_SRC_TEMP_SG;
     for (int q = r0 + lane; q < r1; q += LPG) {
         const int geo = gd.run_geo[q], j = gd.run_row[q];
         const int i0 = geo & 0xffff, len = geo >> 16;
+        if (tsg) {
+            ts0 += sg_run(gd.ax, k, 0, i0, len, j);
+            ts1 += sg_run(gd.ax, k, 1, i0, len, j);
+            continue;
+        }
         for (int i = i0; i < i0 + len; i++) {
             ts0 += src_cell(gd.ax, L.src[1], k, 1, 0, i, j, 0, 1);
             ts1 += src_cell(gd.ax, L.src[1], k, 1, 1, i, j, 0, 1);
@@ -366,6 +390,38 @@ __global__ void k_src_axes(const LayerParams* __restrict__ lp, int nl, int nx, i
         yi[base * ny + i] = ind;
         ya[base * ny + i] = sc * fr;
         yb[base * ny + i] = sc * (1.0 - fr);
+    }
+}
+
+// Running sums of the super-Gaussian x factors: one workgroup per (layer, slot, time)
+// table; xc[0] = 0, xc[i+1] = xc[i] + xa[i].  Each thread sums a contiguous chunk, the
+// chunk totals are scanned in LDS in a fixed order, then each chunk is written out.
+__global__ void __launch_bounds__(256) k_src_cum(const LayerParams* __restrict__ lp, int nx,
+                                                 const double* __restrict__ xa, double* __restrict__ xc)
+{
+    const int b = blockIdx.x;                 // (k * 2 + slot) * 2 + ts
+    const int k = b >> 2, slot = (b >> 1) & 1;
+    if (lp[k].src[slot].kind != MRMG_SRC_TEMP_SG) return;
+    __shared__ double part[257];
+    const int nt = (int)blockDim.x, t = (int)threadIdx.x;
+    const int chunk = (nx + nt - 1) / nt;
+    const int a0 = min(t * chunk, nx), a1 = min(a0 + chunk, nx);
+    const double* x = xa + (size_t)b * nx;
+    double* c = xc + (size_t)b * (nx + 1);
+    double s = 0.0;
+    for (int i = a0; i < a1; i++) s += x[i];
+    part[t + 1] = s;
+    __syncthreads();
+    if (t == 0) {
+        part[0] = 0.0;
+        for (int i = 1; i <= nt; i++) part[i] += part[i - 1];
+    }
+    __syncthreads();
+    s = part[t];
+    if (t == 0) c[0] = 0.0;
+    for (int i = a0; i < a1; i++) {
+        s += x[i];
+        c[i + 1] = s;
     }
 }
 
@@ -692,7 +748,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
     T h[3] = {0, 0, 0};
     double tev[2] = {0, 0};   // grain temperature now and at the next evaluation (moving sources)
     if (MODE != GM_PUSH1) {
-        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hr, a.t, mfield, h, tev, P);
+        field_at<T, LPG, TC>(gd, L, k, g, lane, mf, Hr, a.clk ? a.clk[0] + a.t : a.t, mfield, h, tev, P);
     } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) h[c] = hout[(size_t)c * G + g];
@@ -726,7 +782,7 @@ __global__ void __launch_bounds__(256) k_grain(GrainData<T> gd, T* __restrict__ 
                 if (a.noise == 1) {
                     n[0] = a.normals[3 * (size_t)g]; n[1] = a.normals[3 * (size_t)g + 1]; n[2] = a.normals[3 * (size_t)g + 2];
                 } else {
-                    philox_normals3(a.seed, a.step, g, n);
+                    philox_normals3(a.seed, a.clk ? (unsigned long long)a.clk[1] + a.step : a.step, g, n);
                 }
                 T sg = g_sig;
                 if (tsrc)
@@ -955,9 +1011,10 @@ __global__ void __launch_bounds__(256) k_rb_plane(const TC* __restrict__ Mg, int
     }
     P[i] = s;
 }
+// periodic: minimum-image distances (the periodic media); else plain ones (open boundaries)
 __global__ void __launch_bounds__(256) k_rb_scan(const double* __restrict__ P, int nx, int ny, double dx, double dy,
                                                  double sx, double sy, const double* __restrict__ pos, int n_pos,
-                                                 double* __restrict__ out)
+                                                 double* __restrict__ out, int periodic)
 {
     extern __shared__ double gsh[];   // nx + ny
     __shared__ double red[4];
@@ -967,12 +1024,12 @@ __global__ void __launch_bounds__(256) k_rb_scan(const double* __restrict__ P, i
     const double Lx = nx * dx, Ly = ny * dy;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
         double d = (i + 0.5) * dx - x0;
-        d -= Lx * rint(d / Lx);
+        if (periodic) d -= Lx * rint(d / Lx);
         gsh[i] = exp(-d * d / (2.0 * sx * sx));
     }
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         double d = (j + 0.5) * dy - y0;
-        d -= Ly * rint(d / Ly);
+        if (periodic) d -= Ly * rint(d / Ly);
         gsh[nx + j] = exp(-d * d / (2.0 * sy * sy));
     }
     __syncthreads();
@@ -1037,6 +1094,11 @@ int Engine<T>::src_axes(double t0, double t1)
     hipLaunchKernelGGL(k_src_axes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, lp.p, c->nl, c->nx,
                        c->ny, c->dx, c->dy, t0, t1, ax_xi.p, ax_xa.p, ax_yi.p, ax_ya.p, ax_yb.p, src_pos.p);
     HIPCHK(hipGetLastError());
+    if (any_sg) {
+        hipLaunchKernelGGL(k_src_cum, dim3((unsigned)(c->nl * 4)), dim3(256), 0, c->stream, lp.p, c->nx, ax_xa.p,
+                           ax_xc.p);
+        HIPCHK(hipGetLastError());
+    }
     return MRMG_OK;
 }
 
@@ -1104,7 +1166,7 @@ int Engine<T>::readback_into(const mrmg_sensor* sn, int n_pos, const double* xpo
                        nl, sn->dir[0], sn->dir[1], sn->dir[2], wk, plane);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_rb_scan, dim3(n_pos), dim3(256), sizeof(double) * (nx + ny), c->stream, plane, nx, ny,
-                       c->dx, c->dy, sn->sigma_x, sn->sigma_y, pos, n_pos, dout);
+                       c->dx, c->dy, sn->sigma_x, sn->sigma_y, pos, n_pos, dout, c->open ? 0 : 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * n_pos, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1148,6 +1210,23 @@ int launch_philox_dump(hipStream_t s, unsigned long long seed, unsigned long lon
     const size_t cnt = (size_t)nsteps * G;
     hipLaunchKernelGGL(k_philox_dump, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, seed, step0, nsteps, G,
                        out);
+    HIPCHK(hipGetLastError());
+    return MRMG_OK;
+}
+
+// the device clocks [t, step] of n batch members after a completed step: t = t + dt (the
+// host loop's own addition), step + 1
+__global__ void k_clock_advance(double* __restrict__ clk, int n, double dt)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    clk[2 * b] = clk[2 * b] + dt;
+    clk[2 * b + 1] = clk[2 * b + 1] + 1.0;
+}
+
+int launch_clock_advance(hipStream_t s, double* clk, int n, double dt)
+{
+    hipLaunchKernelGGL(k_clock_advance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, clk, n, dt);
     HIPCHK(hipGetLastError());
     return MRMG_OK;
 }

@@ -111,9 +111,11 @@ static_assert(sizeof(LayerParams) % 8 == 0, "layer parameters are copied to LDS 
 // evaluation, 1 = the next evaluation:
 //   tables:          xi, xa = set()'s node index (1-based) and weight along x;
 //                    yi, ya, yb = index, s*w, s*(1-w) along y (s the time scale)
-//   super-Gaussian:  xa = gx, ya = dtemp*s*gy
+//   super-Gaussian:  xa = gx, ya = dtemp*s*gy, and xc [layer][slot][time][nx+1] the
+//                    running sums of xa (xc[i] = xa[0] + ... + xa[i-1]), so that a run of
+//                    cells i0 .. i0+len-1 in row j sums to ya[j] (xc[i0+len] - xc[i0])
 struct SrcAxes {
-    const int* xi; const double* xa;
+    const int* xi; const double* xa; const double* xc;
     const int* yi; const double* ya; const double* yb;
     const double* pos;    // node positions
     const double* vals;   // table values
@@ -141,6 +143,20 @@ struct RowRuns {
     const int* grain;     // row by row: global grain
     const int* gm;        // row by row: the run's number in the grain-by-grain order
     int nruns;
+};
+
+// The per-context operands of the demag kernels.  A kernel launched for one context takes
+// them by value; a launch over a batch of contexts whose FFT grids and launch shapes are
+// equal takes a device table of them, one entry per context, indexed by blockIdx.z.
+template <typename T> struct DemagOps {
+    const T* Mg;            // [3][G] grain cell values
+    C2<T>* S1;              // spectra of M
+    C2<T>* S2;              // spectra of H
+    const C2<T>* Tspec;     // tensor spectra
+    T* Hcell;               // cell field (mrmg_demag_field only)
+    double* Hr;             // [3][nruns] run sums of the cell field
+    RowRuns R;
+    int G, pad_;
 };
 
 // rows of the grain state buffers
@@ -215,4 +231,7 @@ struct StepArgs {
     double dti;        // 1/sqrt(previous dt * max_dmdt), get_fields' |H| estimate
     int llb;           // push_llb instead of push_llg
     double* tlast;     // adaptive path: grain temperature of the last field evaluation
+    // device clock [t, step] of the current step (a step replayed from a graph): when set,
+    // the evaluation time is clk[0] + t and the thermal counter clk[1] + step
+    const double* clk;
 };

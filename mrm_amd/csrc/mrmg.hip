@@ -66,7 +66,7 @@ int mrmg_device_count(int* n)
     return MRMG_OK;
 }
 
-int mrmg_create(const mrmg_geom* g, mrmg_ctx** out)
+static int create_ctx(const mrmg_geom* g, bool open, mrmg_ctx** out)
 {
     if (!g || !out) return set_err(MRMG_ERR_ARG, "null argument");
     *out = nullptr;
@@ -74,15 +74,20 @@ int mrmg_create(const mrmg_geom* g, mrmg_ctx** out)
         return set_err(MRMG_ERR_ARG, "bad grid %d x %d x %d (n_lay <= %d)", g->nx, g->ny, g->n_lay, MRMG_MAX_LAYERS);
     if (g->precision != MRMG_FP64 && g->precision != MRMG_FP32) return set_err(MRMG_ERR_ARG, "bad precision");
     if (!g->dz) return set_err(MRMG_ERR_ARG, "dz is NULL");
+    if (g->nx > 32767) return set_err(MRMG_ERR_UNSUPPORTED, "nx = %d: runs store x and their length in 16 bits", g->nx);
     std::unique_ptr<mrmg_ctx> c(new mrmg_ctx());
-    c->nx = g->nx; c->ny = g->ny; c->nl = g->n_lay; c->nxh = g->nx / 2 + 1;
+    c->nx = g->nx; c->ny = g->ny; c->nl = g->n_lay;
+    c->open = open;
+    c->fx = open ? 2 * g->nx : g->nx;
+    c->fy = open ? 2 * g->ny : g->ny;
+    c->nxh = c->fx / 2 + 1;
     c->precision = g->precision; c->demag = g->demag;
     c->dx = g->dx; c->dy = g->dy;
     c->dz.assign(g->dz, g->dz + g->n_lay);
     c->il.assign(g->n_lay, 0.0);
     if (g->il_space) c->il.assign(g->il_space, g->il_space + g->n_lay);
-    if (!factorize(c->nx, c->px) || !factorize(c->ny, c->py))
-        return set_err(MRMG_ERR_UNSUPPORTED, "nx=%d, ny=%d must factor into 2, 3, 5", g->nx, g->ny);
+    if (!factorize(c->fx, c->px) || !factorize(c->fy, c->py))
+        return set_err(MRMG_ERR_UNSUPPORTED, "FFT grid %d x %d must factor into 2, 3, 5", c->fx, c->fy);
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return set_err(MRMG_ERR_HIP, "no HIP device");
@@ -102,6 +107,9 @@ int mrmg_create(const mrmg_geom* g, mrmg_ctx** out)
     *out = c.release();
     return MRMG_OK;
 }
+
+int mrmg_create(const mrmg_geom* g, mrmg_ctx** out) { return create_ctx(g, false, out); }
+int mrmg_create_open(const mrmg_geom* g, mrmg_ctx** out) { return create_ctx(g, true, out); }
 
 int mrmg_destroy(mrmg_ctx* c)
 {
@@ -142,7 +150,7 @@ static int tensor_symmetric(mrmg_ctx* c, const double* d_blocks, int nblk, bool*
 {
     DBuf<unsigned long long> cnt;
     RCHK(cnt.alloc(1));
-    RCHK(tensor_sym_count(c->stream, d_blocks, (size_t)c->nx * c->ny, nblk, cnt.p));
+    RCHK(tensor_sym_count(c->stream, d_blocks, (size_t)c->fx * c->fy, nblk, cnt.p));
     unsigned long long h = 1;
     HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -167,7 +175,7 @@ static void offset_pair(int q, int nl, int* ko, int* ks)
 static int tensor_from_full(mrmg_ctx* c, DBuf<double>& full)
 {
     const int nl = c->nl;
-    const size_t blk = (size_t)9 * c->nx * c->ny;
+    const size_t blk = (size_t)9 * c->fx * c->fy;
     bool sym = false, toep = false;
     RCHK(tensor_symmetric(c, full.p, nl * nl, &sym));
     if (nl > 1) {
@@ -198,7 +206,7 @@ static int tensor_from_full(mrmg_ctx* c, DBuf<double>& full)
 int mrmg_set_tensor(mrmg_ctx* c, const double* n_real)
 {
     if (!c || !n_real) return set_err(MRMG_ERR_ARG, "null argument");
-    const size_t count = (size_t)c->nx * c->ny * 9 * c->nl * c->nl;
+    const size_t count = (size_t)c->fx * c->fy * 9 * c->nl * c->nl;
     DBuf<double> d;
     RCHK(d.upload(n_real, count, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -215,7 +223,12 @@ int mrmg_build_tensor_images(mrmg_ctx* c, int num_images, double sul_mu, double 
 {
     if (!c) return set_err(MRMG_ERR_ARG, "null argument");
     if (num_images < 0) return set_err(MRMG_ERR_ARG, "num_images < 0");
-    const int nx = c->nx, ny = c->ny, nl = c->nl;
+    if (c->open && num_images != 0)
+        return set_err(MRMG_ERR_ARG, "open boundaries take no periodic images (num_images %d)", num_images);
+    // the blocks are built on the FFT grid: for open boundaries every displacement of the
+    // media grid once, none wrapped (the grid is padded to twice the media)
+    const int nx = c->fx, ny = c->fy, nl = c->nl;
+    const int open_nx = c->open ? c->nx : 0, open_ny = c->open ? c->ny : 0;
     const size_t plane = (size_t)nx * ny, blk = plane * 9, count = blk * nl * nl;
     // image strengths (mu-1)/(mu+1) of the soft underlayer and the head (tensor.f90:125-126)
     const bool sul = sul_mu > 1.0, head = head_mu > 1.0;
@@ -238,13 +251,14 @@ int mrmg_build_tensor_images(mrmg_ctx* c, int num_images, double sul_mu, double 
         const double rz = 0.5 * (zst - zot + zsb - zob);
         const double norm = c->dx * c->dy * c->dz[ko] * nx * ny;
         RCHK(launch_plane_to_plane(c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko], rz, num_images, norm, dst,
-                                   (sul || head) ? 1 : 0, 0.0, 0.0));
+                                   (sul || head) ? 1 : 0, 0.0, 0.0, open_nx, open_ny));
         if (sul)
             RCHK(launch_plane_to_plane(c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko],
-                                       0.5 * (-zst - zot - zsb - zob), num_images, norm, dst, 2, -mus, mus));
+                                       0.5 * (-zst - zot - zsb - zob), num_images, norm, dst, 2, -mus, mus, open_nx,
+                                       open_ny));
         if (head)
             RCHK(launch_plane_to_plane(c->stream, nx, ny, c->dx, c->dy, c->dz[ks], c->dz[ko], hdz[ks] + hdz[ko],
-                                       num_images, norm, dst, 2, -mush, mush));
+                                       num_images, norm, dst, 2, -mush, mush, open_nx, open_ny));
         if (sul || head) RCHK(launch_tensor_div(c->stream, dst, blk, norm));
         return MRMG_OK;
     };
@@ -533,6 +547,37 @@ int mrmg_step(mrmg_ctx* c, const mrmg_step_params* p, const double* noise, mrmg_
     return c->eng->step(p, noise, diag);
 }
 
+int mrmg_copy_tensor(mrmg_ctx* dst, const mrmg_ctx* src)
+{
+    if (!dst || !src) return set_err(MRMG_ERR_ARG, "null argument");
+    if (dst == src) return MRMG_OK;
+    if (dst->precision != src->precision || dst->nx != src->nx || dst->ny != src->ny || dst->nl != src->nl ||
+        dst->open != src->open || dst->dx != src->dx || dst->dy != src->dy || dst->dz != src->dz ||
+        dst->il != src->il || dst->device != src->device)
+        return set_err(MRMG_ERR_ARG, "the two contexts differ in form, grid, layer stack or device");
+    return dst->eng->copy_tensor(src->eng.get());
+}
+
+int mrmg_step_batch(mrmg_ctx* const* ctxs, const mrmg_step_params* params, int n, int graph, double* t_end)
+{
+    if (!ctxs || !params || n < 1) return set_err(MRMG_ERR_ARG, "null argument or empty batch");
+    for (int b = 0; b < n; b++) {
+        mrmg_ctx* c = ctxs[b];
+        if (!c) return set_err(MRMG_ERR_ARG, "batch member %d is NULL", b);
+        if (c->precision != ctxs[0]->precision) return set_err(MRMG_ERR_ARG, "batch member %d: precision differs", b);
+        if (c->demag && !c->eng->has_tensor()) return set_err(MRMG_ERR_STATE, "batch member %d has no tensor", b);
+        RCHK(ensure_built(c));
+    }
+    if (ctxs[0]->precision == MRMG_FP64) {
+        std::vector<Engine<double>*> M(n);
+        for (int b = 0; b < n; b++) M[b] = static_cast<Engine<double>*>(ctxs[b]->eng.get());
+        return batch_step(M, params, graph, t_end);
+    }
+    std::vector<Engine<float>*> M(n);
+    for (int b = 0; b < n; b++) M[b] = static_cast<Engine<float>*>(ctxs[b]->eng.get());
+    return batch_step(M, params, graph, t_end);
+}
+
 int mrmg_philox_normals(mrmg_ctx* c, uint64_t seed, uint64_t step0, int n_steps, double* out)
 {
     if (!c || !out) return set_err(MRMG_ERR_ARG, "null argument");
@@ -544,7 +589,8 @@ int mrmg_philox_normals(mrmg_ctx* c, uint64_t seed, uint64_t step0, int n_steps,
     RCHK(d.alloc(3 * n));
     RCHK(launch_philox_dump(c->stream, seed, step0, n_steps, G, d.p));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(out, d.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(out, d.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return MRMG_OK;
 }
 

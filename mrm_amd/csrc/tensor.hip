@@ -64,16 +64,24 @@ __device__ void van(double rx, double ry, double rz, double xs, double ys, doubl
 // (an image term, tensor.f90:163-181).
 __global__ void k_plane_to_plane(int nx, int ny, double dx, double dy, double dzs, double dzo, double rz,
                                  int nimg, double norm, double* __restrict__ blk, int mode = 0, double fxy = 0.0,
-                                 double fz = 0.0)
+                                 double fz = 0.0, int onx = 0, int ony = 0)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nx * ny) return;
     const int oi = p % nx, oj = p / nx;
     int is[2], js[2], ni = 0, nj = 0;
-    if (nx % 2 == 0 && oi == nx / 2) { is[0] = -nx / 2; is[1] = nx / 2; ni = 2; }
-    else { is[0] = (oi <= nx / 2) ? -oi : nx - oi; ni = 1; }
-    if (ny % 2 == 0 && oj == ny / 2) { js[0] = -ny / 2; js[1] = ny / 2; nj = 2; }
-    else { js[0] = (oj <= ny / 2) ? -oj : ny - oj; nj = 1; }
+    if (onx > 0) {
+        // open boundaries (nx = 2 onx, ny = 2 ony, nimg = 0): index oi holds the one
+        // displacement -oi (oi < onx) or nx - oi (oi > onx) of the media grid; the middle
+        // line oi = onx is no displacement of it and stays zero
+        if (oi != onx) { is[0] = oi < onx ? -oi : nx - oi; ni = 1; }
+        if (oj != ony) { js[0] = oj < ony ? -oj : ny - oj; nj = 1; }
+    } else {
+        if (nx % 2 == 0 && oi == nx / 2) { is[0] = -nx / 2; is[1] = nx / 2; ni = 2; }
+        else { is[0] = (oi <= nx / 2) ? -oi : nx - oi; ni = 1; }
+        if (ny % 2 == 0 && oj == ny / 2) { js[0] = -ny / 2; js[1] = ny / 2; nj = 2; }
+        else { js[0] = (oj <= ny / 2) ? -oj : ny - oj; nj = 1; }
+    }
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < nj; b++) {
         const int j = js[b];
@@ -228,8 +236,8 @@ int Engine<T>::set_tensor_planes(const double* d_real, bool toeplitz)
 {
     // d_real: device FP64 real-space blocks of 9 (nx,ny) planes: the nl^2 layer pairs
     // (ko + nl ks, the (nx,ny,9,nl,nl) array of calculate_tensor), or with toeplitz the
-    // 2 nl - 1 z offsets (block d + nl - 1 for d = ko - ks)
-    const int nx = c->nx, ny = c->ny, nxh = c->nxh, nl = c->nl;
+    // 2 nl - 1 z offsets (block d + nl - 1 for d = ko - ks); planes of the FFT grid
+    const int nx = c->fx, ny = c->fy, nxh = c->nxh, nl = c->nl;
     const int nq = toeplitz ? 2 * nl - 1 : nl * nl;
     const int P = 9 * nq;
     DBuf<C2<double>> TC, tdx, tdy;
@@ -297,11 +305,11 @@ int Engine<T>::set_tensor_planes(const double* d_real, bool toeplitz)
 
 
 int launch_plane_to_plane(hipStream_t s, int nx, int ny, double dx, double dy, double dzs, double dzo, double rz,
-                          int nimg, double norm, double* blk, int mode, double fxy, double fz)
+                          int nimg, double norm, double* blk, int mode, double fxy, double fz, int onx, int ony)
 {
     const size_t plane = (size_t)nx * ny;
     hipLaunchKernelGGL(k_plane_to_plane, dim3((unsigned)((plane + 127) / 128)), dim3(128), 0, s, nx, ny, dx, dy, dzs,
-                       dzo, rz, nimg, norm, blk, mode, fxy, fz);
+                       dzo, rz, nimg, norm, blk, mode, fxy, fz, onx, ony);
     HIPCHK(hipGetLastError());
     return MRMG_OK;
 }

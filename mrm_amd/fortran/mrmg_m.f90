@@ -12,7 +12,7 @@ module mrmg_m
   implicit none
   private
   public :: mrmg_geom, mrmg_layer, mrmg_ufield, mrmg_source, mrmg_step_params, mrmg_diag, mrmg_sensor
-  public :: mrmg_create, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_build_tensor_images, &
+  public :: mrmg_create, mrmg_create_open, mrmg_destroy, mrmg_set_tensor, mrmg_build_tensor, mrmg_build_tensor_images, &
             mrmg_set_layer, &
             mrmg_set_ufield, mrmg_set_state, mrmg_get_state, mrmg_set_mag, mrmg_get_mag, mrmg_step, mrmg_error, &
             mrmg_readback, mrmg_switch_arm, mrmg_switch_read, mrmg_set_source, mrmg_tune, mrmg_get_tune, &
@@ -102,6 +102,12 @@ module mrmg_m
 
   interface
     integer(c_int) function mrmg_create(g, ctx) bind(C, name='mrmg_create')
+      import :: c_int, c_ptr, mrmg_geom
+      type(mrmg_geom), intent(in) :: g
+      type(c_ptr), intent(out) :: ctx
+    end function
+    ! open in-plane boundaries: the demag over the media alone, on the 2nx x 2ny padded grid
+    integer(c_int) function mrmg_create_open(g, ctx) bind(C, name='mrmg_create_open')
       import :: c_int, c_ptr, mrmg_geom
       type(mrmg_geom), intent(in) :: g
       type(c_ptr), intent(out) :: ctx

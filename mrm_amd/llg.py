@@ -17,13 +17,18 @@ from . import abi, formats
 class Llg:
     """One simulation context (a grid, its layers, tensor and state) on the GPU."""
 
-    def __init__(self, nx, ny, n_lay, dx, dy, dz, il_space=None, precision=abi.MRMG_FP64, demag=True):
+    def __init__(self, nx, ny, n_lay, dx, dy, dz, il_space=None, precision=abi.MRMG_FP64, demag=True, open=False):
+        """open: open in-plane boundaries (the demag convolution over the media alone, on
+        the grid zero-padded to 2nx x 2ny) instead of the reference's periodic ones."""
         self.lib = abi.load()
         self.geom = abi.make_geom(nx, ny, n_lay, dx, dy, dz, il_space, precision, demag)
         h = C.c_void_p()
-        abi.check(self.lib, self.lib.mrmg_create(C.byref(self.geom), C.byref(h)))
+        create = self.lib.mrmg_create_open if open else self.lib.mrmg_create
+        abi.check(self.lib, create(C.byref(self.geom), C.byref(h)))
         self.ctx = h
+        self.open = bool(open)
         self.nx, self.ny, self.n_lay = nx, ny, n_lay
+        self.fx, self.fy = (2 * nx, 2 * ny) if open else (nx, ny)   # the FFT grid of the demag
         self.ngs = [0] * n_lay
         self._keep = {}
         # set_llg defaults (llg.f90:8-13, input.f90:386-412)
@@ -55,16 +60,21 @@ class Llg:
     def build_tensor(self, num_images=2, return_real=False, sul_mu=1.0, head_mu=1.0, fly_height=8e-7,
                      space_to_sul=40e-7):
         """calculate_tensor on the device (tensor.f90:82-216), with the SUL / head images
-        when sul_mu / head_mu > 1."""
+        when sul_mu / head_mu > 1.  The real tensor is on the FFT grid (fx, fy)."""
         out = None
         if return_real:
-            out = np.zeros(self.nx * self.ny * 9 * self.n_lay * self.n_lay)
+            out = np.zeros(self.fx * self.fy * 9 * self.n_lay * self.n_lay)
         if sul_mu > 1.0 or head_mu > 1.0:
             abi.check(self.lib, self.lib.mrmg_build_tensor_images(self.ctx, num_images, sul_mu, head_mu, fly_height,
                                                                   space_to_sul, abi.dptr(out)))
         else:
             abi.check(self.lib, self.lib.mrmg_build_tensor(self.ctx, num_images, abi.dptr(out)))
         return out
+
+    def copy_tensor(self, src):
+        """The tensor spectra of src (an Llg of the same precision, grid, boundaries and
+        layer stack), copied on the device."""
+        abi.check(self.lib, self.lib.mrmg_copy_tensor(self.ctx, src.ctx))
 
     def set_layer(self, k, la):
         s = la.struct()
@@ -312,3 +322,33 @@ class Llg:
         hg = np.zeros((sum(self.ngs), 3))
         abi.check(self.lib, self.lib.mrmg_demag_field(self.ctx, abi.dptr(hc), abi.dptr(hg)))
         return hc, hg
+
+
+def step_batch(xs, n_steps, philox_seeds=None, t0s=None, graph=True):
+    """n_steps fixed-dt steps of several contexts together (mrmg_step_batch): one launch of
+    each demag kernel for all of them, one step captured as a hipGraph and replayed when
+    graph is set.  xs: Llg contexts with equal grids, launch shapes and set_llg settings;
+    philox_seeds: one per context (None: no thermal noise); t0s: start times (default 0).
+    Each context's step counter advances as its own step() would advance it.  Returns the
+    end times."""
+    n = len(xs)
+    if n == 0:
+        return []
+    lib = xs[0].lib
+    ps = (abi.StepParams * n)()
+    for b, x in enumerate(xs):
+        if x.dt_frac != 0.0:
+            raise ValueError("step_batch runs fixed-dt steps only")
+        p = ps[b]
+        p.type, p.n_steps, p.t0, p.dt = x.type, int(n_steps), float(t0s[b] if t0s is not None else 0.0), x.dt
+        p.noise = abi.MRMG_NOISE_NONE if philox_seeds is None else abi.MRMG_NOISE_PHILOX
+        p.seed = 0 if philox_seeds is None else int(philox_seeds[b])
+        p.step0, p.dt_frac, p.dt_max, p.max_dm, p.llb = x.step_index, 0.0, x.dt_max, x.max_dm, int(x.llb)
+    ctxs = (C.c_void_p * n)(*[x.ctx.value for x in xs])
+    t_end = np.zeros(n)
+    abi.check(lib, lib.mrmg_step_batch(ctxs, ps, n, int(bool(graph)), abi.dptr(t_end)))
+    for x in xs:
+        x.step_index += int(n_steps)
+        if n_steps > 0:
+            x.dt = x.fixed_dt()
+    return list(t_end)

@@ -43,6 +43,7 @@ def test_library_loads_without_gpu():
     lib = abi.load()
     assert lib.mrmg_last_error() is not None
     assert lib.mrmg_create(None, None) == 1   # MRMG_ERR_ARG, no device call made
+    assert lib.mrmg_create_open(None, None) == 1
     # the readback and switching-record entry points reject a null context the same way
     assert lib.mrmg_readback(None, None, 0, None, None, None) == 1
     assert lib.mrmg_readback_noalloc(None, None, 0, None, None, None) == 1
