@@ -405,6 +405,9 @@ int mrmg_time_kernel(mrmg_ctx *ctx, const char *name, int reps, double *ms);
  * streaming copy kernel, `reps` times; *ms receives the average duration.  Used as the
  * known-size reference when scaling HBM traffic counters. */
 int mrmg_copy_probe(mrmg_ctx *ctx, size_t bytes, int reps, double *ms);
+/* The same copy by the runtime (hipMemcpyAsync device-to-device on the context's stream),
+ * to cross-check the copy kernel's rate as the reachable bandwidth. */
+int mrmg_memcpy_probe(mrmg_ctx *ctx, size_t bytes, int reps, double *ms);
 
 #ifdef __cplusplus
 }

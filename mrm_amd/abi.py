@@ -265,6 +265,7 @@ def load(path=None):
         ("mrmg_profile", [C.c_void_p, C.c_int]),
         ("mrmg_kernel_stats", [C.c_void_p, C.POINTER(KStat), C.c_int, _ip]),
         ("mrmg_copy_probe", [C.c_void_p, C.c_size_t, C.c_int, _dp]),
+        ("mrmg_memcpy_probe", [C.c_void_p, C.c_size_t, C.c_int, _dp]),
         ("mrmg_time_kernel", [C.c_void_p, C.c_char_p, C.c_int, _dp]),
         ("mrmg_tune", [C.c_void_p, C.c_char_p, C.c_int]),
         ("mrmg_get_tune", [C.c_void_p, C.c_char_p, _ip]),

@@ -719,6 +719,29 @@ int mrmg_copy_probe(mrmg_ctx* c, size_t bytes, int reps, double* ms)
     return MRMG_OK;
 }
 
+int mrmg_memcpy_probe(mrmg_ctx* c, size_t bytes, int reps, double* ms)
+{
+    if (!c || !ms || reps < 1 || bytes == 0) return set_err(MRMG_ERR_ARG, "bad argument");
+    DBuf<unsigned char> a, b;
+    RCHK(a.alloc(bytes));
+    RCHK(b.alloc(bytes));
+    HIPCHK(hipMemsetAsync(a.p, 1, bytes, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, c->stream));   // warm
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return MRMG_OK;
+}
+
 int mrmg_footprint(mrmg_ctx* c, size_t* bytes)
 {
     if (!c || !bytes) return set_err(MRMG_ERR_ARG, "null argument");

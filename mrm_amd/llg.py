@@ -130,6 +130,12 @@ class Llg:
         abi.check(self.lib, self.lib.mrmg_copy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
         return ms.value
 
+    def memcpy_probe(self, nbytes, reps=20):
+        """The same copy by hipMemcpyAsync device-to-device: average ms."""
+        ms = C.c_double()
+        abi.check(self.lib, self.lib.mrmg_memcpy_probe(self.ctx, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
     def time_kernel(self, name, reps=50):
         """Average duration (ms) of `reps` back-to-back launches of one demag kernel class."""
         ms = C.c_double()
