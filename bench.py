@@ -394,7 +394,7 @@ def serving(log, reqs, gen):
     load's."""
     from mrm_amd import serve
     n_requests, rate, steps = len(reqs), SERVING["rate"], SERVING["steps"]
-    srv = serve.Server(workers=4, max_batch=16, chunk=250, graph=True, mem_budget=128 << 30)
+    srv = serve.Server(workers=4, max_batch=16, chunk=250, graph=True, mem_budget=128 << 30, batch_wait=0.05)
     wall = srv.run(reqs, log=log)
     st = serve.latency_stats(reqs, wall)
     bs = np.array(srv.batch_sizes)
@@ -403,8 +403,8 @@ def serving(log, reqs, gen):
                "media_generation_s": round(gen, 1),
                "load": (f"{n_requests} requests, Poisson {rate}/s (seed 11), Nx, Ny log-uniform 64-1024 rounded to "
                         f"powers of two, n_lay 1/2/4, T 0/300/400 K, {steps[0]}-{steps[1]} steps (a twentieth of "
-                        "5k-50k), FP32 form; 4 workers, batches of up to 16, chunks of 250 steps replayed from "
-                        "hipGraphs")})
+                        "5k-50k), FP32 form; 4 workers, batches of up to 16 with a 50 ms window, chunks of 250 "
+                        "steps replayed from hipGraphs")})
     srv.close()
     log(f"serving: p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, {st['cell_updates_per_s']:.3e} cell-updates/s, "
         f"{bs.size} batched calls, mean batch {bs.mean():.2f}, peak {srv.peak / 1e9:.1f} GB admitted")
@@ -533,14 +533,17 @@ def llg_cfg1(a, log):
     return out
 
 
-def open_write(a, log, steps=20):
+def open_write(a, log, shapes, steps=20):
     """The heat-assisted write with open in-plane boundaries: the same 2048 x 2048 x 8
     problem, its demag FFTs on the zero-padded 4096 x 4096 grid (z coupling through the 15
-    layer-offset tensor blocks), FP32 form, default launch shapes."""
+    layer-offset tensor blocks), FP32 form; the grain kernels take the lanes per grain the
+    periodic run's autotuner chose, the FFT kernels their default shapes."""
     med = media.config_media("cfg4")
     t = time.perf_counter()
     x = llb_context(med, abi.MRMG_FP32, open=True)
     setup = time.perf_counter() - t
+    if shapes.get("lanes_per_grain"):
+        x.tune("lanes_per_grain", shapes["lanes_per_grain"])
     x.step(a.warmup, philox_seed=SEED)
     el, d = timed(x, steps, a.warmup * DT_LLB)
     v = med.cells * steps / el
@@ -712,7 +715,7 @@ def main():
         out["llg_cfg1"] = llg_cfg1(a, log)
         out["track_write"] = track_write(log)
         out["serving"] = serving(log, *load)
-        out["open_write"] = open_write(a, log)
+        out["open_write"] = open_write(a, log, f32["shapes"])
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(log)
         log(f"cpu baseline: {out['cpu_baseline']}")

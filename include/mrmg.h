@@ -380,9 +380,11 @@ int mrmg_profile(mrmg_ctx *ctx, int enable);
  * FFTW_EXHAUSTIVE planning plays for the reference, fftw.f90:25): "row_threads" (64, 128,
  * 256), "col_threads" (256, 512, or 1024 with the 6-slot tensor and without wave-local
  * FFTs), "col_groups" (observation-layer groups per column workgroup, 1..n_lay),
- * "col_path" (0 fused, 1 split, 2 fused on column-contiguous spectra), "row_block" (2, 4,
+ * "col_path" (0 fused, 1 split, 2 fused on column-contiguous spectra, 3 split forward
+ * column FFTs then the fused product and inverse, one observation layer per workgroup),
+ * "row_block" (2, 4,
  * 8 rows per transposing row workgroup), "row_radix" / "col_radix" (largest Stockham
- * radix: 4, 8, 16), "wave_fft" (0, 1: wave-local FFTs), "lanes_per_grain" (4, 8, 16).
+ * radix: 4, 8, 16), "wave_fft" (0, 1: wave-local FFTs), "lanes_per_grain" (1, 2, 4, 8, 16).
  * The thread counts, "col_groups" and "row_block" leave results unchanged bit for bit;
  * the others may change the order of floating-point operations (FFT factorisation,
  * gather sums) and so the last bits.  A combination that does not fit the grid is

@@ -21,10 +21,13 @@
 // BIG: 1024-thread workgroups (twice the lanes per FFT pass, at most 128 VGPRs), two
 // product items per thread.
 // BAT: one launch over a batch of contexts (blockIdx.z the member, operands from tab).
+// PRE: the columns in S1 are already transformed along y (by k_colT), so the kernel only
+// forms the products and the inverse transforms of its observation layers.
 // ny: the column length (the FFT grid); nyv: the rows the row kernels write and read
 // (the media rows; below ny for open boundaries, whose padding rows are zero in S1 and
 // not needed in S2).
-template <typename T, int E, bool SYM, bool TL = false, bool WV = false, bool BIG = false, bool BAT = false>
+template <typename T, int E, bool SYM, bool TL = false, bool WV = false, bool BIG = false, bool BAT = false,
+          bool PRE = false>
 __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(DemagOps<T> one, const DemagOps<T>* __restrict__ tab,
                                                      int ny, int nyv, int nl, int nxh,
                                                      int nslot, int toep, int kg, int nb8, FFTPlan py,
@@ -48,12 +51,13 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(DemagOps<T
         ncol * ny,
         [&](int idx) {
             const int col = idx / ny, y = idx - col * ny;
-            if (y >= nyv) return C2<T>{0, 0};
+            if (!PRE && y >= nyv) return C2<T>{0, 0};
             return TL ? S1[((size_t)col * nxh + kx) * ny + y] : S1[((size_t)col * ny + y) * nxh + kx];
         },
         [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; }, tw, twy, ny);
     __syncthreads();
-    if (WV) {
+    if (PRE) {
+    } else if (WV) {
         lds_fft_wave<T, E, false>(re, im, ncol, py, tw);
         __syncthreads();
     } else {
@@ -155,7 +159,14 @@ template <int MB> void Engine<T>::launch_col_sym(int nb8)
 {
     const DemagOps<T> o = ops();
     const dim3 g(nb8, 1, btab ? (unsigned)bn : 1u), b(col_threads);
-    if (btab) {
+    if (pre) {
+        if (col_threads > 512)
+            hipLaunchKernelGGL((k_col<T, MB, true, true, false, true, false, true>), g, b, lds_col, c->stream,
+                               COL_ARGS);
+        else
+            hipLaunchKernelGGL((k_col<T, MB, true, true, false, false, false, true>), g, b, lds_col, c->stream,
+                               COL_ARGS);
+    } else if (btab) {
         if (tl)
             hipLaunchKernelGGL((k_col<T, MB, true, true, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
         else
@@ -178,7 +189,10 @@ template <int MB> void Engine<T>::launch_col_gen(int nb8)
 {
     const DemagOps<T> o = ops();
     const dim3 g(nb8, 1, btab ? (unsigned)bn : 1u), b(col_threads);
-    if (btab) {
+    if (pre)
+        hipLaunchKernelGGL((k_col<T, MB, false, true, false, false, false, true>), g, b, lds_col, c->stream,
+                           COL_ARGS);
+    else if (btab) {
         if (tl)
             hipLaunchKernelGGL((k_col<T, MB, false, true, false, false, true>), g, b, lds_col, c->stream, COL_ARGS);
         else
@@ -213,6 +227,12 @@ template <int MB> int Engine<T>::lds_attr_col()
     HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true, false, false, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, false, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, false, false, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, true, true, false, true, false, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, cl));
+    HIPCHK(hipFuncSetAttribute((const void*)k_col<T, MB, false, true, false, false, false, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, cl));
     return MRMG_OK;
 }

@@ -202,18 +202,24 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(DemagOps<T> one, const DemagOp
     } else {
         lds_fft<T, E, false>(re, im, B, px, tw);
     }
+    // one thread per (kx, row pair): both rows of the pair come from the same four LDS
+    // values, and go out as one 2-element store where the pair is aligned (sy even)
     C2<T>* dst = S + (size_t)kc * nxh * sy;
-    for (int idx = threadIdx.x; idx < nxh * TY; idx += blockDim.x) {
-        const int kx = idx / TY, yy = idx - kx * TY;
-        const int y = y0 + yy;
+    for (int idx = threadIdx.x; idx < nxh * B; idx += blockDim.x) {
+        const int kx = idx / B, p = idx - kx * B;
+        const int y = y0 + 2 * p;
         if (y >= ny) continue;
-        const int p = yy >> 1;
         const int kn = kx == 0 ? 0 : nx - kx;
         const T zr = re[LP(p * nx + kx)], zi = im[LP(p * nx + kx)], nr = re[LP(p * nx + kn)], ni = im[LP(p * nx + kn)];
-        C2<T> v;
-        if ((yy & 1) == 0) v = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
-        else v = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
-        dst[(size_t)kx * sy + y] = v;
+        const C2<T> a = {(zr + nr) * (T)0.5, (zi - ni) * (T)0.5};
+        const C2<T> b = {(zi + ni) * (T)0.5, (nr - zr) * (T)0.5};
+        C2<T>* d = dst + (size_t)kx * sy + y;
+        if (y + 1 < ny && (sy & 1) == 0) {
+            *reinterpret_cast<C4<T>*>(d) = C4<T>{a, b};
+        } else {
+            d[0] = a;
+            if (y + 1 < ny) d[1] = b;
+        }
     }
 }
 
@@ -518,7 +524,8 @@ int Engine<T>::launch_class(int cls)
     }
     case K_COLF:
     case K_COLI:
-        if (!split) return set_err(MRMG_ERR_STATE, "the fused paths have no separate column FFTs");
+        if (!split && !(pre && cls == K_COLF))
+            return set_err(MRMG_ERR_STATE, "the fused paths have no separate column FFTs");
         if (cls == K_COLF) LAUNCH_E(maxb_colT, launch_colT_fwd, gcol, b256);
         else LAUNCH_E(maxb_colT, launch_colT_inv, gcol, b256);
         break;

@@ -306,6 +306,9 @@ struct Engine : EngineBase {
     // column workgroup, and whether it is in use (forced by col_path or by the grid size)
     bool split = false, fused_ok = false, split_ok = false;
     bool tl = false;   // fused column kernel over the split path's column-contiguous spectra
+    // path 3: the forward column FFTs as the split path's kernel (many workgroups), then the
+    // fused kernel's product and inverse FFT, one observation layer group per workgroup
+    bool pre = false;
     bool wv = false;   // wave-local FFTs in the transposing row kernels and the path-2 column kernel
     int TYr = 8, CBc = 1, maxb_rowT = 4, maxb_colT = 4;
     // FFT plans of the step's row and column transforms and their largest radix
@@ -477,12 +480,12 @@ struct Engine : EngineBase {
                 return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: row FFT too large for %d threads", c->fx, ny, nl,
                                row_threads);
         }
-        if (split) {
+        if (split || pre) {
             maxb_colT = pick_e(CBc, c->py, 256);
             if (maxb_rowT < 0 || maxb_colT < 0)
                 return set_err(MRMG_ERR_UNSUPPORTED, "grid %dx%dx%d: split-path FFT too large for %d threads", c->fx, ny,
                                nl, row_threads);
-            return MRMG_OK;
+            if (split) return MRMG_OK;
         }
         maxb_row = pick_e(3, c->px, row_threads);
         const int ncw = col_threads / 64;
@@ -519,17 +522,23 @@ struct Engine : EngineBase {
             old = kg; kg = v;
             if (shapes() != MRMG_OK) { kg = old; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else if (k == "lanes_per_grain") {
-            if (v != 4 && v != 8 && v != 16) return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 4, 8 or 16");
+            if (v != 1 && v != 2 && v != 4 && v != 8 && v != 16)
+                return set_err(MRMG_ERR_ARG, "lanes_per_grain must be 1, 2, 4, 8 or 16");
             lpg = v;
         } else if (k == "col_path") {
-            if (v < 0 || v > 2)
-                return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused), 1 (split) or 2 (fused, column-contiguous)");
+            if (v < 0 || v > 3)
+                return set_err(MRMG_ERR_ARG, "col_path must be 0 (fused), 1 (split), 2 (fused, column-contiguous) or 3 "
+                                             "(split forward column FFTs, fused product and inverse)");
             if (v != 1 && !fused_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the fused column kernel does not fit this grid");
             if (v != 0 && !split_ok) return set_err(MRMG_ERR_UNSUPPORTED, "the transposing row kernels do not fit this grid");
-            const bool os = split, ot = tl;
+            const bool os = split, ot = tl, op = pre;
+            const int ok = kg;
             split = v == 1;
-            tl = v == 2;
-            if (shapes() != MRMG_OK) { split = os; tl = ot; return shapes(), MRMG_ERR_UNSUPPORTED; }
+            tl = v == 2 || v == 3;
+            pre = v == 3;
+            if (pre && !op) kg = c->nl;        // one observation layer per workgroup
+            if (!pre && op) kg = 1;
+            if (shapes() != MRMG_OK) { split = os; tl = ot; pre = op; kg = ok; return shapes(), MRMG_ERR_UNSUPPORTED; }
         } else if (k == "wave_fft") {
             if (v != 0 && v != 1) return set_err(MRMG_ERR_ARG, "wave_fft must be 0 or 1");
             const bool ow = wv;
@@ -564,7 +573,7 @@ struct Engine : EngineBase {
         else if (k == "col_threads") *v = col_threads;
         else if (k == "col_groups") *v = kg;
         else if (k == "lanes_per_grain") *v = lpg;
-        else if (k == "col_path") *v = split ? 1 : tl ? 2 : 0;
+        else if (k == "col_path") *v = split ? 1 : pre ? 3 : tl ? 2 : 0;
         else if (k == "row_block") *v = TYr;
         else if (k == "wave_fft") *v = wv ? 1 : 0;
         else if (k == "row_radix") *v = row_radix;
@@ -1014,8 +1023,9 @@ struct Engine : EngineBase {
     {
         static const int fused[3] = {K_ROW_FWD, K_COL, K_ROW_INV};
         static const int splitc[5] = {K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV};
-        const int* seq = split ? splitc : fused;
-        const int n = split ? 5 : 3;
+        static const int prec[4] = {K_ROW_FWD, K_COLF, K_COL, K_ROW_INV};
+        const int* seq = split ? splitc : pre ? prec : fused;
+        const int n = split ? 5 : pre ? 4 : 3;
         for (int i = 0; i < n; i++) {
             RCHK(ev_begin(seq[i]));
             RCHK(launch_class(seq[i]));
@@ -1488,6 +1498,7 @@ int batch_step(std::vector<Engine<T>*>& M, const mrmg_step_params* P, int graph,
         if (e.c->nx != L.c->nx || e.c->ny != L.c->ny || e.c->nl != L.c->nl || e.c->fx != L.c->fx ||
             e.c->fy != L.c->fy || e.c->device != L.c->device)
             return set_err(MRMG_ERR_ARG, "batch member %d: grid or device differs", b);
+        if (e.pre) return set_err(MRMG_ERR_UNSUPPORTED, "batch member %d: col_path 3 runs unbatched", b);
         if (e.demag_on() != L.demag_on() || e.split != L.split || e.tl != L.tl || e.wv || e.TYr != L.TYr ||
             e.CBc != L.CBc || e.row_threads != L.row_threads || e.col_threads != L.col_threads || e.kg != L.kg ||
             e.row_radix != L.row_radix || e.col_radix != L.col_radix || e.maxb_row != L.maxb_row ||

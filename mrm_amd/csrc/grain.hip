@@ -1134,6 +1134,8 @@ template <int MODE> int Engine<T>::grain(const StepArgs& a, S* mout, double t_ne
     RCHK(ev_begin((MODE == GM_HEUN1 || MODE == GM_ROT1 || MODE == GM_FIELD1 || MODE == GM_PUSH1) ? K_GRAIN1
                                                                                              : K_GRAIN2));
     if (lpg == 4) launch_grain<MODE, 4>(a, mout);
+    else if (lpg == 2) launch_grain<MODE, 2>(a, mout);
+    else if (lpg == 1) launch_grain<MODE, 1>(a, mout);
     else if (lpg == 16) launch_grain<MODE, 16>(a, mout);
     else launch_grain<MODE, 8>(a, mout);
     HIPCHK(hipGetLastError());

@@ -7,7 +7,8 @@ Admitted requests whose contexts share a grid, layer stack and precision form a 
 and a group's members are stepped together in chunks by mrmg_step_batch: each demag
 kernel runs once for all of them and one step, captured as a hipGraph, is replayed.
 Requests join their group between chunks and leave it when their steps are done, so a
-group is never drained to let newcomers in (continuous batching).  Several worker
+group is never drained to let newcomers in (continuous batching); a group that is not
+full may wait a batching window for more members before its first chunk.  Several worker
 threads step different groups (or disjoint parts of one) at once, each batch on its lead
 member's stream, so the work of different groups overlaps on the device.  A new context
 of a grid that is already resident copies its tensor spectra on the device
@@ -83,13 +84,18 @@ class _Live:
         self.left = r.n_steps
         self.t = 0.0
         self.busy = False       # being stepped, or lending its tensor
+        self.ready = time.perf_counter()   # when it joined its group
 
 
 class Server:
     """Continuous batching with admission by device footprint."""
 
-    def __init__(self, workers=4, mem_budget=64 << 30, max_batch=16, chunk=200, graph=True, pool=8, admitters=4):
+    def __init__(self, workers=4, mem_budget=64 << 30, max_batch=16, chunk=200, graph=True, pool=8, admitters=4,
+                 batch_wait=0.0):
+        """batch_wait: seconds a group that is not full waits, from its oldest member's
+        admission, for more members before it is stepped (the batching window)."""
         self.workers = workers
+        self.batch_wait = batch_wait
         self.admitters = admitters
         self.budget = mem_budget
         self.max_batch = max_batch
@@ -278,28 +284,38 @@ class Server:
                 self.cv.notify_all()
 
         def pick():
-            """Under the lock: the members of the group whose oldest waiting member came first."""
-            best = None
+            """Under the lock: the members of the group whose oldest waiting member came
+            first, among the groups that are full or whose oldest member has waited the
+            batching window (or when admission is over); else ([], the next deadline)."""
+            now = time.perf_counter()
+            groups = {}
             for lv in self.live:
-                if not lv.busy and lv.left > 0 and (best is None or lv.r.arrival < best.r.arrival):
-                    best = lv
+                if not lv.busy and lv.left > 0:
+                    groups.setdefault(lv.key, []).append(lv)
+            best, wake = None, None
+            for g in groups.values():
+                g.sort(key=lambda q: q.r.arrival)
+                due = g[0].ready + self.batch_wait
+                if len(g) >= self.max_batch or now >= due or not state["admitting"]:
+                    if best is None or g[0].r.arrival < best[0].r.arrival:
+                        best = g
+                elif wake is None or due < wake:
+                    wake = due
             if best is None:
-                return []
-            group = [lv for lv in self.live if lv.key == best.key and not lv.busy and lv.left > 0]
-            group.sort(key=lambda q: q.r.arrival)
-            group = group[:self.max_batch]
+                return [], wake
+            group = best[:self.max_batch]
             for lv in group:
                 lv.busy = True
-            return group
+            return group, None
 
         def work():
             while True:
                 with self.cv:
                     while True:
-                        group = pick()
+                        group, wake = pick()
                         if group or (not state["admitting"] and not self.live):
                             break
-                        self.cv.wait()
+                        self.cv.wait(None if wake is None else max(wake - time.perf_counter(), 1e-4))
                 if not group:
                     return
                 n = min(self.chunk, min(lv.left for lv in group))

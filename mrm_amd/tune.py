@@ -10,7 +10,8 @@ import numpy as np
 
 DEMAG = ("row_fwd", "col", "row_inv", "col_fwd", "prod", "col_inv")
 SPACE = {
-    "col_path": ([0, 1, 2], DEMAG),           # fused column kernel, split path, fused on column-contiguous spectra
+    "col_path": ([0, 1, 2, 3], DEMAG),        # fused column kernel, split path, fused on column-contiguous spectra,
+                                              # split forward column FFTs then fused product + inverse
     "row_block": ([2, 4, 8], DEMAG),          # rows per transposing row workgroup (paths 1, 2)
     "wave_fft": ([0, 1], DEMAG),              # wave-local FFTs in the transposing row and path-2 column kernels
     "row_threads": ([64, 128, 256], ("row_fwd", "row_inv")),
@@ -18,7 +19,7 @@ SPACE = {
     "col_radix": ([4, 8, 16], ("col", "col_fwd", "col_inv")),
     "col_threads": ([256, 512, 1024], ("col",)),
     "col_groups": (None, ("col",)),          # 1..min(n_lay, 4)
-    "lanes_per_grain": ([4, 8, 16], ("grain1", "grain2")),
+    "lanes_per_grain": ([1, 2, 4, 8, 16], ("grain1", "grain2")),
 }
 
 

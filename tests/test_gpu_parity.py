@@ -435,8 +435,9 @@ def test_launch_shapes_do_not_change_results():
     N = None
     out = {}
     for cfg in ({}, {"row_threads": 64}, {"row_threads": 256, "col_threads": 256}, {"col_groups": 2},
-                {"col_groups": 4, "col_threads": 256}, {"col_threads": 1024}, {"lanes_per_grain": 4},
-                {"lanes_per_grain": 16}):
+                {"col_groups": 4, "col_threads": 256}, {"col_threads": 1024}, {"col_path": 3},
+                {"col_path": 3, "col_groups": 2}, {"lanes_per_grain": 1}, {"lanes_per_grain": 2},
+                {"lanes_per_grain": 4}, {"lanes_per_grain": 16}):
         x = Llg(64, 64, 4, med.dx, med.dy, med.dz)
         if N is None:
             N = x.build_tensor(num_images=1, return_real=True)

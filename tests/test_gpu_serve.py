@@ -24,7 +24,7 @@ def test_results_same_alone_and_in_the_mix():
     srv.run(alone)
     srv.close()
     mixed = _copy(reqs)
-    srv = serve.Server(workers=3, max_batch=8, chunk=50)
+    srv = serve.Server(workers=3, max_batch=8, chunk=50, batch_wait=2.0)   # a window that collects each group
     wall = srv.run(mixed)
     srv.close()
     st = serve.latency_stats(mixed, wall)
