@@ -8,18 +8,43 @@
 #endif
 
 // ------------------------------------------------------------------------- runs <-> LDS rows
+// A thread takes the runs q0 + tid, q0 + tid + nt, ... of its block, RU at a time: the
+// descriptors (and grain values) of RU runs are loaded before any is used, so the
+// dependent loads of a run (descriptor, then grain value) are waited for once per RU runs
+// instead of once per run.
+#define MRMG_RU 4
+
 // The cell magnetisation of rows y0 .. y0+nrows-1 of layer k into LDS, component c0+ci of
 // NC at row pair p = (y - y0)/2: re holds the even rows, im the odd ones, at element
 // (ci*np + p)*nx + x (get_mag, data.f90:344-353: Ms ms_scale m on a grain's cells, 0 on
 // non-magnetic cells).  Mg[c][g] is the grain's cell value.  Also stages the row
 // twiddles.  roff (shared, nrows+1 entries) keeps the block's run offsets for the store.
+// The first RU runs' loads are issued before LDS is cleared and overlap it.
 template <typename T, int NC>
 __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restrict__ Mg, int G, int c0, int k, int ny,
                                             int y0, int nrows, int np, int nx, T* re, T* im, int* roff, C2<T>* tw,
                                             const C2<T>* __restrict__ twx)
 {
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
-    if (tid <= nrows) roff[tid] = R.row_off[k * ny + min(y0 + tid, ny)];
+    const int* ro = R.row_off + k * ny + y0;          // y0 + nrows <= ny
+    const int q0 = ro[0], q1 = ro[nrows];
+    int geo[MRMG_RU];
+    T v[MRMG_RU][NC];
+    auto fetch = [&](int qb) {
+        int g[MRMG_RU];
+#pragma unroll
+        for (int u = 0; u < MRMG_RU; u++) {
+            const int q = qb + u * nt;
+            if (q < q1) { geo[u] = R.geo[q]; g[u] = R.grain[q]; }
+        }
+#pragma unroll
+        for (int u = 0; u < MRMG_RU; u++)
+            if (qb + u * nt < q1)
+#pragma unroll
+                for (int ci = 0; ci < NC; ci++) v[u][ci] = Mg[(size_t)(c0 + ci) * G + g[u]];
+    };
+    fetch(q0 + tid);
+    if (tid <= nrows) roff[tid] = ro[tid];
     for (int i = tid; i < nx; i += nt) tw[i] = twx[i];
     const int n = NC * np * nx;
     for (int i = tid; i < n; i += nt) {
@@ -27,42 +52,71 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
         im[LP(i)] = (T)0;
     }
     __syncthreads();
-    const int q1 = roff[nrows];
-    for (int q = roff[0] + tid; q < q1; q += nt) {
-        int yy = 0;
-        while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
-        const int geo = R.geo[q], g = R.grain[q];
-        const int i0 = geo & 0xffff, len = geo >> 16;
-        T* dst = (yy & 1) ? im : re;
+    for (int qb = q0 + tid; qb < q1; qb += MRMG_RU * nt) {
+        if (qb != q0 + tid) fetch(qb);
 #pragma unroll
-        for (int ci = 0; ci < NC; ci++) {
-            const T v = Mg[(size_t)(c0 + ci) * G + g];
-            const int b = (ci * np + (yy >> 1)) * nx + i0;
-            for (int t = 0; t < len; t++) dst[LP(b + t)] = v;
+        for (int u = 0; u < MRMG_RU; u++) {
+            const int q = qb + u * nt;
+            if (q >= q1) break;
+            int yy = 0;
+            while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
+            const int i0 = geo[u] & 0xffff, len = geo[u] >> 16;
+            T* dst = (yy & 1) ? im : re;
+#pragma unroll
+            for (int ci = 0; ci < NC; ci++) {
+                const int b = (ci * np + (yy >> 1)) * nx + i0;
+                for (int t = 0; t < len; t++) dst[LP(b + t)] = v[u][ci];
+            }
         }
     }
 }
 
+// The run descriptors a thread sums in lds_to_runs, its first RU of them loaded ahead
+// (at kernel start, before the spectra and the FFT) by prefetch.
+struct RunsAhead {
+    int q0, q1;
+    int geo[MRMG_RU], gm[MRMG_RU];
+    __device__ __forceinline__ void fetch(const RowRuns& R, int qb, int nt)
+    {
+#pragma unroll
+        for (int u = 0; u < MRMG_RU; u++) {
+            const int q = qb + u * nt;
+            if (q < q1) { geo[u] = R.geo[q]; gm[u] = R.gm[q]; }
+        }
+    }
+    __device__ __forceinline__ void prefetch(const RowRuns& R, int k, int ny, int y0, int nrows)
+    {
+        const int* ro = R.row_off + k * ny + y0;
+        q0 = ro[0];
+        q1 = ro[nrows];
+        fetch(R, q0 + (int)threadIdx.x, (int)blockDim.x);
+    }
+};
+
 // The cell field of the same rows, summed over each run (in x order, in FP64) into
 // Hr[c][run] (c = c0+ci), with the layout of runs_to_lds.
 template <typename T, int NC>
-__device__ __forceinline__ void lds_to_runs(const RowRuns& R, double* __restrict__ Hr, int c0, int nrows, int np,
-                                            int nx, const T* re, const T* im, const int* roff)
+__device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead& A, double* __restrict__ Hr, int c0,
+                                            int nrows, int np, int nx, const T* re, const T* im, const int* roff)
 {
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
-    const int q1 = roff[nrows];
-    for (int q = roff[0] + tid; q < q1; q += nt) {
-        int yy = 0;
-        while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
-        const int geo = R.geo[q], r = R.gm[q];
-        const int i0 = geo & 0xffff, len = geo >> 16;
-        const T* src = (yy & 1) ? im : re;
+    for (int qb = A.q0 + tid; qb < A.q1; qb += MRMG_RU * nt) {
+        if (qb != A.q0 + tid) A.fetch(R, qb, nt);
 #pragma unroll
-        for (int ci = 0; ci < NC; ci++) {
-            const int b = (ci * np + (yy >> 1)) * nx + i0;
-            double s = 0.0;
-            for (int t = 0; t < len; t++) s += (double)src[LP(b + t)];
-            Hr[(size_t)(c0 + ci) * R.nruns + r] = s;
+        for (int u = 0; u < MRMG_RU; u++) {
+            const int q = qb + u * nt;
+            if (q >= A.q1) break;
+            int yy = 0;
+            while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
+            const int i0 = A.geo[u] & 0xffff, len = A.geo[u] >> 16;
+            const T* src = (yy & 1) ? im : re;
+#pragma unroll
+            for (int ci = 0; ci < NC; ci++) {
+                const int b = (ci * np + (yy >> 1)) * nx + i0;
+                double s = 0.0;
+                for (int t = 0; t < len; t++) s += (double)src[LP(b + t)];
+                Hr[(size_t)(c0 + ci) * R.nruns + A.gm[u]] = s;
+            }
         }
     }
 }
@@ -128,6 +182,8 @@ __global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps
     T* im = re + LPN(3 * nx);
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
+    RunsAhead ra;
+    if (!CELLS) ra.prefetch(R, k, ny, y0, has1 ? 2 : 1);
     staged_load<(E < 4 ? E : 4)>(
         3 * nxh,
         [&](int idx) {
@@ -151,7 +207,7 @@ __global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps
     if (!CELLS && threadIdx.x <= (has1 ? 2 : 1)) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
     lds_fft<T, E, true>(re, im, 3, px, tw);
     if (!CELLS) {
-        lds_to_runs<T, 3>(R, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
+        lds_to_runs<T, 3>(R, ra, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
         return;
     }
     const size_t plane = (size_t)mx * ny;
@@ -400,6 +456,9 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
     const int y0 = (L % nyb) * TY, kc = L / nyb;
     const int k = kc / 3, c = kc - 3 * k;
     const C2<T>* src = S + (size_t)kc * nxh * sy;
+    const int nrows = min(TY, ny - y0);
+    RunsAhead ra;
+    if (!CELLS) ra.prefetch(R, k, ny, y0, nrows);
     // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
     staged_load<(E < 4 ? E : 4)>(
         nxh * B,
@@ -422,7 +481,6 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
         },
         tw, twx, nx);
     __syncthreads();
-    const int nrows = min(TY, ny - y0);
     if (!CELLS && (int)threadIdx.x <= nrows) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
     if (WV) {
         lds_fft_wave<T, E, true>(re, im, B, px, tw);
@@ -431,7 +489,7 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
         lds_fft<T, E, true>(re, im, B, px, tw);
     }
     if (!CELLS) {
-        lds_to_runs<T, 1>(R, Hr, c, nrows, B, nx, re, im, roff);
+        lds_to_runs<T, 1>(R, ra, Hr, c, nrows, B, nx, re, im, roff);
         return;
     }
     T* hc = Hcell + ((size_t)c * nl + k) * (size_t)mx * ny;

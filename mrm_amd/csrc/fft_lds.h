@@ -228,15 +228,28 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
             const int jq = fdiv(j, Ns, mns);
             const int k = j - jq * Ns;
             const int base = off + b * N;
+            if ((NR & 15) == 0) {
+                // a stride that is a multiple of 16 moves the padded index by NR + NR/16
+                const int l0 = LP(base + j), ls = NR + (NR >> 4);
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                v[q][r].x = re[LP(base + j + r * NR)];
-                v[q][r].y = im[LP(base + j + r * NR)];
+                for (int r = 0; r < R; r++) {
+                    v[q][r].x = re[l0 + r * ls];
+                    v[q][r].y = im[l0 + r * ls];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    v[q][r].x = re[LP(base + j + r * NR)];
+                    v[q][r].y = im[LP(base + j + r * NR)];
+                }
             }
             if (Ns > 1) {
+                const int kt = k * tstep;
+                int e = 0;
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    C2<T> w = tw[k * r * tstep];
+                    e += kt;                       // k r tstep
+                    C2<T> w = tw[e];
                     if (INV) w.y = -w.y;
                     v[q][r] = cmul(v[q][r], w);
                 }
@@ -250,10 +263,19 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
         if (dst[q] >= 0) {
+            if ((Ns & 15) == 0) {
+                const int l0 = LP(dst[q]), ls = Ns + (Ns >> 4);
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                re[LP(dst[q] + r * Ns)] = v[q][r].x;
-                im[LP(dst[q] + r * Ns)] = v[q][r].y;
+                for (int r = 0; r < R; r++) {
+                    re[l0 + r * ls] = v[q][r].x;
+                    im[l0 + r * ls] = v[q][r].y;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    re[LP(dst[q] + r * Ns)] = v[q][r].x;
+                    im[LP(dst[q] + r * Ns)] = v[q][r].y;
+                }
             }
         }
     }
