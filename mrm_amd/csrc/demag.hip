@@ -13,6 +13,7 @@
 // dependent loads of a run (descriptor, then grain value) are waited for once per RU runs
 // instead of once per run.
 #define MRMG_RU 4
+#define MRMG_UF 4
 
 // The cell magnetisation of rows y0 .. y0+nrows-1 of layer k into LDS, component c0+ci of
 // NC at row pair p = (y - y0)/2: re holds the even rows, im the odd ones, at element
@@ -65,7 +66,11 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
 #pragma unroll
             for (int ci = 0; ci < NC; ci++) {
                 const int b = (ci * np + (yy >> 1)) * nx + i0;
-                for (int t = 0; t < len; t++) dst[LP(b + t)] = v[u][ci];
+                // MRMG_UF cells per iteration; past the run's end the last cell is written
+                // again (the same value), so no write is predicated
+                for (int t = 0; t < len; t += MRMG_UF)
+#pragma unroll
+                    for (int j = 0; j < MRMG_UF; j++) dst[LP(b + min(t + j, len - 1))] = v[u][ci];
             }
         }
     }
@@ -113,8 +118,16 @@ __device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead& A, doub
 #pragma unroll
             for (int ci = 0; ci < NC; ci++) {
                 const int b = (ci * np + (yy >> 1)) * nx + i0;
+                // in x order, MRMG_UF reads issued per iteration; past the run's end the
+                // last cell is read again and 0 added (exact: s is never -0)
                 double s = 0.0;
-                for (int t = 0; t < len; t++) s += (double)src[LP(b + t)];
+                for (int t = 0; t < len; t += MRMG_UF) {
+                    T a[MRMG_UF];
+#pragma unroll
+                    for (int j = 0; j < MRMG_UF; j++) a[j] = src[LP(b + min(t + j, len - 1))];
+#pragma unroll
+                    for (int j = 0; j < MRMG_UF; j++) s += t + j < len ? (double)a[j] : 0.0;
+                }
                 Hr[(size_t)(c0 + ci) * R.nruns + A.gm[u]] = s;
             }
         }
@@ -309,7 +322,8 @@ __global__ void __launch_bounds__(256) k_colT(DemagOps<T> one, const DemagOps<T>
 
 // The k-space product over column-contiguous spectra: thread per (kx, ky), observation
 // layers in chunks of OC, each output summed over (ks, b) in the reference's order.
-template <typename T, bool SYM, int OC, bool BAT = false>
+// NT: the tensor and the field spectra as non-temporal streams (see prod_streams).
+template <typename T, bool SYM, int OC, bool BAT = false, bool NT = false>
 __global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>* __restrict__ tab, int ny, int nl,
                                               int nxh, int nslot, int toep)
 {
@@ -337,7 +351,9 @@ __global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>
                 const C2<T>* t = Tspec + tensor_block(kx, ko, ks, nl, toep) * nslot * ny + ky;
                 C2<T> a;
                 if (SYM) {
-                    const C2<T> txx = t[0], tyy = t[ny], tzz = t[2 * ny], txy = t[3 * ny], txz = t[4 * ny], tyz = t[5 * ny];
+                    const C2<T> txx = ld_s<NT>(t), tyy = ld_s<NT>(t + ny), tzz = ld_s<NT>(t + 2 * ny),
+                                txy = ld_s<NT>(t + 3 * ny), txz = ld_s<NT>(t + 4 * ny),
+                                tyz = ld_s<NT>(t + 5 * ny);
                     a = cmul(txx, mx); h[j][0].x += a.x; h[j][0].y += a.y;
                     a = cmul(txy, my); h[j][0].x += a.x; h[j][0].y += a.y;
                     a = cmul(txz, mz); h[j][0].x += a.x; h[j][0].y += a.y;
@@ -348,15 +364,15 @@ __global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>
                     a = cmul(tyz, my); h[j][2].x += a.x; h[j][2].y += a.y;
                     a = cmul(tzz, mz); h[j][2].x += a.x; h[j][2].y += a.y;
                 } else {
-                    a = cmul(t[0 * ny], mx); h[j][0].x += a.x; h[j][0].y += a.y;
-                    a = cmul(t[1 * ny], my); h[j][0].x += a.x; h[j][0].y += a.y;
-                    a = cmul(t[2 * ny], mz); h[j][0].x += a.x; h[j][0].y += a.y;
-                    a = cmul(t[3 * ny], mx); h[j][1].x += a.x; h[j][1].y += a.y;
-                    a = cmul(t[4 * ny], my); h[j][1].x += a.x; h[j][1].y += a.y;
-                    a = cmul(t[5 * ny], mz); h[j][1].x += a.x; h[j][1].y += a.y;
-                    a = cmul(t[6 * ny], mx); h[j][2].x += a.x; h[j][2].y += a.y;
-                    a = cmul(t[7 * ny], my); h[j][2].x += a.x; h[j][2].y += a.y;
-                    a = cmul(t[8 * ny], mz); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(ld_s<NT>(t + 0 * ny), mx); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(ld_s<NT>(t + 1 * ny), my); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(ld_s<NT>(t + 2 * ny), mz); h[j][0].x += a.x; h[j][0].y += a.y;
+                    a = cmul(ld_s<NT>(t + 3 * ny), mx); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(ld_s<NT>(t + 4 * ny), my); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(ld_s<NT>(t + 5 * ny), mz); h[j][1].x += a.x; h[j][1].y += a.y;
+                    a = cmul(ld_s<NT>(t + 6 * ny), mx); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(ld_s<NT>(t + 7 * ny), my); h[j][2].x += a.x; h[j][2].y += a.y;
+                    a = cmul(ld_s<NT>(t + 8 * ny), mz); h[j][2].x += a.x; h[j][2].y += a.y;
                 }
             }
         }
@@ -365,7 +381,7 @@ __global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>
             const int ko = kc0 + j;
             if (ko >= nl) break;
 #pragma unroll
-            for (int a = 0; a < 3; a++) S2[((size_t)(3 * ko + a)) * Kc + idx] = h[j][a];
+            for (int a = 0; a < 3; a++) st_s<NT>(S2 + ((size_t)(3 * ko + a)) * Kc + idx, h[j][a]);
         }
     }
 }
@@ -375,7 +391,7 @@ __global__ void __launch_bounds__(256) k_prod(DemagOps<T> one, const DemagOps<T>
 // NL-1 down to -(NL-1), so every field sum H_ko still adds its sources in ascending ks, as
 // k_prod and the reference do (demag.f90:117-121), while each spectrum and each stored
 // tensor value is read from memory once.
-template <typename T, bool SYM, int NL, bool BAT = false>
+template <typename T, bool SYM, int NL, bool BAT = false, bool NT = false>
 __global__ void __launch_bounds__(256) k_prod_offsets(DemagOps<T> one, const DemagOps<T>* __restrict__ tab, int ny,
                                                       int nxh, int nslot)
 {
@@ -391,7 +407,7 @@ __global__ void __launch_bounds__(256) k_prod_offsets(DemagOps<T> one, const Dem
     for (int k = 0; k < NL; k++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            m[k][b] = S1[((size_t)(3 * k + b)) * Kc + idx];
+            m[k][b] = ld_s<NT>(S1 + ((size_t)(3 * k + b)) * Kc + idx);
             h[k][b] = {0, 0};
         }
 #pragma unroll
@@ -399,7 +415,7 @@ __global__ void __launch_bounds__(256) k_prod_offsets(DemagOps<T> one, const Dem
         const C2<T>* t = Tspec + tensor_block(kx, d > 0 ? d : 0, d > 0 ? 0 : -d, NL, 1) * nslot * ny + ky;
         C2<T> tv[SYM ? 6 : 9];
 #pragma unroll
-        for (int sl = 0; sl < (SYM ? 6 : 9); sl++) tv[sl] = t[sl * ny];
+        for (int sl = 0; sl < (SYM ? 6 : 9); sl++) tv[sl] = ld_s<NT>(t + (size_t)sl * ny);
 #pragma unroll
         for (int ko = 0; ko < NL; ko++) {
             const int ks = ko - d;
@@ -429,7 +445,7 @@ __global__ void __launch_bounds__(256) k_prod_offsets(DemagOps<T> one, const Dem
 #pragma unroll
     for (int k = 0; k < NL; k++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) S2[((size_t)(3 * k + b)) * Kc + idx] = h[k][b];
+        for (int b = 0; b < 3; b++) st_s<NT>(S2 + ((size_t)(3 * k + b)) * Kc + idx, h[k][b]);
 }
 
 // c2r of TY rows of one (component, layer) from column-contiguous spectra (FFTW's c2r:
@@ -556,6 +572,20 @@ int Engine<T>::lds_attr()
 // one launch of a demag kernel class (K_ROW_FWD, K_COL, K_ROW_INV on the fused paths;
 // K_ROW_FWD, K_COLF, K_PROD, K_COLI, K_ROW_INV on the split path); with a batch table set
 // (btab, bn members), one launch for all members, blockIdx.z the member
+// k_prod_offsets for NL layers: six or nine tensor slots, one context or a batch, cached or
+// non-temporal streams
+template <typename T, int NL>
+static void launch_prod_offsets(bool sym, bool bat, bool nt, dim3 g, hipStream_t s, const DemagOps<T>& o,
+                                const DemagOps<T>* tab, int fy, int nxh, int nslot)
+{
+#define KPO(S, B, N)                                                                                                 \
+    if (sym == S && bat == B && nt == N)                                                                             \
+        hipLaunchKernelGGL((k_prod_offsets<T, S, NL, B, N>), g, dim3(256), 0, s, o, tab, fy, nxh, nslot);
+    KPO(true, true, true) else KPO(true, true, false) else KPO(true, false, true) else KPO(true, false, false)
+    else KPO(false, true, true) else KPO(false, true, false) else KPO(false, false, true) else KPO(false, false, false)
+#undef KPO
+}
+
 template <typename T>
 int Engine<T>::launch_class(int cls)
 {
@@ -592,38 +622,27 @@ int Engine<T>::launch_class(int cls)
         const size_t Kc = (size_t)nxh * fy;
         const dim3 gp((unsigned)((Kc + 255) / 256), 1, z);
         const DemagOps<T> o = ops();
+        const bool sym = nslot == 6, bat = btab != nullptr, nt = prod_streams();
         if (toep && nl <= 8) {
-#define KPO(NLV)                                                                                                     \
-    if (nslot == 6 && btab)                                                                                          \
-        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV, true>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);   \
-    else if (nslot == 6)                                                                                             \
-        hipLaunchKernelGGL((k_prod_offsets<T, true, NLV>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);         \
-    else if (btab)                                                                                                   \
-        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV, true>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);  \
-    else                                                                                                             \
-        hipLaunchKernelGGL((k_prod_offsets<T, false, NLV>), gp, b256, 0, c->stream, o, btab, fy, nxh, nslot);
             switch (nl) {
-            case 1: KPO(1) break;
-            case 2: KPO(2) break;
-            case 3: KPO(3) break;
-            case 4: KPO(4) break;
-            case 5: KPO(5) break;
-            case 6: KPO(6) break;
-            case 7: KPO(7) break;
-            default: KPO(8) break;
+            case 1: launch_prod_offsets<T, 1>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 2: launch_prod_offsets<T, 2>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 3: launch_prod_offsets<T, 3>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 4: launch_prod_offsets<T, 4>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 5: launch_prod_offsets<T, 5>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 6: launch_prod_offsets<T, 6>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            case 7: launch_prod_offsets<T, 7>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
+            default: launch_prod_offsets<T, 8>(sym, bat, nt, gp, c->stream, o, btab, fy, nxh, nslot); break;
             }
-#undef KPO
-        } else if (nslot == 6 && btab)
-            hipLaunchKernelGGL((k_prod<T, true, 4, true>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
-                               (int)toep);
-        else if (nslot == 6)
-            hipLaunchKernelGGL((k_prod<T, true, 4>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot, (int)toep);
-        else if (btab)
-            hipLaunchKernelGGL((k_prod<T, false, 4, true>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
-                               (int)toep);
-        else
-            hipLaunchKernelGGL((k_prod<T, false, 4>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot,
-                               (int)toep);
+        } else {
+#define KPR(S, B, N)                                                                                                 \
+    if (sym == S && bat == B && nt == N)                                                                             \
+        hipLaunchKernelGGL((k_prod<T, S, 4, B, N>), gp, b256, 0, c->stream, o, btab, fy, nl, nxh, nslot, (int)toep);
+            KPR(true, true, true) else KPR(true, true, false) else KPR(true, false, true) else KPR(true, false, false)
+            else KPR(false, true, true) else KPR(false, true, false) else KPR(false, false, true)
+            else KPR(false, false, false)
+#undef KPR
+        }
         break;
     }
     case K_ROW_INV:

@@ -1071,6 +1071,11 @@ struct Engine : EngineBase {
     int bn = 1;
     DBuf<DemagOps<T>> btab_buf;
     DBuf<double> clk_buf;   // device clocks [t, step] of a batch's members
+    // The product kernel streams its operands (spectra in and out, the tensor) non-temporally
+    // when together they exceed the 256 MB last-level cache of the device, which they then
+    // could not stay in between launches anyway (2048x2048x8 in FP32: 2.3 GB, measured 391
+    // against 454 us); smaller problems keep them cacheable.
+    bool prod_streams() const { return S1.bytes() + S2.bytes() + Tspec.bytes() > (size_t)256 << 20; }
     // transposing row kernels: (ny/TY) * 3 n_lay blocks on a 1D grid padded to a multiple of 8
     unsigned rowT_blocks() const { return (unsigned)(((c->ny + TYr - 1) / TYr * 3 * c->nl + 7) / 8 * 8); }
     template <int MB> void launch_rowT_fwd(dim3 g, dim3 b);

@@ -56,6 +56,37 @@ inline void plan_passes(FFTPlan& p)
 template <typename T> struct C2 { T x, y; };
 template <typename T> struct C4 { C2<T> a, b; };
 
+// Global accesses of the demag's streams (spectra, tensor: each element read or written
+// once per pass, the arrays far larger than the caches), non-temporal when NT.
+template <bool NT, typename T>
+__device__ __forceinline__ C2<T> ld_s(const C2<T>* p)
+{
+    if (NT) return C2<T>{__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
+    return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_s(C2<T>* p, C2<T> v)
+{
+    if (NT) {
+        __builtin_nontemporal_store(v.x, &p->x);
+        __builtin_nontemporal_store(v.y, &p->y);
+    } else {
+        *p = v;
+    }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_s(C4<T>* p, C4<T> v)
+{
+    if (NT) {
+        __builtin_nontemporal_store(v.a.x, &p->a.x);
+        __builtin_nontemporal_store(v.a.y, &p->a.y);
+        __builtin_nontemporal_store(v.b.x, &p->b.x);
+        __builtin_nontemporal_store(v.b.y, &p->b.y);
+    } else {
+        *p = v;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
 {
