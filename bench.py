@@ -124,13 +124,14 @@ def kernel_bytes(med, P, offsets, fft=None):
     }
 
 
-def kernel_table(x, med, P, steps, t0, log, tag):
-    """Per-kernel HIP-event durations over `steps` steps and their algorithmic GB/s."""
+def kernel_table(x, med, P, steps, t0, log, tag, fft=None):
+    """Per-kernel HIP-event durations over `steps` steps and their algorithmic GB/s (fft:
+    the padded FFT grid of open boundaries)."""
     x.profile(True)
     x.step(steps, t0=t0, philox_seed=SEED, diag=False)
     st = x.kernel_stats()
     x.profile(False)
-    kb = kernel_bytes(med, P, x.get_tune("tensor_offsets"))
+    kb = kernel_bytes(med, P, x.get_tune("tensor_offsets"), fft=fft)
     tot = sum(ms for n, ms in st.values() if n > 0)
     kern = {}
     for k, (n, ms) in st.items():
@@ -547,12 +548,8 @@ def open_write(a, log, shapes, steps=20):
     x.step(a.warmup, philox_seed=SEED)
     el, d = timed(x, steps, a.warmup * DT_LLB)
     v = med.cells * steps / el
-    kern = kernel_table(x, med, 4, 5, (a.warmup + steps) * DT_LLB, log, "open write fp32")
-    kb = kernel_bytes(med, 4, x.get_tune("tensor_offsets"), fft=(2 * med.nx, 2 * med.ny))
-    for k in kern:
-        kern[k]["bytes"] = kb[k]
-        kern[k]["gbs"] = round(kb[k] / (kern[k]["avg_us"] * 1e-6) / 1e9, 1)
-        kern[k]["frac"] = round(kern[k]["gbs"] / HBM_PEAK_GBS, 4)
+    kern = kernel_table(x, med, 4, 5, (a.warmup + steps) * DT_LLB, log, "open write fp32",
+                        fft=(2 * med.nx, 2 * med.ny))
     out = {"workload": "2048x2048x8 LLB heat-assisted write, open in-plane boundaries (demag FFTs on the zero-padded "
                        "4096x4096 grid), FP32 form",
            "value": v, "unit": "cell-updates/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
