@@ -224,6 +224,21 @@ static inline std::vector<C2<double>> twiddle_table(int n)
     return w;
 }
 
+// The twiddle table of plan p in its per-pass layout (FFTPlan::toff): entry toff[i] +
+// (r-1) ns[i] + k holds w_n^(k r tstep[i]) of twiddle_table; n entries, the tail unused.
+static inline std::vector<C2<double>> plan_twiddles(const FFTPlan& p)
+{
+    const std::vector<C2<double>> w = twiddle_table(p.n);
+    std::vector<C2<double>> t(p.n, C2<double>{0.0, 0.0});
+    for (int i = 0; i < p.nfac; i++) {
+        const int ns = p.ns[i], R = p.fac[i];
+        if (ns == 1) continue;
+        for (int r = 1; r < R; r++)
+            for (int k = 0; k < ns; k++) t[p.toff[i] + (r - 1) * ns + k] = w[(size_t)k * r * p.tstep[i] % p.n];
+    }
+    return t;
+}
+
 // E: complex elements per thread of one pass, B*N <= E*threads, rounded up to 4..64
 static inline int pick_e(int batch, const FFTPlan& p, int threads)
 {
@@ -419,14 +434,10 @@ struct Engine : EngineBase {
     {
         // the FFT grid (padded to 2 nx x 2 ny for open boundaries)
         const int nx = c->fx, ny = c->fy, nxh = c->nxh, nl = c->nl;
-        auto wx = twiddle_table(nx), wy = twiddle_table(ny);
-        std::vector<C2<T>> tx(nx), ty(ny);
-        for (int i = 0; i < nx; i++) tx[i] = {(T)wx[i].x, (T)wx[i].y};
-        for (int i = 0; i < ny; i++) ty[i] = {(T)wy[i].x, (T)wy[i].y};
-        RCHK(twx.upload(tx.data(), nx, c->stream));
-        RCHK(twy.upload(ty.data(), ny, c->stream));
         prow = c->px;
         pcol = c->py;
+        RCHK(upload_twiddles(prow, twx));
+        RCHK(upload_twiddles(pcol, twy));
         // fused column path: all 3*n_lay columns of one kx in one workgroup's LDS
         lds_row = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN(3 * nx);
         lds_col = sizeof(C2<T>) * ny + sizeof(T) * 2 * LPN(3 * nl * ny);
@@ -462,6 +473,19 @@ struct Engine : EngineBase {
         // path's column kernel reads them as zero
         if (c->open) HIPCHK(hipMemsetAsync(S1.p, 0, S1.bytes(), c->stream));
         kg = 1;
+        return MRMG_OK;
+    }
+
+    // the twiddle table of plan q (plan_twiddles), rounded once to T, into d; the stream
+    // is synchronised so that no kernel in flight still reads the old table
+    int upload_twiddles(const FFTPlan& q, DBuf<C2<T>>& d)
+    {
+        const std::vector<C2<double>> w = plan_twiddles(q);
+        std::vector<C2<T>> t(w.size());
+        for (size_t i = 0; i < w.size(); i++) t[i] = {(T)w[i].x, (T)w[i].y};
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RCHK(d.upload(t.data(), t.size(), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // t is a local of this call
         return MRMG_OK;
     }
 
@@ -549,8 +573,15 @@ struct Engine : EngineBase {
             FFTPlan q;
             if (!factorize(k == "row_radix" ? c->fx : c->fy, q, v) || q.nfac > MRMG_MAX_FAC)
                 return set_err(MRMG_ERR_UNSUPPORTED, "%s %d: no plan", k.c_str(), v);
-            if (k == "row_radix") { prow = q; row_radix = v; }
-            else { pcol = q; col_radix = v; }
+            if (k == "row_radix") {
+                RCHK(upload_twiddles(q, twx));
+                prow = q;
+                row_radix = v;
+            } else {
+                RCHK(upload_twiddles(q, twy));
+                pcol = q;
+                col_radix = v;
+            }
         } else if (k == "row_block") {
             if (v != 2 && v != 4 && v != 8) return set_err(MRMG_ERR_ARG, "row_block must be 2, 4 or 8");
             const size_t l = sizeof(C2<T>) * c->fx + sizeof(T) * 2 * LPN((v / 2) * c->fx);

@@ -22,8 +22,11 @@
 // elements of an LDS array holding n padded entries
 #define LPN(n) ((n) + ((n) >> 4) + 1)
 
-// Pass p has radix fac[p] and span ns[p] (the product of the earlier radices).  The
-// divisions of a pass's index arithmetic use multipliers fixed when the plan is made:
+// Pass p has radix fac[p] and span ns[p] (the product of the earlier radices).  Its
+// twiddles w_n^(k r tstep[p]), k < ns[p], 0 < r < fac[p], sit in the plan's twiddle table at
+// toff[p] + (r-1) ns[p] + k: the lanes of a pass read consecutive entries (no LDS bank
+// conflicts, coalesced global loads).  The divisions of a pass's index arithmetic use
+// multipliers fixed when the plan is made:
 // x / d == umulhi(x, mnr or mns) for the indices of one workgroup (x * d < 2^32), d == 1
 // taken separately.  Reading them per pass also keeps the compiler from hoisting the
 // index arithmetic of every radix branch out of the pass loop.
@@ -33,6 +36,7 @@ struct FFTPlan {
     int fac[MRMG_MAX_FAC];
     int ns[MRMG_MAX_FAC];
     int tstep[MRMG_MAX_FAC];        // twiddle stride n / (ns * fac)
+    int toff[MRMG_MAX_FAC];         // the pass's twiddles in the plan's table
     unsigned mnr[MRMG_MAX_FAC];     // multiplier of n / fac
     unsigned mns[MRMG_MAX_FAC];     // multiplier of ns
 };
@@ -43,10 +47,12 @@ __device__ __forceinline__ int fdiv(int x, int d, unsigned m) { return d == 1 ? 
 // fill the per-pass fields of a plan whose n, nfac and fac are set
 inline void plan_passes(FFTPlan& p)
 {
-    int ns = 1;
+    int ns = 1, off = 0;
     for (int i = 0; i < p.nfac; i++) {
         p.ns[i] = ns;
         p.tstep[i] = p.n / (ns * p.fac[i]);
+        p.toff[i] = off;                      // n - fac[0] entries in all, fewer than n
+        if (ns > 1) off += ns * (p.fac[i] - 1);
         p.mnr[i] = fdiv_mul(p.n / p.fac[i]);
         p.mns[i] = fdiv_mul(ns);
         ns *= p.fac[i];
@@ -243,7 +249,7 @@ __device__ __forceinline__ void wave_sync()
 // (the workgroup, or one wave when WAVE: then only that wave synchronises).
 template <typename T, int R, int MAXB, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, int B, const C2<T>* __restrict__ tw,
-                                         int t, int nt, int tstep, unsigned mnr, unsigned mns)
+                                         int t, int nt, int toff, unsigned mnr, unsigned mns)
 {
     const int NR = N / R;
     const int total = B * NR;
@@ -275,12 +281,10 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
                 }
             }
             if (Ns > 1) {
-                const int kt = k * tstep;
-                int e = 0;
+                const C2<T>* twk = tw + toff + k;
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    e += kt;                       // k r tstep
-                    C2<T> w = tw[e];
+                    C2<T> w = twk[(r - 1) * Ns];   // w_N^(k r tstep)
                     if (INV) w.y = -w.y;
                     v[q][r] = cmul(v[q][r], w);
                 }
@@ -324,7 +328,7 @@ __device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* _
     const int N = plan.n;
     const int t = threadIdx.x, nt = blockDim.x;
     for (int p = 0; p < plan.nfac; p++) {
-        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.tstep[p];
+        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.toff[p];
         const unsigned a = plan.mnr[p], c = plan.mns[p];
         if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
         else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV>(re, im, off, N, Ns, B, tw, t, nt, ts, a, c);
@@ -350,7 +354,7 @@ __device__ void lds_fft_wave(T* re, T* im, int B, const FFTPlan& plan, const C2<
     const int N = plan.n;
     const int o = off + b0 * N;
     for (int p = 0; p < plan.nfac; p++) {
-        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.tstep[p];
+        const int R = plan.fac[p], Ns = plan.ns[p], ts = plan.toff[p];
         const unsigned a = plan.mnr[p], c = plan.mns[p];
         if (R == 16) fft_pass<T, 16, (E + 15) / 16, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);
         else if (R == 8) fft_pass<T, 8, (E + 7) / 8, INV, true>(re, im, o, N, Ns, nb, tw, lane, 64, ts, a, c);

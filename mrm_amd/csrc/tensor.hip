@@ -242,7 +242,7 @@ int Engine<T>::set_tensor_planes(const double* d_real, bool toeplitz)
     const int P = 9 * nq;
     DBuf<C2<double>> TC, tdx, tdy;
     RCHK(TC.alloc((size_t)P * nxh * ny));
-    auto wx = twiddle_table(nx), wy = twiddle_table(ny);
+    auto wx = plan_twiddles(c->px), wy = plan_twiddles(c->py);
     RCHK(tdx.upload(wx.data(), nx, c->stream));
     RCHK(tdy.upload(wy.data(), ny, c->stream));
     const int mbr = pick_e(1, c->px, 256), mbc = pick_e(1, c->py, 256);
