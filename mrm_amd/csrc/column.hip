@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(DemagOps<T
         lds_fft_wave<T, E, false>(re, im, ncol, py, tw);
         __syncthreads();
     } else {
-        lds_fft<T, E, false>(re, im, ncol, py, tw);
+        lds_fft<T, E, false, true>(re, im, ncol, py, tw);
     }
     // product: work item (ko, ky); up to 4 items per thread held in registers.  The tensor
     // values of KSB source layers are loaded before any of them is used, so a thread waits
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(BIG ? 1024 : COL_MAX_THREADS) k_col(DemagOps<T
         lds_fft_wave<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
         __syncthreads();
     } else {
-        lds_fft<T, E, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
+        lds_fft<T, E, true, true>(re, im, 3 * (ko1 - ko0), py, tw, e0);
     }
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const int idx = e0 + i;

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_row_fwd(DemagOps<T> one, const DemagOps
     const bool has1 = y1 < ny;
     runs_to_lds<T, 3>(R, Mg, G, 0, k, ny, y0, has1 ? 2 : 1, 1, nx, re, im, roff, tw, twx);
     __syncthreads();
-    lds_fft<T, E, false>(re, im, 3, px, tw);
+    lds_fft<T, E, false, true>(re, im, 3, px, tw);
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
         const int c = idx / nxh, kx = idx - c * nxh;
         const int kn = kx == 0 ? 0 : nx - kx;
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps
         tw, twx, nx);
     __syncthreads();
     if (!CELLS && threadIdx.x <= (has1 ? 2 : 1)) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
-    lds_fft<T, E, true>(re, im, 3, px, tw);
+    lds_fft<T, E, true, true>(re, im, 3, px, tw);
     if (!CELLS) {
         lds_to_runs<T, 3>(R, ra, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
         return;
@@ -269,13 +269,15 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(DemagOps<T> one, const DemagOp
         lds_fft_wave<T, E, false>(re, im, B, px, tw);
         __syncthreads();
     } else {
-        lds_fft<T, E, false>(re, im, B, px, tw);
+        lds_fft<T, E, false, true>(re, im, B, px, tw);
     }
     // one thread per (kx, row pair): both rows of the pair come from the same four LDS
-    // values, and go out as one 2-element store where the pair is aligned (sy even)
+    // values, and go out as one 2-element store where the pair is aligned (sy even).
+    // B = TY/2 is 1, 2 or 4: kx and p are a shift and a mask of idx
+    const int lb = B == 4 ? 2 : B == 2 ? 1 : 0;
     C2<T>* dst = S + (size_t)kc * nxh * sy;
     for (int idx = threadIdx.x; idx < nxh * B; idx += blockDim.x) {
-        const int kx = idx / B, p = idx - kx * B;
+        const int kx = idx >> lb, p = idx & (B - 1);
         const int y = y0 + 2 * p;
         if (y >= ny) continue;
         const int kn = kx == 0 ? 0 : nx - kx;
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(256) k_colT(DemagOps<T> one, const DemagOps<T>
             nb * ny, [&](int idx) { return idx % ny < nyv ? base[idx] : C2<T>{0, 0}; },
             [&](int idx, C2<T> v) { re[LP(idx)] = v.x; im[LP(idx)] = v.y; });
     __syncthreads();
-    lds_fft<T, E, INV>(re, im, nb, py, twy);
+    lds_fft<T, E, INV, true>(re, im, nb, py, twy);
     for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x)
         if (!INV || nyv == ny || idx % ny < nyv) base[idx] = C2<T>{re[LP(idx)], im[LP(idx)]};
 }
@@ -475,16 +477,18 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
     const int nrows = min(TY, ny - y0);
     RunsAhead ra;
     if (!CELLS) ra.prefetch(R, k, ny, y0, nrows);
-    // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror
+    // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror.  B =
+    // TY/2 is 1, 2 or 4: kx and p are a shift and a mask of idx
+    const int lb = B == 4 ? 2 : B == 2 ? 1 : 0;
     staged_load<(E < 4 ? E : 4)>(
         nxh * B,
         [&](int idx) {
-            const int kx = idx / B, p = idx - kx * B;
+            const int kx = idx >> lb, p = idx & (B - 1);
             const int ya = y0 + 2 * p, yb = ya + 1;
             return C4<T>{src[(size_t)kx * sy + (ya < ny ? ya : ny - 1)], src[(size_t)kx * sy + (yb < ny ? yb : ny - 1)]};
         },
         [&](int idx, C4<T> v) {
-            const int kx = idx / B, p = idx - kx * B;
+            const int kx = idx >> lb, p = idx & (B - 1);
             const int ya = y0 + 2 * p;
             C2<T> A = ya < ny ? v.a : C2<T>{0, 0}, Bv = ya + 1 < ny ? v.b : C2<T>{0, 0};
             if (kx == 0 || 2 * kx == nx) { A.y = 0; Bv.y = 0; }
@@ -502,7 +506,7 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
         lds_fft_wave<T, E, true>(re, im, B, px, tw);
         __syncthreads();
     } else {
-        lds_fft<T, E, true>(re, im, B, px, tw);
+        lds_fft<T, E, true, true>(re, im, B, px, tw);
     }
     if (!CELLS) {
         lds_to_runs<T, 1>(R, ra, Hr, c, nrows, B, nx, re, im, roff);

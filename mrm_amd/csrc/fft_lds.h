@@ -318,13 +318,112 @@ __device__ __forceinline__ void fft_pass(T* re, T* im, int off, int N, int Ns, i
     else __syncthreads();
 }
 
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// One pass of the power-of-two form (lds_fft_p2): fft_pass with the radix R, the span NS,
+// the length N and the first radix R0 fixed at compile time.  The read stride N/R and the
+// span are multiples of 16 (or the span is 1), so with the LDS padding one element's
+// neighbours in a butterfly sit at constant distances from its own padded slot: a pass
+// computes one base address per butterfly and reads, twiddles and writes through
+// immediate offsets.  The arithmetic is fft_pass's, operation for operation.
+template <typename T, int N, int R, int NS, int R0, int MAXB, bool INV>
+__device__ __forceinline__ void fft_pass_p2(T* re, T* im, int off, int B, const C2<T>* __restrict__ tw, int t, int nt)
+{
+    constexpr int NR = N / R, LNR = ilog2c(NR), LNS = ilog2c(NS), LR = ilog2c(R), LN = ilog2c(N);
+    static_assert(NR % 16 == 0 && (NS == 1 || NS % 16 == 0), "padded strides must be constant");
+    constexpr int RS = NR + NR / 16, WS = NS == 1 ? 1 : NS + NS / 16, TOFF = NS == 1 ? 0 : NS - R0;
+    const int total = B << LNR;
+    C2<T> v[MAXB][R];
+    int dst[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int idx = t + q * nt;
+        dst[q] = -1;
+        if (idx < total) {
+            const int b = idx >> LNR, j = idx & (NR - 1);
+            const int jq = j >> LNS, k = j & (NS - 1);
+            const int base = off + (b << LN);
+            const int l0 = LP(base + j);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                v[q][r].x = re[l0 + r * RS];
+                v[q][r].y = im[l0 + r * RS];
+            }
+            if (NS > 1) {
+                const C2<T>* twk = tw + TOFF + k;
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C2<T> w = twk[(r - 1) * NS];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            Dft<T, R, INV>::run(v[q]);
+            dst[q] = LP(base + (jq << (LNS + LR)) + k);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                re[dst[q] + r * WS] = v[q][r].x;
+                im[dst[q] + r * WS] = v[q][r].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int NS, int E, bool INV>
+__device__ __forceinline__ void p2_passes(T* re, T* im, int off, int B, const C2<T>* __restrict__ tw, int t, int nt)
+{
+    if constexpr (NS < N) {
+        constexpr int REM = N / NS, R = REM % 16 == 0 ? 16 : REM;
+        fft_pass_p2<T, N, R, NS, 16, (E + R - 1) / R, INV>(re, im, off, B, tw, t, nt);
+        p2_passes<T, N, NS * R, E, INV>(re, im, off, B, tw, t, nt);
+    }
+}
+
+// lds_fft for the plans of factorize(n, 16) with n a power of two from 256 to 4096 (radix-16
+// passes, then one of 8, 4 or 2): the same transform, bit for bit, with the index
+// arithmetic fixed at compile time.  Returns false (and does nothing) for other plans or
+// an offset that is not a multiple of 16; every thread of the workgroup takes the same
+// branch.
+template <typename T, int E, bool INV>
+__device__ __forceinline__ bool lds_fft_p2(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw,
+                                           int off = 0)
+{
+    if ((off & 15) != 0 || plan.fac[0] != 16) return false;
+    const int t = threadIdx.x, nt = blockDim.x;
+    switch (plan.n) {
+    case 256: if constexpr (256 <= E * 256) { p2_passes<T, 256, 1, E, INV>(re, im, off, B, tw, t, nt); return true; }
+        break;
+    case 512: if constexpr (512 <= E * 256) { p2_passes<T, 512, 1, E, INV>(re, im, off, B, tw, t, nt); return true; }
+        break;
+    case 1024: if constexpr (1024 <= E * 256) { p2_passes<T, 1024, 1, E, INV>(re, im, off, B, tw, t, nt); return true; }
+        break;
+    case 2048: if constexpr (2048 <= E * 256) { p2_passes<T, 2048, 1, E, INV>(re, im, off, B, tw, t, nt); return true; }
+        break;
+    case 4096: if constexpr (4096 <= E * 256) { p2_passes<T, 4096, 1, E, INV>(re, im, off, B, tw, t, nt); return true; }
+        break;
+    default: break;
+    }
+    return false;
+}
+
 // Full transform: every thread of the workgroup must call it (it synchronises).
 // Element i of the B*N array sits at LDS slot LP(i).
 // E bounds the complex elements one thread holds in a pass: B*N <= E*blockDim.x.
 // off: element index of the first transform within the padded LDS arrays.
-template <typename T, int E, bool INV>
-__device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw, int off = 0)
+// P2: take lds_fft_p2 where it applies.
+template <typename T, int E, bool INV, bool P2 = false>
+__device__ __forceinline__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw,
+                                        int off = 0)
 {
+    if constexpr (P2)
+        if (lds_fft_p2<T, E, INV>(re, im, B, plan, tw, off)) return;
     const int N = plan.n;
     const int t = threadIdx.x, nt = blockDim.x;
     for (int p = 0; p < plan.nfac; p++) {
@@ -344,7 +443,8 @@ __device__ void lds_fft(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* _
 // (the runs are disjoint).  E bounds the elements one lane holds: ceil(B/waves)*N <= 64*E.
 // The caller synchronises the workgroup before any wave reads another wave's transforms.
 template <typename T, int E, bool INV>
-__device__ void lds_fft_wave(T* re, T* im, int B, const FFTPlan& plan, const C2<T>* __restrict__ tw, int off = 0)
+__device__ __forceinline__ void lds_fft_wave(T* re, T* im, int B, const FFTPlan& plan,
+                                             const C2<T>* __restrict__ tw, int off = 0)
 {
     const int nw = (int)blockDim.x >> 6, w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const int bpw = (B + nw - 1) / nw;

@@ -464,6 +464,40 @@ def test_launch_shapes_do_not_change_results():
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
 
 
+def test_launch_shapes_agree_at_power_of_two_sizes():
+    """At 256 x 256 the FFTs take the compile-time power-of-two passes: the fused, split and
+    column-contiguous paths still agree bit for bit (row radix 16), and the generic passes
+    (row and column radix 8) agree with them to rounding."""
+    say("launch shapes 256x256x2")
+    med = media.make_media(256, 256, n_lay=2, seed=4)
+    N = None
+    out = {}
+    for cfg in ({}, {"col_path": 1}, {"col_path": 2}, {"col_path": 3}, {"col_path": 1, "row_block": 2},
+                {"row_radix": 8, "col_radix": 8}):
+        x = Llg(256, 256, 2, med.dx, med.dy, med.dz)
+        if N is None:
+            N = x.build_tensor(num_images=1, return_real=True)
+        else:
+            x.set_tensor(N)
+        x.set_media(med)
+        for k, u in enumerate(media.ramp_field(2)):
+            x.set_ufield(k, u)
+        for key, v in cfg.items():
+            x.tune(key, v)
+        x.set_llg(dt=5e-14)
+        x.step(10, philox_seed=3)
+        out[str(cfg)] = [x.get_state(k)[0] for k in range(2)]
+        x.close()
+    base = out["{}"]
+    for key, ms in out.items():
+        d = max(np.abs(p - q).max() for p, q in zip(base, ms))
+        say(f"  {key}: max|dm| {d:.3g}")
+        if "radix" in key:
+            assert d <= 1e-12, key
+        else:
+            assert d == 0.0, key
+
+
 def test_step_rejects_bad_arguments():
     med = media.config_media("cfg0", nx=16, ny=16)
     x = Llg(16, 16, 1, med.dx, med.dy, med.dz)
