@@ -1,7 +1,8 @@
 """Build recipe of libmrmg.so (hipcc, gfx950).  Run by __graft_entry__.build().
 
 The library is several translation units compiled in parallel and linked into one shared
-object; the demag units are compiled once per precision form."""
+object; the demag units are compiled once per precision form, the column unit twice per
+form (two halves of its instantiations)."""
 import os
 import subprocess
 from concurrent.futures import ThreadPoolExecutor
@@ -11,7 +12,10 @@ CSRC = os.path.join(HERE, "csrc")
 # (source, object, extra flags): the demag units once per precision form
 UNITS = [("mrmg.hip", "mrmg.o", []), ("grain.hip", "grain.o", []), ("tensor.hip", "tensor.o", []),
          ("demag.hip", "demag_f32.o", ["-DMRMG_FORM=float"]), ("demag.hip", "demag_f64.o", ["-DMRMG_FORM=double"]),
-         ("column.hip", "column_f32.o", ["-DMRMG_FORM=float"]), ("column.hip", "column_f64.o", ["-DMRMG_FORM=double"])]
+         ("column.hip", "column_f32a.o", ["-DMRMG_FORM=float", "-DMRMG_COL_PART=1"]),
+         ("column.hip", "column_f32b.o", ["-DMRMG_FORM=float", "-DMRMG_COL_PART=2"]),
+         ("column.hip", "column_f64a.o", ["-DMRMG_FORM=double", "-DMRMG_COL_PART=1"]),
+         ("column.hip", "column_f64b.o", ["-DMRMG_FORM=double", "-DMRMG_COL_PART=2"])]
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libmrmg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

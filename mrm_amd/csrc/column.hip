@@ -237,6 +237,9 @@ template <int MB> int Engine<T>::lds_attr_col()
     return MRMG_OK;
 }
 
+// The instantiations per elements-per-lane bound; MRMG_COL_PART 1 or 2 compiles one half
+// of them (the two halves are separate objects, built in parallel), unset both.
+#if !defined(MRMG_COL_PART) || MRMG_COL_PART == 1
 template void Engine<MRMG_FORM>::launch_col_sym<4>(int);
 template void Engine<MRMG_FORM>::launch_col_gen<4>(int);
 template int Engine<MRMG_FORM>::lds_attr_col<4>();
@@ -246,9 +249,12 @@ template int Engine<MRMG_FORM>::lds_attr_col<8>();
 template void Engine<MRMG_FORM>::launch_col_sym<16>(int);
 template void Engine<MRMG_FORM>::launch_col_gen<16>(int);
 template int Engine<MRMG_FORM>::lds_attr_col<16>();
+#endif
+#if !defined(MRMG_COL_PART) || MRMG_COL_PART == 2
 template void Engine<MRMG_FORM>::launch_col_sym<32>(int);
 template void Engine<MRMG_FORM>::launch_col_gen<32>(int);
 template int Engine<MRMG_FORM>::lds_attr_col<32>();
 template void Engine<MRMG_FORM>::launch_col_sym<64>(int);
 template void Engine<MRMG_FORM>::launch_col_gen<64>(int);
 template int Engine<MRMG_FORM>::lds_attr_col<64>();
+#endif
