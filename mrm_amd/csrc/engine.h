@@ -447,7 +447,7 @@ struct Engine : EngineBase {
         const size_t lim = 160 * 1024;
         split_ok = false;
         for (int ty = 8; ty >= 2; ty /= 2) {
-            lds_rowT = sizeof(C2<T>) * nx + sizeof(T) * 2 * LPN((ty / 2) * nx);
+            lds_rowT = sizeof(T) * 2 * LPN((ty / 2) * nx);   // the rows only: twiddles come from L2
             if (lds_rowT <= lim && pick_e(ty / 2, c->px, 256) > 0 && (lds_rowT <= 64 * 1024 || ty == 2)) {
                 TYr = ty;
                 split_ok = true;
@@ -584,7 +584,7 @@ struct Engine : EngineBase {
             }
         } else if (k == "row_block") {
             if (v != 2 && v != 4 && v != 8) return set_err(MRMG_ERR_ARG, "row_block must be 2, 4 or 8");
-            const size_t l = sizeof(C2<T>) * c->fx + sizeof(T) * 2 * LPN((v / 2) * c->fx);
+            const size_t l = sizeof(T) * 2 * LPN((v / 2) * c->fx);
             if (l > 160 * 1024 || pick_e(v / 2, c->px, 256) < 0)
                 return set_err(MRMG_ERR_UNSUPPORTED, "row_block %d does not fit LDS", v);
             const int ot = TYr;
