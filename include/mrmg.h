@@ -386,9 +386,9 @@ int mrmg_profile(mrmg_ctx *ctx, int enable);
  * (largest Stockham radix: 4, 8, 16; radix 16 takes the compile-time passes for
  * power-of-two lengths from 256 to 4096), "wave_fft" (0, 1: wave-local FFTs),
  * "lanes_per_grain" (1, 2, 4, 8, 16).  The thread counts, "col_groups", "row_block" and
- * "col_path" leave results unchanged bit for bit; the others may change the order of
- * floating-point operations (FFT factorisation, gather sums) and so the last bits.  A
- * combination that does not fit the grid is refused. */
+ * (with "wave_fft" 0) "col_path" leave results unchanged bit for bit; the others may
+ * change the order of floating-point operations (FFT factorisation, gather sums) and so
+ * the last bits.  A combination that does not fit the grid is refused. */
 int mrmg_tune(mrmg_ctx *ctx, const char *key, int value);
 /* The value of a tunable, or of the read-only key "tensor_offsets": 1 when the stored
  * tensor spectra are the 2 n_lay - 1 z-offset blocks (the tensor depends on the layer pair
