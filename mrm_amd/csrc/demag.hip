@@ -126,7 +126,7 @@ __device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead& A, doub
 #pragma unroll
                     for (int j = 0; j < MRMG_UF; j++) a[j] = src[LP(b + min(t + j, len - 1))];
 #pragma unroll
-                    for (int j = 0; j < MRMG_UF; j++) s += t + j < len ? (double)a[j] : 0.0;
+                    for (int j = 0; j < MRMG_UF; j++) s += (double)(t + j < len ? a[j] : (T)0);
                 }
                 Hr[(size_t)(c0 + ci) * R.nruns + A.gm[u]] = s;
             }
