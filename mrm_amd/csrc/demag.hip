@@ -20,7 +20,8 @@
 // (ci*np + p)*nx + x (get_mag, data.f90:344-353: Ms ms_scale m on a grain's cells, 0 on
 // non-magnetic cells).  Mg[c][g] is the grain's cell value.  Also stages the row
 // twiddles.  roff (shared, nrows+1 entries) keeps the block's run offsets for the store.
-// The first RU runs' loads are issued before LDS is cleared and overlap it.
+// The first RU runs' loads are issued before LDS is cleared and overlap it.  im must be
+// re + LPN(np * NC * nx), as every caller lays them out.
 template <typename T, int NC>
 __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restrict__ Mg, int G, int c0, int k, int ny,
                                             int y0, int nrows, int np, int nx, T* re, T* im, int* roff, C2<T>* tw,
@@ -48,18 +49,28 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
     if (tid <= nrows) roff[tid] = ro[tid];
     if (tw) for (int i = tid; i < nx; i += nt) tw[i] = twx[i];   // null: no LDS copy wanted
     const int n = NC * np * nx;
-    for (int i = tid; i < n; i += nt) {
-        re[LP(i)] = (T)0;
-        im[LP(i)] = (T)0;
+    if ((reinterpret_cast<size_t>(re) & 15) == 0) {
+        // re and im (= re + LPN(n)) are one span of 2 LPN(n) elements: cleared whole,
+        // padding slots included, with 16-byte stores and a scalar tail
+        const int tot = 2 * LPN(n), per = 16 / (int)sizeof(T), n16 = tot / per;
+        uint4* z = reinterpret_cast<uint4*>(re);
+        for (int i = tid; i < n16; i += nt) z[i] = uint4{0u, 0u, 0u, 0u};
+        for (int i = n16 * per + tid; i < tot; i += nt) re[i] = (T)0;
+    } else {
+        for (int i = tid; i < n; i += nt) {
+            re[LP(i)] = (T)0;
+            im[LP(i)] = (T)0;
+        }
     }
     __syncthreads();
+    // a thread's runs come in increasing q, so its row index yy only moves forward
+    int yy = 0;
     for (int qb = q0 + tid; qb < q1; qb += MRMG_RU * nt) {
         if (qb != q0 + tid) fetch(qb);
 #pragma unroll
         for (int u = 0; u < MRMG_RU; u++) {
             const int q = qb + u * nt;
             if (q >= q1) break;
-            int yy = 0;
             while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
             const int i0 = geo[u] & 0xffff, len = geo[u] >> 16;
             T* dst = (yy & 1) ? im : re;
@@ -105,13 +116,13 @@ __device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead& A, doub
                                             int nrows, int np, int nx, const T* re, const T* im, const int* roff)
 {
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    int yy = 0;   // increasing q: the row index only moves forward
     for (int qb = A.q0 + tid; qb < A.q1; qb += MRMG_RU * nt) {
         if (qb != A.q0 + tid) A.fetch(R, qb, nt);
 #pragma unroll
         for (int u = 0; u < MRMG_RU; u++) {
             const int q = qb + u * nt;
             if (q >= A.q1) break;
-            int yy = 0;
             while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
             const int i0 = A.geo[u] & 0xffff, len = A.geo[u] >> 16;
             const T* src = (yy & 1) ? im : re;

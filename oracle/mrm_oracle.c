@@ -30,12 +30,15 @@
  * cube; the FFT against numpy.fft; the demag product against a brute-force circular
  * sum; the pushes against the analytic macrospin solution (2nd-order convergence for
  * Heun); exchange against reciprocity; thermal sigma against the fluctuation-dissipation
- * value.  The adaptive-dt rejection path (push_m with rot_ratio > 1) is pinned only by
- * invariants: parity unpinned for its exact retry sequence.  push_llb is pinned by its
- * limits (Langevin series, the mean-field fixed point |m| = L(beta(H + lambda |m|))
- * reached at T > 0 without noise, T -> 0 reducing to the LLG precession/damping, the
- * longitudinal relaxation rate), not by a reference trajectory: parity unpinned for the
- * exact thermal LLB trajectory.
+ * value.  The adaptive-dt rejection path (push_m with rot_ratio > 1) is pinned by the
+ * closed-form rotation of one undamped precession step (rejection count, shrink factor,
+ * next-dt factor, elapsed time: tests/test_oracle_pins.py); the stale stage-2 field a
+ * retry reuses (llg.f90:1205-1268) is invisible there, so that detail is parity unpinned.
+ * push_llb is pinned by its limits (Langevin series, the mean-field fixed point
+ * |m| = L(beta(H + lambda |m|)) reached at T > 0 without noise, T -> 0 reducing to the
+ * LLG precession/damping, the longitudinal relaxation rate) and, with noise, by the
+ * closed-form stationary variances of its linearised update (Heun and rotational
+ * branches, tests/test_llb_thermal_equilibrium.py, with planted mistakes that fail).
  */
 #include <math.h>
 #include <stdint.h>
