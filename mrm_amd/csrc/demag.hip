@@ -12,7 +12,7 @@
 // descriptors (and grain values) of RU runs are loaded before any is used, so the
 // dependent loads of a run (descriptor, then grain value) are waited for once per RU runs
 // instead of once per run.
-#define MRMG_RU 4
+#define MRMG_RU 8
 #define MRMG_UF 4
 
 // The cell magnetisation of rows y0 .. y0+nrows-1 of layer k into LDS, component c0+ci of
