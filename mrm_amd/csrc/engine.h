@@ -1322,7 +1322,9 @@ struct Engine : EngineBase {
         const int chunk = pre ? 16 : 64;
         const size_t per = 3 * (size_t)G;
         if (p->noise == MRMG_NOISE_INJECT || pre) {
-            const size_t need = per * std::min(chunk, std::max(p->n_steps, 1));
+            // Philox: always a whole chunk, so that calls of any length reuse one buffer
+            // (no allocation inside a caller's loop of short calls)
+            const size_t need = per * (pre ? chunk : std::min(chunk, std::max(p->n_steps, 1)));
             if (noise_buf.n < need) RCHK(noise_buf.alloc(need));
         }
         double t = p->t0;
