@@ -35,7 +35,7 @@ import numpy as np  # noqa: E402
 
 from mrm_amd import abi, media  # noqa: E402
 from mrm_amd.llg import Llg  # noqa: E402
-from mrm_amd.tune import autotune  # noqa: E402
+from mrm_amd.tune import autotune, tune_keys  # noqa: E402
 
 METRIC = "LLG cell-updates/s FP32 vs FP64 at max|Δm|≤1e-4; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
@@ -538,13 +538,17 @@ def open_write(a, log, shapes, steps=20):
     """The heat-assisted write with open in-plane boundaries: the same 2048 x 2048 x 8
     problem, its demag FFTs on the zero-padded 4096 x 4096 grid (z coupling through the 15
     layer-offset tensor blocks), FP32 form; the grain kernels take the lanes per grain the
-    periodic run's autotuner chose, the FFT kernels their default shapes."""
+    periodic run's autotuner chose, the FFT kernels their default shapes except the rows per
+    row workgroup, timed here over 2, 4 and 8 (the padded rows are twice as long, which
+    moves the best value)."""
     med = media.config_media("cfg4")
     t = time.perf_counter()
     x = llb_context(med, abi.MRMG_FP32, open=True)
     setup = time.perf_counter() - t
     if shapes.get("lanes_per_grain"):
         x.tune("lanes_per_grain", shapes["lanes_per_grain"])
+    if not a.no_tune:
+        tune_keys(x, {"row_block": [2, 4, 8]}, steps=2)
     x.step(a.warmup, philox_seed=SEED)
     el, d = timed(x, steps, a.warmup * DT_LLB)
     v = med.cells * steps / el
@@ -554,7 +558,7 @@ def open_write(a, log, shapes, steps=20):
                        "4096x4096 grid), FP32 form",
            "value": v, "unit": "cell-updates/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
            "setup_s": round(setup, 2), "footprint_bytes": x.footprint(), "finite": bool(np.isfinite(d.mmag)),
-           "tensor_offsets": x.get_tune("tensor_offsets"), "kernels": kern}
+           "tensor_offsets": x.get_tune("tensor_offsets"), "row_block": x.get_tune("row_block"), "kernels": kern}
     x.close()
     log(f"open write fp32: {out['ms_per_step']:.3f} ms/step, {v:.4e} cell-updates/s (setup {setup:.1f} s)")
     return out

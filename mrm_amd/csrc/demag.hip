@@ -11,8 +11,11 @@
 // A thread takes the runs q0 + tid, q0 + tid + nt, ... of its block, RU at a time: the
 // descriptors (and grain values) of RU runs are loaded before any is used, so the
 // dependent loads of a run (descriptor, then grain value) are waited for once per RU runs
-// instead of once per run.
-#define MRMG_RU 8
+// instead of once per run.  RU is 8 for the one-component row kernels on long rows (E >= 16
+// FFT elements per thread, where a thread has several runs) and 4 otherwise: the c2r holds
+// its prefetched descriptors across the FFT, and where they are not used (short rows, or
+// the three rows of the fused kernels) they cost occupancy.
+#define MRMG_RU(NC, E) (((NC) == 1 && (E) >= 16) ? 8 : 4)
 #define MRMG_UF 4
 
 // The cell magnetisation of rows y0 .. y0+nrows-1 of layer k into LDS, component c0+ci of
@@ -22,7 +25,7 @@
 // twiddles.  roff (shared, nrows+1 entries) keeps the block's run offsets for the store.
 // The first RU runs' loads are issued before LDS is cleared and overlap it.  im must be
 // re + LPN(np * NC * nx), as every caller lays them out.
-template <typename T, int NC>
+template <typename T, int NC, int RU>
 __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restrict__ Mg, int G, int c0, int k, int ny,
                                             int y0, int nrows, int np, int nx, T* re, T* im, int* roff, C2<T>* tw,
                                             const C2<T>* __restrict__ twx)
@@ -30,17 +33,17 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
     const int* ro = R.row_off + k * ny + y0;          // y0 + nrows <= ny
     const int q0 = ro[0], q1 = ro[nrows];
-    int geo[MRMG_RU];
-    T v[MRMG_RU][NC];
+    int geo[RU];
+    T v[RU][NC];
     auto fetch = [&](int qb) {
-        int g[MRMG_RU];
+        int g[RU];
 #pragma unroll
-        for (int u = 0; u < MRMG_RU; u++) {
+        for (int u = 0; u < RU; u++) {
             const int q = qb + u * nt;
             if (q < q1) { geo[u] = R.geo[q]; g[u] = R.grain[q]; }
         }
 #pragma unroll
-        for (int u = 0; u < MRMG_RU; u++)
+        for (int u = 0; u < RU; u++)
             if (qb + u * nt < q1)
 #pragma unroll
                 for (int ci = 0; ci < NC; ci++) v[u][ci] = Mg[(size_t)(c0 + ci) * G + g[u]];
@@ -65,10 +68,10 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
     __syncthreads();
     // a thread's runs come in increasing q, so its row index yy only moves forward
     int yy = 0;
-    for (int qb = q0 + tid; qb < q1; qb += MRMG_RU * nt) {
+    for (int qb = q0 + tid; qb < q1; qb += RU * nt) {
         if (qb != q0 + tid) fetch(qb);
 #pragma unroll
-        for (int u = 0; u < MRMG_RU; u++) {
+        for (int u = 0; u < RU; u++) {
             const int q = qb + u * nt;
             if (q >= q1) break;
             while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
@@ -89,13 +92,14 @@ __device__ __forceinline__ void runs_to_lds(const RowRuns& R, const T* __restric
 
 // The run descriptors a thread sums in lds_to_runs, its first RU of them loaded ahead
 // (at kernel start, before the spectra and the FFT) by prefetch.
+template <int RU>
 struct RunsAhead {
     int q0, q1;
-    int geo[MRMG_RU], gm[MRMG_RU];
+    int geo[RU], gm[RU];
     __device__ __forceinline__ void fetch(const RowRuns& R, int qb, int nt)
     {
 #pragma unroll
-        for (int u = 0; u < MRMG_RU; u++) {
+        for (int u = 0; u < RU; u++) {
             const int q = qb + u * nt;
             if (q < q1) { geo[u] = R.geo[q]; gm[u] = R.gm[q]; }
         }
@@ -111,16 +115,16 @@ struct RunsAhead {
 
 // The cell field of the same rows, summed over each run (in x order, in FP64) into
 // Hr[c][run] (c = c0+ci), with the layout of runs_to_lds.
-template <typename T, int NC>
-__device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead& A, double* __restrict__ Hr, int c0,
+template <typename T, int NC, int RU>
+__device__ __forceinline__ void lds_to_runs(const RowRuns& R, RunsAhead<RU>& A, double* __restrict__ Hr, int c0,
                                             int nrows, int np, int nx, const T* re, const T* im, const int* roff)
 {
     const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
     int yy = 0;   // increasing q: the row index only moves forward
-    for (int qb = A.q0 + tid; qb < A.q1; qb += MRMG_RU * nt) {
+    for (int qb = A.q0 + tid; qb < A.q1; qb += RU * nt) {
         if (qb != A.q0 + tid) A.fetch(R, qb, nt);
 #pragma unroll
-        for (int u = 0; u < MRMG_RU; u++) {
+        for (int u = 0; u < RU; u++) {
             const int q = qb + u * nt;
             if (q >= A.q1) break;
             while (yy + 1 < nrows && q >= roff[yy + 1]) yy++;
@@ -170,7 +174,7 @@ __global__ void __launch_bounds__(256) k_row_fwd(DemagOps<T> one, const DemagOps
     T* im = re + LPN(3 * nx);
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
-    runs_to_lds<T, 3>(R, Mg, G, 0, k, ny, y0, has1 ? 2 : 1, 1, nx, re, im, roff, tw, twx);
+    runs_to_lds<T, 3, MRMG_RU(3, E)>(R, Mg, G, 0, k, ny, y0, has1 ? 2 : 1, 1, nx, re, im, roff, tw, twx);
     __syncthreads();
     lds_fft<T, E, false, true>(re, im, 3, px, tw);
     for (int idx = threadIdx.x; idx < 3 * nxh; idx += blockDim.x) {
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps
     T* im = re + LPN(3 * nx);
     const int y0 = 2 * blockIdx.x, y1 = y0 + 1, k = blockIdx.y;
     const bool has1 = y1 < ny;
-    RunsAhead ra;
+    RunsAhead<MRMG_RU(3, E)> ra;
     if (!CELLS) ra.prefetch(R, k, ny, y0, has1 ? 2 : 1);
     staged_load<(E < 4 ? E : 4)>(
         3 * nxh,
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(256) k_row_inv(DemagOps<T> one, const DemagOps
     if (!CELLS && threadIdx.x <= (has1 ? 2 : 1)) roff[threadIdx.x] = R.row_off[k * ny + min(y0 + (int)threadIdx.x, ny)];
     lds_fft<T, E, true, true>(re, im, 3, px, tw);
     if (!CELLS) {
-        lds_to_runs<T, 3>(R, ra, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
+        lds_to_runs<T, 3, MRMG_RU(3, E)>(R, ra, Hr, 0, has1 ? 2 : 1, 1, nx, re, im, roff);
         return;
     }
     const size_t plane = (size_t)mx * ny;
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(256) k_rowT_fwd(DemagOps<T> one, const DemagOp
     if (L >= nyb * 3 * nl) return;
     const int y0 = (L % nyb) * TY, kc = L / nyb;        // kc = k*3 + c
     const int k = kc / 3, c = kc - 3 * k;
-    runs_to_lds<T, 1>(R, Mg, G, c, k, ny, y0, min(TY, ny - y0), B, nx, re, im, roff, (C2<T>*)nullptr, twx);
+    runs_to_lds<T, 1, MRMG_RU(1, E)>(R, Mg, G, c, k, ny, y0, min(TY, ny - y0), B, nx, re, im, roff, (C2<T>*)nullptr, twx);
     __syncthreads();
     if (WV) {
         lds_fft_wave<T, E, false>(re, im, B, px, tw);
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
     const int k = kc / 3, c = kc - 3 * k;
     const C2<T>* src = S + (size_t)kc * nxh * sy;
     const int nrows = min(TY, ny - y0);
-    RunsAhead ra;
+    RunsAhead<MRMG_RU(1, E)> ra;
     if (!CELLS) ra.prefetch(R, k, ny, y0, nrows);
     // each thread takes one (kx, row pair) and forms z = A + iB at kx and its mirror.  B =
     // TY/2 is 1, 2 or 4: kx and p are a shift and a mask of idx
@@ -522,7 +526,7 @@ __global__ void __launch_bounds__(256) k_rowT_inv(DemagOps<T> one, const DemagOp
         lds_fft<T, E, true, true>(re, im, B, px, tw);
     }
     if (!CELLS) {
-        lds_to_runs<T, 1>(R, ra, Hr, c, nrows, B, nx, re, im, roff);
+        lds_to_runs<T, 1, MRMG_RU(1, E)>(R, ra, Hr, c, nrows, B, nx, re, im, roff);
         return;
     }
     T* hc = Hcell + ((size_t)c * nl + k) * (size_t)mx * ny;

@@ -99,5 +99,28 @@ def autotune(x, steps=10, philox_seed=12345, verbose=False):
     return chosen
 
 
+def tune_keys(x, space, steps=3, philox_seed=12345):
+    """Time each listed value of the given tunables ({key: values}) on context x, one key
+    at a time over the demag classes, and keep the fastest; the state, time and thermal
+    counter are restored as in autotune.  Returns {key: value}."""
+    saved = [x.get_state(k)[0] for k in range(x.n_lay)]
+    saved_mag = [x.get_mag(k) for k in range(x.n_lay)]
+    step_index = x.step_index
+    chosen = {}
+    for key, values in space.items():
+        best, best_v = None, x.get_tune(key)
+        for v in values:
+            c = _try(x, {key: v}, DEMAG, steps, philox_seed)
+            if c is not None and (best is None or c < best):
+                best, best_v = c, v
+        x.tune(key, best_v)
+        chosen[key] = best_v
+    for k, m in enumerate(saved):
+        x.set_state(k, m)
+        x.set_mag(k, saved_mag[k])
+    x.step_index = step_index
+    return chosen
+
+
 def state_equal(a, b):
     return all(np.array_equal(p, q) for p, q in zip(a, b))
