@@ -462,6 +462,11 @@ def test_launch_shapes_do_not_change_results():
     autotune(x, steps=3)
     after = [x.get_state(k)[0] for k in range(4)]
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
+    # the one-key sweep of the open-boundary bench leg: it keeps a listed value and the state
+    from mrm_amd.tune import tune_keys
+    got = tune_keys(x, {"row_block": [2, 4, 8]}, steps=2)
+    assert got["row_block"] in (2, 4, 8) and x.get_tune("row_block") == got["row_block"]
+    assert all(np.array_equal(p, x.get_state(k)[0]) for k, p in enumerate(before))
 
 
 def test_launch_shapes_agree_at_power_of_two_sizes():
